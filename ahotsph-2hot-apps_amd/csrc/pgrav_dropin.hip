@@ -1,0 +1,311 @@
+// Kernels behind the drop-in entry points of nln/gravcuda.h (grav_mn_CUDA,
+// grav_mns_CUDA, grav_mnss_CUDA, grav_qns_CUDA, grav_qnss_CUDA).
+//
+// MI355X mapping: one wave64 = 64 sinks, one lane per sink.  Sources that every
+// lane of a wave shares (the M x N kernels) are read through wave-uniform
+// addresses, so the compiler emits scalar loads into SGPRs and the whole wave
+// consumes each source record as a broadcast operand: no LDS traffic and no
+// per-lane gathers.  Long source lists are split over blockIdx.y so that one
+// call fills the chip; partial results meet in the device accumulator with a
+// float atomic add, exactly as the reference's concurrent streams do
+// (nln/pgrav.cu:207-210).
+#include "pgrav_device.h"
+#include "pgrav_internal.h"
+
+using namespace pgrav;
+
+// nln/pgrav.cu:16-47, 66: SoA-8 source field k of source i.
+__device__ __forceinline__ float soa8(const float *__restrict__ f, int i, int k, int sz) {
+    return f[(i >> 3) * 8 * sz + k * 8 + (i & 7)];
+}
+
+// SoA-8 reference records -> (pos4, prepared record).  Zero-mass padding
+// entries (nln/mac.c:486-520 pads every list to a multiple of NSSE with zero
+// records) are parked at 1e8 so they contribute exactly zero even for a sink
+// at the origin (where the reference would form 0 * inf).  1e8 keeps x^4 finite
+// in FP32, so no inf - inf can appear in the order-4 basis.
+__global__ void k_prep_soa8(const float *__restrict__ f, int source_n, int sz, int order,
+                            int dipole, float4 *__restrict__ pos4, float *__restrict__ mom,
+                            int mstride) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= source_n)
+        return;
+    float q[29];
+    const int nf = order == 0 ? 4 : (order == 2 ? 13 : 29);
+    for (int k = 0; k < 29; k++) q[k] = k < nf ? soa8(f, i, k, sz) : 0.0f;
+    float4 p = make_float4(q[1], q[2], q[3], q[0]);
+    bool zero = q[0] == 0.0f;
+    for (int k = 4; k < nf; k++) zero = zero && q[k] == 0.0f;
+    if (zero)
+        p = make_float4(1e8f, 1e8f, 1e8f, 0.0f);
+    pos4[i] = p;
+    if (order > 0) {
+        float rec[PREC4];
+        prep_record(q, order, dipole, rec);
+        for (int k = 0; k < mstride; k++) mom[(size_t)i * mstride + k] = rec[k];
+    }
+}
+
+// AoS hexacell table (nln/physics.h:140-157, byte stride `stride`) -> prepared
+// order-2 table for pQQ1.  Field offsets in floats: mass 0, pos 1..3, bmax 5,
+// qx..qz 12..14, qxx..qyz 15..19.
+__global__ void k_prep_hexacell(const float *__restrict__ tab, int stride_floats, int64_t n,
+                                float4 *__restrict__ pos4, float *__restrict__ mom) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float *c = tab + i * stride_floats;
+    float q[29] = {0};
+    q[0] = c[0]; q[1] = c[1]; q[2] = c[2]; q[3] = c[3]; q[4] = c[5];
+    for (int k = 0; k < 8; k++) q[5 + k] = c[12 + k];
+    pos4[i] = make_float4(q[1], q[2], q[3], q[0]);
+    float rec[PREC4];
+    prep_record(q, 2, 1, rec);
+    for (int k = 0; k < PREC2; k++) mom[i * PREC2 + k] = rec[k];
+}
+
+__device__ __forceinline__ void flush_acc(float *__restrict__ acc, const Acc4 &a) {
+    atomicAdd(acc + 0, a.x);
+    atomicAdd(acc + 1, a.y);
+    atomicAdd(acc + 2, a.z);
+    atomicAdd(acc + 3, a.w);
+}
+
+// pH / pQ / pM / pM_sK1 (nln/pgrav.cu:49-468): m sinks x n sources.
+// ORDER 0 with SOFT == SOFT_K1 is pM_sK1 (e = 1/eps).
+template <int ORDER, int QSIGN, int SOFT>
+__global__ __launch_bounds__(64) void k_mn(const float *__restrict__ sinkpos,
+                                           float *__restrict__ acc, int m,
+                                           const float4 *__restrict__ pos4,
+                                           const float *__restrict__ mom, int n, int chunk,
+                                           float e) {
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    const int i0 = blockIdx.y * chunk;
+    const int i1 = min(n, i0 + chunk);
+    const bool live = s < m;
+    const int sc = live ? s : 0;  // dead lanes compute on a valid sink, never store
+    const float px = sinkpos[3 * sc], py = sinkpos[3 * sc + 1], pz = sinkpos[3 * sc + 2];
+    const float eps2k1 = SOFT == SOFT_K1 ? 1.0f / (e * e) : 0.0f;
+    Acc4 a = {0.f, 0.f, 0.f, 0.f};
+    for (int i = i0; i < i1; i++) {
+        const float4 q = pos4[i];
+        const float x = px - q.x, y = py - q.y, z = pz - q.z;
+        if (ORDER == 0) {
+            if (SOFT == SOFT_K1)
+                k1_interact(q.w, x, y, z, e, eps2k1, a);
+            else
+                mono_interact(q.w, x, y, z, 0.0f, a);
+        } else {
+            const float *M = mom + (size_t)i * (ORDER == 4 ? PREC4 : PREC2);
+            cell_interact<ORDER, true, QSIGN>(M, q.w, x, y, z, 0.0f, a);
+        }
+    }
+    if (live)
+        flush_acc(acc + 4 * s, a);
+}
+
+// pMM_sK1 (nln/pgrav.cu:575-637): every sink against the bodies of a
+// segment list into the sink table.  The segment list is shared by all lanes.
+__global__ __launch_bounds__(64) void k_mns(const float *__restrict__ sinkpos,
+                                            float *__restrict__ acc, int m,
+                                            const uint32_t *__restrict__ seg, int seg_n,
+                                            const float *__restrict__ srcpos, float mmass,
+                                            float eps_inv, int *__restrict__ errflag,
+                                            int source_n) {
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    const bool live = s < m;
+    const int sc = live ? s : 0;
+    const float px = sinkpos[3 * sc], py = sinkpos[3 * sc + 1], pz = sinkpos[3 * sc + 2];
+    const float eps2 = 1.0f / (eps_inv * eps_inv);
+    Acc4 a = {0.f, 0.f, 0.f, 0.f};
+    int count = 0;
+    for (int g = 0; g < seg_n; g++) {
+        const uint32_t base = seg[2 * g], len = seg[2 * g + 1];
+        count += (int)len;
+        for (uint32_t j = 0; j < len; j++) {
+            const float *q = srcpos + 3 * (size_t)(base + j);
+            k1_interact(mmass, px - q[0], py - q[1], pz - q[2], eps_inv, eps2, a);
+        }
+    }
+    if (live)
+        flush_acc(acc + 4 * s, a);
+    if (count != source_n && threadIdx.x == 0 && blockIdx.x == 0)
+        atomicAdd(errflag, 1);  // assert(sum_n == source_n), nln/pgrav.cu:636
+}
+
+// pMM1_mnss_sK1 (nln/pgrav.cu:713-790): sink i uses segment run ss_seg[ss_index[i]].
+__global__ __launch_bounds__(256) void k_mnss(const float *__restrict__ sinkpos,
+                                             float *__restrict__ acc, int m,
+                                             const uint16_t *__restrict__ ss_index,
+                                             const uint32_t *__restrict__ ss_seg, int ss_len,
+                                             const uint32_t *__restrict__ s_seg, int s_seg_len,
+                                             const float *__restrict__ srcpos,
+                                             const int *__restrict__ s_source_n, float mmass,
+                                             float eps_inv, int *__restrict__ errflag) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m)
+        return;
+    const int si = ss_index[s];
+    if (si >= ss_len || ss_seg[2 * si] + ss_seg[2 * si + 1] > (uint32_t)s_seg_len) {
+        atomicAdd(errflag, 1);
+        return;
+    }
+    const uint32_t *seg = s_seg + 2 * (size_t)ss_seg[2 * si];
+    const int seg_len = (int)ss_seg[2 * si + 1];
+    const float px = sinkpos[3 * s], py = sinkpos[3 * s + 1], pz = sinkpos[3 * s + 2];
+    const float eps2 = 1.0f / (eps_inv * eps_inv);
+    Acc4 a = {0.f, 0.f, 0.f, 0.f};
+    int count = 0;
+    for (int g = 0; g < seg_len; g++) {
+        const uint32_t base = seg[2 * g], len = seg[2 * g + 1];
+        count += (int)len;
+        for (uint32_t j = 0; j < len; j++) {
+            const float *q = srcpos + 3 * (size_t)(base + j);
+            k1_interact(mmass, px - q[0], py - q[1], pz - q[2], eps_inv, eps2, a);
+        }
+    }
+    flush_acc(acc + 4 * s, a);
+    if (count != s_source_n[si])
+        atomicAdd(errflag, 1);  // nln/pgrav.cu:779-789
+}
+
+// pQ1_mnss (nln/pgrav.cu:293-368): sink i uses run seg[i] of the packed buffer.
+template <int QSIGN>
+__global__ __launch_bounds__(64) void k_qnss(const float *__restrict__ sinkpos,
+                                             float *__restrict__ acc, int m,
+                                             const uint32_t *__restrict__ seg,
+                                             const float4 *__restrict__ pos4,
+                                             const float *__restrict__ mom) {
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= m)
+        return;
+    const uint32_t base = seg[2 * s], len = seg[2 * s + 1];
+    const float px = sinkpos[3 * s], py = sinkpos[3 * s + 1], pz = sinkpos[3 * s + 2];
+    Acc4 a = {0.f, 0.f, 0.f, 0.f};
+    for (uint32_t j = 0; j < len; j++) {
+        const float4 q = pos4[base + j];
+        cell_interact<2, true, QSIGN>(mom + (size_t)(base + j) * PREC2, q.w, px - q.x, py - q.y,
+                                      pz - q.z, 0.0f, a);
+    }
+    flush_acc(acc + 4 * s, a);
+}
+
+// pQQ1 (nln/pgrav.cu:498-572) with the defects of A5 fixed: bound on the
+// sink count and idx[i] selecting the record.
+template <int QSIGN>
+__global__ __launch_bounds__(64) void k_qns(const float *__restrict__ sinkpos,
+                                            float *__restrict__ acc, int m,
+                                            const int *__restrict__ idx, int source_n,
+                                            const float4 *__restrict__ pos4,
+                                            const float *__restrict__ mom) {
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    const bool live = s < m;
+    const int sc = live ? s : 0;
+    const float px = sinkpos[3 * sc], py = sinkpos[3 * sc + 1], pz = sinkpos[3 * sc + 2];
+    Acc4 a = {0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < source_n; i++) {
+        const int c = idx[i];
+        const float4 q = pos4[c];
+        cell_interact<2, true, QSIGN>(mom + (size_t)c * PREC2, q.w, px - q.x, py - q.y, pz - q.z,
+                                      0.0f, a);
+    }
+    if (live)
+        flush_acc(acc + 4 * s, a);
+}
+
+// ---------------------------------------------------------------- launchers
+namespace pgrav {
+
+hipError_t launch_prep_soa8(const float *f, int source_n, int sz, int order, float4 *pos4,
+                            float *mom, hipStream_t st) {
+    if (source_n <= 0)
+        return hipSuccess;
+    const int mstride = order == 4 ? PREC4 : PREC2;
+    hipLaunchKernelGGL(k_prep_soa8, dim3((source_n + 127) / 128), dim3(128), 0, st, f, source_n,
+                       sz, order, 1, pos4, mom, mstride);
+    return hipGetLastError();
+}
+
+hipError_t launch_prep_hexacell(const float *tab, int stride_floats, int64_t n, float4 *pos4,
+                                float *mom, hipStream_t st) {
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_prep_hexacell, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, tab,
+                       stride_floats, n, pos4, mom);
+    return hipGetLastError();
+}
+
+hipError_t launch_mn(int order, int soft, int qsign, const float *sinkpos, float *acc, int m,
+                     const float4 *pos4, const float *mom, int n, float e, hipStream_t st) {
+    if (m <= 0 || n <= 0)
+        return hipSuccess;
+    const int sink_blocks = (m + 63) / 64;
+    // Split the source list so that one call launches >= ~1024 waves when the
+    // list is long; each wave keeps >= 128 sources to amortise the atomics.
+    int split = 1;
+    while (sink_blocks * split < 1024 && n / (split * 2) >= 128) split *= 2;
+    const int chunk = (n + split - 1) / split;
+    dim3 grid(sink_blocks, split), block(64);
+#define MN(O, Q, S) hipLaunchKernelGGL((k_mn<O, Q, S>), grid, block, 0, st, sinkpos, acc, m, pos4, mom, n, chunk, e)
+    if (order == 0) {
+        if (soft == SOFT_K1) MN(0, 1, SOFT_K1);
+        else MN(0, 1, SOFT_NONE);
+    } else if (order == 2) {
+        if (qsign < 0) MN(2, -1, SOFT_NONE);
+        else MN(2, 1, SOFT_NONE);
+    } else {
+        if (qsign < 0) MN(4, -1, SOFT_NONE);
+        else MN(4, 1, SOFT_NONE);
+    }
+#undef MN
+    return hipGetLastError();
+}
+
+hipError_t launch_mns(const float *sinkpos, float *acc, int m, const uint32_t *seg, int seg_n,
+                      const float *srcpos, float mmass, float eps_inv, int *errflag, int source_n,
+                      hipStream_t st) {
+    if (m <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_mns, dim3((m + 63) / 64), dim3(64), 0, st, sinkpos, acc, m, seg, seg_n,
+                       srcpos, mmass, eps_inv, errflag, source_n);
+    return hipGetLastError();
+}
+
+hipError_t launch_mnss(const float *sinkpos, float *acc, int m, const uint16_t *ss_index,
+                       const uint32_t *ss_seg, int ss_len, const uint32_t *seg, int seg_len,
+                       const float *srcpos, const int *source_n, float mmass, float eps_inv,
+                       int *errflag, hipStream_t st) {
+    if (m <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_mnss, dim3((m + 255) / 256), dim3(256), 0, st, sinkpos, acc, m, ss_index,
+                       ss_seg, ss_len, seg, seg_len, srcpos, source_n, mmass, eps_inv, errflag);
+    return hipGetLastError();
+}
+
+hipError_t launch_qnss(int qsign, const float *sinkpos, float *acc, int m, const uint32_t *seg,
+                       const float4 *pos4, const float *mom, hipStream_t st) {
+    if (m <= 0)
+        return hipSuccess;
+    if (qsign < 0)
+        hipLaunchKernelGGL(k_qnss<-1>, dim3((m + 63) / 64), dim3(64), 0, st, sinkpos, acc, m, seg,
+                           pos4, mom);
+    else
+        hipLaunchKernelGGL(k_qnss<1>, dim3((m + 63) / 64), dim3(64), 0, st, sinkpos, acc, m, seg,
+                           pos4, mom);
+    return hipGetLastError();
+}
+
+hipError_t launch_qns(int qsign, const float *sinkpos, float *acc, int m, const int *idx,
+                      int source_n, const float4 *pos4, const float *mom, hipStream_t st) {
+    if (m <= 0)
+        return hipSuccess;
+    if (qsign < 0)
+        hipLaunchKernelGGL(k_qns<-1>, dim3((m + 63) / 64), dim3(64), 0, st, sinkpos, acc, m, idx,
+                           source_n, pos4, mom);
+    else
+        hipLaunchKernelGGL(k_qns<1>, dim3((m + 63) / 64), dim3(64), 0, st, sinkpos, acc, m, idx,
+                           source_n, pos4, mom);
+    return hipGetLastError();
+}
+
+}  // namespace pgrav

@@ -1,0 +1,37 @@
+// Internal (non-exported) declarations shared by the libpgrav translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pgrav {
+
+// Error reporting (pgrav_abi.hip).  The reference aborts the job on every
+// runtime error (Error() in libsw); pgrav_set_error_mode(1) turns that into a
+// recorded message that pgrav_last_error() returns, for callers that want to
+// handle it.
+void fail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+bool failed();
+void clear_error();
+bool check(hipError_t e, const char *what);
+
+hipError_t launch_prep_soa8(const float *f, int source_n, int sz, int order, float4 *pos4,
+                            float *mom, hipStream_t st);
+hipError_t launch_prep_hexacell(const float *tab, int stride_floats, int64_t n, float4 *pos4,
+                                float *mom, hipStream_t st);
+hipError_t launch_mn(int order, int soft, int qsign, const float *sinkpos, float *acc, int m,
+                     const float4 *pos4, const float *mom, int n, float e, hipStream_t st);
+hipError_t launch_mns(const float *sinkpos, float *acc, int m, const uint32_t *seg, int seg_n,
+                      const float *srcpos, float mmass, float eps_inv, int *errflag,
+                      int source_n, hipStream_t st);
+hipError_t launch_mnss(const float *sinkpos, float *acc, int m, const uint16_t *ss_index,
+                       const uint32_t *ss_seg, int ss_len, const uint32_t *seg, int seg_len,
+                       const float *srcpos, const int *source_n, float mmass, float eps_inv,
+                       int *errflag, hipStream_t st);
+hipError_t launch_qnss(int qsign, const float *sinkpos, float *acc, int m, const uint32_t *seg,
+                       const float4 *pos4, const float *mom, hipStream_t st);
+hipError_t launch_qns(int qsign, const float *sinkpos, float *acc, int m, const int *idx,
+                      int source_n, const float4 *pos4, const float *mom, hipStream_t st);
+
+int quad_phi_sign();
+
+}  // namespace pgrav

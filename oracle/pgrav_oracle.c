@@ -1,0 +1,465 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- FP64 restatement of the reference's
+ * interaction arithmetic.  Used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker; never linked into the product.
+ *
+ * Pins (see tests/test_oracle.py):
+ *   orc_mpole / orc_pmn      -- closed-form multipole convergence against the
+ *                               FP64 direct sum of the cell's own bodies, the
+ *                               single-body special case, the reference's own
+ *                               compiled CPU twins (oracle/_ref) and golden
+ *                               vectors made from them (tests/golden/).
+ *   orc_pm_sk1 / orc_pmm*    -- brute force + K1 continuity at r = eps, golden
+ *                               vectors from the reference's _pgravm_sK1.
+ *   orc_direct               -- Newton's third law, two-body closed forms.
+ *   orc_ewald                -- lattice constant 2.8372975 and symmetry
+ *                               (nln/ewald_le2.c:38-43).
+ *   orc_cell_record          -- parallel-axis identity / tracelessness.
+ */
+#include "pgrav_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* nln/pgrav.cu:66 -- field k of source i in an SoA-nsse block layout */
+static inline double fld(const float *f, int i, int k, int sz, int nsse) {
+    return (double)f[(i / nsse) * nsse * sz + k * nsse + i % nsse];
+}
+
+/* nln/pgrav.cu:66-203 (pH body; pQ stops after the quadrupole, pM after the
+ * monopole) and nln/do_grav_sse4.c:1798-1889 (do_gravph: softened, no dipole). */
+void orc_mpole(const double *q, int order, int dipole, double x, double y, double z, double eps2,
+               int quad_phi_sign, double acc[4]) {
+    double r2 = x * x + y * y + z * z + eps2;
+    double rinv = -1.0 / sqrt(r2); /* -rsqrtf(r2), pgrav.cu:72 */
+    double t = rinv;
+    double eqe = q[ORC_MASS] * t;
+    double rinv2 = t * t;
+    double eq0, eq1, eq2;
+    acc[3] += eqe;
+    eqe *= rinv2;
+    acc[0] += x * eqe;
+    acc[1] += y * eqe;
+    acc[2] += z * eqe;
+    if (order == 0)
+        return;
+    const double R = q[ORC_R];
+
+    if (dipole) { /* pgrav.cu:83-94 */
+        t *= R * rinv2;
+        eq0 = q[ORC_QX] * t;
+        eq1 = q[ORC_QY] * t;
+        eq2 = q[ORC_QZ] * t;
+        eqe = eq0 * x + eq1 * y + eq2 * z;
+        acc[3] += eqe;
+        eqe *= 3.0 * rinv2;
+        acc[0] += x * eqe - eq0;
+        acc[1] += y * eqe - eq1;
+        acc[2] += z * eqe - eq2;
+        t *= 3.0 * R * rinv2;
+    } else { /* do_grav_sse4.c:1836-1837 */
+        t *= rinv2;
+        t *= 3.0 * rinv2 * R * R;
+    }
+
+    /* quadrupole, pgrav.cu:96-120 */
+    eq0 = q[ORC_QXX] * x + q[ORC_QXY] * y + q[ORC_QXZ] * z;
+    eq1 = q[ORC_QYY] * y + q[ORC_QXY] * x + q[ORC_QYZ] * z;
+    eq2 = -(q[ORC_QXX] + q[ORC_QYY]) * z + q[ORC_QXZ] * x + q[ORC_QYZ] * y;
+    eq0 *= t;
+    eq1 *= t;
+    eq2 *= t;
+    eqe = 0.5 * (eq0 * x + eq1 * y + eq2 * z);
+    acc[3] += quad_phi_sign * eqe;
+    eqe *= 5.0 * rinv2;
+    acc[0] += x * eqe - eq0;
+    acc[1] += y * eqe - eq1;
+    acc[2] += z * eqe - eq2;
+    if (order == 2)
+        return;
+
+    /* octupole, pgrav.cu:122-161 */
+    t *= 5.0 * R * rinv2;
+    double xx = 0.5 * x * x, xy = x * y, xz = x * z;
+    double yy = 0.5 * y * y, yz = y * z, zz = 0.5 * z * z;
+    double xxx = ((1.0 / 3.0) * xx - zz) * x;
+    double xxz = (xx - (1.0 / 3.0) * zz) * z;
+    double yyy = ((1.0 / 3.0) * yy - zz) * y;
+    double yyz = (yy - (1.0 / 3.0) * zz) * z;
+    xx -= zz;
+    yy -= zz;
+    eq0 = q[ORC_QXXX] * xx + q[ORC_QXYY] * yy + q[ORC_QXXY] * xy + q[ORC_QXXZ] * xz
+          + q[ORC_QXYZ] * yz;
+    eq1 = q[ORC_QXYY] * xy + q[ORC_QXXY] * xx + q[ORC_QYYY] * yy + q[ORC_QYYZ] * yz
+          + q[ORC_QXYZ] * xz;
+    eq2 = -(q[ORC_QXXX] + q[ORC_QXYY]) * xz - (q[ORC_QXXY] + q[ORC_QYYY]) * yz
+          + q[ORC_QXXZ] * xx + q[ORC_QYYZ] * yy + q[ORC_QXYZ] * xy;
+    eq0 *= t;
+    eq1 *= t;
+    eq2 *= t;
+    eqe = (1.0 / 3.0) * (eq0 * x + eq1 * y + eq2 * z);
+    acc[3] += eqe;
+    eqe *= 7.0 * rinv2;
+    acc[0] += x * eqe - eq0;
+    acc[1] += y * eqe - eq1;
+    acc[2] += z * eqe - eq2;
+
+    /* hexadecapole, pgrav.cu:163-203 */
+    t *= 7.0 * R * rinv2;
+    double xxy = y * xx, xyy = x * yy, xyz = xy * z;
+    eq0 = q[ORC_QXXXX] * xxx + q[ORC_QXYYY] * yyy + q[ORC_QXXXY] * xxy + q[ORC_QXXXZ] * xxz
+          + q[ORC_QXXYY] * xyy + q[ORC_QXXYZ] * xyz + q[ORC_QXYYZ] * yyz;
+    eq1 = q[ORC_QXYYY] * xyy + q[ORC_QXXXY] * xxx + q[ORC_QYYYY] * yyy + q[ORC_QYYYZ] * yyz
+          + q[ORC_QXXYY] * xxy + q[ORC_QXXYZ] * xxz + q[ORC_QXYYZ] * xyz;
+    eq2 = -q[ORC_QXXXX] * xxz - (q[ORC_QXYYY] + q[ORC_QXXXY]) * xyz - q[ORC_QYYYY] * yyz
+          + q[ORC_QXXXZ] * xxx + q[ORC_QYYYZ] * yyy - q[ORC_QXXYY] * (xxz + yyz)
+          + q[ORC_QXXYZ] * xxy + q[ORC_QXYYZ] * xyy;
+    eq0 *= t;
+    eq1 *= t;
+    eq2 *= t;
+    eqe = 0.25 * (eq0 * x + eq1 * y + eq2 * z);
+    acc[3] += eqe;
+    eqe *= 9.0 * rinv2;
+    acc[0] += x * eqe - eq0;
+    acc[1] += y * eqe - eq1;
+    acc[2] += z * eqe - eq2;
+}
+
+/* nln/pgrav.cu:49-410 (pH/pQ/pM) and nln/do_grav_sse4.c:1768-2224 */
+void orc_pmn(int order, int dipole, const float *sink_pos3, int m, const float *f, int source_n,
+             int sz, int nsse, double eps2, int quad_phi_sign, double *out4) {
+    double q[ORC_HSZ];
+    int nf = order == 0 ? 4 : (order == 2 ? 13 : 29);
+    for (int s = 0; s < m; s++) {
+        double acc[4] = {0, 0, 0, 0};
+        for (int i = 0; i < source_n; i++) {
+            memset(q, 0, sizeof(q));
+            for (int k = 0; k < nf; k++) q[k] = fld(f, i, k, sz, nsse);
+            orc_mpole(q, order, dipole, (double)sink_pos3[3 * s] - q[ORC_X],
+                      (double)sink_pos3[3 * s + 1] - q[ORC_Y],
+                      (double)sink_pos3[3 * s + 2] - q[ORC_Z], eps2, quad_phi_sign, acc);
+        }
+        for (int k = 0; k < 4; k++) out4[4 * s + k] = acc[k];
+    }
+}
+
+/* Dehnen K1 pair term (nln/pgrav.cu:745-770, nln/py_pgrav/_pgrav.c:21-42) */
+static void k1_pair(double x, double y, double z, double mass, double eps_inv, double acc[4]) {
+    double eps_inv2 = eps_inv * eps_inv;
+    double eps2 = 1.0 / eps_inv2;
+    double r2 = x * x + y * y + z * z;
+    double eqe;
+    if (r2 > eps2) {
+        double rinv = -1.0 / sqrt(r2);
+        eqe = mass * rinv;
+        acc[3] += eqe;
+        eqe *= rinv * rinv;
+    } else {
+        double u2 = r2 * eps_inv2 - 1.0;
+        double t = (((45.0 / 32.0) * u2 + (-3.0 / 8.0)) * u2 + (1.0 / 2.0)) * u2 - 1.0;
+        t *= mass * eps_inv;
+        acc[3] += t;
+        eqe = ((-135.0 / 16.0) * u2 + (3.0 / 2.0)) * u2 - 1.0;
+        eqe *= mass * eps_inv * eps_inv2;
+    }
+    acc[0] += x * eqe;
+    acc[1] += y * eqe;
+    acc[2] += z * eqe;
+}
+
+/* nln/pgrav.cu:412-468 */
+void orc_pm_sk1(const float *sink_pos3, int m, const float *f, int source_n, int nsse,
+                double eps_inv, double *out4) {
+    for (int s = 0; s < m; s++) {
+        double acc[4] = {0, 0, 0, 0};
+        for (int i = 0; i < source_n; i++) {
+            k1_pair((double)sink_pos3[3 * s] - fld(f, i, 1, 4, nsse),
+                    (double)sink_pos3[3 * s + 1] - fld(f, i, 2, 4, nsse),
+                    (double)sink_pos3[3 * s + 2] - fld(f, i, 3, 4, nsse), fld(f, i, 0, 4, nsse),
+                    eps_inv, acc);
+        }
+        for (int k = 0; k < 4; k++) out4[4 * s + k] = acc[k];
+    }
+}
+
+/* nln/pgrav.cu:575-637 */
+void orc_pmm_sk1(const float *sink_pos3, int m, const uint32_t *seg, int seg_n,
+                 const float *src_pos3, double mmass, double eps_inv, double *out4) {
+    for (int s = 0; s < m; s++) {
+        double acc[4] = {0, 0, 0, 0};
+        for (int g = 0; g < seg_n; g++) {
+            for (uint32_t j = 0; j < seg[2 * g + 1]; j++) {
+                const float *qp = src_pos3 + 3 * (size_t)(seg[2 * g] + j);
+                k1_pair((double)sink_pos3[3 * s] - qp[0], (double)sink_pos3[3 * s + 1] - qp[1],
+                        (double)sink_pos3[3 * s + 2] - qp[2], mmass, eps_inv, acc);
+            }
+        }
+        for (int k = 0; k < 4; k++) out4[4 * s + k] = acc[k];
+    }
+}
+
+/* nln/pgrav.cu:713-790 */
+int orc_pmm1_mnss_sk1(const float *sink_pos3, int m, const uint16_t *ss_index,
+                      const uint32_t *ss_seg, int ss_len, const uint32_t *seg, int seg_len,
+                      const float *src_pos3, const int *source_n, double mmass, double eps_inv,
+                      double *out4) {
+    int bad = 0;
+    for (int s = 0; s < m; s++) {
+        int si = ss_index[s];
+        if (si >= ss_len || ss_seg[2 * si] + ss_seg[2 * si + 1] > (uint32_t)seg_len) {
+            bad++;
+            continue;
+        }
+        const uint32_t *sg = seg + 2 * (size_t)ss_seg[2 * si];
+        int n = (int)ss_seg[2 * si + 1];
+        int cnt = 0;
+        for (int g = 0; g < n; g++) cnt += (int)sg[2 * g + 1];
+        if (cnt != source_n[si])
+            bad++;
+        orc_pmm_sk1(sink_pos3 + 3 * s, 1, sg, n, src_pos3, mmass, eps_inv, out4 + 4 * s);
+    }
+    return bad;
+}
+
+/* nln/pgrav.cu:293-368 */
+void orc_pq1_mnss(const float *sink_pos3, int m, const uint32_t *seg, const float *ff,
+                  int quad_phi_sign, double *out4) {
+    for (int s = 0; s < m; s++) {
+        const float *f = ff + (size_t)seg[2 * s] * 13;
+        orc_pmn(2, 1, sink_pos3 + 3 * s, 1, f, (int)seg[2 * s + 1], 13, 8, 0.0, quad_phi_sign,
+                out4 + 4 * s);
+    }
+}
+
+/* nln/pgrav.cu:498-572 with idx[i] selecting the AoS record (defect A5 fixed) */
+void orc_pqq1(const float *sink_pos3, int m, const int *idx, int source_n, const float *qtab13,
+              int quad_phi_sign, double *out4) {
+    double q[ORC_HSZ];
+    for (int s = 0; s < m; s++) {
+        double acc[4] = {0, 0, 0, 0};
+        for (int i = 0; i < source_n; i++) {
+            memset(q, 0, sizeof(q));
+            for (int k = 0; k < 13; k++) q[k] = qtab13[(size_t)idx[i] * 13 + k];
+            orc_mpole(q, 2, 1, (double)sink_pos3[3 * s] - q[ORC_X],
+                      (double)sink_pos3[3 * s + 1] - q[ORC_Y],
+                      (double)sink_pos3[3 * s + 2] - q[ORC_Z], 0.0, quad_phi_sign, acc);
+        }
+        for (int k = 0; k < 4; k++) out4[4 * s + k] = acc[k];
+    }
+}
+
+/* ---- direct sums (threaded over sinks) ---- */
+typedef struct {
+    const double *sink, *src, *mass;
+    int64_t m0, m1, n;
+    int soft_type;
+    double eps, L;
+    double *out4;
+} dsum_job;
+
+static void *dsum_worker(void *arg) {
+    dsum_job *j = (dsum_job *)arg;
+    for (int64_t s = j->m0; s < j->m1; s++) {
+        double acc[4] = {0, 0, 0, 0};
+        const double px = j->sink[3 * s], py = j->sink[3 * s + 1], pz = j->sink[3 * s + 2];
+        for (int64_t i = 0; i < j->n; i++) {
+            double x = px - j->src[3 * i], y = py - j->src[3 * i + 1], z = pz - j->src[3 * i + 2];
+            double r2 = x * x + y * y + z * z;
+            /* unsoftened: skip the self pair; softened kernels keep it, as the
+             * reference's do_gravp (Plummer) and pMM1_mnss_sK1 (K1) do for a
+             * body on its own list */
+            if (r2 == 0.0 && j->soft_type == 0)
+                continue;
+            double mi = j->mass[i];
+            if (j->soft_type == 4) {
+                k1_pair(x, y, z, mi, 1.0 / j->eps, acc);
+            } else {
+                if (j->soft_type == 1)
+                    r2 += j->eps * j->eps;
+                double rinv = 1.0 / sqrt(r2);
+                double mr = mi * rinv;
+                double mr3 = mr * rinv * rinv;
+                acc[3] -= mr;
+                acc[0] -= x * mr3;
+                acc[1] -= y * mr3;
+                acc[2] -= z * mr3;
+            }
+        }
+        for (int k = 0; k < 4; k++) j->out4[4 * s + k] = acc[k];
+    }
+    return NULL;
+}
+
+static void *ewald_worker(void *arg) {
+    dsum_job *j = (dsum_job *)arg;
+    for (int64_t s = j->m0; s < j->m1; s++) {
+        double acc[4] = {0, 0, 0, 0};
+        for (int64_t i = 0; i < j->n; i++) {
+            double x[3] = {j->src[3 * i] - j->sink[3 * s], j->src[3 * i + 1] - j->sink[3 * s + 1],
+                           j->src[3 * i + 2] - j->sink[3 * s + 2]};
+            double f[3], pot;
+            orc_ewald(x, j->L, f, &pot);
+            /* nln/grav_n2.c:168-170: q->phi += m*pot; q->acc += m*f */
+            acc[0] += j->mass[i] * f[0];
+            acc[1] += j->mass[i] * f[1];
+            acc[2] += j->mass[i] * f[2];
+            acc[3] += j->mass[i] * pot;
+        }
+        for (int k = 0; k < 4; k++) j->out4[4 * s + k] = acc[k];
+    }
+    return NULL;
+}
+
+static void run_threads(void *(*fn)(void *), dsum_job base, int nthreads) {
+    if (nthreads < 1)
+        nthreads = 1;
+    if (nthreads > 64)
+        nthreads = 64;
+    pthread_t th[64];
+    dsum_job jobs[64];
+    int64_t m = base.m1;
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t] = base;
+        jobs[t].m0 = m * t / nthreads;
+        jobs[t].m1 = m * (t + 1) / nthreads;
+        pthread_create(&th[t], NULL, fn, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+}
+
+void orc_direct(const double *sink_pos3, int64_t m, const double *src_pos3,
+                const double *src_mass, int64_t n, int soft_type, double eps, double *out4,
+                int nthreads) {
+    dsum_job b = {sink_pos3, src_pos3, src_mass, 0, m, n, soft_type, eps, 0.0, out4};
+    run_threads(dsum_worker, b, nthreads);
+}
+
+/* nln/ewald.c:13-64 (alpha = 2/L, |n| <= 4 within 4L, |h|^2 <= 16) */
+void orc_ewald(const double x[3], double L, double f[3], double *phi) {
+    const int N = 4;
+    double alpha = 2.0 / L;
+    f[0] = f[1] = f[2] = 0.0;
+    *phi = M_PI / (alpha * alpha * L * L * L);
+    if (x[0] == 0.0 && x[1] == 0.0 && x[2] == 0.0)
+        return;
+    double a1 = 2.0 * alpha / sqrt(M_PI);
+    double a2 = -M_PI * M_PI / (alpha * alpha * L * L);
+    double a3 = 2.0 * M_PI / L;
+    double a4 = 2.0 / (L * L);
+    double a5 = 1.0 / (M_PI * L);
+    for (int i = -N; i <= N; i++)
+        for (int j = -N; j <= N; j++)
+            for (int k = -N; k <= N; k++) {
+                double xx[3] = {x[0] - L * i, x[1] - L * j, x[2] - L * k};
+                double ax2 = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
+                double axx = sqrt(ax2);
+                if (axx > 4.0 * L)
+                    continue;
+                double t1 = erfc(alpha * axx);
+                double t2 = a1 * exp(-alpha * alpha * ax2);
+                double t = (t1 / axx + t2) / ax2;
+                f[0] += t * xx[0];
+                f[1] += t * xx[1];
+                f[2] += t * xx[2];
+                *phi -= t1 / axx;
+            }
+    for (int i = -N; i <= N; i++)
+        for (int j = -N; j <= N; j++)
+            for (int k = -N; k <= N; k++) {
+                double hh = (double)(i * i + j * j + k * k);
+                if (hh > 16 || hh == 0)
+                    continue;
+                double hx = i * x[0] + j * x[1] + k * x[2];
+                double t = a4 * exp(a2 * hh) * sin(a3 * hx) / hh;
+                f[0] += t * i;
+                f[1] += t * j;
+                f[2] += t * k;
+                t = a5 * exp(a2 * hh) * cos(a3 * hx) / hh;
+                *phi -= t;
+            }
+}
+
+void orc_direct_ewald(const double *sink_pos3, int64_t m, const double *src_pos3,
+                      const double *src_mass, int64_t n, double L, double *out4, int nthreads) {
+    dsum_job b = {sink_pos3, src_pos3, src_mass, 0, m, n, 0, 0.0, L, out4};
+    run_threads(ewald_worker, b, nthreads);
+}
+
+/* nln/cofm.c:650-720 (mpole_add_mono), summed over bodies about `center` */
+void orc_raw_moments(const double *pos3, const double *mass, int64_t n, const double center[3],
+                     double raw[35]) {
+    memset(raw, 0, 35 * sizeof(double));
+    for (int64_t i = 0; i < n; i++) {
+        double m = mass[i];
+        double x = pos3[3 * i] - center[0], y = pos3[3 * i + 1] - center[1],
+               z = pos3[3 * i + 2] - center[2];
+        double v[35] = {1,
+                        x, y, z,
+                        x * x, x * y, x * z, y * y, y * z, z * z,
+                        x * x * x, x * x * y, x * x * z, x * y * y, x * y * z, x * z * z,
+                        y * y * y, y * y * z, y * z * z, z * z * z,
+                        x * x * x * x, x * x * x * y, x * x * x * z, x * x * y * y, x * x * y * z,
+                        x * x * z * z, x * y * y * y, x * y * y * z, x * y * z * z, x * z * z * z,
+                        y * y * y * y, y * y * y * z, y * y * z * z, y * z * z * z, z * z * z * z};
+        for (int k = 0; k < 35; k++) raw[k] += m * v[k];
+    }
+}
+
+/* nln/cofm.c:398-562 (CellFromCofm, hexacell branch), FP64 */
+void orc_cell_record(const double *pos3, const double *mass, int64_t n, const double center[3],
+                     double bmax, double rec[ORC_HSZ]) {
+    double u[35];
+    orc_raw_moments(pos3, mass, n, center, u);
+    enum { M, X, Y, Z, X2, XY, XZ, Y2, YZ, Z2, X3, X2Y, X2Z, XY2, XYZ, XZ2, Y3, Y2Z, YZ2, Z3,
+           X4, X3Y, X3Z, X2Y2, X2YZ, X2Z2, XY3, XY2Z, XYZ2, XZ3, Y4, Y3Z, Y2Z2, YZ3, Z4 };
+    double rinv = 1.0 / bmax;
+    rec[ORC_MASS] = u[M];
+    rec[ORC_X] = center[0];
+    rec[ORC_Y] = center[1];
+    rec[ORC_Z] = center[2];
+    rec[ORC_R] = bmax;
+    rec[ORC_QX] = rinv * u[X];
+    rec[ORC_QY] = rinv * u[Y];
+    rec[ORC_QZ] = rinv * u[Z];
+    double t = (u[X2] + u[Y2] + u[Z2]) / 3.0;
+    double r2 = rinv * rinv;
+    rec[ORC_QXX] = (u[X2] - t) * r2;
+    rec[ORC_QXY] = u[XY] * r2;
+    rec[ORC_QYY] = (u[Y2] - t) * r2;
+    rec[ORC_QXZ] = u[XZ] * r2;
+    rec[ORC_QYZ] = u[YZ] * r2;
+    double tx = (u[X3] + u[XY2] + u[XZ2]) / 5.0;
+    double ty = (u[X2Y] + u[Y3] + u[YZ2]) / 5.0;
+    double tz = (u[X2Z] + u[Y2Z] + u[Z3]) / 5.0;
+    double r3 = r2 * rinv;
+    rec[ORC_QXXX] = (u[X3] - 3.0 * tx) * r3;
+    rec[ORC_QXXY] = (u[X2Y] - ty) * r3;
+    rec[ORC_QXYY] = (u[XY2] - tx) * r3;
+    rec[ORC_QYYY] = (u[Y3] - 3.0 * ty) * r3;
+    rec[ORC_QXXZ] = (u[X2Z] - tz) * r3;
+    rec[ORC_QXYZ] = u[XYZ] * r3;
+    rec[ORC_QYYZ] = (u[Y2Z] - tz) * r3;
+    double txx = (u[X4] + u[X2Y2] + u[X2Z2]) / 7.0;
+    double txy = (u[X3Y] + u[XY3] + u[XYZ2]) / 7.0;
+    double txz = (u[X3Z] + u[XY2Z] + u[XZ3]) / 7.0;
+    double tyy = (u[X2Y2] + u[Y4] + u[Y2Z2]) / 7.0;
+    double tyz = (u[X2YZ] + u[Y3Z] + u[YZ3]) / 7.0;
+    double tzz = (u[X2Z2] + u[Y2Z2] + u[Z4]) / 7.0;
+    t = 0.1 * (txx + tyy + tzz);
+    double r4 = r2 * r2;
+    rec[ORC_QXXXX] = (u[X4] - 6.0 * (txx - t)) * r4;
+    rec[ORC_QXXXY] = (u[X3Y] - 3.0 * txy) * r4;
+    rec[ORC_QXXYY] = (u[X2Y2] - (txx + tyy - 2.0 * t)) * r4;
+    rec[ORC_QXYYY] = (u[XY3] - 3.0 * txy) * r4;
+    rec[ORC_QYYYY] = (u[Y4] - 6.0 * (tyy - t)) * r4;
+    rec[ORC_QXXXZ] = (u[X3Z] - 3.0 * txz) * r4;
+    rec[ORC_QXXYZ] = (u[X2YZ] - tyz) * r4;
+    rec[ORC_QXYYZ] = (u[XY2Z] - txz) * r4;
+    rec[ORC_QYYYZ] = (u[Y3Z] - 3.0 * tyz) * r4;
+}

@@ -1,0 +1,272 @@
+"""Parity of the drop-in ABI (nln/gravcuda.h) on the GPU against the FP64 oracle.
+
+Every test goes through the C ABI of libpgrav.so (WalkInitSinkCUDA -> qallocCUDA
+-> grav_*_CUDA -> WalkTerminateSinkCUDA) and compares the accelerations and
+potentials added into the body table with the oracle's FP64 restatement of the
+same reference kernel.  Tolerance: FP32 evaluation, so |gpu - oracle| is
+bounded by 2e-5 of the largest |component| of the sink's result (a sum of
+<= a few hundred terms, each rounded at ~6e-8).
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+STRIDE = 14  # sizeof(body)/sizeof(float), nln/physics.h:33-44
+
+
+def _bodies(rng, n, scale=3.0):
+    b = np.zeros((n, STRIDE), np.float32)
+    b[:, 0] = 1.0 / n
+    b[:, 1:4] = rng.uniform(-scale, scale, (n, 3))
+    return b
+
+
+def _cells(rng, n, spread=0.1, dipole=True):
+    recs = []
+    for _ in range(n):
+        pos = rng.uniform(-1, 1, 3) + rng.normal(size=(30, 3)) * spread
+        mass = rng.uniform(0.5, 1.5, 30) / 30
+        cm = (pos * mass[:, None]).sum(0) / mass.sum()
+        bmax = np.sqrt(((pos - cm) ** 2).sum(1)).max()
+        r = oracle.cell_record(pos, mass, cm, bmax)
+        if dipole:
+            r[5:8] = rng.normal(size=3) * 0.1 * r[0]
+        recs.append(r)
+    return np.array(recs)
+
+
+def _close(got, ref, tol=2e-5):
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    err = np.abs(got - ref) / scale
+    assert err.max() < tol, (err.max(), np.unravel_index(err.argmax(), err.shape))
+
+
+@pytest.mark.parametrize("routine,order,sz", [("pH", 4, 29), ("pQ", 2, 13), ("pM", 0, 4)])
+@pytest.mark.parametrize("qsign", [1, -1])
+def test_grav_mn(gpu_lib, routine, order, sz, qsign):
+    pgrav = gpu_lib
+    rng = np.random.default_rng(100 + order)
+    g = pgrav.GravCuda()
+    pgrav.set_quad_phi_sign(qsign)
+    try:
+        btab = _bodies(rng, 300)
+        btab[:, 7:11] = rng.normal(size=(300, 4))  # results are added (+=)
+        before = btab[:, 7:11].copy()
+        recs = _cells(rng, 77)
+        n = 80  # padded to NSSE with zero records, as nln/mac.c:486-520 does
+        f = oracle.to_soa(recs[:, :sz], 8)
+        flat = btab.reshape(-1)
+        g.walk_init_sink(flat, STRIDE)
+        q, _ = g.qalloc()
+        g.grav_mn(routine, 17, 250, f, n, sz, 0.0, q)
+        q2, _ = g.qalloc()
+        g.grav_mn(routine, 200, 100, f, n, sz, 0.0, q2)  # overlaps 200..266
+        g.walk_terminate_sink(flat, STRIDE)
+        ref = np.zeros((300, 4))
+        o = oracle.pmn(order, btab[:, 1:4], f, n, sz, quad_phi_sign=qsign)
+        ref[17:267] += o[17:267]
+        ref[200:300] += o[200:300]
+        got = btab[:, 7:11] - before
+        _close(got[17:300], ref[17:300])
+        assert np.all(got[:17] == 0)
+    finally:
+        pgrav.set_quad_phi_sign(1)
+
+
+def test_grav_mn_pM_sK1(gpu_lib):
+    pgrav = gpu_lib
+    rng = np.random.default_rng(5)
+    g = pgrav.GravCuda()
+    btab = _bodies(rng, 128, scale=1.0)
+    src = np.zeros((64, 4), np.float32)
+    src[:, 0] = rng.uniform(0.1, 1.0, 64)
+    src[:, 1:4] = btab[rng.integers(0, 128, 64), 1:4] + rng.normal(size=(64, 3)) * 0.2
+    eps = 0.3
+    f = oracle.to_soa(src, 8)
+    flat = btab.reshape(-1)
+    g.walk_init_sink(flat, STRIDE)
+    q, _ = g.qalloc()
+    g.grav_mn("pM_sK1", 0, 128, f, 64, 4, 1.0 / eps, q)
+    g.walk_terminate_sink(flat, STRIDE)
+    ref = oracle.pm_sk1(btab[:, 1:4], f, 64, 1.0 / eps)
+    _close(btab[:, 7:11], ref)
+
+
+def test_grav_mns_pMM_sK1(gpu_lib):
+    pgrav = gpu_lib
+    rng = np.random.default_rng(6)
+    g = pgrav.GravCuda()
+    btab = _bodies(rng, 500, scale=0.5)
+    seg = np.array([[10, 40], [200, 7], [300, 120], [499, 1]], np.uint32)
+    eps = 0.05
+    flat = btab.reshape(-1)
+    g.walk_init_sink(flat, STRIDE)
+    q, _ = g.qalloc()
+    g.grav_mns("pMM_sK1", 64, 130, seg.reshape(-1), int(seg[:, 1].sum()), 0.002, 1.0 / eps, q)
+    g.walk_terminate_sink(flat, STRIDE)
+    ref = oracle.pmm_sk1(btab[64:194, 1:4], seg.reshape(-1), btab[:, 1:4], 0.002, 1.0 / eps)
+    _close(btab[64:194, 7:11], ref)
+
+
+def test_grav_mnss_queue_layout(gpu_lib):
+    """Batched p-p with the queue layout of nln/mac.c:352-392: groups of sinks,
+    a zero-work gap group, per-group segment runs and source counts."""
+    pgrav = gpu_lib
+    rng = np.random.default_rng(8)
+    g = pgrav.GravCuda()
+    N = 2000
+    btab = _bodies(rng, N, scale=0.5)
+    groups = [(100, 37), (137, 64), (230, 20), (250, 300)]  # gap 201..229
+    ss_index, ss_seg, seg, source_n = [], [], [], []
+    m = 0
+    base = groups[0][0]
+    for first, cnt in groups:
+        if first > base + m:
+            gap = first - (base + m)
+            ss_seg.append((len(seg), 0))
+            source_n.append(0)
+            ss_index += [len(ss_seg) - 1] * gap
+            m += gap
+        segs = [(int(b), int(l)) for b, l in zip(rng.integers(0, N - 64, 5), rng.integers(1, 64, 5))]
+        ss_seg.append((len(seg), len(segs)))
+        source_n.append(sum(l for _, l in segs))
+        seg += segs
+        ss_index += [len(ss_seg) - 1] * cnt
+        m += cnt
+    eps = 0.02
+    flat = btab.reshape(-1)
+    g.walk_init_sink(flat, STRIDE)
+    q, _ = g.qalloc()
+    g.grav_mnss("pMM_mnss_sK1", base, m, ss_index, np.array(ss_seg).reshape(-1),
+                np.array(seg).reshape(-1), 0, source_n, 0.0005, 1.0 / eps, q)
+    g.walk_terminate_sink(flat, STRIDE)
+    ref, bad = oracle.pmm1_mnss_sk1(btab[base:base + m, 1:4], ss_index,
+                                    np.array(ss_seg).reshape(-1), np.array(seg).reshape(-1),
+                                    btab[:, 1:4], source_n, 0.0005, 1.0 / eps)
+    assert bad == 0
+    got = btab[base:base + m, 7:11]
+    live = np.abs(ref).max(axis=1) > 0
+    _close(got[live], ref[live])
+    assert np.all(got[~live] == 0)
+
+
+def test_grav_mnss_count_mismatch_is_reported(gpu_lib):
+    pgrav = gpu_lib
+    g = pgrav.GravCuda()
+    rng = np.random.default_rng(9)
+    btab = _bodies(rng, 64)
+    flat = btab.reshape(-1)
+    g.walk_init_sink(flat, STRIDE)
+    q, _ = g.qalloc()
+    g.grav_mnss("pMM_mnss_sK1", 0, 64, np.zeros(64), np.array([0, 1]), np.array([0, 10]), 0,
+                [11], 0.001, 10.0, q)  # source_n says 11, the segments hold 10
+    with pytest.raises(pgrav.PgravError, match="source_n"):
+        g.walk_terminate_sink(flat, STRIDE)
+
+
+def test_grav_qnss(gpu_lib):
+    pgrav = gpu_lib
+    rng = np.random.default_rng(10)
+    g = pgrav.GravCuda()
+    btab = _bodies(rng, 200)
+    recs = _cells(rng, 48)
+    buf = oracle.to_soa(recs[:, :13], 8)  # packed SoA-8 order-2 buffer
+    seg = np.zeros((120, 2), np.uint32)
+    seg[:60] = (0, 24)
+    seg[60:100] = (24, 24)
+    seg[100:] = (8, 40)
+    flat = btab.reshape(-1)
+    g.walk_init_sink(flat, STRIDE)
+    q, _ = g.qalloc()
+    g.grav_qnss("pQ1_qnss", 40, 120, seg.reshape(-1), buf, 48, q)
+    g.walk_terminate_sink(flat, STRIDE)
+    ref = oracle.pq1_mnss(btab[40:160, 1:4], seg.reshape(-1), buf)
+    _close(btab[40:160, 7:11], ref)
+
+
+def _hexacell_table(recs):
+    """nln/physics.h:140-157 byte layout (160 B per record)."""
+    dt = np.dtype([("mass", "<f4"), ("pos", "<f4", 3), ("rcrit_m", "<f4"), ("bmax", "<f4"),
+                   ("lvl_dau", "<i8"), ("bptr", "<u8"), ("rcrit_q", "<f4"), ("rcrit_h", "<f4"),
+                   ("q", "<f4", 3), ("q2", "<f4", 5), ("q3", "<f4", 7), ("q4", "<f4", 9),
+                   ("u", "<f4", 4)])
+    assert dt.itemsize == 160
+    t = np.zeros(len(recs), dt)
+    t["mass"] = recs[:, 0]
+    t["pos"] = recs[:, 1:4]
+    t["bmax"] = recs[:, 4]
+    t["q"] = recs[:, 5:8]
+    t["q2"] = recs[:, 8:13]
+    t["q3"] = recs[:, 13:20]
+    t["q4"] = recs[:, 20:29]
+    return t
+
+
+def test_grav_qns_with_walk_init_src(gpu_lib):
+    pgrav = gpu_lib
+    rng = np.random.default_rng(11)
+    g = pgrav.GravCuda()
+    recs = _cells(rng, 50)
+    tab = _hexacell_table(recs)
+    raw = np.frombuffer(tab.tobytes(), np.uint8).copy()
+    g.walk_init_src(raw, 160, 50)
+    btab = _bodies(rng, 90)
+    idx = rng.permutation(50)[:33].astype(np.int32)
+    flat = btab.reshape(-1)
+    g.walk_init_sink(flat, STRIDE)
+    q, _ = g.qalloc()
+    g.grav_qns("pQ1", 5, 80, idx, q)
+    g.walk_terminate_sink(flat, STRIDE)
+    g.walk_terminate_src()
+    ref = oracle.pqq1(btab[5:85, 1:4], idx, recs[:, :13].astype(np.float32))
+    _close(btab[5:85, 7:11], ref)
+
+
+def test_qalloc_slots_and_unknown_routine(gpu_lib):
+    pgrav = gpu_lib
+    g = pgrav.GravCuda()
+    assert g.vecwidth() == 4
+    q, inuse = g.qalloc()
+    assert 0 <= q < 32 and 0 <= inuse < 32
+    rng = np.random.default_rng(12)
+    btab = _bodies(rng, 16)
+    flat = btab.reshape(-1)
+    g.walk_init_sink(flat, STRIDE)
+    g.grav_mn("pX", 0, 16, np.zeros(32, np.float32), 8, 4, 0.0, q)  # ignored, as nln/pgrav.cu
+    g.walk_terminate_sink(flat, STRIDE)
+    assert np.all(btab[:, 7:11] == 0)
+    with pytest.raises(pgrav.PgravError):
+        g.grav_qnss("pQ2_qnss", 0, 1, np.array([0, 0]), np.zeros(13 * 8, np.float32), 8, q)
+
+
+def test_py_pgrav_monopole_known_answer(gpu_lib):
+    """nln/py_pgrav/test_pgrav.py:6-34 inputs (np.random.seed(123), one sink at
+    the origin, 16 sources of mass 1/16); the FP64 brute force pins the answer."""
+    pgrav = gpu_lib
+    import pgrav_cu
+    rs = np.random.RandomState(123)
+    n = 16
+    source = np.empty((n, 4), np.float32)
+    source[:, 0] = 1.0 / n
+    source[:, 1] = rs.uniform(0.0, 1.0, n)
+    source[:, 2] = rs.uniform(-0.5, 0.5, n)
+    source[:, 3] = rs.uniform(-0.5, 0.5, n)
+    sink = np.array([1.0, 0.0, 0.0, 0.0], np.float32)
+    d = source[:, 1:4].astype(np.float64)  # source - sink
+    r = np.linalg.norm(d, axis=1)
+    m = source[:, 0].astype(np.float64)
+    exact = np.r_[(m[:, None] * d / r[:, None] ** 3).sum(0), -(m / r).sum()]
+    accp = np.zeros(4, np.float32)
+    pgrav.monopole(sink, accp, 4, source.flatten())
+    np.testing.assert_allclose(accp, exact, rtol=0, atol=2e-6 * np.abs(exact).max())
+    swiz = source.reshape(-1, 8, 4).transpose(0, 2, 1).flatten()  # test_pgrav.py:30
+    accp2 = np.zeros(4, np.float32)
+    pgrav.vec_monopole(sink, accp2, 4, np.ascontiguousarray(swiz))
+    np.testing.assert_allclose(accp2, exact, rtol=0, atol=2e-6 * np.abs(exact).max())
+    pgrav_cu.init()
+    accp3 = np.zeros(4, np.float32)
+    pgrav_cu.monopole(sink, accp3, 4, np.ascontiguousarray(swiz))
+    np.testing.assert_allclose(accp3, exact, rtol=0, atol=2e-6 * np.abs(exact).max())
