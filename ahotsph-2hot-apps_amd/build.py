@@ -27,6 +27,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-I" + INCL
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-munsafe-fp-atomics", "-Wno-unused-result"]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-march=x86-64-v2", "-I" + INCLUDE, "-I" + HOST,
+             "-I" + CSRC,
              "-Wall", "-Wno-unused-function", "-pthread"]
 
 

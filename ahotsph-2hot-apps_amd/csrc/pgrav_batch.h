@@ -1,0 +1,50 @@
+// Device view of a prepared force evaluation (see pgrav_batch.hip for the
+// layout) and the launchers of the batched kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define EWALD_N 64  // Ewald correction table points per axis over [0, L/2]
+
+namespace pgrav {
+
+enum Variant {
+    VAR_REL32C = 0,  // FP32 interactions, group-relative coordinates, compensated sums
+    VAR_REL32 = 1,   // FP32, group-relative, plain sums
+    VAR_ABS32 = 2,   // FP32, absolute coordinates, plain sums (the reference's arithmetic)
+    VAR_F64 = 3,     // FP64 everything, one batched launch
+    VAR_PLAIN64 = 4  // FP64, one launch per group (the reference's dispatch)
+};
+
+struct BatchView {
+    const double4 *bpos = nullptr;
+    const double4 *cpos = nullptr;
+    const float *mom32 = nullptr;
+    const double *mom64 = nullptr;
+    const int64_t *gfirst = nullptr;
+    const int *gcount = nullptr;
+    const double4 *gcen = nullptr;
+    const int *lpt = nullptr;
+    const int64_t *off[4] = {nullptr, nullptr, nullptr, nullptr};
+    const uint32_t *idx[4] = {nullptr, nullptr, nullptr, nullptr};
+    float4 *out32 = nullptr;
+    double4 *out64 = nullptr;
+    int *counter = nullptr;
+    const float4 *ewald32 = nullptr;
+    int ngroups = 0;
+    int soft = 0;
+    double eps = 0;
+    double G = 1;
+    int periodic = 0;
+    double box = 1;
+};
+
+hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, hipStream_t st);
+hipError_t launch_group64(const BatchView &v, int g, hipStream_t st);
+hipError_t launch_reset_counter(int *c, hipStream_t st);
+hipError_t launch_ewald_table(double L, double4 *t64, float4 *t32, hipStream_t st);
+hipError_t launch_direct64(const double4 *bpos, int64_t n, const int64_t *sample, int ns, int soft,
+                           double eps, double G, double4 *part, int nsplit, double4 *out,
+                           hipStream_t st);
+
+}  // namespace pgrav

@@ -1,0 +1,504 @@
+// Batched evaluation of every sink group's interaction list in one launch.
+//
+// Layout in HBM (all device-resident for the whole force evaluation):
+//   bpos   double4[N]      Morton-sorted bodies: FP64 x, y, z and mass
+//   cpos   double4[Nnode]  cell centre of mass (FP64) and mass
+//   mom32  float[Nnode*48] prepared multipole records (pgrav_device.h); mom64 FP64 twin
+//   gfirst/gcount/gcen     group body range and FP64 centre + radius
+//   off[k]/idx[k]          CSR lists per group: order-4 cells, order-2 cells,
+//                          order-0 cells, bodies (u32 node / sorted-body slots)
+//   lpt    int[G]          groups by decreasing cost (longest first)
+//
+// Execution: one wave64 per group, one lane per sink (groups hold <= 64).
+// A list is consumed in tiles of 64 entries: each lane fetches one entry's FP64
+// centre, subtracts the FP64 group centre, rounds the small offset to FP32 and
+// parks it in LDS; then every lane sweeps the tile, reading the offset as an
+// LDS broadcast and the 48-float record through wave-uniform scalar loads.
+// Interactions run in FP32 on group-relative coordinates; tile partial sums
+// are folded into the result with an error-free 2Sum (compensated FP32).
+//
+// Variants (template): FP32 or FP64 arithmetic, group-relative or absolute
+// coordinates, compensated or plain accumulation, Plummer / K1 / no
+// softening, periodic (Ewald table) or isolated, persistent work queue or one
+// workgroup per group.
+#include <math.h>
+
+#include "pgrav_batch.h"
+#include "pgrav_device.h"
+
+namespace pgrav {
+
+template <typename T>
+struct V4 {
+    T x, y, z, w;
+};
+
+struct BatchArgs {
+    const double4 *bpos;
+    const double4 *cpos;
+    const void *mom;  // float* or double* prepared records
+    const int64_t *gfirst;
+    const int *gcount;
+    const double4 *gcen;
+    const int *lpt;
+    const int64_t *off[4];
+    const uint32_t *idx[4];
+    void *out;  // float4* or double4*, sorted body order
+    int *counter;
+    const float4 *ewald;  // 64^3 correction table (periodic)
+    int ngroups;
+    int soft;
+    double eps;
+    double G;
+    double box;
+};
+
+// ---- Ewald correction (periodic): trilinear lookup in the FP32 copy of the
+// FP64 table built by k_ewald_table; x = sink - source, wrapped to [-L/2, L/2].
+template <typename T>
+__device__ __forceinline__ void ewald_corr(const float4 *__restrict__ tab, T m, T x, T y, T z,
+                                           T box, Acc4T<T> &a) {
+    const T half = T(0.5) * box;
+    const T inv_h = T(EWALD_N - 1) / half;
+    T sx = x < 0 ? T(-1) : T(1), sy = y < 0 ? T(-1) : T(1), sz = z < 0 ? T(-1) : T(1);
+    T ax = fabs(x), ay = fabs(y), az = fabs(z);
+    ax = ax > half ? box - ax : ax;  // beyond half a box the nearest image is the other side
+    ay = ay > half ? box - ay : ay;
+    az = az > half ? box - az : az;
+    T u = ax * inv_h, v = ay * inv_h, w = az * inv_h;
+    int i = min((int)u, EWALD_N - 2), j = min((int)v, EWALD_N - 2), k = min((int)w, EWALD_N - 2);
+    T fu = u - T(i), fv = v - T(j), fw = w - T(k);
+    const float4 *p = tab + ((size_t)i * EWALD_N + j) * EWALD_N + k;
+    const float4 c000 = p[0], c001 = p[1], c010 = p[EWALD_N], c011 = p[EWALD_N + 1];
+    const float4 c100 = p[EWALD_N * EWALD_N], c101 = p[EWALD_N * EWALD_N + 1];
+    const float4 c110 = p[EWALD_N * EWALD_N + EWALD_N], c111 = p[EWALD_N * EWALD_N + EWALD_N + 1];
+#define LERP3(f)                                                                              \
+    ((T(1) - fu) * ((T(1) - fv) * ((T(1) - fw) * T(c000.f) + fw * T(c001.f)) +               \
+                    fv * ((T(1) - fw) * T(c010.f) + fw * T(c011.f))) +                         \
+     fu * ((T(1) - fv) * ((T(1) - fw) * T(c100.f) + fw * T(c101.f)) +                          \
+           fv * ((T(1) - fw) * T(c110.f) + fw * T(c111.f))))
+    const T cx = LERP3(x), cy = LERP3(y), cz = LERP3(z), cp = LERP3(w);
+#undef LERP3
+    // table holds C(r) = f_ewald(r) - r/|r|^3 for r = source - sink = -x, odd per axis
+    a.x += m * sx * cx;
+    a.y += m * sy * cy;
+    a.z += m * sz * cz;
+    a.w += m * cp;
+}
+
+template <typename T>
+__device__ __forceinline__ void fold(Acc4T<T> &s, Acc4T<T> &c, const Acc4T<T> &p, bool comp) {
+    if (comp) {
+        two_sum_add(s.x, c.x, p.x);
+        two_sum_add(s.y, c.y, p.y);
+        two_sum_add(s.z, c.z, p.z);
+        two_sum_add(s.w, c.w, p.w);
+    } else {
+        s.x += p.x;
+        s.y += p.y;
+        s.z += p.z;
+        s.w += p.w;
+    }
+}
+
+// Group-relative (REL) or absolute offset of a source centre, rounded to T.
+template <typename T, bool REL, bool PERIODIC>
+__device__ __forceinline__ V4<T> source_offset(double4 c, double4 gc, double box) {
+    double dx = c.x, dy = c.y, dz = c.z;
+    if (REL || PERIODIC) {
+        dx -= gc.x;
+        dy -= gc.y;
+        dz -= gc.z;
+    }
+    if (PERIODIC) {
+        dx -= box * rint(dx / box);
+        dy -= box * rint(dy / box);
+        dz -= box * rint(dz / box);
+        if (!REL) {  // absolute variant: re-add the group centre after wrapping
+            dx += gc.x;
+            dy += gc.y;
+            dz += gc.z;
+        }
+    }
+    return V4<T>{T(dx), T(dy), T(dz), T(c.w)};
+}
+
+template <typename T, bool REL, bool COMP, int SOFT, bool PERIODIC>
+__device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds) {
+    const int lane = threadIdx.x;
+    const int64_t first = A.gfirst[g];
+    const int cnt = A.gcount[g];
+    const double4 gc = A.gcen[g];
+    const bool live = lane < cnt;
+    const double4 sp = A.bpos[first + (live ? lane : 0)];
+    T sx, sy, sz;
+    if (REL) {
+        sx = T(sp.x - gc.x);
+        sy = T(sp.y - gc.y);
+        sz = T(sp.z - gc.z);
+    } else {
+        sx = T(sp.x);
+        sy = T(sp.y);
+        sz = T(sp.z);
+    }
+    const T eps = T(A.eps);
+    const T eps2 = eps * eps;
+    const T eps_inv = SOFT == SOFT_K1 ? T(1) / eps : T(0);
+    const T eps2_cell = SOFT == SOFT_PLUMMER ? eps2 : T(0);
+    const T box = T(A.box);
+    const T *__restrict__ mom = (const T *)A.mom;
+    Acc4T<T> s = {0, 0, 0, 0}, c = {0, 0, 0, 0};
+
+#pragma unroll 1
+    for (int kind = 0; kind < 4; kind++) {
+        const int64_t o0 = A.off[kind][g], o1 = A.off[kind][g + 1];
+        const uint32_t *__restrict__ idx = A.idx[kind];
+        for (int64_t t0 = o0; t0 < o1; t0 += 64) {
+            const int nt = (int)min((int64_t)64, o1 - t0);
+            if (lane < nt) {
+                const uint32_t e = idx[t0 + lane];
+                const double4 src = kind == 3 ? A.bpos[e] : A.cpos[e];
+                lds[lane] = source_offset<T, REL, PERIODIC>(src, gc, A.box);
+            }
+            __syncthreads();
+            Acc4T<T> p = {0, 0, 0, 0};
+            if (kind == 0) {
+                for (int k = 0; k < nt; k++) {
+                    const uint32_t e = idx[t0 + k];
+                    const V4<T> r = lds[k];
+                    const T x = sx - r.x, y = sy - r.y, z = sz - r.z;
+                    cell_interact<4, false, 1, T>(mom + (size_t)e * PREC4, r.w, x, y, z, eps2_cell, p);
+                    if (PERIODIC)
+                        ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
+                }
+            } else if (kind == 1) {
+                for (int k = 0; k < nt; k++) {
+                    const uint32_t e = idx[t0 + k];
+                    const V4<T> r = lds[k];
+                    const T x = sx - r.x, y = sy - r.y, z = sz - r.z;
+                    cell_interact<2, false, 1, T>(mom + (size_t)e * PREC4, r.w, x, y, z, eps2_cell, p);
+                    if (PERIODIC)
+                        ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
+                }
+            } else if (kind == 2) {
+                for (int k = 0; k < nt; k++) {
+                    const V4<T> r = lds[k];
+                    const T x = sx - r.x, y = sy - r.y, z = sz - r.z;
+                    mono_interact<T>(r.w, x, y, z, eps2_cell, p);
+                    if (PERIODIC)
+                        ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
+                }
+            } else {
+                for (int k = 0; k < nt; k++) {
+                    const V4<T> r = lds[k];
+                    const T x = sx - r.x, y = sy - r.y, z = sz - r.z;
+                    if (SOFT == SOFT_K1) {
+                        k1_interact<T>(r.w, x, y, z, eps_inv, eps2, p);
+                    } else if (SOFT == SOFT_PLUMMER) {
+                        mono_interact<T>(r.w, x, y, z, eps2, p);
+                    } else {  // unsoftened: drop the self pair
+                        const bool self = x == T(0) && y == T(0) && z == T(0);
+                        mono_interact<T>(self ? T(0) : r.w, x, y, z, self ? T(1) : T(0), p);
+                    }
+                    if (PERIODIC) {
+                        const bool self = x == T(0) && y == T(0) && z == T(0);
+                        if (!self)
+                            ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
+                    }
+                }
+            }
+            __syncthreads();
+            fold<T>(s, c, p, COMP);
+        }
+    }
+    if (live) {
+        const T Gn = T(A.G);  // nln/mac.c:269-277: results scaled by G at the end
+        V4<T> r = {Gn * (s.x + c.x), Gn * (s.y + c.y), Gn * (s.z + c.z), Gn * (s.w + c.w)};
+        ((V4<T> *)A.out)[first + lane] = r;
+    }
+}
+
+// one workgroup per group, groups taken in longest-first order
+template <typename T, bool REL, bool COMP, int SOFT, bool PERIODIC>
+__global__ __launch_bounds__(64) void k_batch(BatchArgs A) {
+    __shared__ V4<T> lds[64];
+    eval_group<T, REL, COMP, SOFT, PERIODIC>(A, A.lpt[blockIdx.x], lds);
+}
+
+// persistent: a fixed grid drains a device work counter; the last workgroup to
+// leave resets the counters so the launch can be replayed (e.g. from a graph)
+template <typename T, bool REL, bool COMP, int SOFT, bool PERIODIC>
+__global__ __launch_bounds__(64) void k_batch_persistent(BatchArgs A) {
+    __shared__ V4<T> lds[64];
+    __shared__ int w;
+    for (;;) {
+        if (threadIdx.x == 0)
+            w = atomicAdd(&A.counter[0], 1);
+        __syncthreads();
+        const int item = w;
+        __syncthreads();
+        if (item >= A.ngroups)
+            break;
+        eval_group<T, REL, COMP, SOFT, PERIODIC>(A, A.lpt[item], lds);
+    }
+    if (threadIdx.x == 0) {
+        const int done = atomicAdd(&A.counter[1], 1);
+        if (done == (int)gridDim.x - 1) {
+            atomicExch(&A.counter[0], 0);
+            atomicExch(&A.counter[1], 0);
+        }
+    }
+}
+
+// the FP64 "plain" reference dispatch: one launch per group (nln/pgrav.cu:1078)
+template <int SOFT, bool PERIODIC>
+__global__ __launch_bounds__(64) void k_group64(BatchArgs A, int g) {
+    __shared__ V4<double> lds[64];
+    eval_group<double, false, false, SOFT, PERIODIC>(A, g, lds);
+}
+
+__global__ void k_reset_counter(int *c) {
+    if (threadIdx.x < 2)
+        c[threadIdx.x] = 0;
+}
+
+// ---------------------------------------------------------------- FP64 direct sum
+// Sampled FP64 direct sum (pattern of nln/grav_n2.c:162-173): grid.y splits the
+// sources; partial sums go to part[split][sink] and are reduced in fixed order.
+template <int SOFT>
+__global__ __launch_bounds__(64) void k_direct64(const double4 *__restrict__ bpos, int64_t n,
+                                                 const int64_t *__restrict__ sample, int ns,
+                                                 double eps, double4 *__restrict__ part,
+                                                 int64_t chunk) {
+    __shared__ double4 lds[64];
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    const bool live = s < ns;
+    const double4 p = bpos[sample[live ? s : 0]];
+    const int64_t j0 = blockIdx.y * chunk, j1 = min(n, j0 + chunk);
+    const double eps2 = eps * eps, eps_inv = eps > 0 ? 1.0 / eps : 0.0;
+    Acc4T<double> a = {0, 0, 0, 0};
+    for (int64_t t0 = j0; t0 < j1; t0 += 64) {
+        const int nt = (int)min((int64_t)64, j1 - t0);
+        if ((int)threadIdx.x < nt)
+            lds[threadIdx.x] = bpos[t0 + threadIdx.x];
+        __syncthreads();
+        for (int k = 0; k < nt; k++) {
+            const double4 q = lds[k];
+            const double x = p.x - q.x, y = p.y - q.y, z = p.z - q.z;
+            if (SOFT == SOFT_K1) {
+                k1_interact<double>(q.w, x, y, z, eps_inv, eps2, a);
+            } else if (SOFT == SOFT_PLUMMER) {
+                mono_interact<double>(q.w, x, y, z, eps2, a);
+            } else {
+                const bool self = x == 0.0 && y == 0.0 && z == 0.0;
+                mono_interact<double>(self ? 0.0 : q.w, x, y, z, self ? 1.0 : 0.0, a);
+            }
+        }
+        __syncthreads();
+    }
+    if (live)
+        part[(size_t)blockIdx.y * ns + s] = make_double4(a.x, a.y, a.z, a.w);
+}
+
+__global__ void k_direct_reduce(const double4 *__restrict__ part, int ns, int nsplit, double G,
+                                double4 *__restrict__ out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns)
+        return;
+    double4 a = make_double4(0, 0, 0, 0);
+    for (int k = 0; k < nsplit; k++) {
+        const double4 q = part[(size_t)k * ns + s];
+        a.x += q.x;
+        a.y += q.y;
+        a.z += q.z;
+        a.w += q.w;
+    }
+    out[s] = make_double4(G * a.x, G * a.y, G * a.z, G * a.w);
+}
+
+// ---------------------------------------------------------------- Ewald table
+// FP64 restatement of nln/ewald.c:13-64 per table point r = (i, j, k) h,
+// h = (L/2)/(EWALD_N-1); stores C(r) = f_ewald(r) - r/|r|^3 and
+// phi_ewald(r) + 1/|r| (at r = 0 the limits 0 and 2 x 1.4186487533 / L).
+__global__ void k_ewald_table(double L, double4 *__restrict__ t64, float4 *__restrict__ t32) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= EWALD_N * EWALD_N * EWALD_N)
+        return;
+    const int i = id / (EWALD_N * EWALD_N), j = (id / EWALD_N) % EWALD_N, k = id % EWALD_N;
+    const double h = 0.5 * L / (EWALD_N - 1);
+    const double x[3] = {i * h, j * h, k * h};
+    double f[3] = {0, 0, 0}, phi;
+    const int NN = 4;
+    const double alpha = 2.0 / L;
+    phi = M_PI / (alpha * alpha * L * L * L);
+    const double r2x = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+    if (r2x == 0.0) {
+        t64[id] = make_double4(0, 0, 0, 2.0 * 1.4186487533 / L);
+        t32[id] = make_float4(0.f, 0.f, 0.f, (float)(2.0 * 1.4186487533 / L));
+        return;
+    }
+    const double a1 = 2.0 * alpha / sqrt(M_PI);
+    const double a2 = -M_PI * M_PI / (alpha * alpha * L * L);
+    const double a3 = 2.0 * M_PI / L;
+    const double a4 = 2.0 / (L * L);
+    const double a5 = 1.0 / (M_PI * L);
+    for (int p = -NN; p <= NN; p++)
+        for (int q = -NN; q <= NN; q++)
+            for (int s = -NN; s <= NN; s++) {
+                const double xx0 = x[0] - L * p, xx1 = x[1] - L * q, xx2 = x[2] - L * s;
+                const double ax2 = xx0 * xx0 + xx1 * xx1 + xx2 * xx2;
+                const double axx = sqrt(ax2);
+                if (axx > 4.0 * L)
+                    continue;
+                const double t1 = erfc(alpha * axx);
+                const double t2 = a1 * exp(-alpha * alpha * ax2);
+                const double t = (t1 / axx + t2) / ax2;
+                f[0] += t * xx0;
+                f[1] += t * xx1;
+                f[2] += t * xx2;
+                phi -= t1 / axx;
+            }
+    for (int p = -NN; p <= NN; p++)
+        for (int q = -NN; q <= NN; q++)
+            for (int s = -NN; s <= NN; s++) {
+                const double hh = (double)(p * p + q * q + s * s);
+                if (hh > 16 || hh == 0)
+                    continue;
+                const double hx = p * x[0] + q * x[1] + s * x[2];
+                const double e = exp(a2 * hh);
+                const double t = a4 * e * sin(a3 * hx) / hh;
+                f[0] += t * p;
+                f[1] += t * q;
+                f[2] += t * s;
+                phi -= a5 * e * cos(a3 * hx) / hh;
+            }
+    const double r = sqrt(r2x), r3 = r * r2x;
+    const double4 c = make_double4(f[0] - x[0] / r3, f[1] - x[1] / r3, f[2] - x[2] / r3, phi + 1.0 / r);
+    t64[id] = c;
+    t32[id] = make_float4((float)c.x, (float)c.y, (float)c.z, (float)c.w);
+}
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_ewald_table(double L, double4 *t64, float4 *t32, hipStream_t st) {
+    const int n = EWALD_N * EWALD_N * EWALD_N;
+    hipLaunchKernelGGL(k_ewald_table, dim3((n + 127) / 128), dim3(128), 0, st, L, t64, t32);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_counter(int *c, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, c);
+    return hipGetLastError();
+}
+
+template <typename T, bool REL, bool COMP, int SOFT, bool PER>
+static void launch_one(const BatchArgs &A, int persistent_blocks, hipStream_t st) {
+    if (persistent_blocks > 0)
+        hipLaunchKernelGGL((k_batch_persistent<T, REL, COMP, SOFT, PER>), dim3(persistent_blocks),
+                           dim3(64), 0, st, A);
+    else
+        hipLaunchKernelGGL((k_batch<T, REL, COMP, SOFT, PER>), dim3(A.ngroups), dim3(64), 0, st, A);
+}
+
+template <typename T, bool REL, bool COMP>
+static void launch_soft(const BatchArgs &A, bool periodic, int pb, hipStream_t st) {
+    switch (A.soft) {
+    case SOFT_K1:
+        if (periodic) launch_one<T, REL, COMP, SOFT_K1, true>(A, pb, st);
+        else launch_one<T, REL, COMP, SOFT_K1, false>(A, pb, st);
+        break;
+    case SOFT_PLUMMER:
+        if (periodic) launch_one<T, REL, COMP, SOFT_PLUMMER, true>(A, pb, st);
+        else launch_one<T, REL, COMP, SOFT_PLUMMER, false>(A, pb, st);
+        break;
+    default:
+        if (periodic) launch_one<T, REL, COMP, SOFT_NONE, true>(A, pb, st);
+        else launch_one<T, REL, COMP, SOFT_NONE, false>(A, pb, st);
+    }
+}
+
+hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, hipStream_t st) {
+    BatchArgs A;
+    A.bpos = v.bpos;
+    A.cpos = v.cpos;
+    A.mom = variant == VAR_F64 ? (const void *)v.mom64 : (const void *)v.mom32;
+    A.gfirst = v.gfirst;
+    A.gcount = v.gcount;
+    A.gcen = v.gcen;
+    A.lpt = v.lpt;
+    for (int k = 0; k < 4; k++) {
+        A.off[k] = v.off[k];
+        A.idx[k] = v.idx[k];
+    }
+    A.out = variant == VAR_F64 ? (void *)v.out64 : (void *)v.out32;
+    A.counter = v.counter;
+    A.ewald = v.ewald32;
+    A.ngroups = v.ngroups;
+    A.soft = v.soft;
+    A.eps = v.eps;
+    A.G = v.G;
+    A.box = v.box;
+    if (A.ngroups <= 0)
+        return hipSuccess;
+    const bool per = v.periodic != 0;
+    switch (variant) {
+    case VAR_REL32C: launch_soft<float, true, true>(A, per, persistent_blocks, st); break;
+    case VAR_REL32: launch_soft<float, true, false>(A, per, persistent_blocks, st); break;
+    case VAR_ABS32: launch_soft<float, false, false>(A, per, persistent_blocks, st); break;
+    case VAR_F64: launch_soft<double, true, false>(A, per, persistent_blocks, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_group64(const BatchView &v, int g, hipStream_t st) {
+    BatchArgs A;
+    A.bpos = v.bpos;
+    A.cpos = v.cpos;
+    A.mom = v.mom64;
+    A.gfirst = v.gfirst;
+    A.gcount = v.gcount;
+    A.gcen = v.gcen;
+    A.lpt = v.lpt;
+    for (int k = 0; k < 4; k++) {
+        A.off[k] = v.off[k];
+        A.idx[k] = v.idx[k];
+    }
+    A.out = v.out64;
+    A.counter = v.counter;
+    A.ewald = v.ewald32;
+    A.ngroups = v.ngroups;
+    A.soft = v.soft;
+    A.eps = v.eps;
+    A.G = v.G;
+    A.box = v.box;
+    const bool per = v.periodic != 0;
+#define G64(S) \
+    do { if (per) hipLaunchKernelGGL((k_group64<S, true>), dim3(1), dim3(64), 0, st, A, g); \
+         else hipLaunchKernelGGL((k_group64<S, false>), dim3(1), dim3(64), 0, st, A, g); } while (0)
+    switch (v.soft) {
+    case SOFT_K1: G64(SOFT_K1); break;
+    case SOFT_PLUMMER: G64(SOFT_PLUMMER); break;
+    default: G64(SOFT_NONE);
+    }
+#undef G64
+    return hipGetLastError();
+}
+
+hipError_t launch_direct64(const double4 *bpos, int64_t n, const int64_t *sample, int ns, int soft,
+                           double eps, double G, double4 *part, int nsplit, double4 *out,
+                           hipStream_t st) {
+    if (ns <= 0)
+        return hipSuccess;
+    const int64_t chunk = (n + nsplit - 1) / nsplit;
+    dim3 grid((ns + 63) / 64, nsplit);
+    if (soft == SOFT_K1)
+        hipLaunchKernelGGL(k_direct64<SOFT_K1>, grid, dim3(64), 0, st, bpos, n, sample, ns, eps, part, chunk);
+    else if (soft == SOFT_PLUMMER)
+        hipLaunchKernelGGL(k_direct64<SOFT_PLUMMER>, grid, dim3(64), 0, st, bpos, n, sample, ns, eps, part, chunk);
+    else
+        hipLaunchKernelGGL(k_direct64<SOFT_NONE>, grid, dim3(64), 0, st, bpos, n, sample, ns, eps, part, chunk);
+    hipLaunchKernelGGL(k_direct_reduce, dim3((ns + 255) / 256), dim3(256), 0, st, part, ns, nsplit, G, out);
+    return hipGetLastError();
+}
+
+}  // namespace pgrav

@@ -1,0 +1,483 @@
+// Force-evaluation plan: the native runtime around the batched kernels.
+//
+// pgrav_plan_create() runs the host side (Morton sort, octree, FP64 moments,
+// sink groups, interaction lists) and uploads everything once; after that a
+// force evaluation is a single kernel launch on device-resident data, which a
+// caller may also capture into a hipGraph and replay.
+#include <stdio.h>
+#include <string.h>
+
+#include <chrono>
+#include <new>
+#include <stdexcept>
+#include <vector>
+
+#include "../../include/pgrav.h"
+#include "pgrav_batch.h"
+#include "pgrav_device.h"
+#include "pgrav_host.h"
+#include "pgrav_internal.h"
+
+namespace pgrav {
+
+static const int NQ_PLAIN = 32;  // streams for the one-launch-per-group path (nln/pgrav.cu:811)
+
+struct Plan {
+    pgrav_params prm;
+    TreeParams tp;
+    Tree tree;
+    Lists lists;
+    std::vector<int64_t> inv;  // original index -> sorted slot
+    hipStream_t stream = nullptr;
+    hipStream_t qs[NQ_PLAIN] = {};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<void *> allocs;
+    BatchView v;
+    double t_tree = 0, t_lists = 0, t_upload = 0;
+    int64_t nnodes = 0;
+
+    template <typename P>
+    P *dalloc(size_t bytes) {
+        void *p = nullptr;
+        if (!check(hipMalloc(&p, bytes ? bytes : 16), "hipMalloc"))
+            throw std::runtime_error("device allocation failed");
+        allocs.push_back(p);
+        return (P *)p;
+    }
+    template <typename P>
+    P *upload(const P *h, size_t count) {
+        P *d = dalloc<P>(count * sizeof(P));
+        if (count && !check(hipMemcpy(d, h, count * sizeof(P), hipMemcpyHostToDevice), "hipMemcpy"))
+            throw std::runtime_error("upload failed");
+        return d;
+    }
+    ~Plan() {
+        if (gexec)
+            (void)hipGraphExecDestroy(gexec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        if (stream)
+            (void)hipStreamSynchronize(stream);
+        for (void *p : allocs) (void)hipFree(p);
+        for (auto &s : qs)
+            if (s)
+                (void)hipStreamDestroy(s);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+        if (ev0)
+            (void)hipEventDestroy(ev0);
+        if (ev1)
+            (void)hipEventDestroy(ev1);
+    }
+};
+
+static double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static int auto_blocks() {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t pr;
+        if (hipGetDeviceProperties(&pr, dev) == hipSuccess)
+            cus = pr.multiProcessorCount;
+    }
+    return cus * 16;  // 16 single-wave workgroups per CU: 4 per SIMD
+}
+
+static hipStream_t stream_of(Plan *P, void *stream) {
+    return stream ? (hipStream_t)stream : P->stream;
+}
+
+static int evaluate(Plan *P, int variant, hipStream_t st) {
+    if (variant == VAR_PLAIN64) {
+        if (!P->v.mom64) {
+            fail("the FP64 tables were not built (pgrav_params.fp64 = 0)");
+            return -1;
+        }
+        // one launch per group, spread over 32 streams as the reference's slot queue
+        if (!check(hipEventRecord(P->ev0, st), "hipEventRecord"))
+            return -1;
+        for (int q = 0; q < NQ_PLAIN; q++)
+            if (!check(hipStreamWaitEvent(P->qs[q], P->ev0, 0), "hipStreamWaitEvent"))
+                return -1;
+        for (int g = 0; g < P->v.ngroups; g++)
+            if (!check(launch_group64(P->v, g, P->qs[g % NQ_PLAIN]), "group kernel"))
+                return -1;
+        for (int q = 0; q < NQ_PLAIN; q++) {
+            if (!check(hipEventRecord(P->ev1, P->qs[q]), "hipEventRecord") ||
+                !check(hipStreamWaitEvent(st, P->ev1, 0), "hipStreamWaitEvent"))
+                return -1;
+        }
+        return 0;
+    }
+    if (variant == VAR_F64 && !P->v.mom64) {
+        fail("the FP64 tables were not built (pgrav_params.fp64 = 0)");
+        return -1;
+    }
+    if (variant < 0 || variant > VAR_F64) {
+        fail("unknown variant %d", variant);
+        return -1;
+    }
+    int pb = P->prm.persistent_blocks;
+    if (pb < 0)
+        pb = auto_blocks();
+    return check(launch_batch(P->v, variant, pb, st), "batched kernel") ? 0 : -1;
+}
+
+}  // namespace pgrav
+
+using namespace pgrav;
+
+extern "C" {
+
+void pgrav_params_default(pgrav_params *p) {
+    memset(p, 0, sizeof(*p));
+    p->theta = 0.6;
+    p->group_max = 64;
+    p->leaf_max = 16;
+    p->order = 4;
+    p->pcut = 0;
+    p->soft_type = 1;
+    p->eps = 1e-4;
+    p->G = 1.0;
+    p->periodic = 0;
+    p->box = 1.0;
+    p->nthreads = 0;
+    p->merge_groups = 1;
+    p->persistent_blocks = 0;
+    p->fp64 = 1;
+}
+
+void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const pgrav_params *p) {
+    clear_error();
+    if (!pos3 || !mass || n <= 0 || !p) {
+        fail("pgrav_plan_create: bad arguments");
+        return nullptr;
+    }
+    if (p->order != 0 && p->order != 2 && p->order != 4) {
+        fail("order must be 0, 2 or 4");
+        return nullptr;
+    }
+    if (p->soft_type != SOFT_NONE && p->soft_type != SOFT_PLUMMER && p->soft_type != SOFT_K1) {
+        fail("soft_type must be 0 (none), 1 (Plummer) or 4 (K1), nln/mac.c:196-237");
+        return nullptr;
+    }
+    if ((p->soft_type != SOFT_NONE && !(p->eps > 0)) || !(p->theta > 0) || p->group_max < 1 ||
+        p->group_max > 64 || (p->periodic && !(p->box > 0))) {
+        fail("pgrav_plan_create: bad parameters (eps %g theta %g group_max %d box %g)", p->eps,
+             p->theta, p->group_max, p->box);
+        return nullptr;
+    }
+    if (n > (int64_t)UINT32_MAX) {
+        fail("more than 2^32 bodies");
+        return nullptr;
+    }
+    int ndev = 0;
+    if (!check(hipGetDeviceCount(&ndev), "hipGetDeviceCount"))
+        return nullptr;
+    Plan *P = new (std::nothrow) Plan();
+    if (!P) {
+        fail("out of host memory");
+        return nullptr;
+    }
+    try {
+        P->prm = *p;
+        P->tp.theta = p->theta;
+        P->tp.group_max = p->group_max;
+        P->tp.leaf_max = p->leaf_max > 0 ? p->leaf_max : 16;
+        P->tp.order = p->order;
+        P->tp.pcut = p->pcut;
+        P->tp.periodic = p->periodic;
+        P->tp.box = p->box;
+        P->tp.nthreads = p->nthreads;
+        P->tp.merge_groups = p->merge_groups;
+        double t0 = now();
+        build_tree(pos3, mass, n, P->tp, P->tree);
+        double t1 = now();
+        build_groups(P->tree, P->tp, P->lists);
+        build_lists(P->tree, P->tp, P->lists);
+        double t2 = now();
+        P->t_tree = t1 - t0;
+        P->t_lists = t2 - t1;
+        P->nnodes = P->tree.nnodes();
+        P->inv.resize(n);
+        for (int64_t i = 0; i < n; i++) P->inv[P->tree.perm[i]] = i;
+
+        if (!check(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking), "hipStreamCreate") ||
+            !check(hipEventCreate(&P->ev0), "hipEventCreate") ||
+            !check(hipEventCreate(&P->ev1), "hipEventCreate"))
+            throw std::runtime_error("stream");
+        for (int q = 0; q < NQ_PLAIN; q++)
+            if (!check(hipStreamCreateWithFlags(&P->qs[q], hipStreamNonBlocking), "hipStreamCreate"))
+                throw std::runtime_error("stream");
+        // bodies
+        std::vector<double> b4(4 * (size_t)n);
+        for (int64_t i = 0; i < n; i++) {
+            b4[4 * i + 0] = P->tree.pos[3 * i + 0];
+            b4[4 * i + 1] = P->tree.pos[3 * i + 1];
+            b4[4 * i + 2] = P->tree.pos[3 * i + 2];
+            b4[4 * i + 3] = P->tree.mass[i];
+        }
+        P->v.bpos = (const double4 *)P->upload(b4.data(), b4.size());
+        std::vector<double>().swap(b4);
+        // cells
+        std::vector<float> r32;
+        std::vector<double> r64, c4;
+        prepare_cells(P->tree, p->order, r32, p->fp64 ? &r64 : nullptr, c4);
+        P->v.cpos = (const double4 *)P->upload(c4.data(), c4.size());
+        P->v.mom32 = P->upload(r32.data(), r32.size());
+        if (p->fp64)
+            P->v.mom64 = P->upload(r64.data(), r64.size());
+        // groups and lists
+        Lists &L = P->lists;
+        P->v.ngroups = (int)L.ngroups;
+        P->v.gfirst = P->upload(L.gfirst.data(), L.gfirst.size());
+        P->v.gcount = P->upload(L.gcount.data(), L.gcount.size());
+        P->v.gcen = (const double4 *)P->upload(L.gcenter.data(), L.gcenter.size());
+        P->v.lpt = P->upload(L.lpt.data(), L.lpt.size());
+        for (int k = 0; k < NLIST; k++) {
+            P->v.off[k] = P->upload(L.off[k].data(), L.off[k].size());
+            P->v.idx[k] = P->upload(L.idx[k].data(), L.idx[k].size());
+            std::vector<uint32_t>().swap(L.idx[k]);  // the device copy is the one used
+        }
+        P->v.out32 = P->dalloc<float4>(sizeof(float4) * (size_t)n);
+        if (p->fp64)
+            P->v.out64 = P->dalloc<double4>(sizeof(double4) * (size_t)n);
+        P->v.counter = P->dalloc<int>(64);
+        if (!check(hipMemset(P->v.counter, 0, 64), "hipMemset"))
+            throw std::runtime_error("memset");
+        P->v.soft = p->soft_type;
+        P->v.eps = p->eps;
+        P->v.G = p->G;
+        P->v.periodic = p->periodic;
+        P->v.box = p->box;
+        if (p->periodic) {
+            const size_t ne = (size_t)EWALD_N * EWALD_N * EWALD_N;
+            double4 *t64 = P->dalloc<double4>(ne * sizeof(double4));
+            float4 *t32 = P->dalloc<float4>(ne * sizeof(float4));
+            if (!check(launch_ewald_table(p->box, t64, t32, P->stream), "Ewald table kernel"))
+                throw std::runtime_error("ewald");
+            P->v.ewald32 = t32;
+        }
+        if (!check(hipStreamSynchronize(P->stream), "hipStreamSynchronize"))
+            throw std::runtime_error("sync");
+        P->t_upload = now() - t2;
+    } catch (const std::exception &e) {
+        if (!failed())
+            fail("pgrav_plan_create: %s", e.what());
+        delete P;
+        return nullptr;
+    }
+    return P;
+}
+
+void pgrav_plan_destroy(void *plan) { delete (Plan *)plan; }
+
+int pgrav_plan_stats(void *plan, pgrav_stats *st) {
+    Plan *P = (Plan *)plan;
+    if (!P || !st) {
+        fail("pgrav_plan_stats: null argument");
+        return -1;
+    }
+    memset(st, 0, sizeof(*st));
+    st->n = P->tree.n;
+    st->nnodes = P->nnodes;
+    st->ngroups = P->lists.ngroups;
+    for (int k = 0; k < NLIST; k++) {
+        st->pairs[k] = P->lists.pairs[k];
+        st->entries[k] = P->lists.off[k].back();
+    }
+    st->t_tree = P->t_tree;
+    st->t_lists = P->t_lists;
+    st->t_upload = P->t_upload;
+    st->mean_group = P->lists.ngroups ? (double)P->tree.n / P->lists.ngroups : 0.0;
+    return 0;
+}
+
+int pgrav_plan_evaluate(void *plan, int variant, void *stream) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P) {
+        fail("null plan");
+        return -1;
+    }
+    return evaluate(P, variant, stream_of(P, stream));
+}
+
+int pgrav_plan_time(void *plan, int variant, int reps, void *stream, double *ms_per_launch) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P || reps < 1 || !ms_per_launch) {
+        fail("pgrav_plan_time: bad arguments");
+        return -1;
+    }
+    hipStream_t st = stream_of(P, stream);
+    hipEvent_t a, b;
+    if (!check(hipEventCreate(&a), "hipEventCreate") || !check(hipEventCreate(&b), "hipEventCreate"))
+        return -1;
+    int rc = check(hipEventRecord(a, st), "hipEventRecord") ? 0 : -1;
+    for (int r = 0; r < reps && rc == 0; r++) rc = evaluate(P, variant, st);
+    if (rc == 0 && check(hipEventRecord(b, st), "hipEventRecord") &&
+        check(hipEventSynchronize(b), "hipEventSynchronize")) {
+        float ms = 0;
+        if (check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime"))
+            *ms_per_launch = (double)ms / reps;
+        else
+            rc = -1;
+    } else {
+        rc = -1;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+}
+
+int pgrav_plan_graph_capture(void *plan, int variant) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P) {
+        fail("null plan");
+        return -1;
+    }
+    if (variant == VAR_PLAIN64) {
+        fail("the one-launch-per-group path is not captured");
+        return -1;
+    }
+    if (P->gexec) {
+        (void)hipGraphExecDestroy(P->gexec);
+        P->gexec = nullptr;
+    }
+    if (P->graph) {
+        (void)hipGraphDestroy(P->graph);
+        P->graph = nullptr;
+    }
+    if (!check(hipStreamSynchronize(P->stream), "hipStreamSynchronize") ||
+        !check(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+        return -1;
+    int rc = evaluate(P, variant, P->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(P->stream, &g);
+    if (rc || !check(e, "hipStreamEndCapture"))
+        return -1;
+    P->graph = g;
+    return check(hipGraphInstantiate(&P->gexec, P->graph, nullptr, nullptr, 0), "hipGraphInstantiate") ? 0 : -1;
+}
+
+int pgrav_plan_graph_launch(void *plan, void *stream) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P || !P->gexec) {
+        fail("no captured graph");
+        return -1;
+    }
+    return check(hipGraphLaunch(P->gexec, stream_of(P, stream)), "hipGraphLaunch") ? 0 : -1;
+}
+
+int pgrav_plan_synchronize(void *plan) {
+    Plan *P = (Plan *)plan;
+    if (!P) {
+        fail("null plan");
+        return -1;
+    }
+    return check(hipStreamSynchronize(P->stream), "hipStreamSynchronize") ? 0 : -1;
+}
+
+int pgrav_plan_result(void *plan, int variant, double *acc4) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P || !acc4) {
+        fail("pgrav_plan_result: null argument");
+        return -1;
+    }
+    const int64_t n = P->tree.n;
+    if (!check(hipStreamSynchronize(P->stream), "hipStreamSynchronize") ||
+        !check(hipDeviceSynchronize(), "hipDeviceSynchronize"))
+        return -1;
+    if (variant == VAR_F64 || variant == VAR_PLAIN64) {
+        if (!P->v.out64) {
+            fail("no FP64 result buffer");
+            return -1;
+        }
+        std::vector<double> h(4 * (size_t)n);
+        if (!check(hipMemcpy(h.data(), P->v.out64, h.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy"))
+            return -1;
+        for (int64_t i = 0; i < n; i++)
+            for (int k = 0; k < 4; k++) acc4[4 * P->tree.perm[i] + k] = h[4 * i + k];
+    } else {
+        std::vector<float> h(4 * (size_t)n);
+        if (!check(hipMemcpy(h.data(), P->v.out32, h.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy"))
+            return -1;
+        for (int64_t i = 0; i < n; i++)
+            for (int k = 0; k < 4; k++) acc4[4 * P->tree.perm[i] + k] = h[4 * i + k];
+    }
+    return 0;
+}
+
+int pgrav_plan_direct(void *plan, const int64_t *sample, int ns, double *out4) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P || !sample || ns <= 0 || !out4) {
+        fail("pgrav_plan_direct: bad arguments");
+        return -1;
+    }
+    if (P->prm.periodic) {
+        fail("pgrav_plan_direct is the isolated direct sum; periodic runs compare with "
+             "the Ewald direct sum of pgrav_ewald_direct");
+        return -1;
+    }
+    std::vector<int64_t> s(ns);
+    for (int i = 0; i < ns; i++) {
+        if (sample[i] < 0 || sample[i] >= P->tree.n) {
+            fail("sample index %lld out of range", (long long)sample[i]);
+            return -1;
+        }
+        s[i] = P->inv[sample[i]];
+    }
+    const int blocks = (ns + 63) / 64;
+    int nsplit = std::max(1, std::min(4096, 8192 / blocks));
+    int64_t *ds = nullptr;
+    double4 *part = nullptr, *out = nullptr;
+    bool ok = check(hipMalloc(&ds, 8 * (size_t)ns), "hipMalloc") &&
+              check(hipMalloc(&part, sizeof(double4) * (size_t)ns * nsplit), "hipMalloc") &&
+              check(hipMalloc(&out, sizeof(double4) * (size_t)ns), "hipMalloc") &&
+              check(hipMemcpy(ds, s.data(), 8 * (size_t)ns, hipMemcpyHostToDevice), "hipMemcpy") &&
+              check(launch_direct64(P->v.bpos, P->tree.n, ds, ns, P->prm.soft_type, P->prm.eps,
+                                    P->prm.G, part, nsplit, out, P->stream), "direct-sum kernel") &&
+              check(hipMemcpyAsync(out4, out, sizeof(double4) * (size_t)ns, hipMemcpyDeviceToHost,
+                                   P->stream), "hipMemcpyAsync") &&
+              check(hipStreamSynchronize(P->stream), "hipStreamSynchronize");
+    for (void *p : {(void *)ds, (void *)part, (void *)out})
+        if (p)
+            (void)hipFree(p);
+    return ok ? 0 : -1;
+}
+
+int pgrav_plan_device_buffers(void *plan, void **out32, void **out64, void **bpos) {
+    Plan *P = (Plan *)plan;
+    if (!P) {
+        fail("null plan");
+        return -1;
+    }
+    if (out32)
+        *out32 = P->v.out32;
+    if (out64)
+        *out64 = P->v.out64;
+    if (bpos)
+        *bpos = (void *)P->v.bpos;
+    return 0;
+}
+
+int pgrav_plan_permutation(void *plan, int64_t *perm) {
+    Plan *P = (Plan *)plan;
+    if (!P || !perm) {
+        fail("null argument");
+        return -1;
+    }
+    memcpy(perm, P->tree.perm.data(), 8 * (size_t)P->tree.n);
+    return 0;
+}
+
+}  // extern "C"

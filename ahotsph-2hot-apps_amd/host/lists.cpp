@@ -1,0 +1,242 @@
+// Sink groups and per-group interaction lists (host).
+//
+// A sink group is a contiguous run of Morton-sorted bodies: the largest cells
+// holding <= group_max bodies, with adjacent small siblings merged up to
+// group_max so that a wave64 has few idle lanes (nln/mac.c:293-339 evaluates a
+// group as the body range [FirstBody, LastBody] under a sink cell).
+//
+// Each group walks the tree once.  A cell is accepted as a multipole source
+// when it is farther than bmax/theta + r_group from the group centre, i.e. the
+// opening-angle test holds for every sink of the group (the Bmax MAC of
+// nln/cofm.c:568-569 with the sink extent of the dual walk, nln/mac.c:1787-1844).
+// A cell that fails the test is opened; a leaf that fails, or a cell with fewer
+// bodies than the cost cut, contributes its bodies to the group's body list
+// (the terminal-traversal optimisation of nln/mac.c:1834-1843).  Every sink
+// therefore interacts with every body exactly once, either directly or inside
+// one accepted cell: sum over a group's sources of their body counts == N (the
+// invariant nln/mac.c:734 checks).
+#include <math.h>
+
+#include <algorithm>
+#include <atomic>
+#include <numeric>
+#include <stdexcept>
+#include <thread>
+
+#include "pgrav_host.h"
+
+namespace pgrav {
+
+static void emit_group(const Tree &t, int64_t first, int64_t count, int gmax, Lists &l) {
+    // a leaf at the maximum depth may hold more than gmax coincident bodies
+    for (int64_t a = first; a < first + count; a += gmax) {
+        l.gfirst.push_back(a);
+        l.gcount.push_back((int32_t)std::min<int64_t>(gmax, first + count - a));
+    }
+}
+
+static void collect(const Tree &t, int32_t nd, const TreeParams &p, Lists &l) {
+    const int gmax = p.group_max;
+    if (t.count[nd] <= gmax || t.child[nd] < 0) {
+        emit_group(t, t.first[nd], t.count[nd], gmax, l);
+        return;
+    }
+    int64_t pf = -1, pc = 0;  // pending merged run of small siblings
+    for (int j = 0; j < t.nchild[nd]; j++) {
+        const int32_t c = t.child[nd] + j;
+        const int32_t cc = t.count[c];
+        if (cc <= gmax) {
+            if (pf >= 0 && p.merge_groups && pc + cc <= gmax) {
+                pc += cc;
+                continue;
+            }
+            if (pf >= 0)
+                emit_group(t, pf, pc, gmax, l);
+            pf = t.first[c];
+            pc = cc;
+        } else {
+            if (pf >= 0)
+                emit_group(t, pf, pc, gmax, l);
+            pf = -1;
+            pc = 0;
+            collect(t, c, p, l);
+        }
+    }
+    if (pf >= 0)
+        emit_group(t, pf, pc, gmax, l);
+}
+
+void build_groups(const Tree &t, const TreeParams &p, Lists &l) {
+    if (p.group_max < 1 || p.group_max > 64)
+        throw std::runtime_error("group_max must be in [1, 64] (one wave64 lane per sink)");
+    l = Lists();
+    if (p.merge_groups == 2) {
+        // fixed runs of group_max consecutive bodies along the Morton curve
+        for (int64_t a = 0; a < t.n; a += p.group_max) {
+            l.gfirst.push_back(a);
+            l.gcount.push_back((int32_t)std::min<int64_t>(p.group_max, t.n - a));
+        }
+    } else {
+        collect(t, 0, p, l);
+    }
+    l.ngroups = (int64_t)l.gfirst.size();
+    l.gcenter.assign(4 * l.ngroups, 0.0);
+    for (int64_t g = 0; g < l.ngroups; g++) {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        const int64_t f = l.gfirst[g];
+        for (int64_t i = f; i < f + l.gcount[g]; i++)
+            for (int a = 0; a < 3; a++) {
+                lo[a] = std::min(lo[a], t.pos[3 * i + a]);
+                hi[a] = std::max(hi[a], t.pos[3 * i + a]);
+            }
+        double c[3], r2 = 0;
+        for (int a = 0; a < 3; a++) c[a] = 0.5 * (lo[a] + hi[a]);
+        for (int64_t i = f; i < f + l.gcount[g]; i++) {
+            double d2 = 0;
+            for (int a = 0; a < 3; a++) d2 += (t.pos[3 * i + a] - c[a]) * (t.pos[3 * i + a] - c[a]);
+            r2 = std::max(r2, d2);
+        }
+        for (int a = 0; a < 3; a++) l.gcenter[4 * g + a] = c[a];
+        l.gcenter[4 * g + 3] = sqrt(r2);
+    }
+}
+
+static int auto_pcut(int order) {
+    // body-count at which one cell interaction (~113 / ~40 VALU for orders 4 / 2)
+    // costs no more than its bodies (~18 VALU each)
+    return order >= 4 ? 8 : (order >= 2 ? 3 : 2);
+}
+
+void build_lists(const Tree &t, const TreeParams &p, Lists &l) {
+    const int64_t G = l.ngroups;
+    const int nth = default_threads(p.nthreads);
+    const int pcut = p.pcut > 0 ? p.pcut : auto_pcut(p.order);
+    const int kind = p.order >= 4 ? L_HEXA : (p.order >= 2 ? L_QUAD : L_MONO);
+    const double inv_theta = 1.0 / p.theta;
+    const double box = p.box;
+    struct Buf {
+        std::vector<uint32_t> v[NLIST];
+        std::vector<int64_t> gid;
+        std::vector<int64_t> start;  // NLIST per processed group
+        int64_t pairs[NLIST] = {0, 0, 0, 0};
+    };
+    std::vector<Buf> bufs(nth);
+    std::atomic<int64_t> next(0);
+    std::atomic<bool> miscount(false);
+    const int64_t chunk = 64;
+    auto worker = [&](int tid) {
+        Buf &B = bufs[tid];
+        std::vector<int32_t> stack;
+        stack.reserve(256);
+        for (;;) {
+            const int64_t g0 = next.fetch_add(chunk);
+            if (g0 >= G)
+                break;
+            for (int64_t g = g0; g < std::min(G, g0 + chunk); g++) {
+                B.gid.push_back(g);
+                for (int k = 0; k < NLIST; k++) B.start.push_back((int64_t)B.v[k].size());
+                const double gx = l.gcenter[4 * g], gy = l.gcenter[4 * g + 1],
+                             gz = l.gcenter[4 * g + 2], rg = l.gcenter[4 * g + 3];
+                int64_t covered = 0;
+                stack.clear();
+                stack.push_back(0);
+                while (!stack.empty()) {
+                    const int32_t c = stack.back();
+                    stack.pop_back();
+                    double dx = t.cofm[3 * c] - gx, dy = t.cofm[3 * c + 1] - gy,
+                           dz = t.cofm[3 * c + 2] - gz;
+                    if (p.periodic) {
+                        dx -= box * nearbyint(dx / box);
+                        dy -= box * nearbyint(dy / box);
+                        dz -= box * nearbyint(dz / box);
+                    }
+                    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+                    const int32_t cnt = t.count[c];
+                    bool accept = cnt >= pcut && d - rg > t.bmax[c] * inv_theta;
+                    // in a periodic box only cells well inside half a box have an
+                    // unambiguous nearest image
+                    if (accept && p.periodic && t.bmax[c] * inv_theta + rg + t.bmax[c] > 0.5 * box)
+                        accept = false;
+                    if (accept) {
+                        B.v[kind].push_back((uint32_t)c);
+                        covered += cnt;
+                    } else if (t.child[c] < 0 || cnt < pcut) {
+                        const int64_t f = t.first[c];
+                        for (int64_t i = f; i < f + cnt; i++) B.v[L_BODY].push_back((uint32_t)i);
+                        covered += cnt;
+                    } else {
+                        for (int j = t.nchild[c] - 1; j >= 0; j--) stack.push_back(t.child[c] + j);
+                    }
+                }
+                if (covered != t.n)
+                    miscount.store(true);
+                const int64_t *st = &B.start[B.start.size() - NLIST];
+                for (int k = 0; k < NLIST; k++)
+                    B.pairs[k] += (int64_t)l.gcount[g] * ((int64_t)B.v[k].size() - st[k]);
+            }
+        }
+    };
+    if (nth <= 1) {
+        worker(0);
+    } else {
+        std::vector<std::thread> th;
+        for (int i = 0; i < nth; i++) th.emplace_back(worker, i);
+        for (auto &x : th) x.join();
+    }
+    if (miscount.load())
+        throw std::runtime_error("a group's sources do not cover N bodies (nln/mac.c:734)");
+    // assemble CSR in group order
+    std::vector<int32_t> owner(G);
+    std::vector<int64_t> slot(G);
+    for (int tid = 0; tid < nth; tid++)
+        for (size_t j = 0; j < bufs[tid].gid.size(); j++) {
+            owner[bufs[tid].gid[j]] = tid;
+            slot[bufs[tid].gid[j]] = (int64_t)j;
+        }
+    for (int k = 0; k < NLIST; k++) {
+        l.off[k].assign(G + 1, 0);
+        l.pairs[k] = 0;
+    }
+    auto seg = [&](int64_t g, int k, int64_t &a, int64_t &b) {
+        const Buf &B = bufs[owner[g]];
+        const int64_t j = slot[g];
+        a = B.start[j * NLIST + k];
+        b = (j + 1 < (int64_t)B.gid.size()) ? B.start[(j + 1) * NLIST + k] : (int64_t)B.v[k].size();
+    };
+    for (int64_t g = 0; g < G; g++)
+        for (int k = 0; k < NLIST; k++) {
+            int64_t a, b;
+            seg(g, k, a, b);
+            l.off[k][g + 1] = l.off[k][g] + (b - a);
+        }
+    for (int k = 0; k < NLIST; k++) {
+        l.idx[k].resize(l.off[k][G]);
+        std::vector<std::thread> th;
+        const int nt = std::min(nth, 16);
+        for (int t0 = 0; t0 < nt; t0++)
+            th.emplace_back([&, t0] {
+                for (int64_t g = G * t0 / nt; g < G * (t0 + 1) / nt; g++) {
+                    int64_t a, b;
+                    seg(g, k, a, b);
+                    std::copy(bufs[owner[g]].v[k].begin() + a, bufs[owner[g]].v[k].begin() + b,
+                              l.idx[k].begin() + l.off[k][g]);
+                }
+            });
+        for (auto &x : th) x.join();
+        for (int tid = 0; tid < nth; tid++) l.pairs[k] += bufs[tid].pairs[k];
+    }
+    bufs.clear();
+    // longest-processing-time-first order (per-wave VALU estimate)
+    std::vector<double> cost(G);
+    for (int64_t g = 0; g < G; g++)
+        cost[g] = 113.0 * (l.off[L_HEXA][g + 1] - l.off[L_HEXA][g]) +
+                  40.0 * (l.off[L_QUAD][g + 1] - l.off[L_QUAD][g]) +
+                  17.0 * (l.off[L_MONO][g + 1] - l.off[L_MONO][g]) +
+                  20.0 * (l.off[L_BODY][g + 1] - l.off[L_BODY][g]);
+    l.lpt.resize(G);
+    std::iota(l.lpt.begin(), l.lpt.end(), 0);
+    std::stable_sort(l.lpt.begin(), l.lpt.end(),
+                     [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+}
+
+}  // namespace pgrav

@@ -1,0 +1,65 @@
+// Host runtime of the batched path: octree, FP64 moments, sink groups and
+// interaction lists.  These stay on the host, as in the reference (nln/mac.c,
+// nln/cofm.c and libtree); the GPU only evaluates the lists.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+namespace pgrav {
+
+struct TreeParams {
+    double theta = 0.6;    // opening angle: accept a cell when d > bmax/theta + r_group
+    int group_max = 64;    // bodies per sink group (one wave64 lane each)
+    int leaf_max = 16;     // bodies per tree leaf
+    int order = 4;         // highest multipole order used for cells (0, 2, 4)
+    int pcut = 0;          // cells with fewer bodies than this go to the body list
+                           // (0: automatic, the point where a cell costs less than its bodies)
+    int periodic = 0;
+    double box = 1.0;      // periodic box size (origin at 0)
+    int nthreads = 0;      // 0: hardware concurrency
+    int merge_groups = 1;  // merge adjacent small sibling groups up to group_max
+};
+
+struct Tree {
+    int64_t n = 0;
+    std::vector<int64_t> perm;    // sorted slot -> original body index
+    std::vector<double> pos;      // 3n, sorted (FP64 absolute positions)
+    std::vector<double> mass;     // n, sorted
+    // nodes in depth-first pre-order; the children of a node are contiguous
+    std::vector<int64_t> first;   // first body
+    std::vector<int32_t> count;   // bodies below
+    std::vector<int32_t> child;   // first child node (-1 for a leaf)
+    std::vector<uint8_t> nchild;
+    std::vector<uint8_t> level;
+    std::vector<int32_t> parent;
+    std::vector<double> cofm;     // 3 per node, FP64
+    std::vector<double> bmax;     // FP64
+    std::vector<double> raw;      // 35 per node: raw moments about cofm (nln/cofm.c:660-720 order)
+    double x0[3] = {0, 0, 0};
+    double size = 1.0;            // root cube edge
+    int64_t nnodes() const { return (int64_t)count.size(); }
+};
+
+enum ListKind { L_HEXA = 0, L_QUAD = 1, L_MONO = 2, L_BODY = 3, NLIST = 4 };
+
+struct Lists {
+    int64_t ngroups = 0;
+    std::vector<int64_t> gfirst;
+    std::vector<int32_t> gcount;
+    std::vector<double> gcenter;  // 4 per group: centre x, y, z and radius
+    std::vector<int64_t> off[NLIST];   // ngroups + 1 offsets per list kind
+    std::vector<uint32_t> idx[NLIST];  // node indices (cells) or sorted body slots
+    int64_t pairs[NLIST] = {0, 0, 0, 0};  // (sink, source) pairs per kind
+    std::vector<int32_t> lpt;     // groups by decreasing estimated cost
+};
+
+void build_tree(const double *pos, const double *mass, int64_t n, const TreeParams &p, Tree &t);
+void build_groups(const Tree &t, const TreeParams &p, Lists &l);
+void build_lists(const Tree &t, const TreeParams &p, Lists &l);
+// Prepared records for every node: FP32 (48 floats) and optionally FP64.
+void prepare_cells(const Tree &t, int order, std::vector<float> &rec32, std::vector<double> *rec64,
+                   std::vector<double> &cpos4);
+int default_threads(int requested);
+
+}  // namespace pgrav

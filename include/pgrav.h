@@ -114,6 +114,81 @@ void pgrav_clear_error(void);
 void pgrav_set_quad_phi_sign(int sign);
 int pgrav_synchronize(void);
 
+/* ------------------------------------------------------------------ part 3
+ * Batched path.  No single reference counterpart: it replaces the host walk's
+ * stream of grav_mn_CUDA / grav_mnss_CUDA / grav_qnss_CUDA calls (nln/mac.c:
+ * 352-475, 877-1272) for one force evaluation by one launch over every sink
+ * group.  The tree, moments and lists are built on the host (as nln/cofm.c
+ * and nln/mac.c do) inside pgrav_plan_create. */
+typedef struct {
+    double theta;          /* opening angle (Bmax MAC, nln/cofm.c:568-569) */
+    int group_max;         /* bodies per sink group, <= 64 */
+    int leaf_max;          /* bodies per tree leaf */
+    int order;             /* cell multipole order: 0, 2 or 4 */
+    int pcut;              /* cells with fewer bodies are evaluated body by body (0: auto) */
+    int soft_type;         /* 0 none, 1 Plummer, 4 Dehnen K1 (nln/mac.c:196-237) */
+    double eps;            /* softening length */
+    double G;              /* Newton's constant applied to the results (nln/mac.c:269-277) */
+    int periodic;          /* 1: unit-cell images + Ewald correction table */
+    double box;            /* periodic box edge */
+    int nthreads;          /* host threads for the build (0: all) */
+    int merge_groups;      /* merge adjacent small sibling groups */
+    int persistent_blocks; /* 0: one workgroup per group; >0: persistent grid; -1: auto */
+    int fp64;              /* also build the FP64 tables (variants 3 and 4) */
+} pgrav_params;
+
+typedef struct {
+    int64_t n, nnodes, ngroups;
+    int64_t pairs[4];   /* (sink, source) pairs: order-4 cells, order-2 cells, order-0 cells, bodies */
+    int64_t entries[4]; /* list entries per kind */
+    double t_tree, t_lists, t_upload; /* host seconds */
+    double mean_group;  /* bodies per group */
+} pgrav_stats;
+
+/* variants of pgrav_plan_evaluate */
+#define PGRAV_REL32C 0  /* FP32, group-relative coordinates, compensated sums (default) */
+#define PGRAV_REL32 1   /* FP32, group-relative, plain sums */
+#define PGRAV_ABS32 2   /* FP32, absolute coordinates, plain sums */
+#define PGRAV_F64 3     /* FP64, one batched launch */
+#define PGRAV_PLAIN64 4 /* FP64, one launch per group on 32 streams */
+
+void pgrav_params_default(pgrav_params *p);
+/* pos3: n x 3 FP64 positions, mass: n FP64.  Returns NULL on error. */
+void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const pgrav_params *p);
+void pgrav_plan_destroy(void *plan);
+int pgrav_plan_stats(void *plan, pgrav_stats *st);
+/* Enqueues one force evaluation on `stream` (NULL: the plan's stream). */
+int pgrav_plan_evaluate(void *plan, int variant, void *stream);
+/* Average device time of one evaluation over `reps` back-to-back launches,
+ * bracketed by HIP events on the launch stream. */
+int pgrav_plan_time(void *plan, int variant, int reps, void *stream, double *ms_per_launch);
+int pgrav_plan_graph_capture(void *plan, int variant);
+int pgrav_plan_graph_launch(void *plan, void *stream);
+int pgrav_plan_synchronize(void *plan);
+/* (ax, ay, az, phi) x n in the caller's body order, after G scaling. */
+int pgrav_plan_result(void *plan, int variant, double *acc4);
+/* FP64 direct sum (nln/grav_n2.c pattern, same softening) for `ns` bodies
+ * given by their original indices; out4 is ns x 4. */
+int pgrav_plan_direct(void *plan, const int64_t *sample, int ns, double *out4);
+/* Device pointers of the FP32 / FP64 results and the sorted FP64 bodies. */
+int pgrav_plan_device_buffers(void *plan, void **out32, void **out64, void **bpos);
+/* perm[i] = original index of sorted slot i. */
+int pgrav_plan_permutation(void *plan, int64_t *perm);
+
+/* Host-only tree and lists (no device): the exact lists a plan evaluates, for
+ * inspection and for checking against the oracle.  List kinds: 0 order-4
+ * cells, 1 order-2 cells, 2 order-0 cells (node indices), 3 bodies (sorted
+ * slots). */
+void *pgrav_host_plan(const double *pos3, const double *mass, int64_t n, const pgrav_params *p);
+void pgrav_host_destroy(void *h);
+int pgrav_host_stats(void *h, pgrav_stats *st);
+int pgrav_host_group(void *h, int64_t g, int64_t *first, int *count, double *center4);
+int64_t pgrav_host_list(void *h, int64_t g, int kind, uint32_t *out, int64_t cap);
+int pgrav_host_cell(void *h, int64_t node, double *cofm3, double *bmax, double *raw35,
+                    int64_t *first, int *count, int *child, int *nchild);
+int pgrav_host_sorted(void *h, int64_t *perm, double *pos3);
+int pgrav_host_lpt(void *h, int32_t *order);
+
 #ifdef __cplusplus
 }
 #endif

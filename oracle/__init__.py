@@ -48,6 +48,7 @@ def lib():
         L.orc_cell_record.argtypes = [_f64p, _f64p, _i64, _f64p, _d, _f64p]
         L.orc_raw_moments.argtypes = [_f64p, _f64p, _i64, _f64p, _f64p]
         L.orc_mpole.argtypes = [_f64p, _i, _i, _d, _d, _d, _d, _i, _f64p]
+        L.orc_eval_records.argtypes = [_f64p, _i64, _f64p, _i64, _i, _i, _d, _i, _f64p]
         _lib = L
     return _lib
 
@@ -166,6 +167,16 @@ def raw_moments(pos, mass, center):
     p = _c(pos, np.float64).reshape(-1)
     lib().orc_raw_moments(p, _c(mass, np.float64), p.size // 3, _c(center, np.float64), raw)
     return raw
+
+
+def eval_records(sinks, recs, order, dipole=False, eps2=0.0, quad_phi_sign=1):
+    """FP64 sum over a list of 29-field records for each sink."""
+    s = _c(sinks, np.float64).reshape(-1)
+    r = _c(recs, np.float64).reshape(-1)
+    out = np.zeros((s.size // 3, 4), np.float64)
+    lib().orc_eval_records(s, s.size // 3, r, r.size // 29, order, int(dipole), eps2,
+                           quad_phi_sign, out)
+    return out
 
 
 def mpole(rec, order, dx, dipole=True, eps2=0.0, quad_phi_sign=1):

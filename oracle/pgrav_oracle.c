@@ -149,6 +149,21 @@ void orc_pmn(int order, int dipole, const float *sink_pos3, int m, const float *
     }
 }
 
+/* m sinks (FP64) against nc AoS records of 29 doubles: the FP64 sum of
+ * orc_mpole over a cell list. */
+void orc_eval_records(const double *sink_pos3, int64_t m, const double *rec29, int64_t nc,
+                      int order, int dipole, double eps2, int quad_phi_sign, double *out4) {
+    for (int64_t s = 0; s < m; s++) {
+        double acc[4] = {0, 0, 0, 0};
+        for (int64_t c = 0; c < nc; c++) {
+            const double *q = rec29 + 29 * c;
+            orc_mpole(q, order, dipole, sink_pos3[3 * s] - q[ORC_X], sink_pos3[3 * s + 1] - q[ORC_Y],
+                      sink_pos3[3 * s + 2] - q[ORC_Z], eps2, quad_phi_sign, acc);
+        }
+        for (int k = 0; k < 4; k++) out4[4 * s + k] = acc[k];
+    }
+}
+
 /* Dehnen K1 pair term (nln/pgrav.cu:745-770, nln/py_pgrav/_pgrav.c:21-42) */
 static void k1_pair(double x, double y, double z, double mass, double eps_inv, double acc[4]) {
     double eps_inv2 = eps_inv * eps_inv;

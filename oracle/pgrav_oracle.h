@@ -33,6 +33,10 @@ enum {
 void orc_mpole(const double *q, int order, int dipole, double x, double y, double z, double eps2,
                int quad_phi_sign, double acc[4]);
 
+/* m FP64 sinks against nc AoS 29-double records (sum of orc_mpole). */
+void orc_eval_records(const double *sink_pos3, int64_t m, const double *rec29, int64_t nc,
+                      int order, int dipole, double eps2, int quad_phi_sign, double *out4);
+
 /* pM / pQ / pH (nln/pgrav.cu:49-410) and the CPU twins do_gravp/pq/ph
  * (nln/do_grav_sse4.c:1768-2224) over an SoA-nsse source list. */
 void orc_pmn(int order, int dipole, const float *sink_pos3, int m, const float *f, int source_n,

@@ -1,0 +1,45 @@
+"""Oracle evaluation of the exact lists a plan evaluates (test helper).
+
+For a sink group of a HostPlan, every accepted cell's record is rebuilt by the
+oracle from the cell's own bodies (oracle.cell_record, independent of the
+library's moment translation) and evaluated in FP64 with oracle.eval_records;
+the group's body list goes through the oracle's FP64 direct sum.  The periodic
+Ewald correction is not included here (see test_periodic_gpu.py).
+"""
+import numpy as np
+
+import oracle
+
+ORDER = {"hexa": 4, "quad": 2, "mono": 0}
+
+
+def group_reference(H, g, pos_sorted, mass_sorted, soft="plummer", eps=1e-4, G=1.0):
+    first, cnt, _ = H.group(g)
+    sinks = pos_sorted[first:first + cnt]
+    acc = np.zeros((cnt, 4))
+    eps2 = eps * eps if soft == "plummer" else 0.0
+    for kind, order in ORDER.items():
+        nodes = H.list(g, kind)
+        if nodes.size == 0:
+            continue
+        recs = []
+        for nd in nodes:
+            c = H.cell(int(nd))
+            b = pos_sorted[c["first"]:c["first"] + c["count"]]
+            m = mass_sorted[c["first"]:c["first"] + c["count"]]
+            recs.append(oracle.cell_record(b, m, c["cofm"], c["bmax"] if c["bmax"] > 0 else 1.0))
+        acc += oracle.eval_records(sinks, np.array(recs), order, dipole=False, eps2=eps2)
+    bodies = H.list(g, "body")
+    if bodies.size:
+        st = {"none": 0, "plummer": 1, "k1": 4}[soft]
+        acc += oracle.direct(sinks, pos_sorted[bodies], mass_sorted[bodies], soft_type=st, eps=eps)
+    return first, G * acc
+
+
+def interactions_cover_all(H, g, n):
+    """Sum of body counts over a group's sources == n (nln/mac.c:734)."""
+    tot = H.list(g, "body").size
+    for kind in ORDER:
+        for nd in H.list(g, kind):
+            tot += H.cell(int(nd))["count"]
+    return tot == n

@@ -1,0 +1,131 @@
+"""Parity of the batched path (pgrav_plan_*) on the GPU.
+
+1. The exact lists of sampled groups are re-evaluated by the FP64 oracle (cell
+   records rebuilt from each cell's bodies); the kernel variants must agree:
+   FP64 to 1e-10, FP32 group-relative to 3e-6 (relative per sink, |da|/|a|).
+2. The library's FP64 direct sum equals the oracle's FP64 direct sum (1e-12).
+3. Graph replays, the persistent grid and the one-launch-per-group FP64
+   dispatch reproduce the eager batched result.
+"""
+import numpy as np
+import pytest
+
+import listoracle
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup(gpu_lib):
+    from pgrav import batch, ics
+    pos, mass = ics.clustered(40000, seed=21, nhalo=24)
+    return batch, ics, pos, mass
+
+
+def _group_errors(batch, pos, mass, plan_kw, variants, ngroups=12, seed=0):
+    plan = batch.Plan(pos, mass, **plan_kw)
+    H = batch.HostPlan(pos, mass, **plan_kw)
+    perm, spos = H.sorted()
+    smass = mass[perm]
+    out = {}
+    for v in variants:
+        plan.evaluate(v)
+        out[v] = plan.result(v)[perm]  # back to sorted order
+    st = H.stats()
+    rng = np.random.default_rng(seed)
+    errs = {v: [] for v in variants}
+    for g in rng.choice(st["ngroups"], ngroups, replace=False):
+        first, ref = listoracle.group_reference(H, int(g), spos, smass,
+                                                soft=plan_kw.get("soft", "plummer"),
+                                                eps=plan_kw.get("eps", 1e-4),
+                                                G=plan_kw.get("G", 1.0))
+        for v in variants:
+            got = out[v][first:first + ref.shape[0]]
+            da = np.linalg.norm(got[:, :3] - ref[:, :3], axis=1) / np.linalg.norm(ref[:, :3], axis=1)
+            dp = np.abs(got[:, 3] - ref[:, 3]) / np.abs(ref[:, 3])
+            errs[v].append(max(da.max(), dp.max()))
+    plan.close()
+    return {v: max(e) for v, e in errs.items()}
+
+
+@pytest.mark.parametrize("soft,eps", [("plummer", 1e-3), ("k1", 2e-3), ("none", 0.0)])
+def test_batched_variants_match_oracle_on_same_lists(setup, soft, eps):
+    batch, ics, pos, mass = setup
+    kw = dict(theta=0.6, order=4, eps=eps, soft=soft, G=1.0)
+    e = _group_errors(batch, pos, mass, kw, ["rel32c", "rel32", "f64", "abs32"])
+    assert e["f64"] < 1e-10, e
+    assert e["rel32c"] < 3e-6, e
+    assert e["rel32"] < 3e-6, e
+    assert e["abs32"] < 1e-3, e  # absolute FP32 coordinates: the precision loss the
+    #                             group-relative path removes
+
+
+@pytest.mark.parametrize("order", [2, 0])
+def test_lower_orders_match_oracle(setup, order):
+    batch, ics, pos, mass = setup
+    kw = dict(theta=0.5, order=order, eps=1e-3, soft="plummer", G=2.0)
+    e = _group_errors(batch, pos, mass, kw, ["rel32c", "f64"], ngroups=8)
+    assert e["f64"] < 1e-10 and e["rel32c"] < 3e-6, e
+
+
+def test_direct_sum_matches_oracle(setup):
+    batch, ics, pos, mass = setup
+    for soft, st, eps in (("plummer", 1, 1e-3), ("k1", 4, 3e-3), ("none", 0, 0.0)):
+        plan = batch.Plan(pos, mass, eps=eps, soft=soft, G=1.5)
+        sample = np.arange(0, pos.shape[0], 997)
+        got = plan.direct(sample)
+        ref = 1.5 * oracle.direct(pos[sample], pos, mass, soft_type=st, eps=eps)
+        np.testing.assert_allclose(got, ref, rtol=1e-11, atol=0)
+        plan.close()
+
+
+def test_accuracy_against_direct_sum(setup):
+    """Order-4 cells at theta = 0.5: the force error is the expansion error."""
+    batch, ics, pos, mass = setup
+    plan = batch.Plan(pos, mass, theta=0.5, order=4, eps=1e-4)
+    plan.evaluate("rel32c")
+    acc = plan.result("rel32c")
+    sample = np.random.default_rng(3).choice(pos.shape[0], 1024, replace=False)
+    da, dp = batch.rel_errors(acc[sample], plan.direct(sample))
+    assert np.median(da) < 1e-4, np.median(da)
+    assert np.percentile(da, 99) < 2e-3
+    plan.close()
+
+
+def test_graph_persistent_and_plain_dispatch_reproduce_eager(setup):
+    batch, ics, pos, mass = setup
+    plan = batch.Plan(pos, mass, theta=0.6, order=4, eps=1e-4)
+    plan.evaluate("rel32c")
+    eager = plan.result("rel32c")
+    plan.graph_capture("rel32c")
+    for rep in range(3):
+        plan.evaluate("f64")  # unrelated eager work between replays
+        plan.synchronize()
+        plan.graph_launch()
+        np.testing.assert_array_equal(plan.result("rel32c"), eager)
+    plan.close()
+    pers = batch.Plan(pos, mass, theta=0.6, order=4, eps=1e-4, persistent_blocks=-1)
+    for rep in range(2):
+        pers.evaluate("rel32c")
+        np.testing.assert_array_equal(pers.result("rel32c"), eager)
+    pers.graph_capture("rel32c")
+    for rep in range(3):
+        pers.graph_launch()
+        np.testing.assert_array_equal(pers.result("rel32c"), eager)
+    pers.evaluate("f64")
+    f64 = pers.result("f64")
+    pers.evaluate("plain64")
+    plain = pers.result("plain64")
+    np.testing.assert_allclose(plain, f64, rtol=1e-12, atol=1e-12 * np.abs(f64).max())
+    pers.close()
+
+
+def test_plan_rejects_bad_input(gpu_lib):
+    from pgrav import batch
+    from pgrav._lib import PgravError
+    pos = np.random.default_rng(0).uniform(size=(100, 3))
+    with pytest.raises(PgravError):
+        batch.Plan(pos, np.full(100, 0.01), soft="plummer", eps=0.0)
+    with pytest.raises(PgravError):
+        batch.Plan(pos, np.full(100, 0.01), soft=3)

@@ -1,0 +1,137 @@
+"""Host half of the batched path (tree, FP64 moments, groups, lists, ICs) on
+the CPU: the invariants the reference's walk relies on, checked against the
+oracle where the oracle has the answer."""
+import numpy as np
+import pytest
+
+import listoracle
+import oracle
+from pgrav import batch, ics
+from pgrav._lib import PgravError
+
+
+@pytest.fixture(scope="module")
+def clustered():
+    pos, mass = ics.clustered(30000, seed=11, nhalo=32)
+    H = batch.HostPlan(pos, mass, theta=0.6, order=4, eps=1e-4)
+    perm, spos = H.sorted()
+    return pos, mass, H, perm, spos
+
+
+def test_sort_is_a_permutation(clustered):
+    pos, mass, H, perm, spos = clustered
+    assert np.array_equal(np.sort(perm), np.arange(pos.shape[0]))
+    np.testing.assert_array_equal(spos, pos[perm])
+
+
+def test_moment_translation_matches_direct_moments(clustered):
+    """FP64 M2M translation (nln/cofm.c:770-964) == moments summed directly
+    from the cell's bodies about its centre of mass (parallel-axis identity)."""
+    pos, mass, H, perm, spos = clustered
+    smass = mass[perm]
+    st = H.stats()
+    rng = np.random.default_rng(0)
+    for nd in [0, 1, 2] + list(rng.integers(0, st["nnodes"], 60)):
+        c = H.cell(int(nd))
+        b = spos[c["first"]:c["first"] + c["count"]]
+        m = smass[c["first"]:c["first"] + c["count"]]
+        cm = (b * m[:, None]).sum(0) / m.sum()
+        np.testing.assert_allclose(c["cofm"], cm, rtol=0, atol=1e-13)
+        raw = oracle.raw_moments(b, m, c["cofm"])
+        scale = np.abs(raw).max()
+        assert np.abs(raw - c["raw"]).max() <= 1e-10 * scale
+        true_bmax = np.sqrt(((b - cm) ** 2).sum(1)).max()
+        assert c["bmax"] >= true_bmax * (1 - 1e-12) - 1e-14  # 1-body cells: 0 vs round-off
+
+
+def test_groups_partition_bodies(clustered):
+    pos, mass, H, perm, spos = clustered
+    st = H.stats()
+    nxt = 0
+    for g in range(st["ngroups"]):
+        first, cnt, c = H.group(g)
+        assert first == nxt and 1 <= cnt <= 64
+        b = spos[first:first + cnt]
+        assert np.sqrt(((b - c[:3]) ** 2).sum(1)).max() <= c[3] * (1 + 1e-12)
+        nxt += cnt
+    assert nxt == pos.shape[0]
+
+
+def test_lists_cover_every_body_once_and_obey_the_mac(clustered):
+    pos, mass, H, perm, spos = clustered
+    st = H.stats()
+    rng = np.random.default_rng(1)
+    for g in rng.choice(st["ngroups"], 25, replace=False):
+        assert listoracle.interactions_cover_all(H, int(g), pos.shape[0])
+        _, _, c = H.group(int(g))
+        for nd in H.list(int(g), "hexa"):
+            cell = H.cell(int(nd))
+            d = np.linalg.norm(cell["cofm"] - c[:3])
+            assert d - c[3] > cell["bmax"] / 0.6
+        body = H.list(int(g), "body")
+        assert np.unique(body).size == body.size
+
+
+def test_build_is_deterministic():
+    pos, mass = ics.clustered(8000, seed=5, nhalo=8)
+    a = batch.HostPlan(pos, mass, theta=0.5, order=4, eps=1e-3)
+    b = batch.HostPlan(pos, mass, theta=0.5, order=4, eps=1e-3, nthreads=3)
+    assert a.stats()["pairs_total"] == b.stats()["pairs_total"]
+    for g in range(0, a.stats()["ngroups"], 7):
+        for kind in batch.KINDS:
+            np.testing.assert_array_equal(a.list(g, kind), b.list(g, kind))
+    np.testing.assert_array_equal(a.lpt(), b.lpt())
+
+
+@pytest.mark.parametrize("order,kind", [(2, "quad"), (0, "mono")])
+def test_lower_orders_fill_their_lists(order, kind):
+    pos, mass = ics.plummer(6000, seed=3)
+    H = batch.HostPlan(pos, mass, theta=0.6, order=order, eps=0.01)
+    st = H.stats()
+    assert st["entries_" + kind] > 0 and st["entries_hexa"] == 0
+
+
+def test_group_modes():
+    pos, mass = ics.clustered(20000, seed=6, nhalo=16)
+    chunks = batch.HostPlan(pos, mass, merge_groups=2).stats()
+    assert chunks["ngroups"] == (20000 + 63) // 64
+    merged = batch.HostPlan(pos, mass, merge_groups=1).stats()
+    cells = batch.HostPlan(pos, mass, merge_groups=0).stats()
+    assert merged["mean_group"] > cells["mean_group"]
+
+
+def test_periodic_lists_use_nearest_images():
+    pos, mass = ics.lattice_grf(16, seed=2)
+    H = batch.HostPlan(pos, mass, periodic=True, box=1.0, theta=0.6, order=4, eps=1e-3)
+    st = H.stats()
+    for g in range(0, st["ngroups"], 5):
+        assert listoracle.interactions_cover_all(H, g, pos.shape[0])
+
+
+def test_bad_parameters_raise():
+    pos, mass = ics.plummer(100, seed=1)
+    with pytest.raises(PgravError):
+        batch.HostPlan(pos, mass, group_max=65)
+    with pytest.raises(PgravError):
+        batch.HostPlan(pos, mass, order=3)
+
+
+def test_ics_shapes_and_statistics():
+    pos, mass = ics.plummer(200000, seed=9)
+    r = np.linalg.norm(pos, axis=1)
+    assert r.max() <= 10.0 + 1e-9
+    # truncated-Plummer median: X = 1/2 of the retained mass fraction
+    xmax = 1.01 ** -1.5
+    r_half = 1.0 / np.sqrt((0.5 * xmax) ** (-2 / 3) - 1)
+    assert abs(np.median(r) / r_half - 1) < 0.02
+    np.testing.assert_allclose(mass.sum(), 1.0)
+    pos, mass = ics.clustered(50000, seed=1, nhalo=64)
+    inside = np.all((pos >= 0) & (pos < 1), axis=1).mean()
+    assert 0.2 <= inside <= 1.0
+    pos, mass = ics.lattice_grf(32, seed=3, rms=0.2)
+    assert pos.shape == (32 ** 3, 3) and np.all((pos >= 0) & (pos < 1))
+    g = (np.arange(32) + 0.5) / 32
+    gx, gy, gz = np.meshgrid(g, g, g, indexing="ij")
+    d = pos - np.stack([gx, gy, gz], -1).reshape(-1, 3)
+    d -= np.rint(d)
+    np.testing.assert_allclose(np.sqrt((d ** 2).sum(1).mean()) * 32, 0.2, rtol=1e-3)
