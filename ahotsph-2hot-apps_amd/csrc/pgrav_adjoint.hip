@@ -1,0 +1,324 @@
+// Adjoint of the batched evaluation, tree and lists held fixed.
+//
+// Loss L = sum_i w_i . a_i + v_i phi_i for per-body weights (w_i, v_i).  With the
+// lists fixed, a_i and phi_i depend on the bodies through (1) the sink position
+// x_i, (2) each body-list source position x_j and mass m_j, and (3) each
+// accepted cell's centre of mass X_c and raw moments M_c, which are functions
+// of the cell's bodies.  Three kernels:
+//
+//  k_adjoint        one wave per sink group, the forward's tiling.  Per pair:
+//                   the sink-position gradient by forward-mode dual numbers
+//                   through the very same cell_interact/mono_interact code;
+//                   for cells the 46 prepared-record coefficients' gradient
+//                   dL/dP (the interaction is linear in P, so this is the
+//                   hand-written transpose of the evaluation) and dL/dX_c;
+//                   for bodies dL/dx_j and dL/dm_j.  Source-side terms are
+//                   summed over the wave's sinks and added atomically.
+//  k_adjoint_nodes  per cell: dL/dP -> dL/d(raw moments) through the fixed
+//                   linear map raw -> record -> prepared record, and the total
+//                   dL/dX_c including the centre dependence of the moments.
+//  k_adjoint_bodies per body: walk up its ancestors, chaining dL/dM_c and dL/dX_c
+//                   to dL/dx_j and dL/dm_j (M_c[a] = sum_j m_j (x_j - X_c)^a,
+//                   X_c = sum_j m_j x_j / sum_j m_j).
+#include "pgrav_batch.h"
+#include "pgrav_device.h"
+
+namespace pgrav {
+
+// ---------------------------------------------------------------- dual numbers
+struct D3 {
+    float v, x, y, z;  // value and gradient with respect to the sink position
+    __device__ D3() = default;
+    __device__ D3(float a) : v(a), x(0.f), y(0.f), z(0.f) {}
+    __device__ D3(float a, float b, float c, float d) : v(a), x(b), y(c), z(d) {}
+};
+__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return D3(a.v + b.v, a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return D3(a.v - b.v, a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ D3 operator-(D3 a) { return D3(-a.v, -a.x, -a.y, -a.z); }
+__device__ __forceinline__ D3 operator*(D3 a, D3 b) {
+    return D3(a.v * b.v, a.x * b.v + a.v * b.x, a.y * b.v + a.v * b.y, a.z * b.v + a.v * b.z);
+}
+__device__ __forceinline__ D3 operator*(float s, D3 b) { return D3(s * b.v, s * b.x, s * b.y, s * b.z); }
+__device__ __forceinline__ D3 operator*(D3 b, float s) { return s * b; }
+__device__ __forceinline__ D3 &operator+=(D3 &a, D3 b) { return a = a + b; }
+__device__ __forceinline__ D3 &operator-=(D3 &a, D3 b) { return a = a - b; }
+__device__ __forceinline__ bool operator>(D3 a, D3 b) { return a.v > b.v; }
+__device__ __forceinline__ bool operator==(D3 a, D3 b) { return a.v == b.v; }
+// -1/sqrt(s): derivative 0.5 s^-3/2 = -0.5 f^3
+__device__ __forceinline__ D3 nrsq(D3 s) {
+    const float f = nrsq(s.v);
+    const float d = -0.5f * f * f * f;
+    return D3(f, d * s.x, d * s.y, d * s.z);
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+#define ADJ_CELL 52  // per node: 46 dL/dP, 3 dL/dX (pair part), pad
+
+// dL/dP for one (sink, cell) pair: the evaluation in cell_interact is linear in
+// the prepared record P; with w, v the sink's weights these are the
+// coefficients of P in w.a + v.phi (see pgrav_device.h for the formulas).
+template <int ORDER>
+__device__ __forceinline__ void cell_beta(float x, float y, float z, float eps2, float wx_,
+                                          float wy_, float wz_, float v, float *beta) {
+    const float x2 = x * x, y2 = y * y, z2 = z * z;
+    const float r2 = x2 + y2 + z2 + eps2;
+    const float rinv = nrsq(r2);
+    const float ri2 = rinv * rinv;
+    const float ra = rinv * ri2;
+    const float wdx = wx_ * x + wy_ * y + wz_ * z;
+    const float as = ra * wdx + v * rinv;  // dL/dS_phi
+    const float rc = 2.0f * ri2, rb = 1.5f * ri2, rq = (4.0f / 3.0f) * ri2;
+    const float at = ra * wdx * rc;        // dL/dT'
+    const float avx = -ra * wx_, avy = -ra * wy_, avz = -ra * wz_;  // dL/dV
+    for (int k = 0; k < 46; k++) beta[k] = 0.0f;
+    beta[3] = as;
+    if (ORDER == 0)
+        return;
+    const float c2 = rc;
+    const float u2 = as * ri2 * ri2 + at * c2;
+    const float G2x = u2 * x + c2 * avx, G2y = u2 * y + c2 * avy, G2z = u2 * z + c2 * avz;
+    beta[4] = G2x * x;
+    beta[5] = G2x * y + G2y * x;
+    beta[6] = G2x * z + G2z * x;
+    beta[7] = G2y * y;
+    beta[8] = G2y * z + G2z * y;
+    beta[9] = G2z * z;
+    if (ORDER == 2)
+        return;
+    const float c3 = rc * rb, c4 = rc * rb * rq;
+    const float ri6 = ri2 * ri2 * ri2;
+    const float u3 = as * ri6 + at * c3;
+    const float u4 = as * ri6 * ri2 + at * c4;
+    const float G3[3] = {u3 * x + c3 * avx, u3 * y + c3 * avy, u3 * z + c3 * avz};
+    const float G4[3] = {u4 * x + c4 * avx, u4 * y + c4 * avy, u4 * z + c4 * avz};
+    const float A = x2 - z2, B = y2 - z2;
+    const float b3[5] = {A, B, x * y, x * z, y * z};
+    const float z3 = 3.0f * z2;
+    const float U[7] = {x * (x2 - z3), y * (y2 - z3), y * A, z * (3.0f * x2 - z2), x * B, x * y * z,
+                        z * (3.0f * y2 - z2)};
+    for (int r = 0; r < 3; r++) {
+        for (int k = 0; k < 5; k++) beta[10 + 5 * r + k] = G3[r] * b3[k];
+        for (int k = 0; k < 7; k++) beta[25 + 7 * r + k] = G4[r] * U[k];
+    }
+}
+
+template <int SOFT>
+__global__ __launch_bounds__(64) void k_adjoint(BatchView V, const float4 *__restrict__ wv,
+                                                float4 *__restrict__ gsink,
+                                                float *__restrict__ gcell,
+                                                float4 *__restrict__ gbody) {
+    __shared__ float4 lds[64];
+    const int g = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t first = V.gfirst[g];
+    const int cnt = V.gcount[g];
+    const double4 gc = V.gcen[g];
+    const bool live = lane < cnt;
+    const int64_t me = first + (live ? lane : 0);
+    const double4 sp = V.bpos[me];
+    const float sx = (float)(sp.x - gc.x), sy = (float)(sp.y - gc.y), sz = (float)(sp.z - gc.z);
+    const float4 w = live ? wv[me] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float eps = (float)V.eps, eps2 = eps * eps;
+    const float eps_inv = SOFT == SOFT_K1 ? 1.0f / eps : 0.0f;
+    const float eps2c = SOFT == SOFT_PLUMMER ? eps2 : 0.0f;
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+
+    for (int kind = 0; kind < 4; kind++) {
+        const int64_t o0 = V.off[kind][g], o1 = V.off[kind][g + 1];
+        const uint32_t *__restrict__ idx = V.idx[kind];
+        for (int64_t t0 = o0; t0 < o1; t0 += 64) {
+            const int nt = (int)min((int64_t)64, o1 - t0);
+            if (lane < nt) {
+                const uint32_t e = idx[t0 + lane];
+                const double4 s = kind == 3 ? V.bpos[e] : V.cpos[e];
+                lds[lane] = make_float4((float)(s.x - gc.x), (float)(s.y - gc.y),
+                                        (float)(s.z - gc.z), (float)s.w);
+            }
+            __syncthreads();
+            for (int k = 0; k < nt; k++) {
+                const uint32_t e = idx[t0 + k];
+                const float4 r = lds[k];
+                const float X = sx - r.x, Y = sy - r.y, Z = sz - r.z;
+                const D3 dx(X, 1.f, 0.f, 0.f), dy(Y, 0.f, 1.f, 0.f), dz(Z, 0.f, 0.f, 1.f);
+                Acc4T<D3> a = {D3(0.f), D3(0.f), D3(0.f), D3(0.f)};
+                if (kind == 0)
+                    cell_interact<4, false, 1, D3, float>(V.mom32 + (size_t)e * PREC4, D3(r.w), dx, dy, dz, D3(eps2c), a);
+                else if (kind == 1)
+                    cell_interact<2, false, 1, D3, float>(V.mom32 + (size_t)e * PREC4, D3(r.w), dx, dy, dz, D3(eps2c), a);
+                else if (kind == 2)
+                    mono_interact<D3>(D3(r.w), dx, dy, dz, D3(eps2c), a);
+                else {
+                    const bool self = X == 0.f && Y == 0.f && Z == 0.f;
+                    if (SOFT == SOFT_K1)
+                        k1_interact<D3>(D3(r.w), dx, dy, dz, D3(eps_inv), D3(eps2), a);
+                    else if (SOFT == SOFT_PLUMMER)
+                        mono_interact<D3>(D3(r.w), dx, dy, dz, D3(eps2), a);
+                    else if (!self)
+                        mono_interact<D3>(D3(r.w), dx, dy, dz, D3(0.f), a);
+                }
+                // d(w.a + v phi)/d(sink position)
+                const float px = w.x * a.x.x + w.y * a.y.x + w.z * a.z.x + w.w * a.w.x;
+                const float py = w.x * a.x.y + w.y * a.y.y + w.z * a.z.y + w.w * a.w.y;
+                const float pz = w.x * a.x.z + w.y * a.y.z + w.z * a.z.z + w.w * a.w.z;
+                gx += px;
+                gy += py;
+                gz += pz;
+                if (kind < 3) {
+                    float beta[46];
+                    if (kind == 0)
+                        cell_beta<4>(X, Y, Z, eps2c, w.x, w.y, w.z, w.w, beta);
+                    else if (kind == 1)
+                        cell_beta<2>(X, Y, Z, eps2c, w.x, w.y, w.z, w.w, beta);
+                    else
+                        cell_beta<0>(X, Y, Z, eps2c, w.x, w.y, w.z, w.w, beta);
+                    const int nb = kind == 0 ? 46 : (kind == 1 ? 10 : 4);
+                    float *gcp = gcell + (size_t)e * ADJ_CELL;
+                    for (int q = 3; q < nb; q++) {
+                        const float s = wave_sum(beta[q]);
+                        if (lane == 0)
+                            atomicAdd(gcp + q, s);
+                    }
+                    const float sx_ = wave_sum(-px), sy_ = wave_sum(-py), sz_ = wave_sum(-pz);
+                    if (lane == 0) {
+                        atomicAdd(gcp + 46, sx_);
+                        atomicAdd(gcp + 47, sy_);
+                        atomicAdd(gcp + 48, sz_);
+                    }
+                } else {
+                    // source body: position gradient is minus the sink's; the mass
+                    // gradient is the pair's (w.a + v phi) per unit source mass
+                    const float lm = r.w != 0.f
+                                         ? (w.x * a.x.v + w.y * a.y.v + w.z * a.z.v + w.w * a.w.v) / r.w
+                                         : 0.f;
+                    const float sx_ = wave_sum(-px), sy_ = wave_sum(-py), sz_ = wave_sum(-pz);
+                    const float sm = wave_sum(lm);
+                    if (lane == 0) {
+                        float *gb = (float *)(gbody + e);
+                        atomicAdd(gb + 0, sx_);
+                        atomicAdd(gb + 1, sy_);
+                        atomicAdd(gb + 2, sz_);
+                        atomicAdd(gb + 3, sm);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (live)
+        gsink[me] = make_float4(gx, gy, gz, 0.f);
+}
+
+// per node: dL/d(raw moments) = Mat^T dL/dP and the total dL/dX_c.  Mat (46 x 35)
+// is the fixed linear map raw moments -> prepared record (R = 1).
+__global__ void k_adjoint_nodes(const float *__restrict__ gcell, const double *__restrict__ raw,
+                                const float *__restrict__ mat, int64_t nnodes,
+                                float *__restrict__ gnode /* 40 per node */) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nnodes)
+        return;
+    const float *gp = gcell + c * ADJ_CELL;
+    float gr[35];
+    for (int a = 0; a < 35; a++) {
+        float s = 0.f;
+        for (int k = 3; k < 46; k++) s += mat[k * 35 + a] * gp[k];
+        gr[a] = s;
+    }
+    // exponents of the 35 raw moments (nln/cofm.c:660-720 order)
+    float gX[3] = {gp[46], gp[47], gp[48]};
+    int a = 0;
+    for (int o = 0; o <= 4; o++)
+        for (int ex = o; ex >= 0; ex--)
+            for (int ey = o - ex; ey >= 0; ey--, a++) {
+                const int ez = o - ex - ey;
+                // dM_a/dX = -(ex M_{a-x}, ey M_{a-y}, ez M_{a-z}); index of a lower
+                // moment found by scanning (tiny table)
+                const int e3[3] = {ex, ey, ez};
+                for (int q = 0; q < 3; q++) {
+                    if (!e3[q])
+                        continue;
+                    int f[3] = {ex, ey, ez};
+                    f[q]--;
+                    const int ol = o - 1;
+                    int b = 0;
+                    for (int oo = 0; oo < ol; oo++) b += (oo + 1) * (oo + 2) / 2;
+                    for (int fx = ol; fx >= 0; fx--)
+                        for (int fy = ol - fx; fy >= 0; fy--, b++)
+                            if (fx == f[0] && fy == f[1])
+                                gX[q] -= gr[a] * (float)e3[q] * (float)raw[c * 35 + b];
+                }
+            }
+    float *o = gnode + c * 40;
+    for (int k = 0; k < 35; k++) o[k] = gr[k];
+    o[35] = gX[0];
+    o[36] = gX[1];
+    o[37] = gX[2];
+}
+
+// per body: chain through every ancestor's moments and centre of mass
+__global__ void k_adjoint_bodies(const double4 *__restrict__ bpos, const double4 *__restrict__ cpos,
+                                 const int *__restrict__ leaf_of, const int *__restrict__ parent,
+                                 const float *__restrict__ gnode, const float4 *__restrict__ gsink,
+                                 const float4 *__restrict__ gbody, int64_t n, double G,
+                                 double4 *__restrict__ out /* dL/dx, dL/dm, sorted order */) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n)
+        return;
+    const double4 p = bpos[j];
+    const float4 s = gsink[j], b = gbody[j];
+    float g[3] = {s.x + b.x, s.y + b.y, s.z + b.z};
+    float gm = b.w;
+    const float mj = (float)p.w;
+    for (int c = leaf_of[j]; c >= 0; c = parent[c]) {
+        const double4 C = cpos[c];
+        const float d[3] = {(float)(p.x - C.x), (float)(p.y - C.y), (float)(p.z - C.z)};
+        const float *gn = gnode + (size_t)c * 40;
+        float pw[3][5];
+        for (int q = 0; q < 3; q++) {
+            pw[q][0] = 1.f;
+            for (int k = 1; k < 5; k++) pw[q][k] = pw[q][k - 1] * d[q];
+        }
+        int a = 0;
+        for (int o = 0; o <= 4; o++)
+            for (int ex = o; ex >= 0; ex--)
+                for (int ey = o - ex; ey >= 0; ey--, a++) {
+                    const int ez = o - ex - ey;
+                    const float ga = gn[a];
+                    gm += ga * pw[0][ex] * pw[1][ey] * pw[2][ez];
+                    if (ex) g[0] += ga * mj * ex * pw[0][ex - 1] * pw[1][ey] * pw[2][ez];
+                    if (ey) g[1] += ga * mj * ey * pw[0][ex] * pw[1][ey - 1] * pw[2][ez];
+                    if (ez) g[2] += ga * mj * ez * pw[0][ex] * pw[1][ey] * pw[2][ez - 1];
+                }
+        const float Mc = (float)C.w;
+        if (Mc != 0.f) {
+            for (int q = 0; q < 3; q++) g[q] += gn[35 + q] * mj / Mc;
+            gm += (gn[35] * d[0] + gn[36] * d[1] + gn[37] * d[2]) / Mc;
+        }
+    }
+    out[j] = make_double4(G * g[0], G * g[1], G * g[2], G * gm);
+}
+
+hipError_t launch_adjoint(const BatchView &v, const float4 *wv, float4 *gsink, float *gcell,
+                          float4 *gbody, const double *raw, const float *mat, int64_t nnodes,
+                          float *gnode, const int *leaf_of, const int *parent, int64_t n,
+                          double4 *out, hipStream_t st) {
+    if (v.ngroups <= 0)
+        return hipSuccess;
+    if (v.soft == SOFT_K1)
+        hipLaunchKernelGGL(k_adjoint<SOFT_K1>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gsink, gcell, gbody);
+    else if (v.soft == SOFT_PLUMMER)
+        hipLaunchKernelGGL(k_adjoint<SOFT_PLUMMER>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gsink, gcell, gbody);
+    else
+        hipLaunchKernelGGL(k_adjoint<SOFT_NONE>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gsink, gcell, gbody);
+    hipLaunchKernelGGL(k_adjoint_nodes, dim3((unsigned)((nnodes + 127) / 128)), dim3(128), 0, st,
+                       gcell, raw, mat, nnodes, gnode);
+    hipLaunchKernelGGL(k_adjoint_bodies, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st,
+                       v.bpos, v.cpos, leaf_of, parent, gnode, gsink, gbody, n, v.G, out);
+    return hipGetLastError();
+}
+
+}  // namespace pgrav

@@ -66,7 +66,9 @@ __device__ __forceinline__ void ewald_corr(const float4 *__restrict__ tab, T m, 
     ay = ay > half ? box - ay : ay;
     az = az > half ? box - az : az;
     T u = ax * inv_h, v = ay * inv_h, w = az * inv_h;
-    int i = min((int)u, EWALD_N - 2), j = min((int)v, EWALD_N - 2), k = min((int)w, EWALD_N - 2);
+    // clamp both ways: a NaN offset must not turn into an out-of-range index
+    int i = max(0, min((int)u, EWALD_N - 2)), j = max(0, min((int)v, EWALD_N - 2)),
+        k = max(0, min((int)w, EWALD_N - 2));
     T fu = u - T(i), fv = v - T(j), fw = w - T(k);
     const float4 *p = tab + ((size_t)i * EWALD_N + j) * EWALD_N + k;
     const float4 c000 = p[0], c001 = p[1], c010 = p[EWALD_N], c011 = p[EWALD_N + 1];

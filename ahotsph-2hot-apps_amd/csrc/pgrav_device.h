@@ -52,8 +52,9 @@ __device__ __forceinline__ double nrsq(double r2) { return -1.0 / sqrt(r2); }
 // Order-4 (and order-2 when ORDER == 2) multipole interaction.  M points at a
 // prepared record; the caller passes m separately (it may come from LDS).
 // QSIGN = -1 reproduces nln/pgrav.cu's "phi -= eqe" for the quadrupole term.
-template <int ORDER, bool DIPOLE, int QSIGN, typename T>
-__device__ __forceinline__ void cell_interact(const T *__restrict__ M, T m, T x, T y, T z, T eps2,
+// TM is the record's storage type (T itself, or float when T is a dual number).
+template <int ORDER, bool DIPOLE, int QSIGN, typename T, typename TM = T>
+__device__ __forceinline__ void cell_interact(const TM *__restrict__ M, T m, T x, T y, T z, T eps2,
                                               Acc4T<T> &a) {
     const T x2 = x * x, y2 = y * y, z2 = z * z;
     const T r2 = x2 + y2 + z2 + eps2;
@@ -89,7 +90,7 @@ __device__ __forceinline__ void cell_interact(const T *__restrict__ M, T m, T x,
     // order 3: basis A, B, xy, xz, yz
     const T A = x2 - z2, B = y2 - z2;
     const T xy = x * y, xz = x * z, yz = y * z;
-    const T *o = M + 10;
+    const TM *o = M + 10;
     const T g3x = o[0] * A + o[1] * B + o[2] * xy + o[3] * xz + o[4] * yz;
     const T g3y = o[5] * A + o[6] * B + o[7] * xy + o[8] * xz + o[9] * yz;
     const T g3z = o[10] * A + o[11] * B + o[12] * xy + o[13] * xz + o[14] * yz;
@@ -103,7 +104,7 @@ __device__ __forceinline__ void cell_interact(const T *__restrict__ M, T m, T x,
     const T U5 = x * B;
     const T U6 = xy * z;
     const T U7 = z * (T(3) * y2 - z2);
-    const T *h = M + 25;
+    const TM *h = M + 25;
     const T g4x = h[0] * U1 + h[1] * U2 + h[2] * U3 + h[3] * U4 + h[4] * U5 + h[5] * U6 + h[6] * U7;
     const T g4y = h[7] * U1 + h[8] * U2 + h[9] * U3 + h[10] * U4 + h[11] * U5 + h[12] * U6 + h[13] * U7;
     const T g4z = h[14] * U1 + h[15] * U2 + h[16] * U3 + h[17] * U4 + h[18] * U5 + h[19] * U6 + h[20] * U7;

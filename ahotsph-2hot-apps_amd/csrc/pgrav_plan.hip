@@ -37,6 +37,12 @@ struct Plan {
     BatchView v;
     double t_tree = 0, t_lists = 0, t_upload = 0;
     int64_t nnodes = 0;
+    // adjoint buffers, allocated on first use
+    float4 *a_wv = nullptr, *a_gsink = nullptr, *a_gbody = nullptr;
+    float *a_gcell = nullptr, *a_gnode = nullptr, *a_mat = nullptr;
+    double *a_raw = nullptr;
+    double4 *a_out = nullptr;
+    int *a_leaf = nullptr, *a_parent = nullptr;
 
     template <typename P>
     P *dalloc(size_t bytes) {
@@ -275,6 +281,103 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
 }
 
 void pgrav_plan_destroy(void *plan) { delete (Plan *)plan; }
+
+int pgrav_plan_update(void *plan, const double *pos3, const double *mass) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P || !pos3 || !mass) {
+        fail("pgrav_plan_update: null argument");
+        return -1;
+    }
+    try {
+        update_bodies(P->tree, pos3, mass, P->prm.periodic, P->prm.box, P->prm.nthreads);
+        const int64_t n = P->tree.n;
+        std::vector<double> b4(4 * (size_t)n);
+        for (int64_t i = 0; i < n; i++) {
+            for (int k = 0; k < 3; k++) b4[4 * i + k] = P->tree.pos[3 * i + k];
+            b4[4 * i + 3] = P->tree.mass[i];
+        }
+        std::vector<float> r32;
+        std::vector<double> r64, c4;
+        prepare_cells(P->tree, P->prm.order, r32, P->v.mom64 ? &r64 : nullptr, c4);
+        if (!check(hipStreamSynchronize(P->stream), "hipStreamSynchronize") ||
+            !check(hipMemcpy((void *)P->v.bpos, b4.data(), b4.size() * 8, hipMemcpyHostToDevice), "hipMemcpy") ||
+            !check(hipMemcpy((void *)P->v.cpos, c4.data(), c4.size() * 8, hipMemcpyHostToDevice), "hipMemcpy") ||
+            !check(hipMemcpy((void *)P->v.mom32, r32.data(), r32.size() * 4, hipMemcpyHostToDevice), "hipMemcpy"))
+            return -1;
+        if (P->v.mom64 &&
+            !check(hipMemcpy((void *)P->v.mom64, r64.data(), r64.size() * 8, hipMemcpyHostToDevice), "hipMemcpy"))
+            return -1;
+    } catch (const std::exception &e) {
+        fail("pgrav_plan_update: %s", e.what());
+        return -1;
+    }
+    return 0;
+}
+
+int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P || !w4 || !grad4) {
+        fail("pgrav_plan_adjoint: null argument");
+        return -1;
+    }
+    if (P->prm.periodic) {
+        fail("the adjoint covers the isolated kernels only");
+        return -1;
+    }
+    try {
+        const int64_t n = P->tree.n, nn = P->nnodes;
+        const Tree &t = P->tree;
+        if (!P->a_wv) {
+            P->a_wv = P->dalloc<float4>(sizeof(float4) * n);
+            P->a_gsink = P->dalloc<float4>(sizeof(float4) * n);
+            P->a_gbody = P->dalloc<float4>(sizeof(float4) * n);
+            P->a_out = P->dalloc<double4>(sizeof(double4) * n);
+            P->a_gcell = P->dalloc<float>(sizeof(float) * 52 * (size_t)nn);
+            P->a_gnode = P->dalloc<float>(sizeof(float) * 40 * (size_t)nn);
+            P->a_raw = P->dalloc<double>(sizeof(double) * 35 * (size_t)nn);
+            // raw moments -> prepared record is linear and centre-independent (R = 1)
+            std::vector<float> mat(46 * 35);
+            for (int a = 0; a < 35; a++) {
+                double raw[35] = {0}, c[3] = {0, 0, 0}, q[29], pr[PREC4];
+                raw[a] = 1.0;
+                raw_to_record<double>(raw, c, 1.0, q);
+                q[5] = q[6] = q[7] = 0.0;
+                prep_record<double>(q, 4, 0, pr);
+                for (int k = 0; k < 46; k++) mat[k * 35 + a] = (float)pr[k];
+            }
+            P->a_mat = P->upload(mat.data(), mat.size());
+            std::vector<int> leaf(n, -1), parent(t.parent.begin(), t.parent.end());
+            for (int64_t c = 0; c < nn; c++)
+                if (t.child[c] < 0)
+                    for (int64_t i = t.first[c]; i < t.first[c] + t.count[c]; i++) leaf[i] = (int)c;
+            P->a_leaf = P->upload(leaf.data(), leaf.size());
+            P->a_parent = P->upload(parent.data(), parent.size());
+        }
+        std::vector<float> wv(4 * (size_t)n);
+        for (int64_t i = 0; i < n; i++)
+            for (int k = 0; k < 4; k++) wv[4 * i + k] = (float)w4[4 * t.perm[i] + k];
+        if (!check(hipMemcpyAsync(P->a_wv, wv.data(), wv.size() * 4, hipMemcpyHostToDevice, P->stream), "hipMemcpyAsync") ||
+            !check(hipMemcpyAsync(P->a_raw, t.raw.data(), t.raw.size() * 8, hipMemcpyHostToDevice, P->stream), "hipMemcpyAsync") ||
+            !check(hipMemsetAsync(P->a_gcell, 0, sizeof(float) * 52 * (size_t)nn, P->stream), "hipMemsetAsync") ||
+            !check(hipMemsetAsync(P->a_gbody, 0, sizeof(float4) * n, P->stream), "hipMemsetAsync") ||
+            !check(launch_adjoint(P->v, P->a_wv, P->a_gsink, P->a_gcell, P->a_gbody, P->a_raw,
+                                  P->a_mat, nn, P->a_gnode, P->a_leaf, P->a_parent, n, P->a_out,
+                                  P->stream), "adjoint kernels"))
+            return -1;
+        std::vector<double> out(4 * (size_t)n);
+        if (!check(hipMemcpyAsync(out.data(), P->a_out, out.size() * 8, hipMemcpyDeviceToHost, P->stream), "hipMemcpyAsync") ||
+            !check(hipStreamSynchronize(P->stream), "hipStreamSynchronize"))
+            return -1;
+        for (int64_t i = 0; i < n; i++)
+            for (int k = 0; k < 4; k++) grad4[4 * t.perm[i] + k] = out[4 * i + k];
+    } catch (const std::exception &e) {
+        fail("pgrav_plan_adjoint: %s", e.what());
+        return -1;
+    }
+    return 0;
+}
 
 int pgrav_plan_stats(void *plan, pgrav_stats *st) {
     Plan *P = (Plan *)plan;

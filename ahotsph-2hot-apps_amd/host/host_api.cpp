@@ -7,6 +7,7 @@
 
 #include "pgrav.h"
 #include "pgrav_host.h"
+#include "pgrav_record.h"
 
 namespace pgrav {
 void fail(const char *fmt, ...);
@@ -127,6 +128,26 @@ int pgrav_host_cell(void *h, int64_t node, double *cofm3, double *bmax, double *
         *child = t.child[node];
     if (nchild)
         *nchild = t.nchild[node];
+    return 0;
+}
+
+int pgrav_host_record(void *h, int64_t node, double *rec29) {
+    HostPlan *H = (HostPlan *)h;
+    if (!H || node < 0 || node >= H->tree.nnodes() || !rec29) {
+        fail("bad node");
+        return -1;
+    }
+    const Tree &t = H->tree;
+    const double bm = t.bmax[node] > 0 ? t.bmax[node] : 1.0;
+    raw_to_record<double>(&t.raw[35 * node], &t.cofm[3 * node], bm, rec29);
+    rec29[5] = rec29[6] = rec29[7] = 0.0;  // zero dipole about the centre of mass
+    return 0;
+}
+
+int pgrav_host_records(void *h, const uint32_t *nodes, int64_t n, double *rec29) {
+    for (int64_t i = 0; i < n; i++)
+        if (pgrav_host_record(h, nodes[i], rec29 + 29 * i))
+            return -1;
     return 0;
 }
 

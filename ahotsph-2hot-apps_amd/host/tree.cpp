@@ -286,8 +286,14 @@ void build_tree(const double *pos, const double *mass, int64_t n, const TreePara
         t.nchild[nd] = (uint8_t)nc;
         for (int c = nc - 1; c >= 0; c--) stack.push_back({c0 + c});
     }
+    compute_moments(t, nth, &key);
+}
 
-    // moments, deepest level first (every child index exceeds its parent's)
+// Moments, centres of mass and bmax of every node, deepest level first (every
+// child index exceeds its parent's).  With `key`, bmax is also bounded by the
+// cell geometry; without it (bodies moved after the build) only the daughters'
+// bound is used, which stays valid wherever the bodies went.
+void compute_moments(Tree &t, int nth, const std::vector<uint64_t> *keyp) {
     const int64_t nn = t.nnodes();
     t.cofm.assign(3 * nn, 0.0);
     t.bmax.assign(nn, 0.0);
@@ -333,6 +339,10 @@ void build_tree(const double *pos, const double *mass, int64_t n, const TreePara
                         shift_add(&t.raw[35 * (size_t)ch], d, raw);
                         bm = std::max(bm, sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + t.bmax[ch]);
                     }
+                    t.bmax[nd] = bm;
+                    if (!keyp)
+                        continue;
+                    const std::vector<uint64_t> &key = *keyp;
                     // geometric bound (nln/cofm.c:214-222): half cell + |cofm - cell centre|
                     const double cs = t.size / (double)(1u << t.level[nd]);
                     // cell corner from the key prefix of its first body
@@ -352,6 +362,24 @@ void build_tree(const double *pos, const double *mass, int64_t n, const TreePara
             }
         });
     }
+}
+
+void update_bodies(Tree &t, const double *pos, const double *mass, int periodic, double box,
+                   int nthreads) {
+    const int nth = default_threads(nthreads);
+    parallel_for(nth, t.n, [&](int64_t a, int64_t b) {
+        for (int64_t i = a; i < b; i++) {
+            const int64_t j = t.perm[i];
+            for (int k = 0; k < 3; k++) {
+                double x = pos[3 * j + k];
+                if (periodic)
+                    x -= box * floor(x / box);
+                t.pos[3 * i + k] = x;
+            }
+            t.mass[i] = mass[j];
+        }
+    });
+    compute_moments(t, nth, nullptr);
 }
 
 // ---------------------------------------------------------------- records

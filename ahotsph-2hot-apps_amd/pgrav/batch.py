@@ -44,6 +44,8 @@ register({
     "pgrav_params_default": (None, [_P]),
     "pgrav_plan_create": (_vp, [_vp, _vp, _i64, _P]),
     "pgrav_plan_destroy": (None, [_vp]),
+    "pgrav_plan_update": (_i, [_vp, _vp, _vp]),
+    "pgrav_plan_adjoint": (_i, [_vp, _vp, _vp]),
     "pgrav_plan_stats": (_i, [_vp, _S]),
     "pgrav_plan_evaluate": (_i, [_vp, _i, _vp]),
     "pgrav_plan_time": (_i, [_vp, _i, _i, _vp, ctypes.POINTER(_d)]),
@@ -60,6 +62,8 @@ register({
     "pgrav_host_group": (_i, [_vp, _i64, _vp, _vp, _vp]),
     "pgrav_host_list": (_i64, [_vp, _i64, _i, _vp, _i64]),
     "pgrav_host_cell": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pgrav_host_record": (_i, [_vp, _i64, _vp]),
+    "pgrav_host_records": (_i, [_vp, _vp, _i64, _vp]),
     "pgrav_host_sorted": (_i, [_vp, _vp, _vp]),
     "pgrav_host_lpt": (_i, [_vp, _vp]),
 })
@@ -125,9 +129,24 @@ class Plan:
         check()
         return _stats_dict(st)
 
+    def update(self, pos, mass):
+        """Move the bodies, keeping the tree topology, groups and lists."""
+        self.pos, self.mass = _as_inputs(pos, mass)
+        lib().pgrav_plan_update(self.h, self.pos.ctypes.data, self.mass.ctypes.data)
+        check()
+
     def evaluate(self, variant=REL32C, stream=None):
         lib().pgrav_plan_evaluate(self.h, _v(variant), stream)
         check()
+
+    def adjoint(self, w4):
+        """Gradient of L = sum_i w_i.a_i + v_i phi_i (w4 = (w, v) per body, caller
+        order) with respect to positions and masses: (n, 4) = (dL/dx, dL/dm)."""
+        w4 = np.ascontiguousarray(w4, np.float64).reshape(self.n, 4)
+        out = np.empty((self.n, 4), np.float64)
+        lib().pgrav_plan_adjoint(self.h, w4.ctypes.data, out.ctypes.data)
+        check()
+        return out
 
     def time(self, variant=REL32C, reps=3, stream=None):
         ms = ctypes.c_double(0)
@@ -227,6 +246,19 @@ class HostPlan:
         check()
         return {"cofm": c, "bmax": bmax.value, "raw": raw, "first": first.value,
                 "count": cnt.value, "child": child.value, "nchild": nch.value}
+
+    def record(self, node):
+        rec = np.empty(29, np.float64)
+        lib().pgrav_host_record(self.h, node, rec.ctypes.data)
+        check()
+        return rec
+
+    def records(self, nodes):
+        nodes = np.ascontiguousarray(nodes, np.uint32)
+        rec = np.empty((nodes.size, 29), np.float64)
+        lib().pgrav_host_records(self.h, nodes.ctypes.data, nodes.size, rec.ctypes.data)
+        check()
+        return rec
 
     def sorted(self):
         perm = np.empty(self.n, np.int64)

@@ -156,6 +156,15 @@ void pgrav_params_default(pgrav_params *p);
 /* pos3: n x 3 FP64 positions, mass: n FP64.  Returns NULL on error. */
 void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const pgrav_params *p);
 void pgrav_plan_destroy(void *plan);
+/* New positions and masses (caller order) with the tree topology, sink groups
+ * and lists held fixed: moments and centres are recomputed in FP64 and the
+ * device tables refreshed (what the adjoint's "tree held fixed" means). */
+int pgrav_plan_update(void *plan, const double *pos3, const double *mass);
+/* Adjoint of the REL32C evaluation with the tree and lists held fixed: for
+ * L = sum_i w_i . a_i + v_i phi_i (w4[i] = (w_i, v_i), caller order) returns
+ * grad4[i] = (dL/dx_i, dL/dy_i, dL/dz_i, dL/dm_i).  No reference counterpart
+ * (the reference has no derivatives). */
+int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4);
 int pgrav_plan_stats(void *plan, pgrav_stats *st);
 /* Enqueues one force evaluation on `stream` (NULL: the plan's stream). */
 int pgrav_plan_evaluate(void *plan, int variant, void *stream);
@@ -186,6 +195,9 @@ int pgrav_host_group(void *h, int64_t g, int64_t *first, int *count, double *cen
 int64_t pgrav_host_list(void *h, int64_t g, int kind, uint32_t *out, int64_t cap);
 int pgrav_host_cell(void *h, int64_t node, double *cofm3, double *bmax, double *raw35,
                     int64_t *first, int *count, int *child, int *nchild);
+/* 29-field record of a node (nln/cofm.c:398-562 normalisation, R = bmax). */
+int pgrav_host_record(void *h, int64_t node, double *rec29);
+int pgrav_host_records(void *h, const uint32_t *nodes, int64_t n, double *rec29);
 int pgrav_host_sorted(void *h, int64_t *perm, double *pos3);
 int pgrav_host_lpt(void *h, int32_t *order);
 

@@ -49,6 +49,7 @@ def lib():
         L.orc_raw_moments.argtypes = [_f64p, _f64p, _i64, _f64p, _f64p]
         L.orc_mpole.argtypes = [_f64p, _i, _i, _d, _d, _d, _d, _i, _f64p]
         L.orc_eval_records.argtypes = [_f64p, _i64, _f64p, _i64, _i, _i, _d, _i, _f64p]
+        L.orc_direct_grad.argtypes = [_f64p, _f64p, _i64, _f64p, _i, _d, _f64p, _i]
         _lib = L
     return _lib
 
@@ -134,6 +135,17 @@ def direct(sink_pos, src_pos, src_mass, soft_type=0, eps=0.0, nthreads=None):
     out = np.zeros((sp.size // 3, 4), np.float64)
     lib().orc_direct(sp, sp.size // 3, q, mm, mm.size, soft_type, eps, out,
                      nthreads or min(8, os.cpu_count() or 1))
+    return out
+
+
+def direct_grad(pos, mass, w4, soft_type=1, eps=0.0, nthreads=None):
+    """(n, 4): dL/dx, dL/dy, dL/dz, dL/dm of L = sum w.a + v phi (FP64 direct sum)."""
+    p = _c(pos, np.float64).reshape(-1)
+    m = _c(mass, np.float64).reshape(-1)
+    w = _c(w4, np.float64).reshape(-1)
+    out = np.zeros((m.size, 4), np.float64)
+    lib().orc_direct_grad(p, m, m.size, w, soft_type, eps, out,
+                          nthreads or min(8, os.cpu_count() or 1))
     return out
 
 

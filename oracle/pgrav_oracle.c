@@ -355,6 +355,72 @@ void orc_direct(const double *sink_pos3, int64_t m, const double *src_pos3,
     run_threads(dsum_worker, b, nthreads);
 }
 
+/* Gradient of L = sum_i w_i . a_i + v_i phi_i for the FP64 direct sum (soft_type
+ * 0 none, 1 Plummer), by differentiating every pair in closed form:
+ * pair (i <- j), r = x_i - x_j, s^2 = r^2 + eps^2:
+ *   a = -m_j r / s^3, phi = -m_j / s
+ *   d(w.a + v phi)/dr = m_j ( -w / s^3 + 3 (w.r) r / s^5 + v r / s^3 )
+ * added to dL/dx_i and subtracted from dL/dx_j; dL/dm_j += w.(-r/s^3) - v/s. */
+typedef struct {
+    const double *pos, *mass, *w4;
+    int64_t n, k0, k1;
+    int soft;
+    double eps;
+    double *grad4;
+} grad_job;
+
+static void *grad_worker(void *arg) {
+    grad_job *J = (grad_job *)arg;
+    const double e2 = J->soft == 1 ? J->eps * J->eps : 0.0;
+    /* body k collects its own sink terms and its source terms (no writes to others) */
+    for (int64_t k = J->k0; k < J->k1; k++) {
+        double gv[3] = {0, 0, 0}, gm = 0;
+        const double *pk = J->pos + 3 * k;
+        for (int64_t j = 0; j < J->n; j++) {
+            const double *pj = J->pos + 3 * j;
+            double r[3] = {pk[0] - pj[0], pk[1] - pj[1], pk[2] - pj[2]};
+            double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            if (r2 == 0.0 && J->soft == 0)
+                continue;
+            double s2 = r2 + e2, s = sqrt(s2), s3 = s2 * s, s5 = s3 * s2;
+            /* k as sink, j as source: d/dx_k */
+            const double *w = J->w4 + 4 * k;
+            double wr = w[0] * r[0] + w[1] * r[1] + w[2] * r[2];
+            for (int q = 0; q < 3; q++)
+                gv[q] += J->mass[j] * (-w[q] / s3 + 3.0 * wr * r[q] / s5 + w[3] * r[q] / s3);
+            /* j as sink, k as source: r' = x_j - x_k = -r; d/dx_k = -d/dr' */
+            const double *wj = J->w4 + 4 * j;
+            double wr2 = -(wj[0] * r[0] + wj[1] * r[1] + wj[2] * r[2]);
+            for (int q = 0; q < 3; q++)
+                gv[q] -= J->mass[k] * (-wj[q] / s3 + 3.0 * wr2 * (-r[q]) / s5 + wj[3] * (-r[q]) / s3);
+            /* dL/dm_k from body j's result */
+            gm += (wj[0] * r[0] + wj[1] * r[1] + wj[2] * r[2]) / s3 - wj[3] / s;
+        }
+        J->grad4[4 * k + 0] = gv[0];
+        J->grad4[4 * k + 1] = gv[1];
+        J->grad4[4 * k + 2] = gv[2];
+        J->grad4[4 * k + 3] = gm;
+    }
+    return NULL;
+}
+
+void orc_direct_grad(const double *pos3, const double *mass, int64_t n, const double *w4,
+                     int soft_type, double eps, double *grad4, int nthreads) {
+    if (nthreads < 1)
+        nthreads = 1;
+    if (nthreads > 64)
+        nthreads = 64;
+    pthread_t th[64];
+    grad_job jobs[64];
+    for (int t = 0; t < nthreads; t++) {
+        grad_job j = {pos3, mass, w4, n, n * t / nthreads, n * (t + 1) / nthreads, soft_type, eps,
+                      grad4};
+        jobs[t] = j;
+        pthread_create(&th[t], NULL, grad_worker, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+}
+
 /* nln/ewald.c:13-64 (alpha = 2/L, |n| <= 4 within 4L, |h|^2 <= 16) */
 void orc_ewald(const double x[3], double L, double f[3], double *phi) {
     const int N = 4;

@@ -76,6 +76,11 @@ void orc_direct(const double *sink_pos3, int64_t m, const double *src_pos3,
                 const double *src_mass, int64_t n, int soft_type, double eps, double *out4,
                 int nthreads);
 
+/* FP64 gradient of L = sum_i w_i.a_i + v_i phi_i of the direct sum (soft_type 0
+ * or 1) with respect to every position and mass; w4[i] = (w_i, v_i). */
+void orc_direct_grad(const double *pos3, const double *mass, int64_t n, const double *w4,
+                     int soft_type, double eps, double *grad4, int nthreads);
+
 /* Classic Ewald sum (nln/ewald.c:13-64): x = source - sink, box L; returns the
  * force per unit mass on the sink (f) and the potential (phi). */
 void orc_ewald(const double x[3], double L, double f[3], double *phi);
