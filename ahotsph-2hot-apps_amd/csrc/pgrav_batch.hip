@@ -183,6 +183,7 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                         ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
                 }
             } else if (kind == 2) {
+#pragma unroll 4
                 for (int k = 0; k < nt; k++) {
                     const V4<T> r = lds[k];
                     const T x = sx - r.x, y = sy - r.y, z = sz - r.z;
@@ -191,6 +192,8 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                         ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
                 }
             } else {
+                // bodies: ~18 VALU per pair, so unroll to keep several LDS reads in flight
+#pragma unroll 4
                 for (int k = 0; k < nt; k++) {
                     const V4<T> r = lds[k];
                     const T x = sx - r.x, y = sy - r.y, z = sz - r.z;

@@ -400,6 +400,16 @@ int pgrav_plan_stats(void *plan, pgrav_stats *st) {
     return 0;
 }
 
+int pgrav_plan_set_launch(void *plan, int persistent_blocks) {
+    Plan *P = (Plan *)plan;
+    if (!P) {
+        fail("null plan");
+        return -1;
+    }
+    P->prm.persistent_blocks = persistent_blocks;
+    return 0;
+}
+
 int pgrav_plan_evaluate(void *plan, int variant, void *stream) {
     Plan *P = (Plan *)plan;
     clear_error();

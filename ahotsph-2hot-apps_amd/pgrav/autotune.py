@@ -1,0 +1,131 @@
+"""Autotuner for the batched evaluation.
+
+Searches the parameters that change the work decomposition but not the
+physics: how sink groups are formed (tree cells with siblings merged, or fixed
+runs along the Morton curve), the group size (lanes per wave used), the leaf
+size, and the launch shape (one workgroup per group in longest-first order,
+or a persistent grid of k single-wave workgroups per CU draining a work
+counter).  The opening angle, multipole order and softening are inputs, so the
+accuracy target is held fixed.  Each candidate is timed with HIP events over
+back-to-back launches on the plan's stream; the fastest wins.
+
+    python -m pgrav.autotune --n 2097152 --out tuned.json
+
+Results are stored per workload key in a JSON file that bench.py reads.
+"""
+import argparse
+import itertools
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import batch
+
+DEFAULT_SPACE = {
+    "merge_groups": [1, 2],
+    "group_max": [48, 64],
+    "leaf_max": [8, 16],
+}
+PERSISTENT = [0, -1, 4, 8, 32]  # 0: one WG per group; -1: 16 per CU; k>0: k x 256 WGs
+TUNED = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tuned.json")
+
+
+def workload_key(n, theta, order, soft, eps, workload):
+    return "%s:n=%d:theta=%.3g:order=%d:soft=%s:eps=%.3g" % (workload, n, theta, order, soft, eps)
+
+
+def tune(pos, mass, base, space=None, reps=3, variant="rel32c", log=print):
+    """Returns (best_params, table).  `base` holds theta/order/eps/soft/G/..."""
+    space = space or DEFAULT_SPACE
+    keys = sorted(space)
+    table = []
+    best = None
+    for combo in itertools.product(*(space[k] for k in keys)):
+        kw = dict(base)
+        kw.update(dict(zip(keys, combo)))
+        t0 = time.perf_counter()
+        plan = batch.Plan(pos, mass, fp64=False, **kw)
+        build = time.perf_counter() - t0
+        st = plan.stats()
+        plan.evaluate(variant)  # warm-up
+        for pb in PERSISTENT:
+            # a persistent grid of k > 0 means k x 256 single-wave workgroups
+            ms = _time_with_blocks(plan, variant, reps, pb * 256 if pb > 0 else pb)
+            row = dict(kw, persistent_blocks=pb, ms=ms, pairs=st["pairs_total"],
+                       ngroups=st["ngroups"], mean_group=st["mean_group"], build_s=build)
+            table.append(row)
+            log("tune %s -> %.3f ms (groups %d, mean %.1f, pairs %.3e)" %
+                ({k: row[k] for k in keys + ["persistent_blocks"]}, ms, st["ngroups"],
+                 st["mean_group"], st["pairs_total"]))
+            if best is None or ms < best["ms"]:
+                best = row
+        plan.close()
+    return best, table
+
+
+def _time_with_blocks(plan, variant, reps, pb):
+    from ._lib import check, lib
+    import ctypes
+    lib().pgrav_plan_set_launch(plan.h, pb)
+    check()
+    ms = ctypes.c_double(0)
+    lib().pgrav_plan_time(plan.h, batch._v(variant), reps, None, ctypes.byref(ms))
+    check()
+    return ms.value
+
+
+def load(key, path=TUNED):
+    try:
+        with open(path) as f:
+            return json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
+
+
+def save(key, best, table, path=TUNED):
+    data = {}
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        pass
+    keep = ("merge_groups", "group_max", "leaf_max", "persistent_blocks", "ms")
+    data[key] = {"best": {k: best[k] for k in keep},
+                 "table": [{k: r[k] for k in keep + ("pairs", "mean_group")} for r in table]}
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=2097152)
+    ap.add_argument("--workload", default="clustered")
+    ap.add_argument("--theta", type=float, default=0.6)
+    ap.add_argument("--order", type=int, default=4)
+    ap.add_argument("--eps", type=float, default=1e-4)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--out", default=TUNED)
+    a = ap.parse_args(argv)
+    import torch  # noqa: F401  (one HIP runtime in the process: torch's)
+    from . import ics
+    if a.workload == "clustered":
+        pos, mass = ics.clustered(a.n, seed=a.seed)
+    elif a.workload == "plummer":
+        pos, mass = ics.plummer(a.n, seed=a.seed)
+    else:
+        pos, mass = ics.lattice_grf(int(round(a.n ** (1 / 3))), seed=a.seed)
+    base = dict(theta=a.theta, order=a.order, eps=a.eps, soft="plummer",
+                periodic=a.workload == "lattice")
+    best, table = tune(pos, mass, base, log=lambda s: print(s, file=sys.stderr, flush=True))
+    key = workload_key(pos.shape[0], a.theta, a.order, "plummer", a.eps, a.workload)
+    save(key, best, table, a.out)
+    print(json.dumps({"key": key, "best": {k: best[k] for k in
+                                           ("merge_groups", "group_max", "leaf_max",
+                                            "persistent_blocks", "ms")}}))
+
+
+if __name__ == "__main__":
+    main()

@@ -48,6 +48,7 @@ register({
     "pgrav_plan_adjoint": (_i, [_vp, _vp, _vp]),
     "pgrav_plan_stats": (_i, [_vp, _S]),
     "pgrav_plan_evaluate": (_i, [_vp, _i, _vp]),
+    "pgrav_plan_set_launch": (_i, [_vp, _i]),
     "pgrav_plan_time": (_i, [_vp, _i, _i, _vp, ctypes.POINTER(_d)]),
     "pgrav_plan_graph_capture": (_i, [_vp, _i]),
     "pgrav_plan_graph_launch": (_i, [_vp, _vp]),

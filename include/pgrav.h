@@ -166,6 +166,10 @@ int pgrav_plan_update(void *plan, const double *pos3, const double *mass);
  * (the reference has no derivatives). */
 int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4);
 int pgrav_plan_stats(void *plan, pgrav_stats *st);
+/* Launch shape for later evaluations: 0 one workgroup per group, -1 auto
+ * persistent grid, k > 0 a persistent grid of k workgroups (a captured graph
+ * keeps the shape it was captured with). */
+int pgrav_plan_set_launch(void *plan, int persistent_blocks);
 /* Enqueues one force evaluation on `stream` (NULL: the plan's stream). */
 int pgrav_plan_evaluate(void *plan, int variant, void *stream);
 /* Average device time of one evaluation over `reps` back-to-back launches,
