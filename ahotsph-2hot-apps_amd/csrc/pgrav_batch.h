@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #define EWALD_N 64  // Ewald correction table points per axis over [0, L/2]
+#define NXCD 8      // MI355X accelerator complex dies (one L2 each)
 
 namespace pgrav {
 
@@ -24,7 +25,9 @@ struct BatchView {
     const int64_t *gfirst = nullptr;
     const int *gcount = nullptr;
     const double4 *gcen = nullptr;
-    const int *lpt = nullptr;
+    const int *lpt = nullptr;   // dispatch order
+    const int *xoff = nullptr;  // NXCD + 1 slice offsets into lpt
+    const int *order_np = nullptr;  // order for one workgroup per group
     const int64_t *off[4] = {nullptr, nullptr, nullptr, nullptr};
     const uint32_t *idx[4] = {nullptr, nullptr, nullptr, nullptr};
     float4 *out32 = nullptr;

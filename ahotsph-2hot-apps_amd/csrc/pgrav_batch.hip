@@ -40,7 +40,9 @@ struct BatchArgs {
     const int64_t *gfirst;
     const int *gcount;
     const double4 *gcen;
-    const int *lpt;
+    const int *lpt;   // dispatch order of the groups
+    const int *xoff;  // NXCD + 1 slice offsets into lpt (persistent launch)
+    const int *order_np;  // dispatch order of the one-workgroup-per-group launch
     const int64_t *off[4];
     const uint32_t *idx[4];
     void *out;  // float4* or double4*, sorted body order
@@ -53,8 +55,18 @@ struct BatchArgs {
     double box;
 };
 
+// nearest periodic image of a displacement (identity when not periodic)
+template <bool PERIODIC, typename T>
+__device__ __forceinline__ T pbc(T x, T box, T ibox) {
+    if (PERIODIC)
+        return x - box * rint(x * ibox);
+    return x;
+}
+
 // ---- Ewald correction (periodic): trilinear lookup in the FP32 copy of the
-// FP64 table built by k_ewald_table; x = sink - source, wrapped to [-L/2, L/2].
+// FP64 table built by k_ewald_table.  x = sink - source, already the nearest
+// image (|x_i| <= L/2), so the Newtonian term the caller added and this
+// correction refer to the same image.
 template <typename T>
 __device__ __forceinline__ void ewald_corr(const float4 *__restrict__ tab, T m, T x, T y, T z,
                                            T box, Acc4T<T> &a) {
@@ -62,9 +74,6 @@ __device__ __forceinline__ void ewald_corr(const float4 *__restrict__ tab, T m, 
     const T inv_h = T(EWALD_N - 1) / half;
     T sx = x < 0 ? T(-1) : T(1), sy = y < 0 ? T(-1) : T(1), sz = z < 0 ? T(-1) : T(1);
     T ax = fabs(x), ay = fabs(y), az = fabs(z);
-    ax = ax > half ? box - ax : ax;  // beyond half a box the nearest image is the other side
-    ay = ay > half ? box - ay : ay;
-    az = az > half ? box - az : az;
     T u = ax * inv_h, v = ay * inv_h, w = az * inv_h;
     // clamp both ways: a NaN offset must not turn into an out-of-range index
     int i = max(0, min((int)u, EWALD_N - 2)), j = max(0, min((int)v, EWALD_N - 2)),
@@ -81,10 +90,11 @@ __device__ __forceinline__ void ewald_corr(const float4 *__restrict__ tab, T m, 
            fv * ((T(1) - fw) * T(c110.f) + fw * T(c111.f))))
     const T cx = LERP3(x), cy = LERP3(y), cz = LERP3(z), cp = LERP3(w);
 #undef LERP3
-    // table holds C(r) = f_ewald(r) - r/|r|^3 for r = source - sink = -x, odd per axis
-    a.x += m * sx * cx;
-    a.y += m * sy * cy;
-    a.z += m * sz * cz;
+    // the table holds C(r) = f_ewald(r) - r/|r|^3 on the positive octant; C is odd
+    // per axis, and the pull on the sink is m C(r) at r = source - sink = -x
+    a.x -= m * sx * cx;
+    a.y -= m * sy * cy;
+    a.z -= m * sz * cz;
     a.w += m * cp;
 }
 
@@ -148,6 +158,7 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
     const T eps_inv = SOFT == SOFT_K1 ? T(1) / eps : T(0);
     const T eps2_cell = SOFT == SOFT_PLUMMER ? eps2 : T(0);
     const T box = T(A.box);
+    const T ibox = T(1) / box;
     const T *__restrict__ mom = (const T *)A.mom;
     Acc4T<T> s = {0, 0, 0, 0}, c = {0, 0, 0, 0};
 
@@ -168,7 +179,9 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                 for (int k = 0; k < nt; k++) {
                     const uint32_t e = idx[t0 + k];
                     const V4<T> r = lds[k];
-                    const T x = sx - r.x, y = sy - r.y, z = sz - r.z;
+                    const T x = pbc<PERIODIC>(sx - r.x, box, ibox);
+                    const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
+                    const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
                     cell_interact<4, false, 1, T>(mom + (size_t)e * PREC4, r.w, x, y, z, eps2_cell, p);
                     if (PERIODIC)
                         ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
@@ -177,7 +190,9 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                 for (int k = 0; k < nt; k++) {
                     const uint32_t e = idx[t0 + k];
                     const V4<T> r = lds[k];
-                    const T x = sx - r.x, y = sy - r.y, z = sz - r.z;
+                    const T x = pbc<PERIODIC>(sx - r.x, box, ibox);
+                    const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
+                    const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
                     cell_interact<2, false, 1, T>(mom + (size_t)e * PREC4, r.w, x, y, z, eps2_cell, p);
                     if (PERIODIC)
                         ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
@@ -186,7 +201,9 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
 #pragma unroll 4
                 for (int k = 0; k < nt; k++) {
                     const V4<T> r = lds[k];
-                    const T x = sx - r.x, y = sy - r.y, z = sz - r.z;
+                    const T x = pbc<PERIODIC>(sx - r.x, box, ibox);
+                    const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
+                    const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
                     mono_interact<T>(r.w, x, y, z, eps2_cell, p);
                     if (PERIODIC)
                         ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
@@ -196,7 +213,9 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
 #pragma unroll 4
                 for (int k = 0; k < nt; k++) {
                     const V4<T> r = lds[k];
-                    const T x = sx - r.x, y = sy - r.y, z = sz - r.z;
+                    const T x = pbc<PERIODIC>(sx - r.x, box, ibox);
+                    const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
+                    const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
                     if (SOFT == SOFT_K1) {
                         k1_interact<T>(r.w, x, y, z, eps_inv, eps2, p);
                     } else if (SOFT == SOFT_PLUMMER) {
@@ -227,30 +246,37 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
 template <typename T, bool REL, bool COMP, int SOFT, bool PERIODIC>
 __global__ __launch_bounds__(64) void k_batch(BatchArgs A) {
     __shared__ V4<T> lds[64];
-    eval_group<T, REL, COMP, SOFT, PERIODIC>(A, A.lpt[blockIdx.x], lds);
+    eval_group<T, REL, COMP, SOFT, PERIODIC>(A, A.order_np[blockIdx.x], lds);
 }
 
-// persistent: a fixed grid drains a device work counter; the last workgroup to
-// leave resets the counters so the launch can be replayed (e.g. from a graph)
+// persistent: a fixed grid drains per-XCD work counters.  Workgroup b runs on
+// XCD b % NXCD (the dispatcher deals workgroups to the XCDs round-robin) and
+// takes groups only from that XCD's slice [xoff[x], xoff[x+1]) of the dispatch
+// order: the host cuts the Morton-ordered groups into NXCD contiguous slices of
+// equal estimated cost, so neighbouring groups -- which share most of their
+// cells -- run under one XCD's L2.  The last workgroup to leave resets the
+// counters so the launch can be replayed (e.g. from a graph).
 template <typename T, bool REL, bool COMP, int SOFT, bool PERIODIC>
 __global__ __launch_bounds__(64) void k_batch_persistent(BatchArgs A) {
     __shared__ V4<T> lds[64];
     __shared__ int w;
+    const int x = blockIdx.x % NXCD;
+    const int lo = A.xoff[x], hi = A.xoff[x + 1];
     for (;;) {
         if (threadIdx.x == 0)
-            w = atomicAdd(&A.counter[0], 1);
+            w = lo + atomicAdd(&A.counter[2 + x], 1);
         __syncthreads();
         const int item = w;
         __syncthreads();
-        if (item >= A.ngroups)
+        if (item >= hi)
             break;
         eval_group<T, REL, COMP, SOFT, PERIODIC>(A, A.lpt[item], lds);
     }
     if (threadIdx.x == 0) {
         const int done = atomicAdd(&A.counter[1], 1);
         if (done == (int)gridDim.x - 1) {
-            atomicExch(&A.counter[0], 0);
             atomicExch(&A.counter[1], 0);
+            for (int k = 0; k < NXCD; k++) atomicExch(&A.counter[2 + k], 0);
         }
     }
 }
@@ -263,7 +289,7 @@ __global__ __launch_bounds__(64) void k_group64(BatchArgs A, int g) {
 }
 
 __global__ void k_reset_counter(int *c) {
-    if (threadIdx.x < 2)
+    if (threadIdx.x < 2 + NXCD)
         c[threadIdx.x] = 0;
 }
 
@@ -397,6 +423,8 @@ hipError_t launch_reset_counter(int *c, hipStream_t st) {
 
 template <typename T, bool REL, bool COMP, int SOFT, bool PER>
 static void launch_one(const BatchArgs &A, int persistent_blocks, hipStream_t st) {
+    // every XCD needs workgroups to drain its slice: round up to a multiple of NXCD
+    persistent_blocks = (persistent_blocks + NXCD - 1) / NXCD * NXCD;
     if (persistent_blocks > 0)
         hipLaunchKernelGGL((k_batch_persistent<T, REL, COMP, SOFT, PER>), dim3(persistent_blocks),
                            dim3(64), 0, st, A);
@@ -430,6 +458,8 @@ hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, 
     A.gcount = v.gcount;
     A.gcen = v.gcen;
     A.lpt = v.lpt;
+    A.xoff = v.xoff;
+    A.order_np = v.order_np;
     for (int k = 0; k < 4; k++) {
         A.off[k] = v.off[k];
         A.idx[k] = v.idx[k];
@@ -464,6 +494,8 @@ hipError_t launch_group64(const BatchView &v, int g, hipStream_t st) {
     A.gcount = v.gcount;
     A.gcen = v.gcen;
     A.lpt = v.lpt;
+    A.xoff = v.xoff;
+    A.order_np = v.order_np;
     for (int k = 0; k < 4; k++) {
         A.off[k] = v.off[k];
         A.idx[k] = v.idx[k];

@@ -155,6 +155,7 @@ void pgrav_params_default(pgrav_params *p) {
     p->merge_groups = 1;
     p->persistent_blocks = 0;
     p->fp64 = 1;
+    p->dispatch = 1;
 }
 
 void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const pgrav_params *p) {
@@ -200,6 +201,7 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         P->tp.box = p->box;
         P->tp.nthreads = p->nthreads;
         P->tp.merge_groups = p->merge_groups;
+        P->tp.dispatch = p->dispatch;
         double t0 = now();
         build_tree(pos3, mass, n, P->tp, P->tree);
         double t1 = now();
@@ -244,6 +246,8 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         P->v.gcount = P->upload(L.gcount.data(), L.gcount.size());
         P->v.gcen = (const double4 *)P->upload(L.gcenter.data(), L.gcenter.size());
         P->v.lpt = P->upload(L.lpt.data(), L.lpt.size());
+        P->v.xoff = P->upload(L.xoff.data(), L.xoff.size());
+        P->v.order_np = P->upload(L.order_np.data(), L.order_np.size());
         for (int k = 0; k < NLIST; k++) {
             P->v.off[k] = P->upload(L.off[k].data(), L.off[k].size());
             P->v.idx[k] = P->upload(L.idx[k].data(), L.idx[k].size());
@@ -398,6 +402,30 @@ int pgrav_plan_stats(void *plan, pgrav_stats *st) {
     st->t_upload = P->t_upload;
     st->mean_group = P->lists.ngroups ? (double)P->tree.n / P->lists.ngroups : 0.0;
     return 0;
+}
+
+int pgrav_plan_set_dispatch(void *plan, int mode) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P || (mode != 0 && mode != 1)) {
+        fail("pgrav_plan_set_dispatch: bad arguments");
+        return -1;
+    }
+    Lists &L = P->lists;
+    const int64_t G = L.ngroups;
+    std::vector<double> cost(G);
+    for (int64_t g = 0; g < G; g++)
+        cost[g] = 113.0 * (L.off[L_HEXA][g + 1] - L.off[L_HEXA][g]) +
+                  40.0 * (L.off[L_QUAD][g + 1] - L.off[L_QUAD][g]) +
+                  17.0 * (L.off[L_MONO][g + 1] - L.off[L_MONO][g]) +
+                  20.0 * (L.off[L_BODY][g + 1] - L.off[L_BODY][g]);
+    make_dispatch(cost, mode, L);
+    P->prm.dispatch = mode;
+    return (check(hipStreamSynchronize(P->stream), "hipStreamSynchronize") &&
+            check(hipMemcpy((void *)P->v.lpt, L.lpt.data(), 4 * G, hipMemcpyHostToDevice), "hipMemcpy") &&
+            check(hipMemcpy((void *)P->v.xoff, L.xoff.data(), 4 * L.xoff.size(), hipMemcpyHostToDevice), "hipMemcpy") &&
+            check(hipMemcpy((void *)P->v.order_np, L.order_np.data(), 4 * G, hipMemcpyHostToDevice), "hipMemcpy"))
+               ? 0 : -1;
 }
 
 int pgrav_plan_set_launch(void *plan, int persistent_blocks) {

@@ -76,6 +76,60 @@ void build_groups(const Tree &t, const TreeParams &p, Lists &l) {
             l.gfirst.push_back(a);
             l.gcount.push_back((int32_t)std::min<int64_t>(p.group_max, t.n - a));
         }
+    } else if (p.merge_groups == 3) {
+        // maximal cells with <= group_max bodies, then consecutive ones (along the
+        // Morton curve, across parents) merged while the merged bounding box
+        // stays within twice the larger box edge
+        TreeParams q = p;
+        q.merge_groups = 0;
+        Lists base;
+        collect(t, 0, q, base);
+        auto box = [&](int64_t f, int64_t c, double *lo, double *hi) {
+            for (int a = 0; a < 3; a++) {
+                lo[a] = 1e300;
+                hi[a] = -1e300;
+            }
+            for (int64_t i = f; i < f + c; i++)
+                for (int a = 0; a < 3; a++) {
+                    lo[a] = std::min(lo[a], t.pos[3 * i + a]);
+                    hi[a] = std::max(hi[a], t.pos[3 * i + a]);
+                }
+        };
+        auto edge = [](const double *lo, const double *hi) {
+            return std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+        };
+        int64_t cf = base.gfirst[0], cc = base.gcount[0];
+        double clo[3], chi[3];
+        box(cf, cc, clo, chi);
+        for (size_t k = 1; k < base.gfirst.size(); k++) {
+            const int64_t f = base.gfirst[k], c = base.gcount[k];
+            double lo[3], hi[3];
+            box(f, c, lo, hi);
+            double mlo[3], mhi[3];
+            for (int a = 0; a < 3; a++) {
+                mlo[a] = std::min(lo[a], clo[a]);
+                mhi[a] = std::max(hi[a], chi[a]);
+            }
+            if (cc + c <= p.group_max &&
+                edge(mlo, mhi) <= 2.0 * std::max(edge(lo, hi), edge(clo, chi))) {
+                cc += c;
+                for (int a = 0; a < 3; a++) {
+                    clo[a] = mlo[a];
+                    chi[a] = mhi[a];
+                }
+            } else {
+                l.gfirst.push_back(cf);
+                l.gcount.push_back((int32_t)cc);
+                cf = f;
+                cc = c;
+                for (int a = 0; a < 3; a++) {
+                    clo[a] = lo[a];
+                    chi[a] = hi[a];
+                }
+            }
+        }
+        l.gfirst.push_back(cf);
+        l.gcount.push_back((int32_t)cc);
     } else {
         collect(t, 0, p, l);
     }
@@ -233,10 +287,56 @@ void build_lists(const Tree &t, const TreeParams &p, Lists &l) {
                   40.0 * (l.off[L_QUAD][g + 1] - l.off[L_QUAD][g]) +
                   17.0 * (l.off[L_MONO][g + 1] - l.off[L_MONO][g]) +
                   20.0 * (l.off[L_BODY][g + 1] - l.off[L_BODY][g]);
-    l.lpt.resize(G);
-    std::iota(l.lpt.begin(), l.lpt.end(), 0);
-    std::stable_sort(l.lpt.begin(), l.lpt.end(),
-                     [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+    make_dispatch(cost, p.dispatch, l);
+}
+
+void make_dispatch(const std::vector<double> &cost, int mode, Lists &l) {
+    const int64_t G = (int64_t)cost.size();
+    const int X = HOST_NXCD;
+    std::vector<std::vector<int32_t>> slice(X);
+    if (mode == 0) {
+        // global longest-first, dealt round-robin to the slices
+        std::vector<int32_t> all(G);
+        std::iota(all.begin(), all.end(), 0);
+        std::stable_sort(all.begin(), all.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+        for (int64_t i = 0; i < G; i++) slice[i % X].push_back(all[i]);
+    } else {
+        // contiguous Morton ranges of equal total cost, longest-first inside
+        double total = 0;
+        for (double c : cost) total += c;
+        double acc = 0;
+        int s = 0;
+        for (int64_t g = 0; g < G; g++) {
+            if (acc >= total * (s + 1) / X && s < X - 1)
+                s++;
+            slice[s].push_back((int32_t)g);
+            acc += cost[g];
+        }
+        for (auto &v : slice)
+            std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+    }
+    l.lpt.clear();
+    l.xoff.assign(X + 1, 0);
+    for (int x = 0; x < X; x++) {
+        l.xoff[x] = (int32_t)l.lpt.size();
+        l.lpt.insert(l.lpt.end(), slice[x].begin(), slice[x].end());
+    }
+    l.xoff[X] = (int32_t)l.lpt.size();
+    // one workgroup per group: block b lands on XCD b % X, so deal the slices
+    // round-robin (a slice that runs out is skipped)
+    l.order_np.clear();
+    std::vector<size_t> pos(X, 0);
+    for (int64_t b = 0; (int64_t)l.order_np.size() < G; b++) {
+        const int x = (int)(b % X);
+        if (pos[x] < slice[x].size())
+            l.order_np.push_back(slice[x][pos[x]++]);
+        else
+            for (int y = 0; y < X; y++)
+                if (pos[y] < slice[y].size()) {
+                    l.order_np.push_back(slice[y][pos[y]++]);
+                    break;
+                }
+    }
 }
 
 }  // namespace pgrav

@@ -19,7 +19,10 @@ struct TreeParams {
     double box = 1.0;      // periodic box size (origin at 0)
     int nthreads = 0;      // 0: hardware concurrency
     int merge_groups = 1;  // merge adjacent small sibling groups up to group_max
+    int dispatch = 1;      // 0: global longest-first; 1: per-XCD Morton slices, longest-first inside
 };
+
+const int HOST_NXCD = 8;  // MI355X XCDs (one L2 each); workgroup b runs on XCD b % 8
 
 struct Tree {
     int64_t n = 0;
@@ -51,8 +54,13 @@ struct Lists {
     std::vector<int64_t> off[NLIST];   // ngroups + 1 offsets per list kind
     std::vector<uint32_t> idx[NLIST];  // node indices (cells) or sorted body slots
     int64_t pairs[NLIST] = {0, 0, 0, 0};  // (sink, source) pairs per kind
-    std::vector<int32_t> lpt;     // groups by decreasing estimated cost
+    std::vector<int32_t> lpt;     // dispatch order: NXCD slices, longest-first within each
+    std::vector<int32_t> xoff;    // NXCD + 1 slice offsets into lpt
+    std::vector<int32_t> order_np;  // one-workgroup-per-group order, slices dealt round-robin
 };
+
+// Dispatch order for a cost estimate per group (see TreeParams::dispatch).
+void make_dispatch(const std::vector<double> &cost, int mode, Lists &l);
 
 void build_tree(const double *pos, const double *mass, int64_t n, const TreeParams &p, Tree &t);
 void compute_moments(Tree &t, int nthreads, const std::vector<uint64_t> *key);

@@ -27,7 +27,7 @@ class Params(ctypes.Structure):
                 ("soft_type", ctypes.c_int), ("eps", ctypes.c_double), ("G", ctypes.c_double),
                 ("periodic", ctypes.c_int), ("box", ctypes.c_double), ("nthreads", ctypes.c_int),
                 ("merge_groups", ctypes.c_int), ("persistent_blocks", ctypes.c_int),
-                ("fp64", ctypes.c_int)]
+                ("fp64", ctypes.c_int), ("dispatch", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -49,6 +49,7 @@ register({
     "pgrav_plan_stats": (_i, [_vp, _S]),
     "pgrav_plan_evaluate": (_i, [_vp, _i, _vp]),
     "pgrav_plan_set_launch": (_i, [_vp, _i]),
+    "pgrav_plan_set_dispatch": (_i, [_vp, _i]),
     "pgrav_plan_time": (_i, [_vp, _i, _i, _vp, ctypes.POINTER(_d)]),
     "pgrav_plan_graph_capture": (_i, [_vp, _i]),
     "pgrav_plan_graph_launch": (_i, [_vp, _vp]),
@@ -72,7 +73,7 @@ register({
 
 def make_params(theta=0.6, order=4, eps=1e-4, soft="plummer", G=1.0, group_max=64, leaf_max=16,
                 pcut=0, periodic=False, box=1.0, nthreads=0, merge_groups=True,
-                persistent_blocks=0, fp64=True):
+                persistent_blocks=0, fp64=True, dispatch=1):
     p = Params()
     lib().pgrav_params_default(ctypes.byref(p))
     p.theta, p.order, p.eps, p.G = theta, order, eps, G
@@ -80,6 +81,7 @@ def make_params(theta=0.6, order=4, eps=1e-4, soft="plummer", G=1.0, group_max=6
     p.group_max, p.leaf_max, p.pcut = group_max, leaf_max, pcut
     p.periodic, p.box, p.nthreads = int(periodic), box, nthreads
     p.merge_groups, p.persistent_blocks, p.fp64 = int(merge_groups), persistent_blocks, int(fp64)
+    p.dispatch = dispatch
     return p
 
 
@@ -148,6 +150,14 @@ class Plan:
         lib().pgrav_plan_adjoint(self.h, w4.ctypes.data, out.ctypes.data)
         check()
         return out
+
+    def set_launch(self, persistent_blocks):
+        lib().pgrav_plan_set_launch(self.h, int(persistent_blocks))
+        check()
+
+    def set_dispatch(self, mode):
+        lib().pgrav_plan_set_dispatch(self.h, int(mode))
+        check()
 
     def time(self, variant=REL32C, reps=3, stream=None):
         ms = ctypes.c_double(0)

@@ -58,11 +58,12 @@ def main():
     ap.add_argument("--workload", default="clustered", choices=["clustered", "plummer", "lattice"])
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--variant", default="rel32c")
-    ap.add_argument("--group-mode", type=int, default=1)
+    ap.add_argument("--group-mode", type=int, default=2)
     ap.add_argument("--persistent", type=int, default=0)
     ap.add_argument("--sample", type=int, default=8192)
     ap.add_argument("--no-plain64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tuned", action="store_true", help="ignore tuned.json")
     ap.add_argument("--quick", action="store_true", help="N = 2^21 for a fast check")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -80,10 +81,22 @@ def main():
     t0 = time.perf_counter()
     pos, mass = make_workload(args)
     log("workload %s N=%d built in %.1fs" % (args.workload, pos.shape[0], time.perf_counter() - t0))
+    # decomposition parameters: the autotuner's choice for this workload when
+    # ahotsph-2hot-apps_amd/tuned.json holds one
+    from pgrav import autotune
+    key = autotune.workload_key(pos.shape[0], args.theta, args.order, "plummer", args.eps,
+                                args.workload)
+    tuned = None if args.no_tuned else autotune.load(key)
+    dec = {"merge_groups": args.group_mode, "group_max": 64, "leaf_max": 16,
+           "persistent_blocks": args.persistent}
+    if tuned:
+        dec.update({k: tuned["best"][k] for k in dec})
+        log("using tuned decomposition %s" % dec)
     t0 = time.perf_counter()
     plan = batch.Plan(pos, mass, theta=args.theta, order=args.order, eps=args.eps,
-                      soft="plummer", merge_groups=args.group_mode,
-                      persistent_blocks=args.persistent, fp64=True)
+                      soft="plummer", merge_groups=dec["merge_groups"],
+                      group_max=dec["group_max"], leaf_max=dec["leaf_max"],
+                      persistent_blocks=dec["persistent_blocks"], fp64=True)
     st = plan.stats()
     log("plan: %.1fs (tree %.1fs lists %.1fs upload %.1fs) groups %d mean %.1f pairs %.3e" %
         (time.perf_counter() - t0, st["t_tree"], st["t_lists"], st["t_upload"], st["ngroups"],
@@ -135,11 +148,16 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        # the reference's own SSE4/AVX CPU kernels (oracle/_ref) on a bounded
+        # sample of the same workload's groups; the checker side, not the product
         try:
-            from cpu_baseline import run_cpu_baseline  # oracle-side harness
-            cpu = run_cpu_baseline(pos, mass, args)
+            from oracle.cpu_baseline import run_cpu_baseline
+            cpu = run_cpu_baseline(pos, mass, args, acc_gpu=acc)
+            log("cpu baseline: %.3e interactions/s on %d threads (%s)" %
+                (cpu["value"], cpu["cores"], cpu["cpu_model"]))
         except Exception as e:  # reported, never silently replaced
             log("cpu baseline unavailable: %s" % e)
+            cpu = {"error": str(e)}
 
     out = {
         "metric": METRIC,
@@ -157,7 +175,8 @@ def main():
         "config": {"workload": "%s N=%d order %d theta %.2f Plummer eps %g G 1" %
                    (args.workload, pos.shape[0], args.order, args.theta, args.eps),
                    "variant": var, "ngroups": st["ngroups"], "pairs": pairs,
-                   "mean_group": st["mean_group"]},
+                   "mean_group": st["mean_group"], "decomposition": dec,
+                   "host_build_s": st["t_tree"] + st["t_lists"]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "kernel_ms": kern_ms, "algorithmic_bytes": nbytes,

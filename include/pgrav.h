@@ -135,6 +135,7 @@ typedef struct {
     int merge_groups;      /* merge adjacent small sibling groups */
     int persistent_blocks; /* 0: one workgroup per group; >0: persistent grid; -1: auto */
     int fp64;              /* also build the FP64 tables (variants 3 and 4) */
+    int dispatch;          /* 0: global longest-first; 1: per-XCD Morton slices (default) */
 } pgrav_params;
 
 typedef struct {
@@ -170,6 +171,8 @@ int pgrav_plan_stats(void *plan, pgrav_stats *st);
  * persistent grid, k > 0 a persistent grid of k workgroups (a captured graph
  * keeps the shape it was captured with). */
 int pgrav_plan_set_launch(void *plan, int persistent_blocks);
+/* Group dispatch order: 0 global longest-first, 1 per-XCD Morton slices. */
+int pgrav_plan_set_dispatch(void *plan, int mode);
 /* Enqueues one force evaluation on `stream` (NULL: the plan's stream). */
 int pgrav_plan_evaluate(void *plan, int variant, void *stream);
 /* Average device time of one evaluation over `reps` back-to-back launches,

@@ -108,6 +108,26 @@ def test_periodic_lists_use_nearest_images():
         assert listoracle.interactions_cover_all(H, g, pos.shape[0])
 
 
+def test_reference_layout_emitters(clustered):
+    """SoA-8 blocks (nln/pgrav.cu:66 indexing) and MMvec segments built from a
+    group's lists hold exactly the list's records and bodies."""
+    from pgrav import reflists
+    pos, mass, H, perm, spos = clustered
+    g = H.stats()["ngroups"] // 2
+    soa, n = reflists.group_soa(H, g, "hexa")
+    nodes = H.list(g, "hexa")
+    assert n % 8 == 0 and n >= nodes.size
+    recs = H.records(nodes)
+    i = nodes.size - 1
+    for k in (0, 1, 4, 12, 28):
+        assert soa[(i // 8) * 8 * 29 + k * 8 + i % 8] == np.float32(recs[i, k])
+    seg, nb = reflists.group_segments(H, g)
+    body = np.sort(H.list(g, "body"))
+    assert nb == body.size
+    expand = np.concatenate([np.arange(b, b + l) for b, l in seg])
+    np.testing.assert_array_equal(expand, body)
+
+
 def test_bad_parameters_raise():
     pos, mass = ics.plummer(100, seed=1)
     with pytest.raises(PgravError):
