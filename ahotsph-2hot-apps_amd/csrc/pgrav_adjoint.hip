@@ -129,8 +129,9 @@ __global__ __launch_bounds__(64) void k_adjoint(BatchView V, const float4 *__res
     float gx = 0.f, gy = 0.f, gz = 0.f;
 
     for (int kind = 0; kind < 4; kind++) {
-        const int64_t o0 = V.off[kind][g], o1 = V.off[kind][g + 1];
-        const uint32_t *__restrict__ idx = V.idx[kind];
+        const int64_t *offk = V.off + (int64_t)kind * (V.ngroups + 1);
+        const int64_t o0 = offk[g], o1 = offk[g + 1];
+        const uint32_t *__restrict__ idx = V.idx;
         for (int64_t t0 = o0; t0 < o1; t0 += 64) {
             const int nt = (int)min((int64_t)64, o1 - t0);
             if (lane < nt) {

@@ -28,8 +28,11 @@ struct BatchView {
     const int *lpt = nullptr;   // dispatch order
     const int *xoff = nullptr;  // NXCD + 1 slice offsets into lpt
     const int *order_np = nullptr;  // order for one workgroup per group
-    const int64_t *off[4] = {nullptr, nullptr, nullptr, nullptr};
-    const uint32_t *idx[4] = {nullptr, nullptr, nullptr, nullptr};
+    // CSR lists of all four kinds in one index array: the entries of kind k
+    // (order-4, order-2, order-0 cells, bodies) for group g are
+    // idx[off[k*(ngroups+1)+g] .. off[k*(ngroups+1)+g+1])
+    const int64_t *off = nullptr;
+    const uint32_t *idx = nullptr;
     float4 *out32 = nullptr;
     double4 *out64 = nullptr;
     int *counter = nullptr;

@@ -43,8 +43,8 @@ struct BatchArgs {
     const int *lpt;   // dispatch order of the groups
     const int *xoff;  // NXCD + 1 slice offsets into lpt (persistent launch)
     const int *order_np;  // dispatch order of the one-workgroup-per-group launch
-    const int64_t *off[4];
-    const uint32_t *idx[4];
+    const int64_t *off;  // [4][ngroups+1], absolute positions in idx
+    const uint32_t *idx;
     void *out;  // float4* or double4*, sorted body order
     int *counter;
     const float4 *ewald;  // 64^3 correction table (periodic)
@@ -164,8 +164,9 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
 
 #pragma unroll 1
     for (int kind = 0; kind < 4; kind++) {
-        const int64_t o0 = A.off[kind][g], o1 = A.off[kind][g + 1];
-        const uint32_t *__restrict__ idx = A.idx[kind];
+        const int64_t *offk = A.off + (int64_t)kind * (A.ngroups + 1);
+        const uint32_t *__restrict__ idx = A.idx;
+        const int64_t o0 = offk[g], o1 = offk[g + 1];
         for (int64_t t0 = o0; t0 < o1; t0 += 64) {
             const int nt = (int)min((int64_t)64, o1 - t0);
             if (lane < nt) {
@@ -256,8 +257,31 @@ __global__ __launch_bounds__(64) void k_batch(BatchArgs A) {
 // equal estimated cost, so neighbouring groups -- which share most of their
 // cells -- run under one XCD's L2.  The last workgroup to leave resets the
 // counters so the launch can be replayed (e.g. from a graph).
+//
+// Every buffer also arrives as its own __restrict__ argument: the loop stores
+// a group's results and bumps the counters before it reads the next group's
+// lists and records, and only pointers known to be distinct let the compiler
+// keep those reads on wave-uniform scalar loads.
 template <typename T, bool REL, bool COMP, int SOFT, bool PERIODIC>
-__global__ __launch_bounds__(64) void k_batch_persistent(BatchArgs A) {
+__global__ __launch_bounds__(64) void k_batch_persistent(
+    BatchArgs A, const double4 *__restrict__ bpos, const double4 *__restrict__ cpos,
+    const T *__restrict__ mom, const int64_t *__restrict__ gfirst, const int *__restrict__ gcount,
+    const double4 *__restrict__ gcen, const int *__restrict__ lpt, const int *__restrict__ xoff,
+    const int64_t *__restrict__ off, const uint32_t *__restrict__ idx, V4<T> *__restrict__ out,
+    int *__restrict__ counter, const float4 *__restrict__ ewald) {
+    A.bpos = bpos;
+    A.cpos = cpos;
+    A.mom = mom;
+    A.gfirst = gfirst;
+    A.gcount = gcount;
+    A.gcen = gcen;
+    A.lpt = lpt;
+    A.xoff = xoff;
+    A.off = off;
+    A.idx = idx;
+    A.out = out;
+    A.counter = counter;
+    A.ewald = ewald;
     __shared__ V4<T> lds[64];
     __shared__ int w;
     const int x = blockIdx.x % NXCD;
@@ -266,7 +290,10 @@ __global__ __launch_bounds__(64) void k_batch_persistent(BatchArgs A) {
         if (threadIdx.x == 0)
             w = lo + atomicAdd(&A.counter[2 + x], 1);
         __syncthreads();
-        const int item = w;
+        // the LDS value is the same in every lane: say so, so that the group's
+        // scalars (list offsets, record addresses) stay in SGPRs and the
+        // records keep coming through scalar loads
+        const int item = __builtin_amdgcn_readfirstlane(w);
         __syncthreads();
         if (item >= hi)
             break;
@@ -427,7 +454,9 @@ static void launch_one(const BatchArgs &A, int persistent_blocks, hipStream_t st
     persistent_blocks = (persistent_blocks + NXCD - 1) / NXCD * NXCD;
     if (persistent_blocks > 0)
         hipLaunchKernelGGL((k_batch_persistent<T, REL, COMP, SOFT, PER>), dim3(persistent_blocks),
-                           dim3(64), 0, st, A);
+                           dim3(64), 0, st, A, A.bpos, A.cpos, (const T *)A.mom, A.gfirst,
+                           A.gcount, A.gcen, A.lpt, A.xoff, A.off, A.idx, (V4<T> *)A.out,
+                           A.counter, A.ewald);
     else
         hipLaunchKernelGGL((k_batch<T, REL, COMP, SOFT, PER>), dim3(A.ngroups), dim3(64), 0, st, A);
 }
@@ -460,10 +489,8 @@ hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, 
     A.lpt = v.lpt;
     A.xoff = v.xoff;
     A.order_np = v.order_np;
-    for (int k = 0; k < 4; k++) {
-        A.off[k] = v.off[k];
-        A.idx[k] = v.idx[k];
-    }
+    A.off = v.off;
+    A.idx = v.idx;
     A.out = variant == VAR_F64 ? (void *)v.out64 : (void *)v.out32;
     A.counter = v.counter;
     A.ewald = v.ewald32;
@@ -496,10 +523,8 @@ hipError_t launch_group64(const BatchView &v, int g, hipStream_t st) {
     A.lpt = v.lpt;
     A.xoff = v.xoff;
     A.order_np = v.order_np;
-    for (int k = 0; k < 4; k++) {
-        A.off[k] = v.off[k];
-        A.idx[k] = v.idx[k];
-    }
+    A.off = v.off;
+    A.idx = v.idx;
     A.out = v.out64;
     A.counter = v.counter;
     A.ewald = v.ewald32;

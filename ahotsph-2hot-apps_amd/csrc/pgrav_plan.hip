@@ -248,10 +248,27 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         P->v.lpt = P->upload(L.lpt.data(), L.lpt.size());
         P->v.xoff = P->upload(L.xoff.data(), L.xoff.size());
         P->v.order_np = P->upload(L.order_np.data(), L.order_np.size());
-        for (int k = 0; k < NLIST; k++) {
-            P->v.off[k] = P->upload(L.off[k].data(), L.off[k].size());
-            P->v.idx[k] = P->upload(L.idx[k].data(), L.idx[k].size());
-            std::vector<uint32_t>().swap(L.idx[k]);  // the device copy is the one used
+        {
+            // one CSR over all kinds: offsets [NLIST][G+1] into one index array
+            const size_t G1 = (size_t)L.ngroups + 1;
+            std::vector<int64_t> off(NLIST * G1);
+            size_t total = 0;
+            for (int k = 0; k < NLIST; k++) {
+                for (size_t g = 0; g < G1; g++) off[k * G1 + g] = L.off[k][g] + (int64_t)total;
+                total += L.idx[k].size();
+            }
+            P->v.off = P->upload(off.data(), off.size());
+            uint32_t *idx = P->dalloc<uint32_t>(std::max<size_t>(1, total) * sizeof(uint32_t));
+            size_t at = 0;
+            for (int k = 0; k < NLIST; k++) {
+                const size_t m = L.idx[k].size();
+                if (m && !check(hipMemcpy(idx + at, L.idx[k].data(), m * sizeof(uint32_t),
+                                          hipMemcpyHostToDevice), "hipMemcpy"))
+                    throw std::runtime_error("upload failed");
+                at += m;
+                std::vector<uint32_t>().swap(L.idx[k]);  // the device copy is the one used
+            }
+            P->v.idx = idx;
         }
         P->v.out32 = P->dalloc<float4>(sizeof(float4) * (size_t)n);
         if (p->fp64)

@@ -25,11 +25,11 @@ import numpy as np
 from . import batch
 
 DEFAULT_SPACE = {
-    "merge_groups": [1, 2],
+    "merge_groups": [1, 3],
     "group_max": [48, 64],
     "leaf_max": [8, 16],
 }
-PERSISTENT = [0, -1, 4, 8, 32]  # 0: one WG per group; -1: 16 per CU; k>0: k x 256 WGs
+PERSISTENT = [0, -1, 32]  # 0: one WG per group; -1: 16 per CU; k>0: k x 256 WGs
 TUNED = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tuned.json")
 
 

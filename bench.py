@@ -46,6 +46,16 @@ def make_workload(args):
     return ics.lattice_grf(ng, seed=args.seed)
 
 
+def workload_name(args, n, variant=None, dec=None):
+    name = "%s N=%d order %d theta %.2f Plummer eps %g G 1" % (args.workload, n, args.order,
+                                                              args.theta, args.eps)
+    if variant is not None:  # key of a measured-traffic record: the launch too
+        name += " | %s groups %d/%d/%d persistent %d" % (
+            variant, dec["merge_groups"], dec["group_max"], dec["leaf_max"],
+            dec["persistent_blocks"])
+    return name
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,13 +68,16 @@ def main():
     ap.add_argument("--workload", default="clustered", choices=["clustered", "plummer", "lattice"])
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--variant", default="rel32c")
-    ap.add_argument("--group-mode", type=int, default=2)
+    ap.add_argument("--group-mode", type=int, default=1)
     ap.add_argument("--persistent", type=int, default=0)
     ap.add_argument("--sample", type=int, default=8192)
     ap.add_argument("--no-plain64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tuned", action="store_true", help="ignore tuned.json")
     ap.add_argument("--quick", action="store_true", help="N = 2^21 for a fast check")
+    ap.add_argument("--copy-check-mib", type=int, default=0,
+                    help="first run a device-to-device copy of this many MiB three times "
+                         "(the known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path "
@@ -77,6 +90,16 @@ def main():
     assert torch.cuda.is_available(), "no GPU"
     torch.cuda.init()
     from pgrav import batch
+
+    if args.copy_check_mib > 0:
+        n4 = args.copy_check_mib * (1 << 20) // 4
+        src = torch.ones(n4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy check: 3 x %d MiB device-to-device copies" % args.copy_check_mib)
+        del src, dst
 
     t0 = time.perf_counter()
     pos, mass = make_workload(args)
@@ -146,6 +169,18 @@ def main():
         log("plain FP64 one-launch-per-group %.1f ms -> speedup %.1fx" % (plain_ms, plain_ms / kern_ms))
     baselines["speedup_vs_batched_fp64"] = f64_ms / kern_ms
 
+    # HBM bytes per launch of the batched kernel, measured with rocprofv3
+    # FETCH_SIZE / WRITE_SIZE for this exact workload and scaled by the same
+    # counters on a copy of known size (profiles/traffic.json records how)
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            rec = json.load(f).get(workload_name(args, pos.shape[0], var, dec))
+        if rec:
+            traffic = rec["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        traffic = None
+
     cpu = None
     if not args.no_cpu:
         # the reference's own SSE4/AVX CPU kernels (oracle/_ref) on a bounded
@@ -172,13 +207,12 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "%s N=%d order %d theta %.2f Plummer eps %g G 1" %
-                   (args.workload, pos.shape[0], args.order, args.theta, args.eps),
+        "config": {"workload": workload_name(args, pos.shape[0]),
                    "variant": var, "ngroups": st["ngroups"], "pairs": pairs,
                    "mean_group": st["mean_group"], "decomposition": dec,
                    "host_build_s": st["t_tree"] + st["t_lists"]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel_ms": kern_ms, "algorithmic_bytes": nbytes,
                      "hbm_GBs_algorithmic": nbytes / (kern_ms * 1e-3) / 1e9},
         "pct_fp32_peak": 100.0 * achieved / PEAK_FP32_TFLOPS,
