@@ -107,6 +107,36 @@ int64_t pgrav_host_list(void *h, int64_t g, int kind, uint32_t *out, int64_t cap
     return b - a;
 }
 
+int pgrav_host_groups(void *h, int64_t *first, int *count, double *center4) {
+    HostPlan *H = (HostPlan *)h;
+    if (!H) {
+        fail("null plan");
+        return -1;
+    }
+    const Lists &L = H->lists;
+    if (first)
+        memcpy(first, L.gfirst.data(), sizeof(int64_t) * L.ngroups);
+    if (count)
+        memcpy(count, L.gcount.data(), sizeof(int) * L.ngroups);
+    if (center4)
+        memcpy(center4, L.gcenter.data(), sizeof(double) * 4 * L.ngroups);
+    return 0;
+}
+
+int64_t pgrav_host_csr(void *h, int kind, int64_t *off, uint32_t *idx) {
+    HostPlan *H = (HostPlan *)h;
+    if (!H || kind < 0 || kind >= NLIST) {
+        fail("bad list kind");
+        return -1;
+    }
+    const Lists &L = H->lists;
+    if (off)
+        memcpy(off, L.off[kind].data(), sizeof(int64_t) * (L.ngroups + 1));
+    if (idx)
+        memcpy(idx, L.idx[kind].data(), sizeof(uint32_t) * L.idx[kind].size());
+    return (int64_t)L.idx[kind].size();
+}
+
 int pgrav_host_cell(void *h, int64_t node, double *cofm3, double *bmax, double *raw35,
                     int64_t *first, int *count, int *child, int *nchild) {
     HostPlan *H = (HostPlan *)h;

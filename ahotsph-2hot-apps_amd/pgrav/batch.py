@@ -63,6 +63,8 @@ register({
     "pgrav_host_stats": (_i, [_vp, _S]),
     "pgrav_host_group": (_i, [_vp, _i64, _vp, _vp, _vp]),
     "pgrav_host_list": (_i64, [_vp, _i64, _i, _vp, _i64]),
+    "pgrav_host_groups": (_i, [_vp, _vp, _vp, _vp]),
+    "pgrav_host_csr": (_i64, [_vp, _i, _vp, _vp]),
     "pgrav_host_cell": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pgrav_host_record": (_i, [_vp, _i64, _vp]),
     "pgrav_host_records": (_i, [_vp, _vp, _i64, _vp]),
@@ -263,6 +265,28 @@ class HostPlan:
         lib().pgrav_host_record(self.h, node, rec.ctypes.data)
         check()
         return rec
+
+    def groups(self):
+        """(first, count, centre4) of every group."""
+        G = self.stats()["ngroups"]
+        first = np.empty(G, np.int64)
+        cnt = np.empty(G, np.int32)
+        cen = np.empty((G, 4), np.float64)
+        lib().pgrav_host_groups(self.h, first.ctypes.data, cnt.ctypes.data, cen.ctypes.data)
+        check()
+        return first, cnt, cen
+
+    def csr(self, kind):
+        """One list kind for all groups: (off[G+1], idx)."""
+        k = KINDS.index(kind) if isinstance(kind, str) else kind
+        G = self.stats()["ngroups"]
+        off = np.empty(G + 1, np.int64)
+        n = lib().pgrav_host_csr(self.h, k, off.ctypes.data, None)
+        check()
+        idx = np.empty(max(n, 0), np.uint32)
+        lib().pgrav_host_csr(self.h, k, None, idx.ctypes.data)
+        check()
+        return off, idx
 
     def records(self, nodes):
         nodes = np.ascontiguousarray(nodes, np.uint32)

@@ -187,7 +187,7 @@ def main():
         # sample of the same workload's groups; the checker side, not the product
         try:
             from oracle.cpu_baseline import run_cpu_baseline
-            cpu = run_cpu_baseline(pos, mass, args, acc_gpu=acc)
+            cpu = run_cpu_baseline(pos, mass, args, acc_gpu=acc, dec=dec)
             log("cpu baseline: %.3e interactions/s on %d threads (%s)" %
                 (cpu["value"], cpu["cores"], cpu["cpu_model"]))
         except Exception as e:  # reported, never silently replaced

@@ -200,6 +200,11 @@ void pgrav_host_destroy(void *h);
 int pgrav_host_stats(void *h, pgrav_stats *st);
 int pgrav_host_group(void *h, int64_t g, int64_t *first, int *count, double *center4);
 int64_t pgrav_host_list(void *h, int64_t g, int kind, uint32_t *out, int64_t cap);
+/* All groups at once: first body, count and FP64 centre + radius (arrays of ngroups). */
+int pgrav_host_groups(void *h, int64_t *first, int *count, double *center4);
+/* One list kind for all groups as CSR: off[ngroups+1], idx[off[ngroups]]; returns
+ * the entry count (either pointer may be NULL). */
+int64_t pgrav_host_csr(void *h, int kind, int64_t *off, uint32_t *idx);
 int pgrav_host_cell(void *h, int64_t node, double *cofm3, double *bmax, double *raw35,
                     int64_t *first, int *count, int *child, int *nchild);
 /* 29-field record of a node (nln/cofm.c:398-562 normalisation, R = bmax). */

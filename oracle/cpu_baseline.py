@@ -15,7 +15,7 @@ import time
 
 import numpy as np
 
-from . import HERE, REF_DIR, lib, ref_available, to_soa
+from . import REF_DIR, lib, ref_available
 
 
 def _cpu_model():
@@ -28,89 +28,95 @@ def _cpu_model():
     return platform.processor() or "unknown"
 
 
-def _prepare(H, groups, spos, smass, nsse):
-    sink_off, cell_off, body_off = [0], [0], [0]
-    sinks, cells, bodies = [], [], []
-    inter = 0
-    for g in groups:
-        first, cnt, _ = H.group(int(g))
-        sinks.append(spos[first:first + cnt].astype(np.float32))
-        nodes = H.list(int(g), "hexa")
-        recs = H.records(nodes)
-        body = H.list(int(g), "body")
-        b4 = np.concatenate([smass[body, None], spos[body]], axis=1) if body.size else \
-            np.zeros((0, 4))
-        cells.append(to_soa(recs.astype(np.float32), nsse) if len(recs) else np.zeros(0, np.float32))
-        bodies.append(to_soa(b4.astype(np.float32), nsse) if body.size else np.zeros(0, np.float32))
-        sink_off.append(sink_off[-1] + cnt)
-        cell_off.append(cell_off[-1] + (len(recs) + nsse - 1) // nsse)
-        body_off.append(body_off[-1] + (body.size + nsse - 1) // nsse)
-        inter += cnt * (len(recs) + body.size)
-    cat = lambda xs: np.ascontiguousarray(np.concatenate(xs) if xs else np.zeros(0), np.float32)
-    return (np.array(sink_off, np.int64), cat([s.reshape(-1) for s in sinks]),
-            np.array(cell_off, np.int64), cat(cells), np.array(body_off, np.int64), cat(bodies),
-            inter)
-
-
-def _run(arch, prep, ngroups, eps2, nthreads, reps):
+def _run_lists(arch, D, groups, eps2, nthreads):
     L = lib()
-    fn = L.orc_cpu_ref_run
+    fn = L.orc_cpu_ref_run_lists
     fn.restype = ctypes.c_double
-    fn.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
-                   ctypes.c_int] + [ctypes.c_void_p] * 6 + [ctypes.c_float, ctypes.c_int,
-                                                            ctypes.c_int, ctypes.c_void_p]
-    sink_off, sinks, cell_off, cells, body_off, bodies, inter = prep
+    fn.argtypes = ([ctypes.c_char_p] * 3 + [ctypes.c_int] * 2 + [ctypes.c_void_p] * 10 +
+                   [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
     nsse = 4 if arch == "sse4" else 8
-    out = np.zeros((sink_off[-1], 4), np.float32)
+    out = np.zeros((D["sinks"].shape[0], 4), np.float32)
+    ks = ctypes.c_double(0)
+    groups = np.ascontiguousarray(groups, np.int32)
     path = os.path.join(REF_DIR, "libref_%s.so" % arch).encode()
-    t = fn(path, ("do_gravph_" + arch).encode(), ("do_gravp_" + arch).encode(), nsse, 29, ngroups,
-           sink_off.ctypes.data, sinks.ctypes.data, cell_off.ctypes.data, cells.ctypes.data,
-           body_off.ctypes.data, bodies.ctypes.data, eps2, nthreads, reps, out.ctypes.data)
-    if t < 0:
+    wall = fn(path, ("do_gravph_" + arch).encode(), ("do_gravp_" + arch).encode(), nsse, 29,
+              D["rec"].ctypes.data, D["sinks"].ctypes.data, D["bodies"].ctypes.data,
+              D["first"].ctypes.data, D["count"].ctypes.data, D["hoff"].ctypes.data,
+              D["hidx"].ctypes.data, D["boff"].ctypes.data, D["bidx"].ctypes.data,
+              groups.ctypes.data, groups.size, eps2, nthreads, out.ctypes.data, ctypes.byref(ks))
+    if wall < 0:
         raise RuntimeError("reference CPU kernels could not be loaded from %s" % path)
-    return t, out
+    return wall, ks.value, out
 
 
-def run_cpu_baseline(pos, mass, args, acc_gpu=None, target_s=15.0, nthreads=None):
-    """Returns the bench.py cpu_baseline object (interactions/s on the host cores)."""
+def _interactions(D, groups):
+    c = D["count"][groups].astype(np.int64)
+    nh = D["hoff"][groups + 1] - D["hoff"][groups]
+    nb = D["boff"][groups + 1] - D["boff"][groups]
+    return int((c * (nh + nb)).sum())
+
+
+def run_cpu_baseline(pos, mass, args, acc_gpu=None, target_s=15.0, nthreads=None, dec=None):
+    """Returns the bench.py cpu_baseline object (interactions/s on the host cores).
+
+    The sample is every group when the two timed passes fit in about
+    `target_s` seconds of wall time, else a seeded random subset sized to fit.
+    The rate is interactions / (kernel-call seconds summed over threads /
+    threads): the gathers into SoA blocks are not counted against the CPU."""
     if not ref_available():
         raise RuntimeError("oracle/_ref is not built (make -C oracle ref)")
     from pgrav import batch
     nthreads = nthreads or min(16, os.cpu_count() or 1)
     t0 = time.perf_counter()
+    dec = dec or {"merge_groups": args.group_mode, "group_max": 64, "leaf_max": 16}
     H = batch.HostPlan(pos, mass, theta=args.theta, order=args.order, eps=args.eps,
-                       soft="plummer", merge_groups=args.group_mode)
+                       soft="plummer", merge_groups=dec["merge_groups"],
+                       group_max=dec["group_max"], leaf_max=dec["leaf_max"])
+    st = H.stats()
     perm, spos = H.sorted()
     smass = np.asarray(mass, np.float64)[perm]
-    ng = H.stats()["ngroups"]
+    first, count, _ = H.groups()
+    hoff, hidx = H.csr("hexa")
+    boff, bidx = H.csr("body")
+    if H.csr("quad")[1].size or H.csr("mono")[1].size:
+        raise RuntimeError("the CPU baseline covers order-4 cells and bodies only")
+    D = {"rec": np.ascontiguousarray(H.records(np.arange(st["nnodes"], dtype=np.uint32)),
+                                     np.float32),
+         "sinks": np.ascontiguousarray(spos, np.float32),
+         "bodies": np.ascontiguousarray(np.concatenate([smass[:, None], spos], axis=1), np.float32),
+         "first": first, "count": count, "hoff": hoff, "hidx": hidx, "boff": boff, "bidx": bidx}
+    del H
+    ng = first.size
     rng = np.random.default_rng(777)
     eps2 = float(args.eps) ** 2
     result = {"unit": "interactions/s", "cores": nthreads, "kind": "reference",
               "cpu_model": _cpu_model()}
     # calibrate on a small sample, then size the timed sample to ~target_s seconds
-    k = min(ng, 16 * nthreads)
-    groups = rng.choice(ng, k, replace=False)
-    prep = _prepare(H, groups, spos, smass, 4)
-    t, _ = _run("sse4", prep, k, eps2, nthreads, 1)
-    rate = prep[-1] / t
-    k2 = int(min(ng, max(k, k * target_s / 2 / max(t, 1e-3))))
-    groups = rng.choice(ng, k2, replace=False)
+    cal = rng.choice(ng, min(ng, 64 * nthreads), replace=False).astype(np.int32)
+    wall, _, _ = _run_lists("sse4", D, cal, eps2, nthreads)
+    per_int = wall / max(1, _interactions(D, cal))
+    total = _interactions(D, np.arange(ng))
+    # SSE4 then AVX (about half the time): ~1.5 SSE4 passes in the budget
+    frac = min(1.0, target_s / 1.5 / max(per_int * total, 1e-9))
+    groups = (np.arange(ng) if frac >= 1.0 else
+              np.sort(rng.choice(ng, max(len(cal), int(frac * ng)), replace=False))).astype(np.int32)
+    inter = _interactions(D, groups)
     for arch in ("sse4", "avx8"):
-        nsse = 4 if arch == "sse4" else 8
-        prep = _prepare(H, groups, spos, smass, nsse)
-        t, out = _run(arch, prep, k2, eps2, nthreads, 1)
-        result[arch] = prep[-1] / t
-        result[arch + "_seconds"] = t
-        if acc_gpu is not None and arch == "sse4":
-            # the same bodies' results from the GPU (caller order -> sorted order)
-            sidx = np.concatenate([np.arange(*(lambda f, c: (f, f + c))(*H.group(int(g))[:2]))
-                                   for g in groups])
+        wall, ks, out = _run_lists(arch, D, groups, eps2, nthreads)
+        result[arch] = inter / (ks / nthreads)
+        result[arch + "_wall_seconds"] = wall
+        result[arch + "_kernel_seconds_per_thread"] = ks / nthreads
+        if acc_gpu is not None and arch == "avx8":
+            sidx = np.concatenate([np.arange(first[g], first[g] + count[g]) for g in groups[:4096]])
             ref = np.asarray(acc_gpu)[perm[sidx]]
-            d = np.linalg.norm(out[:, :3] - ref[:, :3], axis=1) / np.linalg.norm(ref[:, :3], axis=1)
+            d = (np.linalg.norm(out[sidx, :3] - ref[:, :3], axis=1) /
+                 np.linalg.norm(ref[:, :3], axis=1))
             result["median_rel_diff_vs_gpu"] = float(np.median(d))
     result["value"] = max(result["sse4"], result["avx8"])
-    result["sample"] = ("%d of %d sink groups (%d sinks, %.3e interactions) of the benchmark "
-                        "workload, do_gravph+do_gravp per sink, %d threads"
-                        % (k2, ng, prep[0][-1], prep[-1], nthreads))
+    result["sample"] = ("%s %d of %d sink groups (%d sinks, %.3e interactions) of the benchmark "
+                        "workload: reference do_gravph+do_gravp (SSE4 and AVX builds) per sink "
+                        "over the same lists, %d threads, kernel calls timed"
+                        % ("all" if groups.size == ng else "random", groups.size, ng,
+                           int(count[groups].sum()), inter, nthreads))
     result["prep_seconds"] = time.perf_counter() - t0
     return result

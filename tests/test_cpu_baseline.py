@@ -1,0 +1,46 @@
+"""The CPU baseline harness (oracle/cpu_baseline.*) runs the reference's own
+SSE4/AVX kernels over the plan's lists; its per-sink results must match the
+FP64 list oracle on the same groups, or the rate it reports is not a rate of
+the same work."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import cpu_baseline
+from pgrav import batch, ics
+from listoracle import group_reference
+
+pytestmark = pytest.mark.skipif(not oracle.ref_available(),
+                                reason="oracle/_ref not built (needs the reference sources)")
+
+
+@pytest.mark.parametrize("arch", ["sse4", "avx8"])
+def test_harness_results_match_list_oracle(arch):
+    pos, mass = ics.clustered(6000, seed=5)
+    H = batch.HostPlan(pos, mass, theta=0.6, order=4, eps=1e-3, soft="plummer", merge_groups=1)
+    st = H.stats()
+    perm, spos = H.sorted()
+    smass = mass[perm]
+    first, count, _ = H.groups()
+    hoff, hidx = H.csr("hexa")
+    boff, bidx = H.csr("body")
+    assert hoff[-1] == st["entries_hexa"] and boff[-1] == st["entries_body"]
+    D = {"rec": np.ascontiguousarray(H.records(np.arange(st["nnodes"], dtype=np.uint32)), np.float32),
+         "sinks": np.ascontiguousarray(spos, np.float32),
+         "bodies": np.ascontiguousarray(np.concatenate([smass[:, None], spos], 1), np.float32),
+         "first": first, "count": count, "hoff": hoff, "hidx": hidx, "boff": boff, "bidx": bidx}
+    groups = np.arange(0, first.size, 7, dtype=np.int32)
+    wall, ks, out = cpu_baseline._run_lists(arch, D, groups, 1e-6, 4)
+    assert wall > 0 and ks > 0
+    for g in groups[:12]:
+        f, ref = group_reference(H, int(g), spos, smass, soft="plummer", eps=1e-3)
+        got = out[f:f + count[g]]
+        rel = np.linalg.norm(got[:, :3] - ref[:, :3], axis=1) / np.linalg.norm(ref[:, :3], axis=1)
+        # FP32 absolute coordinates in the reference kernels: ~1e-5 relative
+        assert np.median(rel) < 1e-4, (g, np.median(rel))
+        assert np.allclose(got[:, 3], ref[:, 3], rtol=1e-4)
+    # groups not in the sample are left untouched
+    mask = np.ones(out.shape[0], bool)
+    for g in groups:
+        mask[first[g]:first[g] + count[g]] = False
+    assert not out[mask].any()
