@@ -2,6 +2,9 @@
 
     python ahotsph-2hot-apps_amd/build.py            # build in-tree
     python ahotsph-2hot-apps_amd/build.py --asm      # also dump gfx950 assembly
+    python ahotsph-2hot-apps_amd/build.py --define NAME[=VALUE] --out PATH
+                                       # a variant library elsewhere (A/B timing;
+                                       # load it with PGRAV_LIBRARY=PATH)
 
 HIP sources are compiled by hipcc for gfx950 only; host-only C++ (tree, lists,
 generators) by g++.  Everything links into one shared library next to this file
@@ -45,40 +48,52 @@ def _run(cmd):
     return r.returncode, " ".join(cmd), out
 
 
-def build(verbose=False, asm=False, jobs=None):
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose=False, asm=False, jobs=None, defines=(), out=None):
+    out = os.path.abspath(out) if out else LIB
+    bdir = BUILD if out == LIB else os.path.join(BUILD, "variant_" + os.path.basename(out))
+    os.makedirs(bdir, exist_ok=True)
+    dflags = ["-D" + d for d in defines]
     hip, cpp = _sources()
     cmds, objs = [], []
     for src in hip:
-        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+        obj = os.path.join(bdir, os.path.basename(src) + ".o")
         objs.append(obj)
-        cmds.append([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
+        cmds.append([HIPCC] + HIP_FLAGS + dflags + ["-c", src, "-o", obj])
         if asm:
-            cmds.append([HIPCC] + HIP_FLAGS + ["--cuda-device-only", "-S", src, "-o",
-                                               os.path.join(BUILD, os.path.basename(src) + ".s")])
+            cmds.append([HIPCC] + HIP_FLAGS + dflags +
+                        ["--cuda-device-only", "-S", src, "-o",
+                         os.path.join(bdir, os.path.basename(src) + ".s")])
     for src in cpp:
-        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+        obj = os.path.join(bdir, os.path.basename(src) + ".o")
         objs.append(obj)
         cmds.append(["g++"] + CXX_FLAGS + ["-c", src, "-o", obj])
     jobs = jobs or min(8, os.cpu_count() or 1)
     with cf.ThreadPoolExecutor(jobs) as ex:
-        for rc, cmd, out in ex.map(_run, cmds):
+        for rc, cmd, log in ex.map(_run, cmds):
             if verbose or rc:
                 print(cmd)
-                if out:
-                    print(out)
+                if log:
+                    print(log)
             if rc:
                 raise RuntimeError("compile failed: " + cmd)
-    tmp = LIB + ".tmp"
-    rc, cmd, out = _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    tmp = out + ".tmp"
+    rc, cmd, log = _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
                         + ["-pthread"])
     if rc:
         print(cmd)
-        print(out)
+        print(log)
         raise RuntimeError("link failed")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv, asm="--asm" in sys.argv))
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-v", action="store_true")
+    ap.add_argument("--asm", action="store_true")
+    ap.add_argument("--define", action="append", default=[])
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    print(build(verbose=a.v, asm=a.asm, defines=a.define, out=a.out))

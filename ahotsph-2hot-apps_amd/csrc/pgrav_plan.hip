@@ -90,7 +90,7 @@ static int auto_blocks() {
         if (hipGetDeviceProperties(&pr, dev) == hipSuccess)
             cus = pr.multiProcessorCount;
     }
-    return cus * 16;  // 16 single-wave workgroups per CU: 4 per SIMD
+    return cus * 32;  // 32 single-wave workgroups per CU: 8 per SIMD, a full CU
 }
 
 static hipStream_t stream_of(Plan *P, void *stream) {

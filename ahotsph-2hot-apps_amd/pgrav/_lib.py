@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG, "libpgrav.so")
+# PGRAV_LIBRARY selects another build of the same ABI (e.g. a variant built
+# with build.py --out); the default is the in-tree library
+LIB_PATH = os.environ.get("PGRAV_LIBRARY") or os.path.join(PKG, "libpgrav.so")
 
 
 class PgravError(RuntimeError):
@@ -62,7 +64,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise PgravError("libpgrav.so is not built: run "
+            raise PgravError(LIB_PATH + " is not built: run "
                              "python ahotsph-2hot-apps_amd/build.py (no CPU fallback exists)")
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():

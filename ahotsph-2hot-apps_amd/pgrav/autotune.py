@@ -29,7 +29,11 @@ DEFAULT_SPACE = {
     "group_max": [48, 64],
     "leaf_max": [8, 16],
 }
-PERSISTENT = [0, -1, 32]  # 0: one WG per group; -1: 16 per CU; k>0: k x 256 WGs
+# launch shapes: 0 = one workgroup per group; -1 = persistent grid of 32
+# single-wave workgroups per CU; k > 0 = persistent grid of k per CU.  The
+# table and tuned.json store the resulting workgroup count (what
+# pgrav_plan_set_launch and Plan(persistent_blocks=...) take).
+PERSISTENT = [0, -1, 16]
 TUNED = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tuned.json")
 
 
@@ -52,9 +56,9 @@ def tune(pos, mass, base, space=None, reps=3, variant="rel32c", log=print):
         st = plan.stats()
         plan.evaluate(variant)  # warm-up
         for pb in PERSISTENT:
-            # a persistent grid of k > 0 means k x 256 single-wave workgroups
-            ms = _time_with_blocks(plan, variant, reps, pb * 256 if pb > 0 else pb)
-            row = dict(kw, persistent_blocks=pb, ms=ms, pairs=st["pairs_total"],
+            blocks = pb * _num_cus() if pb > 0 else pb
+            ms = _time_with_blocks(plan, variant, reps, blocks)
+            row = dict(kw, persistent_blocks=blocks, ms=ms, pairs=st["pairs_total"],
                        ngroups=st["ngroups"], mean_group=st["mean_group"], build_s=build)
             table.append(row)
             log("tune %s -> %.3f ms (groups %d, mean %.1f, pairs %.3e)" %
@@ -64,6 +68,11 @@ def tune(pos, mass, base, space=None, reps=3, variant="rel32c", log=print):
                 best = row
         plan.close()
     return best, table
+
+
+def _num_cus():
+    import torch
+    return torch.cuda.get_device_properties(0).multi_processor_count
 
 
 def _time_with_blocks(plan, variant, reps, pb):

@@ -76,8 +76,9 @@ def main():
     ap.add_argument("--no-tuned", action="store_true", help="ignore tuned.json")
     ap.add_argument("--quick", action="store_true", help="N = 2^21 for a fast check")
     ap.add_argument("--copy-check-mib", type=int, default=0,
-                    help="first run a device-to-device copy of this many MiB three times "
-                         "(the known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+                    help="first run an elementwise kernel that reads and writes this many "
+                         "MiB, three times (the known-size kernel that scales rocprofv3 "
+                         "FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path "
@@ -96,9 +97,10 @@ def main():
         src = torch.ones(n4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.mul(src, 2.0, out=dst)  # reads and writes copy_check_mib MiB each
         torch.cuda.synchronize()
-        log("copy check: 3 x %d MiB device-to-device copies" % args.copy_check_mib)
+        log("copy check: 3 x (read %d MiB, write %d MiB)" % (args.copy_check_mib,
+                                                           args.copy_check_mib))
         del src, dst
 
     t0 = time.perf_counter()
