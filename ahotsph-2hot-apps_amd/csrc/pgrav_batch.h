@@ -37,6 +37,7 @@ struct BatchView {
     double4 *out64 = nullptr;
     int *counter = nullptr;
     const float4 *ewald32 = nullptr;
+    const double4 *ewald64 = nullptr;  // FP64 table (direct-sum check)
     int ngroups = 0;
     int soft = 0;
     double eps = 0;
@@ -53,8 +54,9 @@ hipError_t launch_adjoint(const BatchView &v, const float4 *wv, float4 *gsink, f
                           float4 *gbody, const double *raw, const float *mat, int64_t nnodes,
                           float *gnode, const int *leaf_of, const int *parent, int64_t n,
                           double4 *out, hipStream_t st);
+// ewald64 non-null: periodic (nearest image + Ewald correction, box edge `box`)
 hipError_t launch_direct64(const double4 *bpos, int64_t n, const int64_t *sample, int ns, int soft,
                            double eps, double G, double4 *part, int nsplit, double4 *out,
-                           hipStream_t st);
+                           const double4 *ewald64, double box, hipStream_t st);
 
 }  // namespace pgrav

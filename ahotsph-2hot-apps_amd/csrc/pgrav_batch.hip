@@ -67,8 +67,8 @@ __device__ __forceinline__ T pbc(T x, T box, T ibox) {
 // FP64 table built by k_ewald_table.  x = sink - source, already the nearest
 // image (|x_i| <= L/2), so the Newtonian term the caller added and this
 // correction refer to the same image.
-template <typename T>
-__device__ __forceinline__ void ewald_corr(const float4 *__restrict__ tab, T m, T x, T y, T z,
+template <typename T, typename TV>
+__device__ __forceinline__ void ewald_corr(const TV *__restrict__ tab, T m, T x, T y, T z,
                                            T box, Acc4T<T> &a) {
     const T half = T(0.5) * box;
     const T inv_h = T(EWALD_N - 1) / half;
@@ -79,10 +79,10 @@ __device__ __forceinline__ void ewald_corr(const float4 *__restrict__ tab, T m, 
     int i = max(0, min((int)u, EWALD_N - 2)), j = max(0, min((int)v, EWALD_N - 2)),
         k = max(0, min((int)w, EWALD_N - 2));
     T fu = u - T(i), fv = v - T(j), fw = w - T(k);
-    const float4 *p = tab + ((size_t)i * EWALD_N + j) * EWALD_N + k;
-    const float4 c000 = p[0], c001 = p[1], c010 = p[EWALD_N], c011 = p[EWALD_N + 1];
-    const float4 c100 = p[EWALD_N * EWALD_N], c101 = p[EWALD_N * EWALD_N + 1];
-    const float4 c110 = p[EWALD_N * EWALD_N + EWALD_N], c111 = p[EWALD_N * EWALD_N + EWALD_N + 1];
+    const TV *p = tab + ((size_t)i * EWALD_N + j) * EWALD_N + k;
+    const TV c000 = p[0], c001 = p[1], c010 = p[EWALD_N], c011 = p[EWALD_N + 1];
+    const TV c100 = p[EWALD_N * EWALD_N], c101 = p[EWALD_N * EWALD_N + 1];
+    const TV c110 = p[EWALD_N * EWALD_N + EWALD_N], c111 = p[EWALD_N * EWALD_N + EWALD_N + 1];
 #define LERP3(f)                                                                              \
     ((T(1) - fu) * ((T(1) - fv) * ((T(1) - fw) * T(c000.f) + fw * T(c001.f)) +               \
                     fv * ((T(1) - fw) * T(c010.f) + fw * T(c011.f))) +                         \
@@ -357,11 +357,15 @@ __global__ void k_reset_counter(int *c) {
 // ---------------------------------------------------------------- FP64 direct sum
 // Sampled FP64 direct sum (pattern of nln/grav_n2.c:162-173): grid.y splits the
 // sources; partial sums go to part[split][sink] and are reduced in fixed order.
-template <int SOFT>
+// Periodic: nearest image of every source plus the Ewald correction from the
+// FP64 table (the tree kernels read its FP32 copy), so the difference to the
+// tree result is the tree's own approximation error.
+template <int SOFT, bool PERIODIC>
 __global__ __launch_bounds__(64) void k_direct64(const double4 *__restrict__ bpos, int64_t n,
                                                  const int64_t *__restrict__ sample, int ns,
                                                  double eps, double4 *__restrict__ part,
-                                                 int64_t chunk) {
+                                                 int64_t chunk, const double4 *__restrict__ tab,
+                                                 double box) {
     __shared__ double4 lds[64];
     const int s = blockIdx.x * 64 + threadIdx.x;
     const bool live = s < ns;
@@ -376,15 +380,19 @@ __global__ __launch_bounds__(64) void k_direct64(const double4 *__restrict__ bpo
         __syncthreads();
         for (int k = 0; k < nt; k++) {
             const double4 q = lds[k];
-            const double x = p.x - q.x, y = p.y - q.y, z = p.z - q.z;
+            const double ib = PERIODIC ? 1.0 / box : 0.0;
+            const double x = pbc<PERIODIC>(p.x - q.x, box, ib), y = pbc<PERIODIC>(p.y - q.y, box, ib),
+                         z = pbc<PERIODIC>(p.z - q.z, box, ib);
+            const bool self = x == 0.0 && y == 0.0 && z == 0.0;
             if (SOFT == SOFT_K1) {
                 k1_interact<double>(q.w, x, y, z, eps_inv, eps2, a);
             } else if (SOFT == SOFT_PLUMMER) {
                 mono_interact<double>(q.w, x, y, z, eps2, a);
             } else {
-                const bool self = x == 0.0 && y == 0.0 && z == 0.0;
                 mono_interact<double>(self ? 0.0 : q.w, x, y, z, self ? 1.0 : 0.0, a);
             }
+            if (PERIODIC && !self)
+                ewald_corr<double>(tab, q.w, x, y, z, box, a);
         }
         __syncthreads();
     }
@@ -580,19 +588,32 @@ hipError_t launch_group64(const BatchView &v, int g, hipStream_t st) {
     return hipGetLastError();
 }
 
+template <bool PER>
+static void launch_direct_soft(dim3 grid, const double4 *bpos, int64_t n, const int64_t *sample,
+                               int ns, int soft, double eps, double4 *part, int64_t chunk,
+                               const double4 *tab, double box, hipStream_t st) {
+    if (soft == SOFT_K1)
+        hipLaunchKernelGGL((k_direct64<SOFT_K1, PER>), grid, dim3(64), 0, st, bpos, n, sample, ns,
+                           eps, part, chunk, tab, box);
+    else if (soft == SOFT_PLUMMER)
+        hipLaunchKernelGGL((k_direct64<SOFT_PLUMMER, PER>), grid, dim3(64), 0, st, bpos, n, sample,
+                           ns, eps, part, chunk, tab, box);
+    else
+        hipLaunchKernelGGL((k_direct64<SOFT_NONE, PER>), grid, dim3(64), 0, st, bpos, n, sample,
+                           ns, eps, part, chunk, tab, box);
+}
+
 hipError_t launch_direct64(const double4 *bpos, int64_t n, const int64_t *sample, int ns, int soft,
                            double eps, double G, double4 *part, int nsplit, double4 *out,
-                           hipStream_t st) {
+                           const double4 *ewald64, double box, hipStream_t st) {
     if (ns <= 0)
         return hipSuccess;
     const int64_t chunk = (n + nsplit - 1) / nsplit;
     dim3 grid((ns + 63) / 64, nsplit);
-    if (soft == SOFT_K1)
-        hipLaunchKernelGGL(k_direct64<SOFT_K1>, grid, dim3(64), 0, st, bpos, n, sample, ns, eps, part, chunk);
-    else if (soft == SOFT_PLUMMER)
-        hipLaunchKernelGGL(k_direct64<SOFT_PLUMMER>, grid, dim3(64), 0, st, bpos, n, sample, ns, eps, part, chunk);
+    if (ewald64)
+        launch_direct_soft<true>(grid, bpos, n, sample, ns, soft, eps, part, chunk, ewald64, box, st);
     else
-        hipLaunchKernelGGL(k_direct64<SOFT_NONE>, grid, dim3(64), 0, st, bpos, n, sample, ns, eps, part, chunk);
+        launch_direct_soft<false>(grid, bpos, n, sample, ns, soft, eps, part, chunk, nullptr, box, st);
     hipLaunchKernelGGL(k_direct_reduce, dim3((ns + 255) / 256), dim3(256), 0, st, part, ns, nsplit, G, out);
     return hipGetLastError();
 }

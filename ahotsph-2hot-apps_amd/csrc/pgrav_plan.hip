@@ -288,6 +288,7 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
             if (!check(launch_ewald_table(p->box, t64, t32, P->stream), "Ewald table kernel"))
                 throw std::runtime_error("ewald");
             P->v.ewald32 = t32;
+            P->v.ewald64 = t64;
         }
         if (!check(hipStreamSynchronize(P->stream), "hipStreamSynchronize"))
             throw std::runtime_error("sync");
@@ -581,11 +582,6 @@ int pgrav_plan_direct(void *plan, const int64_t *sample, int ns, double *out4) {
         fail("pgrav_plan_direct: bad arguments");
         return -1;
     }
-    if (P->prm.periodic) {
-        fail("pgrav_plan_direct is the isolated direct sum; periodic runs compare with "
-             "the Ewald direct sum of pgrav_ewald_direct");
-        return -1;
-    }
     std::vector<int64_t> s(ns);
     for (int i = 0; i < ns; i++) {
         if (sample[i] < 0 || sample[i] >= P->tree.n) {
@@ -603,7 +599,9 @@ int pgrav_plan_direct(void *plan, const int64_t *sample, int ns, double *out4) {
               check(hipMalloc(&out, sizeof(double4) * (size_t)ns), "hipMalloc") &&
               check(hipMemcpy(ds, s.data(), 8 * (size_t)ns, hipMemcpyHostToDevice), "hipMemcpy") &&
               check(launch_direct64(P->v.bpos, P->tree.n, ds, ns, P->prm.soft_type, P->prm.eps,
-                                    P->prm.G, part, nsplit, out, P->stream), "direct-sum kernel") &&
+                                    P->prm.G, part, nsplit, out,
+                                    P->prm.periodic ? P->v.ewald64 : nullptr, P->prm.box,
+                                    P->stream), "direct-sum kernel") &&
               check(hipMemcpyAsync(out4, out, sizeof(double4) * (size_t)ns, hipMemcpyDeviceToHost,
                                    P->stream), "hipMemcpyAsync") &&
               check(hipStreamSynchronize(P->stream), "hipStreamSynchronize");

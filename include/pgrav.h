@@ -184,7 +184,8 @@ int pgrav_plan_synchronize(void *plan);
 /* (ax, ay, az, phi) x n in the caller's body order, after G scaling. */
 int pgrav_plan_result(void *plan, int variant, double *acc4);
 /* FP64 direct sum (nln/grav_n2.c pattern, same softening) for `ns` bodies
- * given by their original indices; out4 is ns x 4. */
+ * given by their original indices; out4 is ns x 4.  Periodic plans: nearest
+ * image plus the Ewald correction interpolated from the FP64 table. */
 int pgrav_plan_direct(void *plan, const int64_t *sample, int ns, double *out4);
 /* Device pointers of the FP32 / FP64 results and the sorted FP64 bodies. */
 int pgrav_plan_device_buffers(void *plan, void **out32, void **out64, void **bpos);
