@@ -153,9 +153,10 @@ void pgrav_params_default(pgrav_params *p) {
     p->box = 1.0;
     p->nthreads = 0;
     p->merge_groups = 1;
-    p->persistent_blocks = 0;
+    p->persistent_blocks = -1;  // persistent grid, 32 workgroups per CU
     p->fp64 = 1;
     p->dispatch = 1;
+    p->mixed_order = 0;
 }
 
 void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const pgrav_params *p) {
@@ -202,6 +203,7 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         P->tp.nthreads = p->nthreads;
         P->tp.merge_groups = p->merge_groups;
         P->tp.dispatch = p->dispatch;
+        P->tp.mixed_order = p->mixed_order;
         double t0 = now();
         build_tree(pos3, mass, n, P->tp, P->tree);
         double t1 = now();

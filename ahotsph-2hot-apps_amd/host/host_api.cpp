@@ -49,6 +49,7 @@ void *pgrav_host_plan(const double *pos3, const double *mass, int64_t n, const p
         H->tp.nthreads = p->nthreads;
         H->tp.merge_groups = p->merge_groups;
         H->tp.dispatch = p->dispatch;
+        H->tp.mixed_order = p->mixed_order;
         build_tree(pos3, mass, n, H->tp, H->tree);
         build_groups(H->tree, H->tp, H->lists);
         build_lists(H->tree, H->tp, H->lists);

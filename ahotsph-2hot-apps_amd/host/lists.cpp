@@ -168,6 +168,12 @@ void build_lists(const Tree &t, const TreeParams &p, Lists &l) {
     const int kind = p.order >= 4 ? L_HEXA : (p.order >= 2 ? L_QUAD : L_MONO);
     const double inv_theta = 1.0 / p.theta;
     const double box = p.box;
+    // mixed order: distance (in bmax) beyond which order 2 / order 0 meets the
+    // truncation bound theta^(order+1) that the top order meets at bmax/theta
+    const int top = p.order >= 4 ? 4 : (p.order >= 2 ? 2 : 0);
+    const bool mixed = p.mixed_order && top > 0;
+    const double r_quad = top == 4 ? pow(p.theta, -5.0 / 3.0) : 1e300;
+    const double r_mono = pow(p.theta, -(double)(top + 1));
     struct Buf {
         std::vector<uint32_t> v[NLIST];
         std::vector<int64_t> gid;
@@ -212,7 +218,12 @@ void build_lists(const Tree &t, const TreeParams &p, Lists &l) {
                     if (accept && p.periodic && t.bmax[c] * inv_theta + rg + t.bmax[c] > 0.5 * box)
                         accept = false;
                     if (accept) {
-                        B.v[kind].push_back((uint32_t)c);
+                        int k = kind;
+                        if (mixed) {
+                            const double s = (d - rg) / t.bmax[c];
+                            k = s > r_mono ? L_MONO : (s > r_quad ? L_QUAD : kind);
+                        }
+                        B.v[k].push_back((uint32_t)c);
                         covered += cnt;
                     } else if (t.child[c] < 0 || cnt < pcut) {
                         const int64_t f = t.first[c];

@@ -20,6 +20,8 @@ struct TreeParams {
     int nthreads = 0;      // 0: hardware concurrency
     int merge_groups = 1;  // merge adjacent small sibling groups up to group_max
     int dispatch = 1;      // 0: global longest-first; 1: per-XCD Morton slices, longest-first inside
+    int mixed_order = 0;   // 1: an accepted cell far enough for a lower order to meet the same
+                           // truncation bound is listed at that order (order-2 / order-0 lists)
 };
 
 const int HOST_NXCD = 8;  // MI355X XCDs (one L2 each); workgroup b runs on XCD b % 8

@@ -27,7 +27,7 @@ class Params(ctypes.Structure):
                 ("soft_type", ctypes.c_int), ("eps", ctypes.c_double), ("G", ctypes.c_double),
                 ("periodic", ctypes.c_int), ("box", ctypes.c_double), ("nthreads", ctypes.c_int),
                 ("merge_groups", ctypes.c_int), ("persistent_blocks", ctypes.c_int),
-                ("fp64", ctypes.c_int), ("dispatch", ctypes.c_int)]
+                ("fp64", ctypes.c_int), ("dispatch", ctypes.c_int), ("mixed_order", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -75,7 +75,7 @@ register({
 
 def make_params(theta=0.6, order=4, eps=1e-4, soft="plummer", G=1.0, group_max=64, leaf_max=16,
                 pcut=0, periodic=False, box=1.0, nthreads=0, merge_groups=True,
-                persistent_blocks=0, fp64=True, dispatch=1):
+                persistent_blocks=-1, fp64=True, dispatch=1, mixed_order=False):
     p = Params()
     lib().pgrav_params_default(ctypes.byref(p))
     p.theta, p.order, p.eps, p.G = theta, order, eps, G
@@ -84,6 +84,7 @@ def make_params(theta=0.6, order=4, eps=1e-4, soft="plummer", G=1.0, group_max=6
     p.periodic, p.box, p.nthreads = int(periodic), box, nthreads
     p.merge_groups, p.persistent_blocks, p.fp64 = int(merge_groups), persistent_blocks, int(fp64)
     p.dispatch = dispatch
+    p.mixed_order = int(mixed_order)
     return p
 
 

@@ -1,0 +1,193 @@
+"""Measurements of the path on the configurations it is specified on (one GPU).
+
+    python -m pgrav.study c3 [--n 16777216] [--thetas 0.5 0.6 0.7]
+    python -m pgrav.study c4 [--ng 256]
+    python -m pgrav.study c5 [--n 1048576]
+
+c3  clustered bodies (pgrav.ics.clustered), order 4, Plummer eps 1e-4, for each
+    opening angle: kernel time of the FP32 group-relative compensated (rel32c),
+    plain-sum (rel32) and absolute-coordinate (abs32, the reference's arithmetic)
+    variants and of the batched FP64 variant, each with its force error against
+    the FP64 direct sum on 8192 sampled bodies.
+c4  periodic unit box, ng^3 lattice displaced by a Gaussian random field
+    (pgrav.ics.lattice_grf), 64^3 Ewald table: eager and graph-replayed kernel
+    time, error against the periodic FP64 direct sum (nearest image + Ewald
+    table) on 1024 sampled bodies, net momentum.
+c5  clustered N bodies, tree and lists fixed: forward and adjoint wall time, and
+    the adjoint against central finite differences (h = 1e-5, FP64 forward) of
+    L = sum_i w_i.a_i + v_i phi_i on 64 sampled coordinates.
+
+Each configuration prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import autotune, batch, ics
+
+
+def _log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _errors(acc, ref):
+    da, dp = batch.rel_errors(acc, ref)
+    return {"median": float(np.median(da)), "p90": float(np.percentile(da, 90)),
+            "p99": float(np.percentile(da, 99)), "phi_median": float(np.median(dp))}
+
+
+def _decomposition(n, theta, workload, eps=1e-4):
+    dec = {"merge_groups": 1, "group_max": 64, "leaf_max": 16, "persistent_blocks": -1}
+    t = autotune.load(autotune.workload_key(n, theta, 4, "plummer", eps, workload))
+    if t:
+        dec.update({k: t["best"][k] for k in dec})
+    return dec
+
+
+def c3(args):
+    pos, mass = ics.clustered(args.n, seed=2)
+    rng = np.random.default_rng(12345)
+    sample = rng.choice(pos.shape[0], min(8192, pos.shape[0]), replace=False)
+    ref = None
+    for theta in args.thetas:
+        dec = _decomposition(pos.shape[0], theta, "clustered")
+        t0 = time.perf_counter()
+        plan = batch.Plan(pos, mass, theta=theta, order=4, eps=1e-4, soft="plummer", fp64=True,
+                          **dec)
+        st = plan.stats()
+        _log("c3 theta %.2f: plan %.1fs, %d groups, %.3e pairs" %
+             (theta, time.perf_counter() - t0, st["ngroups"], st["pairs_total"]))
+        if ref is None:  # the direct sum does not depend on theta
+            ref = plan.direct(sample)
+        row = {"config": "c3", "n": int(pos.shape[0]), "theta": theta, "decomposition": dec,
+               "groups": st["ngroups"], "pairs": st["pairs_total"],
+               "pairs_hexa": st["pairs_hexa"], "pairs_body": st["pairs_body"], "variants": {}}
+        for var in ("rel32c", "rel32", "abs32", "f64"):
+            plan.evaluate(var)
+            ms = plan.time(var, reps=3)
+            e = _errors(plan.result(var)[sample], ref)
+            row["variants"][var] = {"kernel_ms": ms, "interactions_per_s": st["pairs_total"] / (ms * 1e-3),
+                                    "errors": e}
+            _log("   %-6s %9.3f ms  median %.2e  p99 %.2e" % (var, ms, e["median"], e["p99"]))
+        plan.close()
+        print(json.dumps(row), flush=True)
+
+
+def c4(args):
+    pos, mass = ics.lattice_grf(args.ng, seed=4)
+    n = pos.shape[0]
+    dec = _decomposition(n, args.theta, "lattice")
+    t0 = time.perf_counter()
+    plan = batch.Plan(pos, mass, periodic=True, box=1.0, theta=args.theta, order=4, eps=args.eps,
+                      soft="plummer", fp64=False, **dec)
+    st = plan.stats()
+    _log("c4 %d^3: plan %.1fs, %d groups, %.3e pairs" % (args.ng, time.perf_counter() - t0,
+                                                         st["ngroups"], st["pairs_total"]))
+    plan.evaluate("rel32c")
+    eager = plan.time("rel32c", reps=3)
+    plan.graph_capture("rel32c")
+    plan.graph_launch()
+    plan.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.graph_launch()
+    plan.synchronize()
+    graph = (time.perf_counter() - t0) / reps * 1e3
+    acc = plan.result("rel32c")
+    rng = np.random.default_rng(7)
+    sample = rng.choice(n, 1024, replace=False)
+    ref = plan.direct(sample)
+    err = np.linalg.norm(acc[sample, :3] - ref[:, :3], axis=1)
+    rms = float(np.sqrt((ref[:, :3] ** 2).sum(1).mean()))
+    net = np.abs((mass[:, None] * acc[:, :3]).sum(0)).max()
+    typical = (mass[:, None] * np.abs(acc[:, :3])).sum()
+    row = {"config": "c4", "n": int(n), "theta": args.theta, "eps": args.eps, "decomposition": dec,
+           "groups": st["ngroups"], "pairs": st["pairs_total"], "kernel_ms_eager": eager,
+           "ms_graph_replay_wall": graph, "interactions_per_s": st["pairs_total"] / (eager * 1e-3),
+           "errors_vs_direct_ewald": {"median_over_rms": float(np.median(err) / rms),
+                                      "p99_over_rms": float(np.percentile(err, 99) / rms),
+                                      "median_rel": _errors(acc[sample], ref)["median"]},
+           "net_force_over_sum_abs": float(net / typical)}
+    _log("c4: eager %.3f ms, graph %.3f ms, median |da|/rms %.2e" %
+         (eager, graph, row["errors_vs_direct_ewald"]["median_over_rms"]))
+    plan.close()
+    print(json.dumps(row), flush=True)
+
+
+def c5(args):
+    pos, mass = ics.clustered(args.n, seed=31)
+    n = pos.shape[0]
+    rng = np.random.default_rng(5)
+    w4 = rng.normal(size=(n, 4))
+    plan = batch.Plan(pos, mass, theta=0.6, order=4, eps=1e-4, soft="plummer", fp64=True)
+    st = plan.stats()
+    plan.evaluate("rel32c")
+    fwd = plan.time("rel32c", reps=3)
+    plan.adjoint(w4)  # warm-up
+    t0 = time.perf_counter()
+    grad = plan.adjoint(w4)
+    adj = (time.perf_counter() - t0) * 1e3
+    h = 1e-5
+    picks = rng.choice(n, 16, replace=False)
+    fd, ad = [], []
+
+    def loss():
+        plan.evaluate("f64")
+        return float((plan.result("f64") * w4).sum())
+
+    for k in picks:
+        for q in range(4):
+            p1, p2, m1, m2 = pos.copy(), pos.copy(), mass.copy(), mass.copy()
+            if q < 3:
+                p1[k, q] += h
+                p2[k, q] -= h
+                step = h
+            else:
+                m1[k] *= 1 + h
+                m2[k] *= 1 - h
+                step = mass[k] * h
+            plan.update(p1, m1)
+            lp = loss()
+            plan.update(p2, m2)
+            lm = loss()
+            fd.append((lp - lm) / (2 * step))
+            ad.append(grad[k, q])
+    fd, ad = np.array(fd).reshape(-1, 4), np.array(ad).reshape(-1, 4)
+    out = {}
+    for q, name in ((slice(0, 3), "position"), (slice(3, 4), "mass")):
+        scale = np.sqrt((fd[:, q] ** 2).mean())
+        out[name] = float(np.abs(ad[:, q] - fd[:, q]).max() / scale)
+    row = {"config": "c5", "n": int(n), "groups": st["ngroups"], "pairs": st["pairs_total"],
+           "forward_kernel_ms": fwd, "adjoint_wall_ms": adj,
+           "fd_max_err_over_rms": out, "fd_samples": int(picks.size)}
+    _log("c5: forward %.3f ms, adjoint %.1f ms (wall, incl. transfers), FD %s" % (fwd, adj, out))
+    plan.close()
+    print(json.dumps(row), flush=True)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("config", choices=["c3", "c4", "c5"])
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--ng", type=int, default=256)
+    ap.add_argument("--theta", type=float, default=0.6)
+    ap.add_argument("--eps", type=float, default=1e-4)
+    ap.add_argument("--thetas", type=float, nargs="+", default=[0.5, 0.6, 0.7])
+    a = ap.parse_args(argv)
+    import torch  # noqa: F401  (one HIP runtime in the process: torch's)
+    if a.config == "c3":
+        a.n = a.n or 16777216
+        c3(a)
+    elif a.config == "c4":
+        c4(a)
+    else:
+        a.n = a.n or 1048576
+        c5(a)
+
+
+if __name__ == "__main__":
+    main()

@@ -69,7 +69,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--variant", default="rel32c")
     ap.add_argument("--group-mode", type=int, default=1)
-    ap.add_argument("--persistent", type=int, default=0)
+    ap.add_argument("--persistent", type=int, default=-1)
     ap.add_argument("--sample", type=int, default=8192)
     ap.add_argument("--no-plain64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")

@@ -133,9 +133,15 @@ typedef struct {
     double box;            /* periodic box edge */
     int nthreads;          /* host threads for the build (0: all) */
     int merge_groups;      /* merge adjacent small sibling groups */
-    int persistent_blocks; /* 0: one workgroup per group; >0: persistent grid; -1: auto */
+    int persistent_blocks; /* 0: one workgroup per group; >0: persistent grid of that many
+                              single-wave workgroups; -1 (default): 32 per CU */
     int fp64;              /* also build the FP64 tables (variants 3 and 4) */
     int dispatch;          /* 0: global longest-first; 1: per-XCD Morton slices (default) */
+    int mixed_order;       /* 1: list an accepted cell at the lowest order whose truncation
+                              bound (bmax/d)^(q+1) is within the order-`order` bound at the
+                              opening-angle boundary, theta^(order+1): order 2 beyond
+                              bmax * theta^(-(order+1)/3), order 0 beyond bmax * theta^-(order+1)
+                              (the order-by-distance choice of nln/mac.c:1797-1810); 0: off */
 } pgrav_params;
 
 typedef struct {

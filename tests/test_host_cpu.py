@@ -155,3 +155,38 @@ def test_ics_shapes_and_statistics():
     d = pos - np.stack([gx, gy, gz], -1).reshape(-1, 3)
     d -= np.rint(d)
     np.testing.assert_allclose(np.sqrt((d ** 2).sum(1).mean()) * 32, 0.2, rtol=1e-3)
+
+
+def test_mixed_order_lists_keep_the_truncation_bound():
+    """mixed_order: the same accepted cells, each listed at the lowest order whose
+    (bmax/d')^(q+1) bound (d' = d - r_group) is within theta^5; evaluated by the
+    FP64 list oracle the force error stays at the single-order level."""
+    pos, mass = ics.clustered(12000, seed=21, nhalo=12)
+    theta = 0.6
+    one = batch.HostPlan(pos, mass, theta=theta, order=4, eps=1e-3)
+    mix = batch.HostPlan(pos, mass, theta=theta, order=4, eps=1e-3, mixed_order=True)
+    s1, s2 = one.stats(), mix.stats()
+    # identical acceptance: only the order of accepted cells changes
+    assert s1["ngroups"] == s2["ngroups"] and s1["entries_body"] == s2["entries_body"]
+    assert s2["entries_hexa"] + s2["entries_quad"] + s2["entries_mono"] == s1["entries_hexa"]
+    assert s2["entries_quad"] > 0.3 * s1["entries_hexa"]
+    perm, spos = mix.sorted()
+    smass = mass[perm]
+    rng = np.random.default_rng(4)
+    e1, e2 = [], []
+    for g in rng.choice(s2["ngroups"], 12, replace=False):
+        g = int(g)
+        _, _, c = mix.group(g)
+        for kind, q in (("quad", 2), ("mono", 0)):
+            for nd in mix.list(g, kind):
+                cell = mix.cell(int(nd))
+                d = np.linalg.norm(cell["cofm"] - c[:3]) - c[3]
+                assert (cell["bmax"] / d) ** (q + 1) <= theta ** 5 * (1 + 1e-12)
+        f, r1 = listoracle.group_reference(one, g, spos, smass, eps=1e-3)
+        _, r2 = listoracle.group_reference(mix, g, spos, smass, eps=1e-3)
+        ex = oracle.direct(spos[f:f + r1.shape[0]], spos, smass, soft_type=1, eps=1e-3)
+        n = np.linalg.norm(ex[:, :3], axis=1)
+        e1.append(np.linalg.norm(r1[:, :3] - ex[:, :3], axis=1) / n)
+        e2.append(np.linalg.norm(r2[:, :3] - ex[:, :3], axis=1) / n)
+    m1, m2 = np.median(np.concatenate(e1)), np.median(np.concatenate(e2))
+    assert m2 < 3 * m1 and m2 < 1e-3, (m1, m2)
