@@ -4,7 +4,8 @@ Times the reference's SSE4 (4-wide) and AVX (8-wide) CPU kernels, compiled
 from the reference sources into oracle/_ref, on a bounded sample of sink
 groups of the benchmark workload, over exactly the interaction lists the GPU
 evaluates: do_gravph_{sse4,avx8} (Plummer-softened order-4 cells, no dipole,
-nln/do_grav_sse4.c:1768) over the group's cells and do_gravp_{sse4,avx8}
+nln/do_grav_sse4.c:1768) or do_gravpq_{sse4,avx8} (order 2, :2090) over the
+group's cells and do_gravp_{sse4,avx8}
 (nln/do_grav_sse4.c:2167) over its bodies, per sink, as the leaf evaluation
 of nln/mac.c:477-630 does.  Positions are absolute FP32, as in the reference.
 """
@@ -28,7 +29,7 @@ def _cpu_model():
     return platform.processor() or "unknown"
 
 
-def _run_lists(arch, D, groups, eps2, nthreads):
+def _run_lists(arch, D, groups, eps2, nthreads, cell="ph", csz=29):
     L = lib()
     fn = L.orc_cpu_ref_run_lists
     fn.restype = ctypes.c_double
@@ -39,7 +40,7 @@ def _run_lists(arch, D, groups, eps2, nthreads):
     ks = ctypes.c_double(0)
     groups = np.ascontiguousarray(groups, np.int32)
     path = os.path.join(REF_DIR, "libref_%s.so" % arch).encode()
-    wall = fn(path, ("do_gravph_" + arch).encode(), ("do_gravp_" + arch).encode(), nsse, 29,
+    wall = fn(path, ("do_grav%s_%s" % (cell, arch)).encode(), ("do_gravp_" + arch).encode(), nsse, csz,
               D["rec"].ctypes.data, D["sinks"].ctypes.data, D["bodies"].ctypes.data,
               D["first"].ctypes.data, D["count"].ctypes.data, D["hoff"].ctypes.data,
               D["hidx"].ctypes.data, D["boff"].ctypes.data, D["bidx"].ctypes.data,
@@ -76,11 +77,15 @@ def run_cpu_baseline(pos, mass, args, acc_gpu=None, target_s=15.0, nthreads=None
     perm, spos = H.sorted()
     smass = np.asarray(mass, np.float64)[perm]
     first, count, _ = H.groups()
-    hoff, hidx = H.csr("hexa")
+    # order-4 cells go to do_gravph (29-float records), order-2 cells to do_gravpq (their
+    # first 13 floats: mass, centre, bmax, dipole, quadrupole); one cell order per plan
+    kind, cell, csz = ("hexa", "ph", 29) if args.order >= 4 else ("quad", "pq", 13)
+    hoff, hidx = H.csr(kind)
     boff, bidx = H.csr("body")
-    if H.csr("quad")[1].size or H.csr("mono")[1].size:
-        raise RuntimeError("the CPU baseline covers order-4 cells and bodies only")
-    D = {"rec": np.ascontiguousarray(H.records(np.arange(st["nnodes"], dtype=np.uint32)),
+    others = [k for k in ("hexa", "quad", "mono") if k != kind]
+    if any(H.csr(k)[1].size for k in others):
+        raise RuntimeError("the CPU baseline covers one cell order plus bodies")
+    D = {"rec": np.ascontiguousarray(H.records(np.arange(st["nnodes"], dtype=np.uint32))[:, :csz],
                                      np.float32),
          "sinks": np.ascontiguousarray(spos, np.float32),
          "bodies": np.ascontiguousarray(np.concatenate([smass[:, None], spos], axis=1), np.float32),
@@ -93,7 +98,7 @@ def run_cpu_baseline(pos, mass, args, acc_gpu=None, target_s=15.0, nthreads=None
               "cpu_model": _cpu_model()}
     # calibrate on a small sample, then size the timed sample to ~target_s seconds
     cal = rng.choice(ng, min(ng, 64 * nthreads), replace=False).astype(np.int32)
-    wall, _, _ = _run_lists("sse4", D, cal, eps2, nthreads)
+    wall, _, _ = _run_lists("sse4", D, cal, eps2, nthreads, cell, csz)
     per_int = wall / max(1, _interactions(D, cal))
     total = _interactions(D, np.arange(ng))
     # SSE4 then AVX (about half the time): ~1.5 SSE4 passes in the budget
@@ -102,7 +107,7 @@ def run_cpu_baseline(pos, mass, args, acc_gpu=None, target_s=15.0, nthreads=None
               np.sort(rng.choice(ng, max(len(cal), int(frac * ng)), replace=False))).astype(np.int32)
     inter = _interactions(D, groups)
     for arch in ("sse4", "avx8"):
-        wall, ks, out = _run_lists(arch, D, groups, eps2, nthreads)
+        wall, ks, out = _run_lists(arch, D, groups, eps2, nthreads, cell, csz)
         result[arch] = inter / (ks / nthreads)
         result[arch + "_wall_seconds"] = wall
         result[arch + "_kernel_seconds_per_thread"] = ks / nthreads
@@ -114,9 +119,9 @@ def run_cpu_baseline(pos, mass, args, acc_gpu=None, target_s=15.0, nthreads=None
             result["median_rel_diff_vs_gpu"] = float(np.median(d))
     result["value"] = max(result["sse4"], result["avx8"])
     result["sample"] = ("%s %d of %d sink groups (%d sinks, %.3e interactions) of the benchmark "
-                        "workload: reference do_gravph+do_gravp (SSE4 and AVX builds) per sink "
+                        "workload: reference do_grav%s+do_gravp (SSE4 and AVX builds) per sink "
                         "over the same lists, %d threads, kernel calls timed"
                         % ("all" if groups.size == ng else "random", groups.size, ng,
-                           int(count[groups].sum()), inter, nthreads))
+                           int(count[groups].sum()), inter, cell, nthreads))
     result["prep_seconds"] = time.perf_counter() - t0
     return result

@@ -13,7 +13,8 @@ import oracle
 ORDER = {"hexa": 4, "quad": 2, "mono": 0}
 
 
-def group_reference(H, g, pos_sorted, mass_sorted, soft="plummer", eps=1e-4, G=1.0):
+def group_reference(H, g, pos_sorted, mass_sorted, soft="plummer", eps=1e-4, G=1.0,
+                    quad_phi_sign=1):
     first, cnt, _ = H.group(g)
     sinks = pos_sorted[first:first + cnt]
     acc = np.zeros((cnt, 4))
@@ -28,7 +29,8 @@ def group_reference(H, g, pos_sorted, mass_sorted, soft="plummer", eps=1e-4, G=1
             b = pos_sorted[c["first"]:c["first"] + c["count"]]
             m = mass_sorted[c["first"]:c["first"] + c["count"]]
             recs.append(oracle.cell_record(b, m, c["cofm"], c["bmax"] if c["bmax"] > 0 else 1.0))
-        acc += oracle.eval_records(sinks, np.array(recs), order, dipole=False, eps2=eps2)
+        acc += oracle.eval_records(sinks, np.array(recs), order, dipole=False, eps2=eps2,
+                                   quad_phi_sign=quad_phi_sign)
     bodies = H.list(g, "body")
     if bodies.size:
         st = {"none": 0, "plummer": 1, "k1": 4}[soft]
