@@ -435,10 +435,8 @@ int pgrav_plan_set_dispatch(void *plan, int mode) {
     const int64_t G = L.ngroups;
     std::vector<double> cost(G);
     for (int64_t g = 0; g < G; g++)
-        cost[g] = 113.0 * (L.off[L_HEXA][g + 1] - L.off[L_HEXA][g]) +
-                  40.0 * (L.off[L_QUAD][g + 1] - L.off[L_QUAD][g]) +
-                  17.0 * (L.off[L_MONO][g + 1] - L.off[L_MONO][g]) +
-                  20.0 * (L.off[L_BODY][g + 1] - L.off[L_BODY][g]);
+        cost[g] = wave_cost(L.off[L_HEXA][g + 1] - L.off[L_HEXA][g], L.off[L_QUAD][g + 1] - L.off[L_QUAD][g],
+                            L.off[L_MONO][g + 1] - L.off[L_MONO][g], L.off[L_BODY][g + 1] - L.off[L_BODY][g]);
     make_dispatch(cost, mode, L);
     P->prm.dispatch = mode;
     return (check(hipStreamSynchronize(P->stream), "hipStreamSynchronize") &&

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <functional>
 #include <numeric>
 #include <stdexcept>
 #include <thread>
@@ -64,6 +65,182 @@ static void collect(const Tree &t, int32_t nd, const TreeParams &p, Lists &l) {
     }
     if (pf >= 0)
         emit_group(t, pf, pc, gmax, l);
+}
+
+static int auto_pcut(int order) {
+    // body-count at which one cell interaction (~110 / ~38 VALU for orders 4 / 2)
+    // costs no more than its bodies (~15 VALU each)
+    return order >= 4 ? 8 : (order >= 2 ? 3 : 2);
+}
+
+// The acceptance test and the walk of one sink group (centre g, radius rg).
+struct Mac {
+    const Tree &t;
+    int pcut, kind;
+    double inv_theta, box, r_quad, r_mono;
+    bool periodic, mixed;
+    Mac(const Tree &tree, const TreeParams &p) : t(tree) {
+        pcut = p.pcut > 0 ? p.pcut : auto_pcut(p.order);
+        kind = p.order >= 4 ? L_HEXA : (p.order >= 2 ? L_QUAD : L_MONO);
+        inv_theta = 1.0 / p.theta;
+        box = p.box;
+        periodic = p.periodic != 0;
+        // mixed order: distance (in bmax) beyond which order 2 / order 0 meets the
+        // truncation bound theta^(order+1) that the top order meets at bmax/theta
+        const int top = p.order >= 4 ? 4 : (p.order >= 2 ? 2 : 0);
+        mixed = p.mixed_order && top > 0;
+        r_quad = top == 4 ? pow(p.theta, -5.0 / 3.0) : 1e300;
+        r_mono = pow(p.theta, -(double)(top + 1));
+    }
+    // cell(kind, node) for every accepted cell, bodies(first, count) for every run of
+    // bodies listed directly; returns the number of bodies covered (== N when whole)
+    template <class C, class B>
+    int64_t walk(const double *gc, std::vector<int32_t> &stack, C cell, B bodies) const {
+        const double gx = gc[0], gy = gc[1], gz = gc[2], rg = gc[3];
+        int64_t covered = 0;
+        stack.clear();
+        stack.push_back(0);
+        while (!stack.empty()) {
+            const int32_t c = stack.back();
+            stack.pop_back();
+            double dx = t.cofm[3 * c] - gx, dy = t.cofm[3 * c + 1] - gy, dz = t.cofm[3 * c + 2] - gz;
+            if (periodic) {
+                dx -= box * nearbyint(dx / box);
+                dy -= box * nearbyint(dy / box);
+                dz -= box * nearbyint(dz / box);
+            }
+            const double d = sqrt(dx * dx + dy * dy + dz * dz);
+            const int32_t cnt = t.count[c];
+            bool accept = cnt >= pcut && d - rg > t.bmax[c] * inv_theta;
+            // in a periodic box only cells well inside half a box have an
+            // unambiguous nearest image
+            if (accept && periodic && t.bmax[c] * inv_theta + rg + t.bmax[c] > 0.5 * box)
+                accept = false;
+            if (accept) {
+                int k = kind;
+                if (mixed) {
+                    const double s = (d - rg) / t.bmax[c];
+                    k = s > r_mono ? L_MONO : (s > r_quad ? L_QUAD : kind);
+                }
+                cell(k, c);
+                covered += cnt;
+            } else if (t.child[c] < 0 || cnt < pcut) {
+                bodies(t.first[c], cnt);
+                covered += cnt;
+            } else {
+                for (int j = t.nchild[c] - 1; j >= 0; j--) stack.push_back(t.child[c] + j);
+            }
+        }
+        return covered;
+    }
+};
+
+double wave_cost(int64_t hexa, int64_t quad, int64_t mono, int64_t body) {
+    // VALU instructions per wave of the batched kernel: 110 per order-4 cell,
+    // 38 per order-2 cell, 16 per order-0 cell, 15.25 per body, ~40 per 64-entry
+    // tile (fetch, offsets, barriers, compensated fold) and ~60 per group
+    auto tiles = [](int64_t n) { return (double)((n + 63) / 64); };
+    return 110.0 * hexa + 38.0 * quad + 16.0 * mono + 15.25 * body +
+           40.0 * (tiles(hexa) + tiles(quad) + tiles(mono) + tiles(body)) + 60.0;
+}
+
+// bounding-box centre and enclosing radius of a run of sorted bodies
+static void group_sphere(const Tree &t, int64_t f, int64_t n, double *out4) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t i = f; i < f + n; i++)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], t.pos[3 * i + a]);
+            hi[a] = std::max(hi[a], t.pos[3 * i + a]);
+        }
+    double r2 = 0;
+    for (int a = 0; a < 3; a++) out4[a] = 0.5 * (lo[a] + hi[a]);
+    for (int64_t i = f; i < f + n; i++) {
+        double d2 = 0;
+        for (int a = 0; a < 3; a++) d2 += (t.pos[3 * i + a] - out4[a]) * (t.pos[3 * i + a] - out4[a]);
+        r2 = std::max(r2, d2);
+    }
+    out4[3] = sqrt(r2);
+}
+
+static void run_threads(int nth, const std::function<void(int)> &fn) {
+    if (nth <= 1) {
+        fn(0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int i = 0; i < nth; i++) th.emplace_back(fn, i);
+    for (auto &x : th) x.join();
+}
+
+// Cost-driven merging (mode 4): start from sibling-merged cells and merge two
+// Morton-adjacent groups when their joint list costs less than the two lists
+// (both evaluated with the MAC above); the most profitable disjoint pairs first,
+// two rounds.
+static void merge_by_cost(const Tree &t, const TreeParams &p, Lists &l) {
+    const Mac mac(t, p);
+    const int nth = default_threads(p.nthreads);
+    auto cost_of = [&](int64_t f, int64_t n, std::vector<int32_t> &stack) {
+        double gc[4];
+        group_sphere(t, f, n, gc);
+        int64_t cnt[NLIST] = {0, 0, 0, 0};
+        mac.walk(gc, stack, [&](int k, int32_t) { cnt[k]++; },
+                 [&](int64_t, int32_t m) { cnt[L_BODY] += m; });
+        return wave_cost(cnt[L_HEXA], cnt[L_QUAD], cnt[L_MONO], cnt[L_BODY]);
+    };
+    for (int round = 0; round < 2; round++) {
+        const int64_t G = (int64_t)l.gfirst.size();
+        if (G < 2)
+            return;
+        std::vector<double> own(G), pair(G, 0.0);
+        std::atomic<int64_t> next(0);
+        run_threads(nth, [&](int) {
+            std::vector<int32_t> stack;
+            stack.reserve(256);
+            for (;;) {
+                const int64_t g0 = next.fetch_add(256);
+                if (g0 >= G)
+                    break;
+                for (int64_t g = g0; g < std::min(G, g0 + 256); g++) {
+                    own[g] = cost_of(l.gfirst[g], l.gcount[g], stack);
+                    if (g + 1 < G && l.gcount[g] + l.gcount[g + 1] <= p.group_max &&
+                        l.gfirst[g] + l.gcount[g] == l.gfirst[g + 1])
+                        pair[g] = cost_of(l.gfirst[g], l.gcount[g] + l.gcount[g + 1], stack);
+                }
+            }
+        });
+        std::vector<std::pair<double, int64_t>> gain;
+        for (int64_t g = 0; g + 1 < G; g++)
+            if (pair[g] > 0 && pair[g] < own[g] + own[g + 1])
+                gain.push_back({own[g] + own[g + 1] - pair[g], g});
+        if (gain.empty())
+            return;
+        std::sort(gain.begin(), gain.end(), [](const std::pair<double, int64_t> &a,
+                                               const std::pair<double, int64_t> &b) {
+            return a.first > b.first || (a.first == b.first && a.second < b.second);
+        });
+        std::vector<char> take(G, 0), used(G, 0);
+        for (const auto &x : gain) {
+            const int64_t g = x.second;
+            if (!used[g] && !used[g + 1]) {
+                used[g] = used[g + 1] = 1;
+                take[g] = 1;
+            }
+        }
+        std::vector<int64_t> f2;
+        std::vector<int32_t> c2;
+        for (int64_t g = 0; g < G; g++) {
+            if (take[g]) {
+                f2.push_back(l.gfirst[g]);
+                c2.push_back(l.gcount[g] + l.gcount[g + 1]);
+                g++;
+            } else {
+                f2.push_back(l.gfirst[g]);
+                c2.push_back(l.gcount[g]);
+            }
+        }
+        l.gfirst.swap(f2);
+        l.gcount.swap(c2);
+    }
 }
 
 void build_groups(const Tree &t, const TreeParams &p, Lists &l) {
@@ -130,50 +307,23 @@ void build_groups(const Tree &t, const TreeParams &p, Lists &l) {
         }
         l.gfirst.push_back(cf);
         l.gcount.push_back((int32_t)cc);
+    } else if (p.merge_groups == 4) {
+        TreeParams q = p;
+        q.merge_groups = 1;
+        collect(t, 0, q, l);
+        merge_by_cost(t, p, l);
     } else {
         collect(t, 0, p, l);
     }
     l.ngroups = (int64_t)l.gfirst.size();
     l.gcenter.assign(4 * l.ngroups, 0.0);
-    for (int64_t g = 0; g < l.ngroups; g++) {
-        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-        const int64_t f = l.gfirst[g];
-        for (int64_t i = f; i < f + l.gcount[g]; i++)
-            for (int a = 0; a < 3; a++) {
-                lo[a] = std::min(lo[a], t.pos[3 * i + a]);
-                hi[a] = std::max(hi[a], t.pos[3 * i + a]);
-            }
-        double c[3], r2 = 0;
-        for (int a = 0; a < 3; a++) c[a] = 0.5 * (lo[a] + hi[a]);
-        for (int64_t i = f; i < f + l.gcount[g]; i++) {
-            double d2 = 0;
-            for (int a = 0; a < 3; a++) d2 += (t.pos[3 * i + a] - c[a]) * (t.pos[3 * i + a] - c[a]);
-            r2 = std::max(r2, d2);
-        }
-        for (int a = 0; a < 3; a++) l.gcenter[4 * g + a] = c[a];
-        l.gcenter[4 * g + 3] = sqrt(r2);
-    }
-}
-
-static int auto_pcut(int order) {
-    // body-count at which one cell interaction (~113 / ~40 VALU for orders 4 / 2)
-    // costs no more than its bodies (~18 VALU each)
-    return order >= 4 ? 8 : (order >= 2 ? 3 : 2);
+    for (int64_t g = 0; g < l.ngroups; g++) group_sphere(t, l.gfirst[g], l.gcount[g], &l.gcenter[4 * g]);
 }
 
 void build_lists(const Tree &t, const TreeParams &p, Lists &l) {
     const int64_t G = l.ngroups;
     const int nth = default_threads(p.nthreads);
-    const int pcut = p.pcut > 0 ? p.pcut : auto_pcut(p.order);
-    const int kind = p.order >= 4 ? L_HEXA : (p.order >= 2 ? L_QUAD : L_MONO);
-    const double inv_theta = 1.0 / p.theta;
-    const double box = p.box;
-    // mixed order: distance (in bmax) beyond which order 2 / order 0 meets the
-    // truncation bound theta^(order+1) that the top order meets at bmax/theta
-    const int top = p.order >= 4 ? 4 : (p.order >= 2 ? 2 : 0);
-    const bool mixed = p.mixed_order && top > 0;
-    const double r_quad = top == 4 ? pow(p.theta, -5.0 / 3.0) : 1e300;
-    const double r_mono = pow(p.theta, -(double)(top + 1));
+    const Mac mac(t, p);
     struct Buf {
         std::vector<uint32_t> v[NLIST];
         std::vector<int64_t> gid;
@@ -195,44 +345,11 @@ void build_lists(const Tree &t, const TreeParams &p, Lists &l) {
             for (int64_t g = g0; g < std::min(G, g0 + chunk); g++) {
                 B.gid.push_back(g);
                 for (int k = 0; k < NLIST; k++) B.start.push_back((int64_t)B.v[k].size());
-                const double gx = l.gcenter[4 * g], gy = l.gcenter[4 * g + 1],
-                             gz = l.gcenter[4 * g + 2], rg = l.gcenter[4 * g + 3];
-                int64_t covered = 0;
-                stack.clear();
-                stack.push_back(0);
-                while (!stack.empty()) {
-                    const int32_t c = stack.back();
-                    stack.pop_back();
-                    double dx = t.cofm[3 * c] - gx, dy = t.cofm[3 * c + 1] - gy,
-                           dz = t.cofm[3 * c + 2] - gz;
-                    if (p.periodic) {
-                        dx -= box * nearbyint(dx / box);
-                        dy -= box * nearbyint(dy / box);
-                        dz -= box * nearbyint(dz / box);
-                    }
-                    const double d = sqrt(dx * dx + dy * dy + dz * dz);
-                    const int32_t cnt = t.count[c];
-                    bool accept = cnt >= pcut && d - rg > t.bmax[c] * inv_theta;
-                    // in a periodic box only cells well inside half a box have an
-                    // unambiguous nearest image
-                    if (accept && p.periodic && t.bmax[c] * inv_theta + rg + t.bmax[c] > 0.5 * box)
-                        accept = false;
-                    if (accept) {
-                        int k = kind;
-                        if (mixed) {
-                            const double s = (d - rg) / t.bmax[c];
-                            k = s > r_mono ? L_MONO : (s > r_quad ? L_QUAD : kind);
-                        }
-                        B.v[k].push_back((uint32_t)c);
-                        covered += cnt;
-                    } else if (t.child[c] < 0 || cnt < pcut) {
-                        const int64_t f = t.first[c];
-                        for (int64_t i = f; i < f + cnt; i++) B.v[L_BODY].push_back((uint32_t)i);
-                        covered += cnt;
-                    } else {
-                        for (int j = t.nchild[c] - 1; j >= 0; j--) stack.push_back(t.child[c] + j);
-                    }
-                }
+                const int64_t covered = mac.walk(
+                    &l.gcenter[4 * g], stack, [&](int k, int32_t c) { B.v[k].push_back((uint32_t)c); },
+                    [&](int64_t f, int32_t n) {
+                        for (int64_t i = f; i < f + n; i++) B.v[L_BODY].push_back((uint32_t)i);
+                    });
                 if (covered != t.n)
                     miscount.store(true);
                 const int64_t *st = &B.start[B.start.size() - NLIST];
@@ -291,13 +408,11 @@ void build_lists(const Tree &t, const TreeParams &p, Lists &l) {
         for (int tid = 0; tid < nth; tid++) l.pairs[k] += bufs[tid].pairs[k];
     }
     bufs.clear();
-    // longest-processing-time-first order (per-wave VALU estimate)
+    // dispatch order from the per-wave VALU estimate
     std::vector<double> cost(G);
     for (int64_t g = 0; g < G; g++)
-        cost[g] = 113.0 * (l.off[L_HEXA][g + 1] - l.off[L_HEXA][g]) +
-                  40.0 * (l.off[L_QUAD][g + 1] - l.off[L_QUAD][g]) +
-                  17.0 * (l.off[L_MONO][g + 1] - l.off[L_MONO][g]) +
-                  20.0 * (l.off[L_BODY][g + 1] - l.off[L_BODY][g]);
+        cost[g] = wave_cost(l.off[L_HEXA][g + 1] - l.off[L_HEXA][g], l.off[L_QUAD][g + 1] - l.off[L_QUAD][g],
+                            l.off[L_MONO][g + 1] - l.off[L_MONO][g], l.off[L_BODY][g + 1] - l.off[L_BODY][g]);
     make_dispatch(cost, p.dispatch, l);
 }
 

@@ -18,7 +18,9 @@ struct TreeParams {
     int periodic = 0;
     double box = 1.0;      // periodic box size (origin at 0)
     int nthreads = 0;      // 0: hardware concurrency
-    int merge_groups = 1;  // merge adjacent small sibling groups up to group_max
+    int merge_groups = 1;  // 0: maximal cells <= group_max; 1: adjacent small siblings merged;
+                           // 2: fixed Morton runs; 3: box-limited Morton merge; 4: mode 1, then
+                           // Morton-adjacent groups merged where the joint list costs less
     int dispatch = 1;      // 0: global longest-first; 1: per-XCD Morton slices, longest-first inside
     int mixed_order = 0;   // 1: an accepted cell far enough for a lower order to meet the same
                            // truncation bound is listed at that order (order-2 / order-0 lists)
@@ -75,5 +77,7 @@ void build_lists(const Tree &t, const TreeParams &p, Lists &l);
 void prepare_cells(const Tree &t, int order, std::vector<float> &rec32, std::vector<double> *rec64,
                    std::vector<double> &cpos4);
 int default_threads(int requested);
+// estimated VALU instructions of one wave over a group's lists (entries per kind)
+double wave_cost(int64_t hexa, int64_t quad, int64_t mono, int64_t body);
 
 }  // namespace pgrav

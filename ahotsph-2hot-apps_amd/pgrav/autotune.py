@@ -1,9 +1,9 @@
 """Autotuner for the batched evaluation.
 
 Searches the parameters that change the work decomposition but not the
-physics: how sink groups are formed (tree cells with siblings merged, or fixed
-runs along the Morton curve), the group size (lanes per wave used), the leaf
-size, and the launch shape (one workgroup per group in longest-first order,
+physics: how sink groups are formed (tree cells with siblings merged, then
+optionally merged further where the joint list costs less), the group size
+(lanes per wave used), the leaf size, and the launch shape (one workgroup per group in longest-first order,
 or a persistent grid of k single-wave workgroups per CU draining a work
 counter).  The opening angle, multipole order and softening are inputs, so the
 accuracy target is held fixed.  Each candidate is timed with HIP events over
@@ -25,15 +25,15 @@ import numpy as np
 from . import batch
 
 DEFAULT_SPACE = {
-    "merge_groups": [1, 3],
-    "group_max": [48, 64],
-    "leaf_max": [8, 16],
+    "merge_groups": [1, 4],
+    "group_max": [64],
+    "leaf_max": [16, 32, 48, 64],
 }
 # launch shapes: 0 = one workgroup per group; -1 = persistent grid of 32
 # single-wave workgroups per CU; k > 0 = persistent grid of k per CU.  The
 # table and tuned.json store the resulting workgroup count (what
 # pgrav_plan_set_launch and Plan(persistent_blocks=...) take).
-PERSISTENT = [0, -1, 16]
+PERSISTENT = [0, -1]
 TUNED = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tuned.json")
 
 
