@@ -40,6 +40,7 @@ __device__ __forceinline__ D3 operator*(D3 a, D3 b) {
 }
 __device__ __forceinline__ D3 operator*(float s, D3 b) { return D3(s * b.v, s * b.x, s * b.y, s * b.z); }
 __device__ __forceinline__ D3 operator*(D3 b, float s) { return s * b; }
+__device__ __forceinline__ D3 fma(D3 a, D3 b, D3 c) { return a * b + c; }
 __device__ __forceinline__ D3 &operator+=(D3 &a, D3 b) { return a = a + b; }
 __device__ __forceinline__ D3 &operator-=(D3 &a, D3 b) { return a = a - b; }
 __device__ __forceinline__ bool operator>(D3 a, D3 b) { return a.v > b.v; }

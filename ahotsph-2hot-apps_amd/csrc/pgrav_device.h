@@ -131,7 +131,8 @@ __device__ __forceinline__ void cell_interact(const TM *__restrict__ M, T m, T x
 // nln/do_grav_sse4.c:2188-2215.
 template <typename T>
 __device__ __forceinline__ void mono_interact(T m, T x, T y, T z, T eps2, Acc4T<T> &a) {
-    const T r2 = x * x + y * y + z * z + eps2;
+    // eps2 seeds the sum: three FMAs
+    const T r2 = fma(x, x, fma(y, y, fma(z, z, eps2)));
     const T rinv = nrsq(r2);
     const T mr = m * rinv;
     const T mr3 = mr * rinv * rinv;
