@@ -1,7 +1,7 @@
 """Measurements of the path on the configurations it is specified on (one GPU).
 
     python -m pgrav.study c3 [--n 16777216] [--thetas 0.5 0.6 0.7]
-    python -m pgrav.study c4 [--ng 256]
+    python -m pgrav.study c4 [--ng 256] [--theta 0.6 | --thetas 0.4 0.5 0.6]
     python -m pgrav.study c5 [--n 1048576]
 
 c3  clustered bodies (pgrav.ics.clustered), order 4, Plummer eps 1e-4, for each
@@ -14,8 +14,9 @@ c4  periodic unit box, ng^3 lattice displaced by a Gaussian random field
     time, error against the periodic FP64 direct sum (nearest image + Ewald
     table) on 1024 sampled bodies, net momentum.
 c5  clustered N bodies, tree and lists fixed: forward and adjoint wall time, and
-    the adjoint against central finite differences (h = 1e-5, FP64 forward) of
-    L = sum_i w_i.a_i + v_i phi_i on 64 sampled coordinates.
+    the adjoint against central finite differences (h = 1e-5 and h/2, combined
+    by Richardson extrapolation; FP64 forward) of L = sum_i w_i.a_i + v_i phi_i
+    on 64 sampled coordinates.
 
 Each configuration prints one JSON line on stdout; progress goes to stderr.
 """
@@ -133,34 +134,44 @@ def c5(args):
     adj = (time.perf_counter() - t0) * 1e3
     h = 1e-5
     picks = rng.choice(n, 16, replace=False)
-    fd, ad = [], []
 
     def loss():
         plan.evaluate("f64")
         return float((plan.result("f64") * w4).sum())
 
+    def central(k, q, hh):
+        p1, p2, m1, m2 = pos.copy(), pos.copy(), mass.copy(), mass.copy()
+        if q < 3:
+            p1[k, q] += hh
+            p2[k, q] -= hh
+            step = hh
+        else:
+            m1[k] *= 1 + hh
+            m2[k] *= 1 - hh
+            step = mass[k] * hh
+        plan.update(p1, m1)
+        lp = loss()
+        plan.update(p2, m2)
+        lm = loss()
+        return (lp - lm) / (2 * step)
+
+    # central differences at h and h/2; Richardson's combination removes the h^2
+    # term, which at h = 1e-5 is not small next to bodies 1e-4 apart in the halos
+    fd, fd2, ad = [], [], []
     for k in picks:
         for q in range(4):
-            p1, p2, m1, m2 = pos.copy(), pos.copy(), mass.copy(), mass.copy()
-            if q < 3:
-                p1[k, q] += h
-                p2[k, q] -= h
-                step = h
-            else:
-                m1[k] *= 1 + h
-                m2[k] *= 1 - h
-                step = mass[k] * h
-            plan.update(p1, m1)
-            lp = loss()
-            plan.update(p2, m2)
-            lm = loss()
-            fd.append((lp - lm) / (2 * step))
+            fd.append(central(k, q, h))
+            fd2.append(central(k, q, h / 2))
             ad.append(grad[k, q])
-    fd, ad = np.array(fd).reshape(-1, 4), np.array(ad).reshape(-1, 4)
+    plan.update(pos, mass)
+    fd, fd2, ad = (np.array(v).reshape(-1, 4) for v in (fd, fd2, ad))
+    rich = (4 * fd2 - fd) / 3
     out = {}
     for q, name in ((slice(0, 3), "position"), (slice(3, 4), "mass")):
-        scale = np.sqrt((fd[:, q] ** 2).mean())
+        scale = np.sqrt((rich[:, q] ** 2).mean())
         out[name] = float(np.abs(ad[:, q] - fd[:, q]).max() / scale)
+        out[name + "_richardson"] = float(np.abs(ad[:, q] - rich[:, q]).max() / scale)
+        out[name + "_richardson_median"] = float(np.median(np.abs(ad[:, q] - rich[:, q])) / scale)
     row = {"config": "c5", "n": int(n), "groups": st["ngroups"], "pairs": st["pairs_total"],
            "forward_kernel_ms": fwd, "adjoint_wall_ms": adj,
            "fd_max_err_over_rms": out, "fd_samples": int(picks.size)}
@@ -176,14 +187,18 @@ def main(argv=None):
     ap.add_argument("--ng", type=int, default=256)
     ap.add_argument("--theta", type=float, default=0.6)
     ap.add_argument("--eps", type=float, default=1e-4)
-    ap.add_argument("--thetas", type=float, nargs="+", default=[0.5, 0.6, 0.7])
+    ap.add_argument("--thetas", type=float, nargs="+", default=None)
     a = ap.parse_args(argv)
+    a.thetas_given = a.thetas is not None
+    a.thetas = a.thetas or [0.5, 0.6, 0.7]
     import torch  # noqa: F401  (one HIP runtime in the process: torch's)
     if a.config == "c3":
         a.n = a.n or 16777216
         c3(a)
     elif a.config == "c4":
-        c4(a)
+        for theta in (a.thetas if a.thetas_given else [a.theta]):
+            a.theta = theta
+            c4(a)
     else:
         a.n = a.n or 1048576
         c5(a)

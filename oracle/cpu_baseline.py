@@ -107,7 +107,12 @@ def run_cpu_baseline(pos, mass, args, acc_gpu=None, target_s=15.0, nthreads=None
               np.sort(rng.choice(ng, max(len(cal), int(frac * ng)), replace=False))).astype(np.int32)
     inter = _interactions(D, groups)
     for arch in ("sse4", "avx8"):
+        # a workload that finishes in milliseconds is repeated (best pass kept)
         wall, ks, out = _run_lists(arch, D, groups, eps2, nthreads, cell, csz)
+        for _ in range(min(50, int(1.0 / max(wall, 1e-4)))):
+            w2, k2, _ = _run_lists(arch, D, groups, eps2, nthreads, cell, csz)
+            if k2 < ks:
+                wall, ks = w2, k2
         result[arch] = inter / (ks / nthreads)
         result[arch + "_wall_seconds"] = wall
         result[arch + "_kernel_seconds_per_thread"] = ks / nthreads
