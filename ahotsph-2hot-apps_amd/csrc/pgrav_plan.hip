@@ -16,6 +16,7 @@
 #include "pgrav_batch.h"
 #include "pgrav_device.h"
 #include "pgrav_host.h"
+#include "pgrav_moments.h"
 #include "pgrav_internal.h"
 
 namespace pgrav {
@@ -40,8 +41,11 @@ struct Plan {
     // adjoint buffers, allocated on first use
     float4 *a_wv = nullptr, *a_gsink = nullptr, *a_gbody = nullptr;
     float *a_gcell = nullptr, *a_gnode = nullptr, *a_mat = nullptr;
-    double *a_raw = nullptr;
     double4 *a_out = nullptr;
+    // FP64 upward pass on the device (pgrav_moments.hip), set up on first use
+    MomentsView mv;
+    std::vector<int> lev_off;  // nodes of level L: lev_nodes[lev_off[L] .. lev_off[L+1])
+    void ensure_moments_view();
     int *a_leaf = nullptr, *a_parent = nullptr;
 
     template <typename P>
@@ -81,6 +85,39 @@ struct Plan {
 
 static double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Device copies of the tree structure and of the host's moments, so that a
+// later update recomputes the moments on the GPU from the moved bodies.
+void Plan::ensure_moments_view() {
+    if (mv.raw)
+        return;
+    const Tree &t = tree;
+    const int64_t nn = t.nnodes();
+    std::vector<int> cnt(t.count.begin(), t.count.end()), ch(t.child.begin(), t.child.end()),
+        nch(t.nchild.begin(), t.nchild.end());
+    const int NL = 23;
+    lev_off.assign(NL + 1, 0);
+    for (int64_t i = 0; i < nn; i++) lev_off[t.level[i] + 1]++;
+    for (int L = 0; L < NL; L++) lev_off[L + 1] += lev_off[L];
+    std::vector<int> lev_nodes(nn), fill(lev_off.begin(), lev_off.end() - 1);
+    for (int64_t i = 0; i < nn; i++) lev_nodes[fill[t.level[i]]++] = (int)i;
+    std::vector<double> c4((size_t)nn * 4);
+    for (int64_t i = 0; i < nn; i++) {
+        for (int q = 0; q < 3; q++) c4[4 * i + q] = t.cofm[3 * i + q];
+        c4[4 * i + 3] = t.raw[35 * (size_t)i];
+    }
+    mv.bpos = v.bpos;
+    mv.first = upload(t.first.data(), t.first.size());
+    mv.count = upload(cnt.data(), cnt.size());
+    mv.child = upload(ch.data(), ch.size());
+    mv.nchild = upload(nch.data(), nch.size());
+    mv.lev_nodes = upload(lev_nodes.data(), lev_nodes.size());
+    mv.cpos = (double4 *)v.cpos;
+    mv.bmax = upload(t.bmax.data(), t.bmax.size());
+    mv.raw = upload(t.raw.data(), t.raw.size());
+    mv.mom32 = (float *)v.mom32;
+    mv.mom64 = (double *)v.mom64;
 }
 
 static int auto_blocks() {
@@ -314,23 +351,31 @@ int pgrav_plan_update(void *plan, const double *pos3, const double *mass) {
         return -1;
     }
     try {
-        update_bodies(P->tree, pos3, mass, P->prm.periodic, P->prm.box, P->prm.nthreads);
-        const int64_t n = P->tree.n;
+        // the bodies move into their sorted slots on the host; the moments, centres,
+        // bmax and records are recomputed from them on the GPU in FP64
+        Tree &t = P->tree;
+        const int64_t n = t.n;
         std::vector<double> b4(4 * (size_t)n);
+        const bool per = P->prm.periodic != 0;
+        const double box = P->prm.box;
         for (int64_t i = 0; i < n; i++) {
-            for (int k = 0; k < 3; k++) b4[4 * i + k] = P->tree.pos[3 * i + k];
-            b4[4 * i + 3] = P->tree.mass[i];
+            const int64_t j = t.perm[i];
+            for (int k = 0; k < 3; k++) {
+                double x = pos3[3 * j + k];
+                if (per)
+                    x -= box * floor(x / box);
+                t.pos[3 * i + k] = x;
+                b4[4 * i + k] = x;
+            }
+            t.mass[i] = mass[j];
+            b4[4 * i + 3] = mass[j];
         }
-        std::vector<float> r32;
-        std::vector<double> r64, c4;
-        prepare_cells(P->tree, P->prm.order, r32, P->v.mom64 ? &r64 : nullptr, c4);
+        P->ensure_moments_view();
         if (!check(hipStreamSynchronize(P->stream), "hipStreamSynchronize") ||
             !check(hipMemcpy((void *)P->v.bpos, b4.data(), b4.size() * 8, hipMemcpyHostToDevice), "hipMemcpy") ||
-            !check(hipMemcpy((void *)P->v.cpos, c4.data(), c4.size() * 8, hipMemcpyHostToDevice), "hipMemcpy") ||
-            !check(hipMemcpy((void *)P->v.mom32, r32.data(), r32.size() * 4, hipMemcpyHostToDevice), "hipMemcpy"))
-            return -1;
-        if (P->v.mom64 &&
-            !check(hipMemcpy((void *)P->v.mom64, r64.data(), r64.size() * 8, hipMemcpyHostToDevice), "hipMemcpy"))
+            !check(launch_moments(P->mv, P->lev_off.data(), (int)P->lev_off.size() - 1, P->nnodes,
+                                  P->prm.order, P->stream), "moment kernels") ||
+            !check(hipStreamSynchronize(P->stream), "hipStreamSynchronize"))
             return -1;
     } catch (const std::exception &e) {
         fail("pgrav_plan_update: %s", e.what());
@@ -353,6 +398,7 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
     try {
         const int64_t n = P->tree.n, nn = P->nnodes;
         const Tree &t = P->tree;
+        P->ensure_moments_view();  // the device raw moments are the current ones
         if (!P->a_wv) {
             P->a_wv = P->dalloc<float4>(sizeof(float4) * n);
             P->a_gsink = P->dalloc<float4>(sizeof(float4) * n);
@@ -360,7 +406,6 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
             P->a_out = P->dalloc<double4>(sizeof(double4) * n);
             P->a_gcell = P->dalloc<float>(sizeof(float) * 52 * (size_t)nn);
             P->a_gnode = P->dalloc<float>(sizeof(float) * 40 * (size_t)nn);
-            P->a_raw = P->dalloc<double>(sizeof(double) * 35 * (size_t)nn);
             // raw moments -> prepared record is linear and centre-independent (R = 1)
             std::vector<float> mat(46 * 35);
             for (int a = 0; a < 35; a++) {
@@ -383,10 +428,9 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
         for (int64_t i = 0; i < n; i++)
             for (int k = 0; k < 4; k++) wv[4 * i + k] = (float)w4[4 * t.perm[i] + k];
         if (!check(hipMemcpyAsync(P->a_wv, wv.data(), wv.size() * 4, hipMemcpyHostToDevice, P->stream), "hipMemcpyAsync") ||
-            !check(hipMemcpyAsync(P->a_raw, t.raw.data(), t.raw.size() * 8, hipMemcpyHostToDevice, P->stream), "hipMemcpyAsync") ||
             !check(hipMemsetAsync(P->a_gcell, 0, sizeof(float) * 52 * (size_t)nn, P->stream), "hipMemsetAsync") ||
             !check(hipMemsetAsync(P->a_gbody, 0, sizeof(float4) * n, P->stream), "hipMemsetAsync") ||
-            !check(launch_adjoint(P->v, P->a_wv, P->a_gsink, P->a_gcell, P->a_gbody, P->a_raw,
+            !check(launch_adjoint(P->v, P->a_wv, P->a_gsink, P->a_gcell, P->a_gbody, P->mv.raw,
                                   P->a_mat, nn, P->a_gnode, P->a_leaf, P->a_parent, n, P->a_out,
                                   P->stream), "adjoint kernels"))
             return -1;
@@ -624,6 +668,30 @@ int pgrav_plan_device_buffers(void *plan, void **out32, void **out64, void **bpo
     if (bpos)
         *bpos = (void *)P->v.bpos;
     return 0;
+}
+
+int pgrav_plan_cells(void *plan, int64_t node0, int64_t count, double *cofm4, double *bmax,
+                     double *raw35) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P || node0 < 0 || count < 0 || node0 + count > P->nnodes) {
+        fail("pgrav_plan_cells: bad node range");
+        return -1;
+    }
+    try {
+        P->ensure_moments_view();
+        bool ok = check(hipStreamSynchronize(P->stream), "hipStreamSynchronize");
+        if (ok && cofm4)
+            ok = check(hipMemcpy(cofm4, P->mv.cpos + node0, sizeof(double4) * count, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (ok && bmax)
+            ok = check(hipMemcpy(bmax, P->mv.bmax + node0, sizeof(double) * count, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (ok && raw35)
+            ok = check(hipMemcpy(raw35, P->mv.raw + 35 * node0, sizeof(double) * 35 * count, hipMemcpyDeviceToHost), "hipMemcpy");
+        return ok ? 0 : -1;
+    } catch (const std::exception &e) {
+        fail("pgrav_plan_cells: %s", e.what());
+        return -1;
+    }
 }
 
 int pgrav_plan_permutation(void *plan, int64_t *perm) {

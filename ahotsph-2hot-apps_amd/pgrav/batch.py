@@ -50,6 +50,7 @@ register({
     "pgrav_plan_evaluate": (_i, [_vp, _i, _vp]),
     "pgrav_plan_set_launch": (_i, [_vp, _i]),
     "pgrav_plan_set_dispatch": (_i, [_vp, _i]),
+    "pgrav_plan_cells": (_i, [_vp, _i64, _i64, _vp, _vp, _vp]),
     "pgrav_plan_time": (_i, [_vp, _i, _i, _vp, ctypes.POINTER(_d)]),
     "pgrav_plan_graph_capture": (_i, [_vp, _i]),
     "pgrav_plan_graph_launch": (_i, [_vp, _vp]),
@@ -200,6 +201,17 @@ class Plan:
                                         ctypes.byref(bp))
         check()
         return o32.value, o64.value, bp.value
+
+    def cells(self, node0=0, count=None):
+        """Device FP64 (cofm4, bmax, raw35) of nodes node0 .. node0+count."""
+        count = self.stats()["nnodes"] - node0 if count is None else count
+        c4 = np.empty((count, 4), np.float64)
+        bm = np.empty(count, np.float64)
+        raw = np.empty((count, 35), np.float64)
+        lib().pgrav_plan_cells(self.h, node0, count, c4.ctypes.data, bm.ctypes.data,
+                               raw.ctypes.data)
+        check()
+        return c4, bm, raw
 
     def permutation(self):
         perm = np.empty(self.n, np.int64)

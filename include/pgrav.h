@@ -195,6 +195,10 @@ int pgrav_plan_result(void *plan, int variant, double *acc4);
 int pgrav_plan_direct(void *plan, const int64_t *sample, int ns, double *out4);
 /* Device pointers of the FP32 / FP64 results and the sorted FP64 bodies. */
 int pgrav_plan_device_buffers(void *plan, void **out32, void **out64, void **bpos);
+/* The device's FP64 centre of mass + mass, bmax and 35 raw moments of nodes
+ * [node0, node0 + count) (after pgrav_plan_update: as recomputed on the GPU). */
+int pgrav_plan_cells(void *plan, int64_t node0, int64_t count, double *cofm4, double *bmax,
+                     double *raw35);
 /* perm[i] = original index of sorted slot i. */
 int pgrav_plan_permutation(void *plan, int64_t *perm);
 
