@@ -277,9 +277,18 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
     }
 }
 
+// Minimum waves per SIMD asked of the register allocator: left to it, the FP32
+// periodic kernels keep the Ewald lookup's eight corners in flight with 120+
+// VGPRs (4 waves); capped at 6 waves they fit in 80 without spilling.
+template <typename T, bool PERIODIC>
+struct MinWaves {
+    static constexpr int value = (sizeof(T) == 4 && PERIODIC) ? 6 : 1;
+};
+
 // one workgroup per group, groups taken in longest-first order
 template <typename T, bool REL, bool COMP, int SOFT, bool PERIODIC>
-__global__ __launch_bounds__(64) void k_batch(BatchArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PERIODIC>::value)))
+void k_batch(BatchArgs A) {
     __shared__ V4<T> lds[64];
     eval_group<T, REL, COMP, SOFT, PERIODIC>(A, A.order_np[blockIdx.x], lds);
 }
@@ -297,7 +306,8 @@ __global__ __launch_bounds__(64) void k_batch(BatchArgs A) {
 // lists and records, and only pointers known to be distinct let the compiler
 // keep those reads on wave-uniform scalar loads.
 template <typename T, bool REL, bool COMP, int SOFT, bool PERIODIC>
-__global__ __launch_bounds__(64) void k_batch_persistent(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PERIODIC>::value)))
+void k_batch_persistent(
     BatchArgs A, const double4 *__restrict__ bpos, const double4 *__restrict__ cpos,
     const T *__restrict__ mom, const int64_t *__restrict__ gfirst, const int *__restrict__ gcount,
     const double4 *__restrict__ gcen, const int *__restrict__ lpt, const int *__restrict__ xoff,
