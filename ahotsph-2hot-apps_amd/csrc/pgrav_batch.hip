@@ -135,19 +135,6 @@ __device__ __forceinline__ V4<T> source_offset(double4 c, double4 gc, double box
     return V4<T>{T(dx), T(dy), T(dz), T(c.w)};
 }
 
-#define PGRAV_PRAGMA(x) _Pragma(#x)
-
-// The k-th entry of the current tile (k wave-uniform).
-__device__ __forceinline__ uint32_t entry(uint32_t e_lane, int k, const uint32_t *tile) {
-#ifndef PGRAV_IDX_FROM_MEMORY
-    (void)tile;
-    return (uint32_t)__builtin_amdgcn_readlane((int)e_lane, k);
-#else
-    (void)e_lane;
-    return tile[k];
-#endif
-}
-
 template <typename T, bool REL, bool COMP, int SOFT, bool PERIODIC>
 __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds) {
     const int lane = threadIdx.x;
@@ -182,37 +169,16 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
         const int64_t o0 = offk[g], o1 = offk[g + 1];
         for (int64_t t0 = o0; t0 < o1; t0 += 64) {
             const int nt = (int)min((int64_t)64, o1 - t0);
-            // lane j holds the tile's j-th entry; the sweep reads it back with
-            // v_readlane, so a cell's record address needs no memory round trip
-            uint32_t e_lane = 0;
             if (lane < nt) {
-                e_lane = idx[t0 + lane];
-                const double4 src = kind == 3 ? A.bpos[e_lane] : A.cpos[e_lane];
+                const uint32_t e = idx[t0 + lane];
+                const double4 src = kind == 3 ? A.bpos[e] : A.cpos[e];
                 lds[lane] = source_offset<T, REL, PERIODIC>(src, gc, A.box);
             }
             __syncthreads();
             Acc4T<T> p = {0, 0, 0, 0};
             if (kind == 0) {
-                int k0 = 0;
-#ifdef PGRAV_HEXA_PAIR
-                // two cells per trip: independent chains for the scheduler
-                for (; !PERIODIC && k0 + 1 < nt; k0 += 2) {
-                    const uint32_t ea = entry(e_lane, k0, idx + t0);
-                    const uint32_t eb = entry(e_lane, k0 + 1, idx + t0);
-                    const V4<T> ra = lds[k0], rb = lds[k0 + 1];
-                    Acc4T<T> q = {0, 0, 0, 0};
-                    cell_interact<4, false, 1, T>(mom + (size_t)ea * PREC4, ra.w, sx - ra.x, sy - ra.y,
-                                                  sz - ra.z, eps2_cell, p);
-                    cell_interact<4, false, 1, T>(mom + (size_t)eb * PREC4, rb.w, sx - rb.x, sy - rb.y,
-                                                  sz - rb.z, eps2_cell, q);
-                    p.x += q.x;
-                    p.y += q.y;
-                    p.z += q.z;
-                    p.w += q.w;
-                }
-#endif
-                for (int k = k0; k < nt; k++) {
-                    const uint32_t e = entry(e_lane, k, idx + t0);
+                for (int k = 0; k < nt; k++) {
+                    const uint32_t e = idx[t0 + k];
                     const V4<T> r = lds[k];
                     const T x = pbc<PERIODIC>(sx - r.x, box, ibox);
                     const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
@@ -223,7 +189,7 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                 }
             } else if (kind == 1) {
                 for (int k = 0; k < nt; k++) {
-                    const uint32_t e = entry(e_lane, k, idx + t0);
+                    const uint32_t e = idx[t0 + k];
                     const V4<T> r = lds[k];
                     const T x = pbc<PERIODIC>(sx - r.x, box, ibox);
                     const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
