@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <chrono>
+#include <cmath>
 #include <new>
 #include <stdexcept>
 #include <vector>
@@ -358,6 +359,14 @@ int pgrav_plan_update(void *plan, const double *pos3, const double *mass) {
         std::vector<double> b4(4 * (size_t)n);
         const bool per = P->prm.periodic != 0;
         const double box = P->prm.box;
+        for (int64_t j = 0; j < n; j++) {
+            if (!(mass[j] > 0) || !std::isfinite(mass[j]) || !std::isfinite(pos3[3 * j]) ||
+                !std::isfinite(pos3[3 * j + 1]) || !std::isfinite(pos3[3 * j + 2])) {
+                fail("pgrav_plan_update: body %lld has a non-positive mass or a non-finite value",
+                     (long long)j);
+                return -1;
+            }
+        }
         for (int64_t i = 0; i < n; i++) {
             const int64_t j = t.perm[i];
             for (int k = 0; k < 3; k++) {

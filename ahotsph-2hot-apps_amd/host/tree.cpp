@@ -14,7 +14,9 @@
 #include <algorithm>
 #include <atomic>
 #include <functional>
+#include <cmath>
 #include <stdexcept>
+#include <string>
 #include <thread>
 
 #include "pgrav_host.h"
@@ -180,6 +182,15 @@ void build_tree(const double *pos, const double *mass, int64_t n, const TreePara
     t.n = n;
     if (n <= 0)
         throw std::runtime_error("build_tree: no bodies");
+    // every mass positive and finite, every coordinate finite (the reference
+    // aborts on a bad particle mass, nln/mac.c:1838-1843)
+    for (int64_t i = 0; i < n; i++) {
+        if (!(mass[i] > 0) || !std::isfinite(mass[i]))
+            throw std::runtime_error("body " + std::to_string(i) + ": mass must be positive and finite");
+        for (int a = 0; a < 3; a++)
+            if (!std::isfinite(pos[3 * i + a]))
+                throw std::runtime_error("body " + std::to_string(i) + ": position is not finite");
+    }
     // bounding cube
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     if (p.periodic) {
