@@ -6,14 +6,18 @@
 // accepted cell's centre of mass X_c and raw moments M_c, which are functions
 // of the cell's bodies.  Three kernels:
 //
-//  k_adjoint        one wave per sink group, the forward's tiling.  Per pair:
-//                   the sink-position gradient by forward-mode dual numbers
-//                   through the very same cell_interact/mono_interact code;
-//                   for cells the 46 prepared-record coefficients' gradient
-//                   dL/dP (the interaction is linear in P, so this is the
-//                   hand-written transpose of the evaluation) and dL/dX_c;
-//                   for bodies dL/dx_j and dL/dm_j.  Source-side terms are
-//                   summed over the wave's sinks and added atomically.
+//  k_adjoint        one wave per sink group, the forward's tiling (lane = sink).
+//                   Per pair: the sink-position gradient by forward-mode dual
+//                   numbers through the very same cell_interact/mono_interact
+//                   code; for cells also the pair's share of dL/dX_c, summed
+//                   over the wave's sinks and added atomically.
+//  k_adjoint_sources one wave per sink group, lane = source (cell or body) of a
+//                   list tile, looping over the group's sinks: for cells the 43
+//                   prepared-record coefficients' gradient dL/dP (the
+//                   interaction is linear in P, so this is the hand-written
+//                   transpose of the evaluation); for bodies dL/dx_j and
+//                   dL/dm_j.  Lane-private sums, one atomic add per term and
+//                   tile.
 //  k_adjoint_nodes  per cell: dL/dP -> dL/d(raw moments) through the fixed
 //                   linear map raw -> record -> prepared record, and the total
 //                   dL/dX_c including the centre dependence of the moments.
@@ -76,6 +80,7 @@ __device__ __forceinline__ void cell_beta(float x, float y, float z, float eps2,
     const float rc = 2.0f * ri2, rb = 1.5f * ri2, rq = (4.0f / 3.0f) * ri2;
     const float at = ra * wdx * rc;        // dL/dT'
     const float avx = -ra * wx_, avy = -ra * wy_, avz = -ra * wz_;  // dL/dV
+#pragma unroll
     for (int k = 0; k < 46; k++) beta[k] = 0.0f;
     beta[3] = as;
     if (ORDER == 0)
@@ -171,40 +176,15 @@ __global__ __launch_bounds__(64) void k_adjoint(BatchView V, const float4 *__res
                 gy += py;
                 gz += pz;
                 if (kind < 3) {
-                    float beta[46];
-                    if (kind == 0)
-                        cell_beta<4>(X, Y, Z, eps2c, w.x, w.y, w.z, w.w, beta);
-                    else if (kind == 1)
-                        cell_beta<2>(X, Y, Z, eps2c, w.x, w.y, w.z, w.w, beta);
-                    else
-                        cell_beta<0>(X, Y, Z, eps2c, w.x, w.y, w.z, w.w, beta);
-                    const int nb = kind == 0 ? 46 : (kind == 1 ? 10 : 4);
-                    float *gcp = gcell + (size_t)e * ADJ_CELL;
-                    for (int q = 3; q < nb; q++) {
-                        const float s = wave_sum(beta[q]);
-                        if (lane == 0)
-                            atomicAdd(gcp + q, s);
-                    }
+                    // the pair's share of dL/dX_c is minus the sink's; summed over
+                    // the wave's sinks (the record gradient dL/dP is accumulated
+                    // per cell by k_adjoint_sources without cross-lane sums)
                     const float sx_ = wave_sum(-px), sy_ = wave_sum(-py), sz_ = wave_sum(-pz);
                     if (lane == 0) {
+                        float *gcp = gcell + (size_t)e * ADJ_CELL;
                         atomicAdd(gcp + 46, sx_);
                         atomicAdd(gcp + 47, sy_);
                         atomicAdd(gcp + 48, sz_);
-                    }
-                } else {
-                    // source body: position gradient is minus the sink's; the mass
-                    // gradient is the pair's (w.a + v phi) per unit source mass
-                    const float lm = r.w != 0.f
-                                         ? (w.x * a.x.v + w.y * a.y.v + w.z * a.z.v + w.w * a.w.v) / r.w
-                                         : 0.f;
-                    const float sx_ = wave_sum(-px), sy_ = wave_sum(-py), sz_ = wave_sum(-pz);
-                    const float sm = wave_sum(lm);
-                    if (lane == 0) {
-                        float *gb = (float *)(gbody + e);
-                        atomicAdd(gb + 0, sx_);
-                        atomicAdd(gb + 1, sy_);
-                        atomicAdd(gb + 2, sz_);
-                        atomicAdd(gb + 3, sm);
                     }
                 }
             }
@@ -213,6 +193,98 @@ __global__ __launch_bounds__(64) void k_adjoint(BatchView V, const float4 *__res
     }
     if (live)
         gsink[me] = make_float4(gx, gy, gz, 0.f);
+}
+
+// One cell's dL/dP over the group's sinks (sink offsets sk, weights sw in LDS).
+template <int ORDER>
+__device__ __forceinline__ void cell_source(const float4 *sk, const float4 *sw, int cnt, float rx,
+                                            float ry, float rz, float eps2c, float *gcp) {
+    constexpr int NB = ORDER == 4 ? 46 : (ORDER == 2 ? 10 : 4);
+    float acc[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++) acc[q] = 0.f;
+    for (int i = 0; i < cnt; i++) {
+        const float4 x = sk[i], w = sw[i];
+        float beta[46];
+        cell_beta<ORDER>(x.x - rx, x.y - ry, x.z - rz, eps2c, w.x, w.y, w.z, w.w, beta);
+#pragma unroll
+        for (int q = 3; q < NB; q++) acc[q] += beta[q];
+    }
+#pragma unroll
+    for (int q = 3; q < NB; q++) atomicAdd(gcp + q, acc[q]);
+}
+
+// Source-major pass: one wave per sink group, lane j = the j-th source of a
+// list tile, looping over the group's sinks (broadcast from LDS).  Each lane
+// accumulates its own source's terms over all the group's sinks in registers
+// and adds them once per tile: for a cell the 43 record coefficients dL/dP
+// (cell_beta), for a body its pair position gradient (minus the sink-side one,
+// by the same dual numbers) and dL/dm_j.
+template <int SOFT>
+__global__ __launch_bounds__(64) void k_adjoint_sources(BatchView V, const float4 *__restrict__ wv,
+                                                        float *__restrict__ gcell,
+                                                        float4 *__restrict__ gbody) {
+    __shared__ float4 sk[64], sw[64];
+    const int g = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t first = V.gfirst[g];
+    const int cnt = V.gcount[g];
+    const double4 gc = V.gcen[g];
+    if (lane < cnt) {
+        const double4 sp = V.bpos[first + lane];
+        sk[lane] = make_float4((float)(sp.x - gc.x), (float)(sp.y - gc.y), (float)(sp.z - gc.z), 0.f);
+        sw[lane] = wv[first + lane];
+    }
+    __syncthreads();
+    const float eps = (float)V.eps, eps2 = eps * eps;
+    const float eps_inv = SOFT == SOFT_K1 ? 1.0f / eps : 0.0f;
+    const float eps2c = SOFT == SOFT_PLUMMER ? eps2 : 0.0f;
+    for (int kind = 0; kind < 4; kind++) {
+        const int64_t *offk = V.off + (int64_t)kind * (V.ngroups + 1);
+        const int64_t o0 = offk[g], o1 = offk[g + 1];
+        for (int64_t t0 = o0; t0 < o1; t0 += 64) {
+            const int nt = (int)min((int64_t)64, o1 - t0);
+            if (lane >= nt)
+                continue;
+            const uint32_t e = V.idx[t0 + lane];
+            const double4 s = kind == 3 ? V.bpos[e] : V.cpos[e];
+            const float rx = (float)(s.x - gc.x), ry = (float)(s.y - gc.y), rz = (float)(s.z - gc.z);
+            const float rm = (float)s.w;
+            if (kind == 0) {
+                cell_source<4>(sk, sw, cnt, rx, ry, rz, eps2c, gcell + (size_t)e * ADJ_CELL);
+            } else if (kind == 1) {
+                cell_source<2>(sk, sw, cnt, rx, ry, rz, eps2c, gcell + (size_t)e * ADJ_CELL);
+            } else if (kind == 2) {
+                cell_source<0>(sk, sw, cnt, rx, ry, rz, eps2c, gcell + (size_t)e * ADJ_CELL);
+            } else {
+                float bx = 0.f, by = 0.f, bz = 0.f, bm = 0.f;
+                for (int i = 0; i < cnt; i++) {
+                    const float4 x = sk[i], w = sw[i];
+                    const float X = x.x - rx, Y = x.y - ry, Z = x.z - rz;
+                    const D3 dx(X, 1.f, 0.f, 0.f), dy(Y, 0.f, 1.f, 0.f), dz(Z, 0.f, 0.f, 1.f);
+                    Acc4T<D3> a = {D3(0.f), D3(0.f), D3(0.f), D3(0.f)};
+                    const bool self = X == 0.f && Y == 0.f && Z == 0.f;
+                    if (SOFT == SOFT_K1)
+                        k1_interact<D3>(D3(rm), dx, dy, dz, D3(eps_inv), D3(eps2), a);
+                    else if (SOFT == SOFT_PLUMMER)
+                        mono_interact<D3>(D3(rm), dx, dy, dz, D3(eps2), a);
+                    else if (!self)
+                        mono_interact<D3>(D3(rm), dx, dy, dz, D3(0.f), a);
+                    bx -= w.x * a.x.x + w.y * a.y.x + w.z * a.z.x + w.w * a.w.x;
+                    by -= w.x * a.x.y + w.y * a.y.y + w.z * a.z.y + w.w * a.w.y;
+                    bz -= w.x * a.x.z + w.y * a.y.z + w.z * a.z.z + w.w * a.w.z;
+                    // the pair is linear in the source mass
+                    if (rm != 0.f)
+                        bm += (w.x * a.x.v + w.y * a.y.v + w.z * a.z.v + w.w * a.w.v) / rm;
+                }
+                float *gb = (float *)(gbody + e);
+                atomicAdd(gb + 0, bx);
+                atomicAdd(gb + 1, by);
+                atomicAdd(gb + 2, bz);
+                atomicAdd(gb + 3, bm);
+            }
+        }
+    }
 }
 
 // per node: dL/d(raw moments) = Mat^T dL/dP and the total dL/dX_c.  Mat (46 x 35)
@@ -310,12 +382,16 @@ hipError_t launch_adjoint(const BatchView &v, const float4 *wv, float4 *gsink, f
                           double4 *out, hipStream_t st) {
     if (v.ngroups <= 0)
         return hipSuccess;
-    if (v.soft == SOFT_K1)
+    if (v.soft == SOFT_K1) {
         hipLaunchKernelGGL(k_adjoint<SOFT_K1>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gsink, gcell, gbody);
-    else if (v.soft == SOFT_PLUMMER)
+        hipLaunchKernelGGL(k_adjoint_sources<SOFT_K1>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gcell, gbody);
+    } else if (v.soft == SOFT_PLUMMER) {
         hipLaunchKernelGGL(k_adjoint<SOFT_PLUMMER>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gsink, gcell, gbody);
-    else
+        hipLaunchKernelGGL(k_adjoint_sources<SOFT_PLUMMER>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gcell, gbody);
+    } else {
         hipLaunchKernelGGL(k_adjoint<SOFT_NONE>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gsink, gcell, gbody);
+        hipLaunchKernelGGL(k_adjoint_sources<SOFT_NONE>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gcell, gbody);
+    }
     hipLaunchKernelGGL(k_adjoint_nodes, dim3((unsigned)((nnodes + 127) / 128)), dim3(128), 0, st,
                        gcell, raw, mat, nnodes, gnode);
     hipLaunchKernelGGL(k_adjoint_bodies, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st,
