@@ -63,35 +63,34 @@ __device__ __forceinline__ T pbc(T x, T box, T ibox) {
     return x;
 }
 
-// ---- Ewald correction (periodic): trilinear lookup in the FP32 copy of the
-// FP64 table built by k_ewald_table.  x = sink - source, already the nearest
-// image (|x_i| <= L/2), so the Newtonian term the caller added and this
-// correction refer to the same image.
+// ---- Ewald correction (periodic).  The table holds, at the nodes of a
+// EWALD_N^3 grid over the positive octant [0, L/2]^3, the correction
+// T = (C(r), psi(r)) -- C = f_ewald(r) - r/|r|^3, psi = phi_ewald(r) + 1/|r| --
+// and its three partial derivatives (4 x TV per node, built in FP64 by
+// k_ewald_table).  The lookup takes the nearest node and its first-order
+// Taylor step: one 64-byte record, 12 FMAs, error O(h^2 |T''|) as trilinear
+// interpolation over 8 nodes would have.  x = sink - source, already the
+// nearest image (|x_i| <= L/2), so the Newtonian term the caller added and
+// this correction refer to the same image.
 template <typename T, typename TV>
 __device__ __forceinline__ void ewald_corr(const TV *__restrict__ tab, T m, T x, T y, T z,
                                            T box, Acc4T<T> &a) {
-    const T half = T(0.5) * box;
-    const T inv_h = T(EWALD_N - 1) / half;
-    T sx = x < 0 ? T(-1) : T(1), sy = y < 0 ? T(-1) : T(1), sz = z < 0 ? T(-1) : T(1);
-    T ax = fabs(x), ay = fabs(y), az = fabs(z);
-    T u = ax * inv_h, v = ay * inv_h, w = az * inv_h;
+    const T h = T(0.5) * box / T(EWALD_N - 1);
+    const T inv_h = T(1) / h;
+    const T sx = x < 0 ? T(-1) : T(1), sy = y < 0 ? T(-1) : T(1), sz = z < 0 ? T(-1) : T(1);
+    const T ax = fabs(x), ay = fabs(y), az = fabs(z);
     // clamp both ways: a NaN offset must not turn into an out-of-range index
-    int i = max(0, min((int)u, EWALD_N - 2)), j = max(0, min((int)v, EWALD_N - 2)),
-        k = max(0, min((int)w, EWALD_N - 2));
-    T fu = u - T(i), fv = v - T(j), fw = w - T(k);
-    const TV *p = tab + ((size_t)i * EWALD_N + j) * EWALD_N + k;
-    const TV c000 = p[0], c001 = p[1], c010 = p[EWALD_N], c011 = p[EWALD_N + 1];
-    const TV c100 = p[EWALD_N * EWALD_N], c101 = p[EWALD_N * EWALD_N + 1];
-    const TV c110 = p[EWALD_N * EWALD_N + EWALD_N], c111 = p[EWALD_N * EWALD_N + EWALD_N + 1];
-#define LERP3(f)                                                                              \
-    ((T(1) - fu) * ((T(1) - fv) * ((T(1) - fw) * T(c000.f) + fw * T(c001.f)) +               \
-                    fv * ((T(1) - fw) * T(c010.f) + fw * T(c011.f))) +                         \
-     fu * ((T(1) - fv) * ((T(1) - fw) * T(c100.f) + fw * T(c101.f)) +                          \
-           fv * ((T(1) - fw) * T(c110.f) + fw * T(c111.f))))
-    const T cx = LERP3(x), cy = LERP3(y), cz = LERP3(z), cp = LERP3(w);
-#undef LERP3
-    // the table holds C(r) = f_ewald(r) - r/|r|^3 on the positive octant; C is odd
-    // per axis, and the pull on the sink is m C(r) at r = source - sink = -x
+    const int i = max(0, min((int)rint(ax * inv_h), EWALD_N - 1));
+    const int j = max(0, min((int)rint(ay * inv_h), EWALD_N - 1));
+    const int k = max(0, min((int)rint(az * inv_h), EWALD_N - 1));
+    const T du = ax - T(i) * h, dv = ay - T(j) * h, dw = az - T(k) * h;
+    const TV *p = tab + 4 * (((size_t)i * EWALD_N + j) * EWALD_N + k);
+    const TV t0 = p[0], gu = p[1], gv = p[2], gw = p[3];
+    const T cx = T(t0.x) + du * T(gu.x) + dv * T(gv.x) + dw * T(gw.x);
+    const T cy = T(t0.y) + du * T(gu.y) + dv * T(gv.y) + dw * T(gw.y);
+    const T cz = T(t0.z) + du * T(gu.z) + dv * T(gv.z) + dw * T(gw.z);
+    const T cp = T(t0.w) + du * T(gu.w) + dv * T(gv.w) + dw * T(gw.w);
+    // C is odd per axis; the pull on the sink is m C(r) at r = source - sink = -x
     a.x -= m * sx * cx;
     a.y -= m * sy * cy;
     a.z -= m * sz * cz;
@@ -244,11 +243,11 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
 }
 
 // Minimum waves per SIMD asked of the register allocator: left to it, the FP32
-// periodic kernels keep the Ewald lookup's eight corners in flight with 120+
-// VGPRs (4 waves); capped at 6 waves they fit in 80 without spilling.
+// periodic kernels keep the Ewald records in flight with ~80 VGPRs (6 waves);
+// asked for 8 they fit in 64 (the K1 persistent variant spills 4 registers).
 template <typename T, bool PERIODIC>
 struct MinWaves {
-    static constexpr int value = (sizeof(T) == 4 && PERIODIC) ? 6 : 1;
+    static constexpr int value = (sizeof(T) == 4 && PERIODIC) ? 8 : 1;
 };
 
 // one workgroup per group, groups taken in longest-first order
@@ -393,26 +392,17 @@ __global__ void k_direct_reduce(const double4 *__restrict__ part, int ns, int ns
 }
 
 // ---------------------------------------------------------------- Ewald table
-// FP64 restatement of nln/ewald.c:13-64 per table point r = (i, j, k) h,
-// h = (L/2)/(EWALD_N-1); stores C(r) = f_ewald(r) - r/|r|^3 and
-// phi_ewald(r) + 1/|r| (at r = 0 the limits 0 and 2 x 1.4186487533 / L).
-__global__ void k_ewald_table(double L, double4 *__restrict__ t64, float4 *__restrict__ t32) {
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= EWALD_N * EWALD_N * EWALD_N)
-        return;
-    const int i = id / (EWALD_N * EWALD_N), j = (id / EWALD_N) % EWALD_N, k = id % EWALD_N;
-    const double h = 0.5 * L / (EWALD_N - 1);
-    const double x[3] = {i * h, j * h, k * h};
+// FP64 restatement of nln/ewald.c:13-64 at one point r: C(r) = f_ewald(r) - r/|r|^3
+// and psi(r) = phi_ewald(r) + 1/|r| (at r = 0 the limits 0 and 2 x 1.4186487533 / L).
+__device__ double4 ewald_point(double x0, double x1, double x2, double L) {
+    const double x[3] = {x0, x1, x2};
     double f[3] = {0, 0, 0}, phi;
     const int NN = 4;
     const double alpha = 2.0 / L;
     phi = M_PI / (alpha * alpha * L * L * L);
     const double r2x = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
-    if (r2x == 0.0) {
-        t64[id] = make_double4(0, 0, 0, 2.0 * 1.4186487533 / L);
-        t32[id] = make_float4(0.f, 0.f, 0.f, (float)(2.0 * 1.4186487533 / L));
-        return;
-    }
+    if (r2x == 0.0)
+        return make_double4(0, 0, 0, 2.0 * 1.4186487533 / L);
     const double a1 = 2.0 * alpha / sqrt(M_PI);
     const double a2 = -M_PI * M_PI / (alpha * alpha * L * L);
     const double a3 = 2.0 * M_PI / L;
@@ -449,9 +439,31 @@ __global__ void k_ewald_table(double L, double4 *__restrict__ t64, float4 *__res
                 phi -= a5 * e * cos(a3 * hx) / hh;
             }
     const double r = sqrt(r2x), r3 = r * r2x;
-    const double4 c = make_double4(f[0] - x[0] / r3, f[1] - x[1] / r3, f[2] - x[2] / r3, phi + 1.0 / r);
-    t64[id] = c;
-    t32[id] = make_float4((float)c.x, (float)c.y, (float)c.z, (float)c.w);
+    return make_double4(f[0] - x[0] / r3, f[1] - x[1] / r3, f[2] - x[2] / r3, phi + 1.0 / r);
+}
+
+// Table node (i, j, k) at r = (i, j, k) h, h = (L/2)/(EWALD_N-1): the correction
+// and its partial derivatives by FP64 central differences of step h/64 (T is
+// smooth there; the removed Newtonian singularity cancels inside C).
+__global__ void k_ewald_table(double L, double4 *__restrict__ t64, float4 *__restrict__ t32) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= EWALD_N * EWALD_N * EWALD_N)
+        return;
+    const int i = id / (EWALD_N * EWALD_N), j = (id / EWALD_N) % EWALD_N, k = id % EWALD_N;
+    const double h = 0.5 * L / (EWALD_N - 1);
+    const double x = i * h, y = j * h, z = k * h, d = h / 64;
+    double4 out[4];
+    out[0] = ewald_point(x, y, z, L);
+    for (int b = 0; b < 3; b++) {
+        const double4 cp = ewald_point(x + (b == 0) * d, y + (b == 1) * d, z + (b == 2) * d, L);
+        const double4 cm = ewald_point(x - (b == 0) * d, y - (b == 1) * d, z - (b == 2) * d, L);
+        const double s = 1.0 / (2.0 * d);
+        out[1 + b] = make_double4((cp.x - cm.x) * s, (cp.y - cm.y) * s, (cp.z - cm.z) * s, (cp.w - cm.w) * s);
+    }
+    for (int c = 0; c < 4; c++) {
+        t64[4 * (size_t)id + c] = out[c];
+        t32[4 * (size_t)id + c] = make_float4((float)out[c].x, (float)out[c].y, (float)out[c].z, (float)out[c].w);
+    }
 }
 
 // ---------------------------------------------------------------- launchers

@@ -322,7 +322,7 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         P->v.periodic = p->periodic;
         P->v.box = p->box;
         if (p->periodic) {
-            const size_t ne = (size_t)EWALD_N * EWALD_N * EWALD_N;
+            const size_t ne = 4 * (size_t)EWALD_N * EWALD_N * EWALD_N;  // value + 3 derivatives
             double4 *t64 = P->dalloc<double4>(ne * sizeof(double4));
             float4 *t32 = P->dalloc<float4>(ne * sizeof(float4));
             if (!check(launch_ewald_table(p->box, t64, t32, P->stream), "Ewald table kernel"))
