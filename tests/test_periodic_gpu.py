@@ -69,7 +69,8 @@ def test_momentum_is_nearly_conserved(gpu_lib):
 
 def test_periodic_direct_sum_matches_exact_ewald(gpu_lib):
     """The GPU FP64 direct sum of a periodic plan (nearest image + the FP64
-    Ewald table, trilinear) against the oracle's exact lattice sum per pair:
+    Ewald table: nearest node plus its derivatives) against the oracle's exact
+    lattice sum per pair:
     what remains is the table's interpolation error."""
     from pgrav import batch, ics
     pos, mass = ics.lattice_grf(12, seed=11, rms=0.3)
