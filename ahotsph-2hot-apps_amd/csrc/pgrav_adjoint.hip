@@ -11,13 +11,15 @@
 //                   numbers through the very same cell_interact/mono_interact
 //                   code; for cells also the pair's share of dL/dX_c, summed
 //                   over the wave's sinks and added atomically.
-//  k_adjoint_sources one wave per sink group, lane = source (cell or body) of a
-//                   list tile, looping over the group's sinks: for cells the 43
-//                   prepared-record coefficients' gradient dL/dP (the
-//                   interaction is linear in P, so this is the hand-written
-//                   transpose of the evaluation); for bodies dL/dx_j and
-//                   dL/dm_j.  Lane-private sums, one atomic add per term and
-//                   tile.
+//  k_adjoint_bodysrc one wave per sink group, lane = body of a body-list tile,
+//                   looping over the group's sinks: dL/dx_j and dL/dm_j of the
+//                   source bodies, lane-private sums, one atomic add per term
+//                   and tile.
+//  k_adjoint_cells  one wave per listed cell over the transposed lists (the
+//                   groups that list it): the 43 prepared-record coefficients'
+//                   gradient dL/dP (the interaction is linear in P, so this is
+//                   the hand-written transpose of the evaluation), summed in
+//                   registers and once across the wave; no atomics.
 //  k_adjoint_nodes  per cell: dL/dP -> dL/d(raw moments) through the fixed
 //                   linear map raw -> record -> prepared record, and the total
 //                   dL/dX_c including the centre dependence of the moments.
@@ -63,6 +65,7 @@ __device__ __forceinline__ float wave_sum(float v) {
 }
 
 #define ADJ_CELL 52  // per node: 46 dL/dP, 3 dL/dX (pair part), pad
+#define L_BODY_KIND 3  // list kinds 0..2 are cells (orders 4, 2, 0), 3 bodies
 
 // dL/dP for one (sink, cell) pair: the evaluation in cell_interact is linear in
 // the prepared record P; with w, v the sink's weights these are the
@@ -178,7 +181,7 @@ __global__ __launch_bounds__(64) void k_adjoint(BatchView V, const float4 *__res
                 if (kind < 3) {
                     // the pair's share of dL/dX_c is minus the sink's; summed over
                     // the wave's sinks (the record gradient dL/dP is accumulated
-                    // per cell by k_adjoint_sources without cross-lane sums)
+                    // per cell by k_adjoint_cells over the transposed lists)
                     const float sx_ = wave_sum(-px), sy_ = wave_sum(-py), sz_ = wave_sum(-pz);
                     if (lane == 0) {
                         float *gcp = gcell + (size_t)e * ADJ_CELL;
@@ -195,34 +198,13 @@ __global__ __launch_bounds__(64) void k_adjoint(BatchView V, const float4 *__res
         gsink[me] = make_float4(gx, gy, gz, 0.f);
 }
 
-// One cell's dL/dP over the group's sinks (sink offsets sk, weights sw in LDS).
-template <int ORDER>
-__device__ __forceinline__ void cell_source(const float4 *sk, const float4 *sw, int cnt, float rx,
-                                            float ry, float rz, float eps2c, float *gcp) {
-    constexpr int NB = ORDER == 4 ? 46 : (ORDER == 2 ? 10 : 4);
-    float acc[NB];
-#pragma unroll
-    for (int q = 0; q < NB; q++) acc[q] = 0.f;
-    for (int i = 0; i < cnt; i++) {
-        const float4 x = sk[i], w = sw[i];
-        float beta[46];
-        cell_beta<ORDER>(x.x - rx, x.y - ry, x.z - rz, eps2c, w.x, w.y, w.z, w.w, beta);
-#pragma unroll
-        for (int q = 3; q < NB; q++) acc[q] += beta[q];
-    }
-#pragma unroll
-    for (int q = 3; q < NB; q++) atomicAdd(gcp + q, acc[q]);
-}
-
-// Source-major pass: one wave per sink group, lane j = the j-th source of a
-// list tile, looping over the group's sinks (broadcast from LDS).  Each lane
-// accumulates its own source's terms over all the group's sinks in registers
-// and adds them once per tile: for a cell the 43 record coefficients dL/dP
-// (cell_beta), for a body its pair position gradient (minus the sink-side one,
-// by the same dual numbers) and dL/dm_j.
+// Body sources, source-major: one wave per sink group, lane j = the j-th body
+// of a body-list tile, looping over the group's sinks (broadcast from LDS).
+// Each lane sums its body's pair position gradient (minus the sink-side one,
+// by the same dual numbers) and dL/dm_j over the group's sinks in registers
+// and adds them once per tile.
 template <int SOFT>
-__global__ __launch_bounds__(64) void k_adjoint_sources(BatchView V, const float4 *__restrict__ wv,
-                                                        float *__restrict__ gcell,
+__global__ __launch_bounds__(64) void k_adjoint_bodysrc(BatchView V, const float4 *__restrict__ wv,
                                                         float4 *__restrict__ gbody) {
     __shared__ float4 sk[64], sw[64];
     const int g = blockIdx.x;
@@ -238,52 +220,108 @@ __global__ __launch_bounds__(64) void k_adjoint_sources(BatchView V, const float
     __syncthreads();
     const float eps = (float)V.eps, eps2 = eps * eps;
     const float eps_inv = SOFT == SOFT_K1 ? 1.0f / eps : 0.0f;
-    const float eps2c = SOFT == SOFT_PLUMMER ? eps2 : 0.0f;
-    for (int kind = 0; kind < 4; kind++) {
-        const int64_t *offk = V.off + (int64_t)kind * (V.ngroups + 1);
-        const int64_t o0 = offk[g], o1 = offk[g + 1];
-        for (int64_t t0 = o0; t0 < o1; t0 += 64) {
-            const int nt = (int)min((int64_t)64, o1 - t0);
-            if (lane >= nt)
-                continue;
-            const uint32_t e = V.idx[t0 + lane];
-            const double4 s = kind == 3 ? V.bpos[e] : V.cpos[e];
-            const float rx = (float)(s.x - gc.x), ry = (float)(s.y - gc.y), rz = (float)(s.z - gc.z);
-            const float rm = (float)s.w;
-            if (kind == 0) {
-                cell_source<4>(sk, sw, cnt, rx, ry, rz, eps2c, gcell + (size_t)e * ADJ_CELL);
-            } else if (kind == 1) {
-                cell_source<2>(sk, sw, cnt, rx, ry, rz, eps2c, gcell + (size_t)e * ADJ_CELL);
-            } else if (kind == 2) {
-                cell_source<0>(sk, sw, cnt, rx, ry, rz, eps2c, gcell + (size_t)e * ADJ_CELL);
-            } else {
-                float bx = 0.f, by = 0.f, bz = 0.f, bm = 0.f;
-                for (int i = 0; i < cnt; i++) {
-                    const float4 x = sk[i], w = sw[i];
-                    const float X = x.x - rx, Y = x.y - ry, Z = x.z - rz;
-                    const D3 dx(X, 1.f, 0.f, 0.f), dy(Y, 0.f, 1.f, 0.f), dz(Z, 0.f, 0.f, 1.f);
-                    Acc4T<D3> a = {D3(0.f), D3(0.f), D3(0.f), D3(0.f)};
-                    const bool self = X == 0.f && Y == 0.f && Z == 0.f;
-                    if (SOFT == SOFT_K1)
-                        k1_interact<D3>(D3(rm), dx, dy, dz, D3(eps_inv), D3(eps2), a);
-                    else if (SOFT == SOFT_PLUMMER)
-                        mono_interact<D3>(D3(rm), dx, dy, dz, D3(eps2), a);
-                    else if (!self)
-                        mono_interact<D3>(D3(rm), dx, dy, dz, D3(0.f), a);
-                    bx -= w.x * a.x.x + w.y * a.y.x + w.z * a.z.x + w.w * a.w.x;
-                    by -= w.x * a.x.y + w.y * a.y.y + w.z * a.z.y + w.w * a.w.y;
-                    bz -= w.x * a.x.z + w.y * a.y.z + w.z * a.z.z + w.w * a.w.z;
-                    // the pair is linear in the source mass
-                    if (rm != 0.f)
-                        bm += (w.x * a.x.v + w.y * a.y.v + w.z * a.z.v + w.w * a.w.v) / rm;
-                }
-                float *gb = (float *)(gbody + e);
-                atomicAdd(gb + 0, bx);
-                atomicAdd(gb + 1, by);
-                atomicAdd(gb + 2, bz);
-                atomicAdd(gb + 3, bm);
-            }
+    const int64_t *offk = V.off + (int64_t)L_BODY_KIND * (V.ngroups + 1);
+    const int64_t o0 = offk[g], o1 = offk[g + 1];
+    for (int64_t t0 = o0 + lane; t0 < o1; t0 += 64) {
+        const uint32_t e = V.idx[t0];
+        const double4 s = V.bpos[e];
+        const float rx = (float)(s.x - gc.x), ry = (float)(s.y - gc.y), rz = (float)(s.z - gc.z);
+        const float rm = (float)s.w;
+        float bx = 0.f, by = 0.f, bz = 0.f, bm = 0.f;
+        for (int i = 0; i < cnt; i++) {
+            const float4 x = sk[i], w = sw[i];
+            const float X = x.x - rx, Y = x.y - ry, Z = x.z - rz;
+            const D3 dx(X, 1.f, 0.f, 0.f), dy(Y, 0.f, 1.f, 0.f), dz(Z, 0.f, 0.f, 1.f);
+            Acc4T<D3> a = {D3(0.f), D3(0.f), D3(0.f), D3(0.f)};
+            const bool self = X == 0.f && Y == 0.f && Z == 0.f;
+            if (SOFT == SOFT_K1)
+                k1_interact<D3>(D3(rm), dx, dy, dz, D3(eps_inv), D3(eps2), a);
+            else if (SOFT == SOFT_PLUMMER)
+                mono_interact<D3>(D3(rm), dx, dy, dz, D3(eps2), a);
+            else if (!self)
+                mono_interact<D3>(D3(rm), dx, dy, dz, D3(0.f), a);
+            bx -= w.x * a.x.x + w.y * a.y.x + w.z * a.z.x + w.w * a.w.x;
+            by -= w.x * a.x.y + w.y * a.y.y + w.z * a.z.y + w.w * a.w.y;
+            bz -= w.x * a.x.z + w.y * a.y.z + w.z * a.z.z + w.w * a.w.z;
+            // the pair is linear in the source mass
+            if (rm != 0.f)
+                bm += (w.x * a.x.v + w.y * a.y.v + w.z * a.z.v + w.w * a.w.v) / rm;
         }
+        float *gb = (float *)(gbody + e);
+        atomicAdd(gb + 0, bx);
+        atomicAdd(gb + 1, by);
+        atomicAdd(gb + 2, bz);
+        atomicAdd(gb + 3, bm);
+    }
+}
+
+// ---- transposed cell lists: for every cell, the (group, list kind) pairs that
+// list it, so that a cell's record gradient is summed by one wave alone.
+#define TKIND_SHIFT 30  // entry = group | kind << 30
+
+__global__ __launch_bounds__(64) void k_transpose_count(BatchView V, int *__restrict__ cnt) {
+    const int g = blockIdx.x;
+    for (int kind = 0; kind < L_BODY_KIND; kind++) {
+        const int64_t *offk = V.off + (int64_t)kind * (V.ngroups + 1);
+        for (int64_t t = offk[g] + threadIdx.x; t < offk[g + 1]; t += 64) atomicAdd(cnt + V.idx[t], 1);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_transpose_fill(BatchView V,
+                                                       unsigned long long *__restrict__ cursor,
+                                                       uint32_t *__restrict__ tlist) {
+    const int g = blockIdx.x;
+    for (int kind = 0; kind < L_BODY_KIND; kind++) {
+        const int64_t *offk = V.off + (int64_t)kind * (V.ngroups + 1);
+        for (int64_t t = offk[g] + threadIdx.x; t < offk[g + 1]; t += 64) {
+            const unsigned long long at = atomicAdd(cursor + V.idx[t], 1ull);
+            tlist[at] = (uint32_t)g | ((uint32_t)kind << TKIND_SHIFT);
+        }
+    }
+}
+
+// One wave per listed cell: lanes take a listing group's sinks (coalesced), the
+// wave walks the cell's groups; every lane keeps its partial dL/dP in registers
+// across groups and the wave sums them once, at the end.  No atomics: the cell's
+// record gradient has one writer.
+template <int SOFT>
+__global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, const float4 *__restrict__ wv,
+                                                      const int *__restrict__ tcells,
+                                                      const int64_t *__restrict__ toff,
+                                                      const uint32_t *__restrict__ tlist,
+                                                      float *__restrict__ gcell) {
+    const int c = tcells[blockIdx.x];
+    const int lane = threadIdx.x;
+    const double4 X = V.cpos[c];
+    const float eps2c = SOFT == SOFT_PLUMMER ? (float)(V.eps * V.eps) : 0.0f;
+    float acc[46];
+#pragma unroll
+    for (int q = 0; q < 46; q++) acc[q] = 0.f;
+    for (int64_t t = toff[c]; t < toff[c + 1]; t++) {
+        const uint32_t ent = tlist[t];
+        const int g = (int)(ent & ((1u << TKIND_SHIFT) - 1)), kind = (int)(ent >> TKIND_SHIFT);
+        if (lane >= V.gcount[g])
+            continue;
+        const int64_t i = V.gfirst[g] + lane;
+        const double4 sp = V.bpos[i];
+        const float4 w = wv[i];
+        const float x = (float)(sp.x - X.x), y = (float)(sp.y - X.y), z = (float)(sp.z - X.z);
+        float beta[46];
+        if (kind == 0)
+            cell_beta<4>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
+        else if (kind == 1)
+            cell_beta<2>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
+        else
+            cell_beta<0>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
+#pragma unroll
+        for (int q = 3; q < 46; q++) acc[q] += beta[q];
+    }
+    float *gcp = gcell + (size_t)c * ADJ_CELL;
+#pragma unroll
+    for (int q = 3; q < 46; q++) {
+        const float s = wave_sum(acc[q]);
+        if (lane == 0)
+            gcp[q] = s;
     }
 }
 
@@ -376,22 +414,40 @@ __global__ void k_adjoint_bodies(const double4 *__restrict__ bpos, const double4
     out[j] = make_double4(G * g[0], G * g[1], G * g[2], G * gm);
 }
 
+hipError_t launch_transpose_count(const BatchView &v, int *cnt, hipStream_t st) {
+    if (v.ngroups > 0)
+        hipLaunchKernelGGL(k_transpose_count, dim3(v.ngroups), dim3(64), 0, st, v, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_fill(const BatchView &v, unsigned long long *cursor, uint32_t *tlist,
+                                 hipStream_t st) {
+    if (v.ngroups > 0)
+        hipLaunchKernelGGL(k_transpose_fill, dim3(v.ngroups), dim3(64), 0, st, v, cursor, tlist);
+    return hipGetLastError();
+}
+
 hipError_t launch_adjoint(const BatchView &v, const float4 *wv, float4 *gsink, float *gcell,
                           float4 *gbody, const double *raw, const float *mat, int64_t nnodes,
                           float *gnode, const int *leaf_of, const int *parent, int64_t n,
-                          double4 *out, hipStream_t st) {
+                          const int *tcells, int ntcells, const int64_t *toff,
+                          const uint32_t *tlist, double4 *out, hipStream_t st) {
     if (v.ngroups <= 0)
         return hipSuccess;
+#define ADJ(S)                                                                                     \
+    hipLaunchKernelGGL(k_adjoint<S>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gsink, gcell, gbody); \
+    hipLaunchKernelGGL(k_adjoint_bodysrc<S>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gbody);      \
+    if (ntcells > 0)                                                                               \
+        hipLaunchKernelGGL(k_adjoint_cells<S>, dim3(ntcells), dim3(64), 0, st, v, wv, tcells, toff, \
+                           tlist, gcell);
     if (v.soft == SOFT_K1) {
-        hipLaunchKernelGGL(k_adjoint<SOFT_K1>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gsink, gcell, gbody);
-        hipLaunchKernelGGL(k_adjoint_sources<SOFT_K1>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gcell, gbody);
+        ADJ(SOFT_K1)
     } else if (v.soft == SOFT_PLUMMER) {
-        hipLaunchKernelGGL(k_adjoint<SOFT_PLUMMER>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gsink, gcell, gbody);
-        hipLaunchKernelGGL(k_adjoint_sources<SOFT_PLUMMER>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gcell, gbody);
+        ADJ(SOFT_PLUMMER)
     } else {
-        hipLaunchKernelGGL(k_adjoint<SOFT_NONE>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gsink, gcell, gbody);
-        hipLaunchKernelGGL(k_adjoint_sources<SOFT_NONE>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gcell, gbody);
+        ADJ(SOFT_NONE)
     }
+#undef ADJ
     hipLaunchKernelGGL(k_adjoint_nodes, dim3((unsigned)((nnodes + 127) / 128)), dim3(128), 0, st,
                        gcell, raw, mat, nnodes, gnode);
     hipLaunchKernelGGL(k_adjoint_bodies, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st,

@@ -48,6 +48,11 @@ struct Plan {
     std::vector<int> lev_off;  // nodes of level L: lev_nodes[lev_off[L] .. lev_off[L+1])
     void ensure_moments_view();
     int *a_leaf = nullptr, *a_parent = nullptr;
+    // transposed cell lists (groups listing each cell), built on the first adjoint
+    int64_t *a_toff = nullptr;
+    uint32_t *a_tlist = nullptr;
+    int *a_tcells = nullptr;
+    int a_ntcells = 0;
 
     template <typename P>
     P *dalloc(size_t bytes) {
@@ -432,6 +437,30 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
                     for (int64_t i = t.first[c]; i < t.first[c] + t.count[c]; i++) leaf[i] = (int)c;
             P->a_leaf = P->upload(leaf.data(), leaf.size());
             P->a_parent = P->upload(parent.data(), parent.size());
+            // transposed cell lists: count, scan on the host, fill
+            if (P->lists.ngroups >= (int64_t)1 << 30)
+                throw std::runtime_error("too many groups for the transposed lists");
+            int *cnt = P->dalloc<int>(sizeof(int) * (size_t)nn);
+            std::vector<int> hcnt(nn);
+            if (!check(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)nn, P->stream), "hipMemsetAsync") ||
+                !check(launch_transpose_count(P->v, cnt, P->stream), "transpose kernel") ||
+                !check(hipMemcpyAsync(hcnt.data(), cnt, sizeof(int) * (size_t)nn, hipMemcpyDeviceToHost, P->stream), "hipMemcpyAsync") ||
+                !check(hipStreamSynchronize(P->stream), "hipStreamSynchronize"))
+                return -1;
+            std::vector<int64_t> toff(nn + 1, 0);
+            std::vector<int> tcells;
+            for (int64_t c = 0; c < nn; c++) {
+                toff[c + 1] = toff[c] + hcnt[c];
+                if (hcnt[c])
+                    tcells.push_back((int)c);
+            }
+            P->a_toff = P->upload(toff.data(), toff.size());
+            unsigned long long *cursor = (unsigned long long *)P->upload(toff.data(), (size_t)nn);
+            P->a_tlist = P->dalloc<uint32_t>(sizeof(uint32_t) * std::max<int64_t>(1, toff[nn]));
+            P->a_tcells = P->upload(tcells.data(), tcells.size());
+            P->a_ntcells = (int)tcells.size();
+            if (!check(launch_transpose_fill(P->v, cursor, P->a_tlist, P->stream), "transpose kernel"))
+                return -1;
         }
         std::vector<float> wv(4 * (size_t)n);
         for (int64_t i = 0; i < n; i++)
@@ -440,7 +469,8 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
             !check(hipMemsetAsync(P->a_gcell, 0, sizeof(float) * 52 * (size_t)nn, P->stream), "hipMemsetAsync") ||
             !check(hipMemsetAsync(P->a_gbody, 0, sizeof(float4) * n, P->stream), "hipMemsetAsync") ||
             !check(launch_adjoint(P->v, P->a_wv, P->a_gsink, P->a_gcell, P->a_gbody, P->mv.raw,
-                                  P->a_mat, nn, P->a_gnode, P->a_leaf, P->a_parent, n, P->a_out,
+                                  P->a_mat, nn, P->a_gnode, P->a_leaf, P->a_parent, n,
+                                  P->a_tcells, P->a_ntcells, P->a_toff, P->a_tlist, P->a_out,
                                   P->stream), "adjoint kernels"))
             return -1;
         std::vector<double> out(4 * (size_t)n);
