@@ -297,24 +297,49 @@ __global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, const float4 
     float acc[46];
 #pragma unroll
     for (int q = 0; q < 46; q++) acc[q] = 0.f;
-    for (int64_t t = toff[c]; t < toff[c + 1]; t++) {
+    // the next group's sink is loaded while the current one is evaluated: the
+    // chain list entry -> group range -> sink is three dependent loads
+    const int64_t t0 = toff[c], t1 = toff[c + 1];
+    int kind = 0;
+    bool live = false;
+    double4 sp = make_double4(0, 0, 0, 0);
+    float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto fetch = [&](int64_t t, int &k, bool &lv, double4 &p, float4 &ww) {
         const uint32_t ent = tlist[t];
-        const int g = (int)(ent & ((1u << TKIND_SHIFT) - 1)), kind = (int)(ent >> TKIND_SHIFT);
-        if (lane >= V.gcount[g])
-            continue;
-        const int64_t i = V.gfirst[g] + lane;
-        const double4 sp = V.bpos[i];
-        const float4 w = wv[i];
-        const float x = (float)(sp.x - X.x), y = (float)(sp.y - X.y), z = (float)(sp.z - X.z);
-        float beta[46];
-        if (kind == 0)
-            cell_beta<4>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
-        else if (kind == 1)
-            cell_beta<2>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
-        else
-            cell_beta<0>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
+        const int g = (int)(ent & ((1u << TKIND_SHIFT) - 1));
+        k = (int)(ent >> TKIND_SHIFT);
+        lv = lane < V.gcount[g];
+        if (lv) {
+            const int64_t i = V.gfirst[g] + lane;
+            p = V.bpos[i];
+            ww = wv[i];
+        }
+    };
+    if (t0 < t1)
+        fetch(t0, kind, live, sp, w);
+    for (int64_t t = t0; t < t1; t++) {
+        int kn = 0;
+        bool ln = false;
+        double4 spn = make_double4(0, 0, 0, 0);
+        float4 wn = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (t + 1 < t1)
+            fetch(t + 1, kn, ln, spn, wn);
+        if (live) {
+            const float x = (float)(sp.x - X.x), y = (float)(sp.y - X.y), z = (float)(sp.z - X.z);
+            float beta[46];
+            if (kind == 0)
+                cell_beta<4>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
+            else if (kind == 1)
+                cell_beta<2>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
+            else
+                cell_beta<0>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
 #pragma unroll
-        for (int q = 3; q < 46; q++) acc[q] += beta[q];
+            for (int q = 3; q < 46; q++) acc[q] += beta[q];
+        }
+        kind = kn;
+        live = ln;
+        sp = spn;
+        w = wn;
     }
     float *gcp = gcell + (size_t)c * ADJ_CELL;
 #pragma unroll
