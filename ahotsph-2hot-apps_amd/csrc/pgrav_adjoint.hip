@@ -257,7 +257,7 @@ __global__ __launch_bounds__(64) void k_adjoint_bodysrc(BatchView V, const float
 
 // ---- transposed cell lists: for every cell, the (group, list kind) pairs that
 // list it, so that a cell's record gradient is summed by one wave alone.
-#define TKIND_SHIFT 30  // entry = group | kind << 30
+// entry = (first sink, sink count | list kind << 8) of one listing group
 
 __global__ __launch_bounds__(64) void k_transpose_count(BatchView V, int *__restrict__ cnt) {
     const int g = blockIdx.x;
@@ -269,13 +269,14 @@ __global__ __launch_bounds__(64) void k_transpose_count(BatchView V, int *__rest
 
 __global__ __launch_bounds__(64) void k_transpose_fill(BatchView V,
                                                        unsigned long long *__restrict__ cursor,
-                                                       uint32_t *__restrict__ tlist) {
+                                                       uint2 *__restrict__ tlist) {
     const int g = blockIdx.x;
+    const uint2 me = make_uint2((uint32_t)V.gfirst[g], (uint32_t)V.gcount[g]);
     for (int kind = 0; kind < L_BODY_KIND; kind++) {
         const int64_t *offk = V.off + (int64_t)kind * (V.ngroups + 1);
         for (int64_t t = offk[g] + threadIdx.x; t < offk[g + 1]; t += 64) {
             const unsigned long long at = atomicAdd(cursor + V.idx[t], 1ull);
-            tlist[at] = (uint32_t)g | ((uint32_t)kind << TKIND_SHIFT);
+            tlist[at] = make_uint2(me.x, me.y | ((uint32_t)kind << 8));
         }
     }
 }
@@ -283,12 +284,14 @@ __global__ __launch_bounds__(64) void k_transpose_fill(BatchView V,
 // One wave per listed cell: lanes take a listing group's sinks (coalesced), the
 // wave walks the cell's groups; every lane keeps its partial dL/dP in registers
 // across groups and the wave sums them once, at the end.  No atomics: the cell's
-// record gradient has one writer.
+// record gradient has one writer.  The cell's list entries arrive 64 at a time
+// in one vector load and are read back with v_readlane; the next group's sinks
+// load while the current group is evaluated.
 template <int SOFT>
 __global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, const float4 *__restrict__ wv,
                                                       const int *__restrict__ tcells,
                                                       const int64_t *__restrict__ toff,
-                                                      const uint32_t *__restrict__ tlist,
+                                                      const uint2 *__restrict__ tlist,
                                                       float *__restrict__ gcell) {
     const int c = tcells[blockIdx.x];
     const int lane = threadIdx.x;
@@ -297,49 +300,55 @@ __global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, const float4 
     float acc[46];
 #pragma unroll
     for (int q = 0; q < 46; q++) acc[q] = 0.f;
-    // the next group's sink is loaded while the current one is evaluated: the
-    // chain list entry -> group range -> sink is three dependent loads
     const int64_t t0 = toff[c], t1 = toff[c + 1];
-    int kind = 0;
-    bool live = false;
-    double4 sp = make_double4(0, 0, 0, 0);
-    float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
-    auto fetch = [&](int64_t t, int &k, bool &lv, double4 &p, float4 &ww) {
-        const uint32_t ent = tlist[t];
-        const int g = (int)(ent & ((1u << TKIND_SHIFT) - 1));
-        k = (int)(ent >> TKIND_SHIFT);
-        lv = lane < V.gcount[g];
-        if (lv) {
-            const int64_t i = V.gfirst[g] + lane;
-            p = V.bpos[i];
-            ww = wv[i];
+    for (int64_t tb = t0; tb < t1; tb += 64) {
+        const int nt = (int)min((int64_t)64, t1 - tb);
+        const uint2 mine = lane < nt ? tlist[tb + lane] : make_uint2(0u, 0u);
+        // entry k: first sink, count and kind, wave-uniform
+        auto entry = [&](int k, uint32_t &first, int &cnt, int &kind) {
+            first = (uint32_t)__builtin_amdgcn_readlane((int)mine.x, k);
+            const uint32_t y = (uint32_t)__builtin_amdgcn_readlane((int)mine.y, k);
+            cnt = (int)(y & 255u);
+            kind = (int)(y >> 8);
+        };
+        uint32_t f;
+        int cnt, kind;
+        entry(0, f, cnt, kind);
+        double4 sp = make_double4(0, 0, 0, 0);
+        float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane < cnt) {
+            sp = V.bpos[f + lane];
+            w = wv[f + lane];
         }
-    };
-    if (t0 < t1)
-        fetch(t0, kind, live, sp, w);
-    for (int64_t t = t0; t < t1; t++) {
-        int kn = 0;
-        bool ln = false;
-        double4 spn = make_double4(0, 0, 0, 0);
-        float4 wn = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (t + 1 < t1)
-            fetch(t + 1, kn, ln, spn, wn);
-        if (live) {
-            const float x = (float)(sp.x - X.x), y = (float)(sp.y - X.y), z = (float)(sp.z - X.z);
-            float beta[46];
-            if (kind == 0)
-                cell_beta<4>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
-            else if (kind == 1)
-                cell_beta<2>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
-            else
-                cell_beta<0>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
+        for (int k = 0; k < nt; k++) {
+            uint32_t fn = 0;
+            int cn = 0, kn = 0;
+            double4 spn = make_double4(0, 0, 0, 0);
+            float4 wn = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (k + 1 < nt) {
+                entry(k + 1, fn, cn, kn);
+                if (lane < cn) {
+                    spn = V.bpos[fn + lane];
+                    wn = wv[fn + lane];
+                }
+            }
+            if (lane < cnt) {
+                const float x = (float)(sp.x - X.x), y = (float)(sp.y - X.y), z = (float)(sp.z - X.z);
+                float beta[46];
+                if (kind == 0)
+                    cell_beta<4>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
+                else if (kind == 1)
+                    cell_beta<2>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
+                else
+                    cell_beta<0>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
 #pragma unroll
-            for (int q = 3; q < 46; q++) acc[q] += beta[q];
+                for (int q = 3; q < 46; q++) acc[q] += beta[q];
+            }
+            cnt = cn;
+            kind = kn;
+            sp = spn;
+            w = wn;
         }
-        kind = kn;
-        live = ln;
-        sp = spn;
-        w = wn;
     }
     float *gcp = gcell + (size_t)c * ADJ_CELL;
 #pragma unroll
@@ -445,7 +454,7 @@ hipError_t launch_transpose_count(const BatchView &v, int *cnt, hipStream_t st) 
     return hipGetLastError();
 }
 
-hipError_t launch_transpose_fill(const BatchView &v, unsigned long long *cursor, uint32_t *tlist,
+hipError_t launch_transpose_fill(const BatchView &v, unsigned long long *cursor, uint2 *tlist,
                                  hipStream_t st) {
     if (v.ngroups > 0)
         hipLaunchKernelGGL(k_transpose_fill, dim3(v.ngroups), dim3(64), 0, st, v, cursor, tlist);
@@ -456,7 +465,7 @@ hipError_t launch_adjoint(const BatchView &v, const float4 *wv, float4 *gsink, f
                           float4 *gbody, const double *raw, const float *mat, int64_t nnodes,
                           float *gnode, const int *leaf_of, const int *parent, int64_t n,
                           const int *tcells, int ntcells, const int64_t *toff,
-                          const uint32_t *tlist, double4 *out, hipStream_t st) {
+                          const uint2 *tlist, double4 *out, hipStream_t st) {
     if (v.ngroups <= 0)
         return hipSuccess;
 #define ADJ(S)                                                                                     \

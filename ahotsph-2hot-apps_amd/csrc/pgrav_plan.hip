@@ -50,7 +50,7 @@ struct Plan {
     int *a_leaf = nullptr, *a_parent = nullptr;
     // transposed cell lists (groups listing each cell), built on the first adjoint
     int64_t *a_toff = nullptr;
-    uint32_t *a_tlist = nullptr;
+    uint2 *a_tlist = nullptr;
     int *a_tcells = nullptr;
     int a_ntcells = 0;
 
@@ -438,8 +438,6 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
             P->a_leaf = P->upload(leaf.data(), leaf.size());
             P->a_parent = P->upload(parent.data(), parent.size());
             // transposed cell lists: count, scan on the host, fill
-            if (P->lists.ngroups >= (int64_t)1 << 30)
-                throw std::runtime_error("too many groups for the transposed lists");
             int *cnt = P->dalloc<int>(sizeof(int) * (size_t)nn);
             std::vector<int> hcnt(nn);
             if (!check(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)nn, P->stream), "hipMemsetAsync") ||
@@ -456,7 +454,7 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
             }
             P->a_toff = P->upload(toff.data(), toff.size());
             unsigned long long *cursor = (unsigned long long *)P->upload(toff.data(), (size_t)nn);
-            P->a_tlist = P->dalloc<uint32_t>(sizeof(uint32_t) * std::max<int64_t>(1, toff[nn]));
+            P->a_tlist = P->dalloc<uint2>(sizeof(uint2) * std::max<int64_t>(1, toff[nn]));
             P->a_tcells = P->upload(tcells.data(), tcells.size());
             P->a_ntcells = (int)tcells.size();
             if (!check(launch_transpose_fill(P->v, cursor, P->a_tlist, P->stream), "transpose kernel"))
