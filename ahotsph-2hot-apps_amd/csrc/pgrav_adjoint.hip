@@ -255,31 +255,9 @@ __global__ __launch_bounds__(64) void k_adjoint_bodysrc(BatchView V, const float
     }
 }
 
-// ---- transposed cell lists: for every cell, the (group, list kind) pairs that
-// list it, so that a cell's record gradient is summed by one wave alone.
-// entry = (first sink, sink count | list kind << 8) of one listing group
-
-__global__ __launch_bounds__(64) void k_transpose_count(BatchView V, int *__restrict__ cnt) {
-    const int g = blockIdx.x;
-    for (int kind = 0; kind < L_BODY_KIND; kind++) {
-        const int64_t *offk = V.off + (int64_t)kind * (V.ngroups + 1);
-        for (int64_t t = offk[g] + threadIdx.x; t < offk[g + 1]; t += 64) atomicAdd(cnt + V.idx[t], 1);
-    }
-}
-
-__global__ __launch_bounds__(64) void k_transpose_fill(BatchView V,
-                                                       unsigned long long *__restrict__ cursor,
-                                                       uint2 *__restrict__ tlist) {
-    const int g = blockIdx.x;
-    const uint2 me = make_uint2((uint32_t)V.gfirst[g], (uint32_t)V.gcount[g]);
-    for (int kind = 0; kind < L_BODY_KIND; kind++) {
-        const int64_t *offk = V.off + (int64_t)kind * (V.ngroups + 1);
-        for (int64_t t = offk[g] + threadIdx.x; t < offk[g + 1]; t += 64) {
-            const unsigned long long at = atomicAdd(cursor + V.idx[t], 1ull);
-            tlist[at] = make_uint2(me.x, me.y | ((uint32_t)kind << 8));
-        }
-    }
-}
+// ---- transposed cell lists (built by the host): for every cell, the groups
+// that list it, entry = (first sink, sink count | list kind << 8), so that a
+// cell's record gradient is summed by one wave alone.
 
 // One wave per listed cell: lanes take a listing group's sinks (coalesced), the
 // wave walks the cell's groups; every lane keeps its partial dL/dP in registers
@@ -446,19 +424,6 @@ __global__ void k_adjoint_bodies(const double4 *__restrict__ bpos, const double4
         }
     }
     out[j] = make_double4(G * g[0], G * g[1], G * g[2], G * gm);
-}
-
-hipError_t launch_transpose_count(const BatchView &v, int *cnt, hipStream_t st) {
-    if (v.ngroups > 0)
-        hipLaunchKernelGGL(k_transpose_count, dim3(v.ngroups), dim3(64), 0, st, v, cnt);
-    return hipGetLastError();
-}
-
-hipError_t launch_transpose_fill(const BatchView &v, unsigned long long *cursor, uint2 *tlist,
-                                 hipStream_t st) {
-    if (v.ngroups > 0)
-        hipLaunchKernelGGL(k_transpose_fill, dim3(v.ngroups), dim3(64), 0, st, v, cursor, tlist);
-    return hipGetLastError();
 }
 
 hipError_t launch_adjoint(const BatchView &v, const float4 *wv, float4 *gsink, float *gcell,

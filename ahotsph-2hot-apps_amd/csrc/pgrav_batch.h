@@ -50,11 +50,6 @@ hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, 
 hipError_t launch_group64(const BatchView &v, int g, hipStream_t st);
 hipError_t launch_reset_counter(int *c, hipStream_t st);
 hipError_t launch_ewald_table(double L, double4 *t64, float4 *t32, hipStream_t st);
-// transposed cell lists: counts per node, then per listing group the entry
-// (first sink, sink count | list kind << 8)
-hipError_t launch_transpose_count(const BatchView &v, int *cnt, hipStream_t st);
-hipError_t launch_transpose_fill(const BatchView &v, unsigned long long *cursor, uint2 *tlist,
-                                 hipStream_t st);
 hipError_t launch_adjoint(const BatchView &v, const float4 *wv, float4 *gsink, float *gcell,
                           float4 *gbody, const double *raw, const float *mat, int64_t nnodes,
                           float *gnode, const int *leaf_of, const int *parent, int64_t n,

@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <new>
@@ -437,28 +438,50 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
                     for (int64_t i = t.first[c]; i < t.first[c] + t.count[c]; i++) leaf[i] = (int)c;
             P->a_leaf = P->upload(leaf.data(), leaf.size());
             P->a_parent = P->upload(parent.data(), parent.size());
-            // transposed cell lists: count, scan on the host, fill
-            int *cnt = P->dalloc<int>(sizeof(int) * (size_t)nn);
-            std::vector<int> hcnt(nn);
-            if (!check(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)nn, P->stream), "hipMemsetAsync") ||
-                !check(launch_transpose_count(P->v, cnt, P->stream), "transpose kernel") ||
-                !check(hipMemcpyAsync(hcnt.data(), cnt, sizeof(int) * (size_t)nn, hipMemcpyDeviceToHost, P->stream), "hipMemcpyAsync") ||
-                !check(hipStreamSynchronize(P->stream), "hipStreamSynchronize"))
+            // transposed cell lists, built on the host from the device's lists: for
+            // each cell, the groups listing it in group (Morton) order, so that the
+            // cells processed together walk neighbouring groups and the sums have
+            // a fixed order
+            const Lists &L = P->lists;
+            const int64_t G = L.ngroups;
+            int64_t ncell_entries = 0;
+            for (int k = 0; k < L_BODY; k++) ncell_entries += L.off[k][G];
+            std::vector<uint32_t> idx(std::max<int64_t>(1, ncell_entries));
+            if (ncell_entries &&
+                (!check(hipMemcpyAsync(idx.data(), P->v.idx, sizeof(uint32_t) * ncell_entries,
+                                       hipMemcpyDeviceToHost, P->stream), "hipMemcpyAsync") ||
+                 !check(hipStreamSynchronize(P->stream), "hipStreamSynchronize")))
                 return -1;
             std::vector<int64_t> toff(nn + 1, 0);
+            for (int64_t e = 0; e < ncell_entries; e++) toff[idx[e] + 1]++;
             std::vector<int> tcells;
             for (int64_t c = 0; c < nn; c++) {
-                toff[c + 1] = toff[c] + hcnt[c];
-                if (hcnt[c])
+                if (toff[c + 1])
                     tcells.push_back((int)c);
+                toff[c + 1] += toff[c];
+            }
+            std::vector<int64_t> cursor(toff.begin(), toff.end() - 1);
+            std::vector<uint2> tl(std::max<int64_t>(1, ncell_entries));
+            int64_t base = 0;
+            for (int k = 0; k < L_BODY; k++) {
+                for (int64_t g = 0; g < G; g++) {
+                    const uint2 ent = make_uint2((uint32_t)L.gfirst[g],
+                                                 (uint32_t)L.gcount[g] | ((uint32_t)k << 8));
+                    for (int64_t e = base + L.off[k][g]; e < base + L.off[k][g + 1]; e++)
+                        tl[cursor[idx[e]]++] = ent;
+                }
+                base += L.off[k][G];
+            }
+            // group order within a cell across kinds: a cell has one kind per group
+            for (size_t j = 0; j < tcells.size(); j++) {
+                const int c = tcells[j];
+                std::sort(tl.begin() + toff[c], tl.begin() + toff[c + 1],
+                          [](const uint2 &a, const uint2 &b) { return a.x < b.x; });
             }
             P->a_toff = P->upload(toff.data(), toff.size());
-            unsigned long long *cursor = (unsigned long long *)P->upload(toff.data(), (size_t)nn);
-            P->a_tlist = P->dalloc<uint2>(sizeof(uint2) * std::max<int64_t>(1, toff[nn]));
+            P->a_tlist = P->upload(tl.data(), tl.size());
             P->a_tcells = P->upload(tcells.data(), tcells.size());
             P->a_ntcells = (int)tcells.size();
-            if (!check(launch_transpose_fill(P->v, cursor, P->a_tlist, P->stream), "transpose kernel"))
-                return -1;
         }
         std::vector<float> wv(4 * (size_t)n);
         for (int64_t i = 0; i < n; i++)
