@@ -15,11 +15,11 @@
 //                   looping over the group's sinks: dL/dx_j and dL/dm_j of the
 //                   source bodies, lane-private sums, one atomic add per term
 //                   and tile.
-//  k_adjoint_cells  one wave per listed cell over the transposed lists (the
-//                   groups that list it): the 43 prepared-record coefficients'
-//                   gradient dL/dP (the interaction is linear in P, so this is
-//                   the hand-written transpose of the evaluation), summed in
-//                   registers and once across the wave; no atomics.
+//  k_adjoint_cells  one wave per listed cell (per chunk of 128 listing groups)
+//                   over the transposed lists: the 43 prepared-record
+//                   coefficients' gradient dL/dP (the interaction is linear in
+//                   P, so this is the hand-written transpose of the
+//                   evaluation), summed in registers and once across the wave.
 //  k_adjoint_nodes  per cell: dL/dP -> dL/d(raw moments) through the fixed
 //                   linear map raw -> record -> prepared record, and the total
 //                   dL/dX_c including the centre dependence of the moments.
@@ -262,23 +262,24 @@ __global__ __launch_bounds__(64) void k_adjoint_bodysrc(BatchView V, const float
 // One wave per listed cell: lanes take a listing group's sinks (coalesced), the
 // wave walks the cell's groups; every lane keeps its partial dL/dP in registers
 // across groups and the wave sums them once, at the end.  No atomics: the cell's
-// record gradient has one writer.  The cell's list entries arrive 64 at a time
+// record gradient has one writer per chunk.  The cell's list entries arrive 64 at a time
 // in one vector load and are read back with v_readlane; the next group's sinks
 // load while the current group is evaluated.
 template <int SOFT>
 __global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, const float4 *__restrict__ wv,
                                                       const int *__restrict__ tcells,
-                                                      const int64_t *__restrict__ toff,
+                                                      const longlong2 *__restrict__ trange,
                                                       const uint2 *__restrict__ tlist,
                                                       float *__restrict__ gcell) {
+    // work item: a cell and one chunk [t0, t1) of its transposed list
     const int c = tcells[blockIdx.x];
+    const int64_t t0 = trange[blockIdx.x].x, t1 = trange[blockIdx.x].y;
     const int lane = threadIdx.x;
     const double4 X = V.cpos[c];
     const float eps2c = SOFT == SOFT_PLUMMER ? (float)(V.eps * V.eps) : 0.0f;
     float acc[46];
 #pragma unroll
     for (int q = 0; q < 46; q++) acc[q] = 0.f;
-    const int64_t t0 = toff[c], t1 = toff[c + 1];
     for (int64_t tb = t0; tb < t1; tb += 64) {
         const int nt = (int)min((int64_t)64, t1 - tb);
         const uint2 mine = lane < nt ? tlist[tb + lane] : make_uint2(0u, 0u);
@@ -328,12 +329,13 @@ __global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, const float4 
             w = wn;
         }
     }
+    // a cell listed by many groups is split over several waves: each adds its sums
     float *gcp = gcell + (size_t)c * ADJ_CELL;
 #pragma unroll
     for (int q = 3; q < 46; q++) {
         const float s = wave_sum(acc[q]);
         if (lane == 0)
-            gcp[q] = s;
+            atomicAdd(gcp + q, s);
     }
 }
 
@@ -429,7 +431,7 @@ __global__ void k_adjoint_bodies(const double4 *__restrict__ bpos, const double4
 hipError_t launch_adjoint(const BatchView &v, const float4 *wv, float4 *gsink, float *gcell,
                           float4 *gbody, const double *raw, const float *mat, int64_t nnodes,
                           float *gnode, const int *leaf_of, const int *parent, int64_t n,
-                          const int *tcells, int ntcells, const int64_t *toff,
+                          const int *tcells, int ntcells, const longlong2 *trange,
                           const uint2 *tlist, double4 *out, hipStream_t st) {
     if (v.ngroups <= 0)
         return hipSuccess;
@@ -437,7 +439,7 @@ hipError_t launch_adjoint(const BatchView &v, const float4 *wv, float4 *gsink, f
     hipLaunchKernelGGL(k_adjoint<S>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gsink, gcell, gbody); \
     hipLaunchKernelGGL(k_adjoint_bodysrc<S>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gbody);      \
     if (ntcells > 0)                                                                               \
-        hipLaunchKernelGGL(k_adjoint_cells<S>, dim3(ntcells), dim3(64), 0, st, v, wv, tcells, toff, \
+        hipLaunchKernelGGL(k_adjoint_cells<S>, dim3(ntcells), dim3(64), 0, st, v, wv, tcells, trange, \
                            tlist, gcell);
     if (v.soft == SOFT_K1) {
         ADJ(SOFT_K1)

@@ -53,7 +53,7 @@ hipError_t launch_ewald_table(double L, double4 *t64, float4 *t32, hipStream_t s
 hipError_t launch_adjoint(const BatchView &v, const float4 *wv, float4 *gsink, float *gcell,
                           float4 *gbody, const double *raw, const float *mat, int64_t nnodes,
                           float *gnode, const int *leaf_of, const int *parent, int64_t n,
-                          const int *tcells, int ntcells, const int64_t *toff,
+                          const int *tcells, int ntcells, const longlong2 *trange,
                           const uint2 *tlist, double4 *out, hipStream_t st);
 // ewald64 non-null: periodic (nearest image + Ewald correction, box edge `box`)
 hipError_t launch_direct64(const double4 *bpos, int64_t n, const int64_t *sample, int ns, int soft,

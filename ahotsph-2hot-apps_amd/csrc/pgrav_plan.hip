@@ -50,7 +50,7 @@ struct Plan {
     void ensure_moments_view();
     int *a_leaf = nullptr, *a_parent = nullptr;
     // transposed cell lists (groups listing each cell), built on the first adjoint
-    int64_t *a_toff = nullptr;
+    longlong2 *a_trange = nullptr;  // per work item: [t0, t1) into a_tlist
     uint2 *a_tlist = nullptr;
     int *a_tcells = nullptr;
     int a_ntcells = 0;
@@ -454,10 +454,15 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
                 return -1;
             std::vector<int64_t> toff(nn + 1, 0);
             for (int64_t e = 0; e < ncell_entries; e++) toff[idx[e] + 1]++;
+            // work items: a cell and a chunk of at most 128 of its listing groups
             std::vector<int> tcells;
+            std::vector<longlong2> trange;
             for (int64_t c = 0; c < nn; c++) {
-                if (toff[c + 1])
+                const int64_t t0 = toff[c], m = toff[c + 1];
+                for (int64_t a = 0; a < m; a += 128) {
                     tcells.push_back((int)c);
+                    trange.push_back(make_longlong2(t0 + a, t0 + std::min<int64_t>(m, a + 128)));
+                }
                 toff[c + 1] += toff[c];
             }
             std::vector<int64_t> cursor(toff.begin(), toff.end() - 1);
@@ -473,12 +478,11 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
                 base += L.off[k][G];
             }
             // group order within a cell across kinds: a cell has one kind per group
-            for (size_t j = 0; j < tcells.size(); j++) {
-                const int c = tcells[j];
-                std::sort(tl.begin() + toff[c], tl.begin() + toff[c + 1],
-                          [](const uint2 &a, const uint2 &b) { return a.x < b.x; });
-            }
-            P->a_toff = P->upload(toff.data(), toff.size());
+            for (int64_t c = 0; c < nn; c++)
+                if (toff[c + 1] - toff[c] > 1)
+                    std::sort(tl.begin() + toff[c], tl.begin() + toff[c + 1],
+                              [](const uint2 &a, const uint2 &b) { return a.x < b.x; });
+            P->a_trange = P->upload(trange.data(), trange.size());
             P->a_tlist = P->upload(tl.data(), tl.size());
             P->a_tcells = P->upload(tcells.data(), tcells.size());
             P->a_ntcells = (int)tcells.size();
@@ -491,7 +495,7 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
             !check(hipMemsetAsync(P->a_gbody, 0, sizeof(float4) * n, P->stream), "hipMemsetAsync") ||
             !check(launch_adjoint(P->v, P->a_wv, P->a_gsink, P->a_gcell, P->a_gbody, P->mv.raw,
                                   P->a_mat, nn, P->a_gnode, P->a_leaf, P->a_parent, n,
-                                  P->a_tcells, P->a_ntcells, P->a_toff, P->a_tlist, P->a_out,
+                                  P->a_tcells, P->a_ntcells, P->a_trange, P->a_tlist, P->a_out,
                                   P->stream), "adjoint kernels"))
             return -1;
         std::vector<double> out(4 * (size_t)n);
