@@ -33,6 +33,7 @@ struct BatchView {
     // idx[off[k*(ngroups+1)+g] .. off[k*(ngroups+1)+g+1])
     const int64_t *off = nullptr;
     const uint32_t *idx = nullptr;
+    const uint8_t *img = nullptr;  // periodic = 2: image of each idx entry
     float4 *out32 = nullptr;
     double4 *out64 = nullptr;
     int *counter = nullptr;
@@ -42,7 +43,8 @@ struct BatchView {
     int soft = 0;
     double eps = 0;
     double G = 1;
-    int periodic = 0;
+    int periodic = 0;  // 1 nearest image + Ewald table; 2 explicit images (img)
+    int nimage = 0;
     double box = 1;
 };
 

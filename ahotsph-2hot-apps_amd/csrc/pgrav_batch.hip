@@ -48,6 +48,8 @@ struct BatchArgs {
     void *out;  // float4* or double4*, sorted body order
     int *counter;
     const float4 *ewald;  // 64^3 correction table (periodic)
+    const uint8_t *img;  // explicit images: the image of each entry, parallel to idx (or null)
+    int img_f;           // 2 nimage + 1 images per side
     int ngroups;
     int soft;
     double eps;
@@ -170,7 +172,15 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
             const int nt = (int)min((int64_t)64, o1 - t0);
             if (lane < nt) {
                 const uint32_t e = idx[t0 + lane];
-                const double4 src = kind == 3 ? A.bpos[e] : A.cpos[e];
+                double4 src = kind == 3 ? A.bpos[e] : A.cpos[e];
+                if (A.img) {
+                    // image (i, j, k) in nln/mac.c:91-111 order: the source sits at
+                    // x + (i, j, k) box, added in FP64
+                    const int im = A.img[t0 + lane], f = A.img_f, h = f >> 1;
+                    src.x += (double)(im / (f * f) - h) * A.box;
+                    src.y += (double)((im / f) % f - h) * A.box;
+                    src.z += (double)(im % f - h) * A.box;
+                }
                 lds[lane] = source_offset<T, REL, PERIODIC>(src, gc, A.box);
             }
             __syncthreads();
@@ -529,9 +539,12 @@ hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, 
     A.eps = v.eps;
     A.G = v.G;
     A.box = v.box;
+    A.img = v.periodic == 2 ? v.img : nullptr;
+    A.img_f = 2 * v.nimage + 1;
     if (A.ngroups <= 0)
         return hipSuccess;
-    const bool per = v.periodic != 0;
+    // explicit images run the isolated kernels (the offsets come with the entries)
+    const bool per = v.periodic == 1;
     switch (variant) {
     case VAR_REL32C: launch_soft<float, true, true>(A, per, persistent_blocks, st); break;
     case VAR_REL32: launch_soft<float, true, false>(A, per, persistent_blocks, st); break;
@@ -563,7 +576,9 @@ hipError_t launch_group64(const BatchView &v, int g, hipStream_t st) {
     A.eps = v.eps;
     A.G = v.G;
     A.box = v.box;
-    const bool per = v.periodic != 0;
+    A.img = v.periodic == 2 ? v.img : nullptr;
+    A.img_f = 2 * v.nimage + 1;
+    const bool per = v.periodic == 1;
 #define G64(S) \
     do { if (per) hipLaunchKernelGGL((k_group64<S, true>), dim3(1), dim3(64), 0, st, A, g); \
          else hipLaunchKernelGGL((k_group64<S, false>), dim3(1), dim3(64), 0, st, A, g); } while (0)

@@ -1,0 +1,155 @@
+// Far field of the explicit-image periodic scheme on the GPU, in FP64: the
+// pass nln/main.c:1811-1862 (FixCubeEwaldLE) runs on the host after the walk.
+// Three launches, all graph-capturable (no host round trip, no memset):
+//
+//   k_le_partials  global moments mu_gamma = sum_j m_j x_j^gamma (165 monomials,
+//                  degree <= 8, box units about the box centre): each workgroup
+//                  sums a contiguous range of the sorted bodies, 128 at a time
+//                  staged in LDS as m x^a, y^b, z^c; one thread per monomial
+//   k_le_coeffs    fixed-order sum of the partials, then C = W mu (W: the
+//                  lattice expansion and background, built on the host)
+//   k_le_apply     per body: the degree-8 polynomial sum C_beta x^beta and its
+//                  gradient, added to the force result with the G / L and
+//                  G / L^2 scales and the self term -xi m_i (nln/ewald_le2.c:44)
+//
+// Algorithmic work: 165 x 2 flops per body for the moments, ~4 x 165 FMA per
+// body for the evaluation; 32 B read per body per pass plus the result
+// read-modify-write -- a few tenths of a millisecond at N = 16.8 M.
+#include <hip/hip_runtime.h>
+
+#include "pgrav_le.h"
+#include "pgrav_le_launch.h"
+
+namespace pgrav {
+
+namespace {
+
+constexpr int LE_TILE = 128;
+
+__global__ __launch_bounds__(256) void k_le_partials(const double4 *__restrict__ bpos, int64_t n,
+                                                     double box, int64_t chunk,
+                                                     double *__restrict__ partial) {
+    __shared__ double pm[LE_TILE][LE_DEG + 1], py[LE_TILE][LE_DEG + 1], pz[LE_TILE][LE_DEG + 1];
+    const int t = threadIdx.x;
+    const int64_t b0 = (int64_t)blockIdx.x * chunk, b1 = min(n, b0 + chunk);
+    int ea = 0, eb = 0, ec = 0;
+    if (t < LE_NMON) {
+        // the thread's monomial (a runtime index into the LDS tables)
+#pragma unroll
+        for (int k = 0; k < LE_NMON; k++)
+            if (k == t) {
+                ea = LE_MON.e[k][0];
+                eb = LE_MON.e[k][1];
+                ec = LE_MON.e[k][2];
+            }
+    }
+    const double ibox = 1.0 / box;
+    double acc = 0.0;
+    for (int64_t s = b0; s < b1; s += LE_TILE) {
+        const int m = (int)min((int64_t)LE_TILE, b1 - s);
+        if (t < m) {
+            const double4 b = bpos[s + t];
+            const double x = (b.x - 0.5 * box) * ibox, y = (b.y - 0.5 * box) * ibox,
+                         z = (b.z - 0.5 * box) * ibox;
+            double px = b.w, qy = 1.0, qz = 1.0;
+            for (int k = 0; k <= LE_DEG; k++) {
+                pm[t][k] = px;
+                py[t][k] = qy;
+                pz[t][k] = qz;
+                px *= x;
+                qy *= y;
+                qz *= z;
+            }
+        }
+        __syncthreads();
+        if (t < LE_NMON)
+            for (int j = 0; j < m; j++) acc += pm[j][ea] * py[j][eb] * pz[j][ec];
+        __syncthreads();
+    }
+    if (t < LE_NMON)
+        partial[(size_t)blockIdx.x * LE_NMON + t] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_le_coeffs(const double *__restrict__ partial, int nblocks,
+                                                   const double *__restrict__ W,
+                                                   double *__restrict__ coef) {
+    __shared__ double mu[LE_NMON];
+    const int t = threadIdx.x;
+    if (t < LE_NMON) {
+        double s = 0.0;
+        for (int b = 0; b < nblocks; b++) s += partial[(size_t)b * LE_NMON + t];
+        mu[t] = s;
+    }
+    __syncthreads();
+    if (t < LE_NMON) {
+        double s = 0.0;
+        for (int g = 0; g < LE_NMON; g++) s += W[(size_t)t * LE_NMON + g] * mu[g];
+        coef[t] = s;
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void k_le_apply(const double4 *__restrict__ bpos, int64_t n,
+                                                  double box, double G,
+                                                  const double *__restrict__ coef,
+                                                  V *__restrict__ out) {
+    __shared__ double c[LE_NMON];
+    for (int k = threadIdx.x; k < LE_NMON; k += blockDim.x) c[k] = coef[k];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const double4 b = bpos[i];
+    const double ibox = 1.0 / box;
+    double px[LE_DEG + 1], py[LE_DEG + 1], pz[LE_DEG + 1];
+    px[0] = py[0] = pz[0] = 1.0;
+    const double x = (b.x - 0.5 * box) * ibox, y = (b.y - 0.5 * box) * ibox, z = (b.z - 0.5 * box) * ibox;
+#pragma unroll
+    for (int k = 1; k <= LE_DEG; k++) {
+        px[k] = px[k - 1] * x;
+        py[k] = py[k - 1] * y;
+        pz[k] = pz[k - 1] * z;
+    }
+    double phi = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+    for (int k = 0; k < LE_NMON; k++) {
+        const int a = LE_MON.e[k][0], bb = LE_MON.e[k][1], cc = LE_MON.e[k][2];
+        const double yz = py[bb] * pz[cc];
+        phi += c[k] * px[a] * yz;
+        if (a)
+            gx += c[k] * a * px[a - 1] * yz;
+        if (bb)
+            gy += c[k] * bb * px[a] * py[bb - 1] * pz[cc];
+        if (cc)
+            gz += c[k] * cc * px[a] * py[bb] * pz[cc - 1];
+    }
+    const double sa = -G * ibox * ibox, sp = G * ibox;
+    V r = out[i];
+    r.x += sa * gx;
+    r.y += sa * gy;
+    r.z += sa * gz;
+    r.w += sp * (phi - LE_XI * b.w);
+    out[i] = r;
+}
+
+}  // namespace
+
+hipError_t launch_ewald_le(const double4 *bpos, int64_t n, double box, double G, const double *W,
+                           double *partial, int nblocks, double *coef, float4 *out32,
+                           double4 *out64, hipStream_t st) {
+    if (n <= 0)
+        return hipSuccess;
+    if (nblocks < 1 || nblocks > LE_MAX_BLOCKS || (out32 == nullptr) == (out64 == nullptr))
+        return hipErrorInvalidValue;
+    const int64_t chunk = (n + nblocks - 1) / nblocks;
+    hipLaunchKernelGGL(k_le_partials, dim3(nblocks), dim3(256), 0, st, bpos, n, box, chunk, partial);
+    hipLaunchKernelGGL(k_le_coeffs, dim3(1), dim3(256), 0, st, partial, nblocks, W, coef);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (out32)
+        hipLaunchKernelGGL(k_le_apply<float4>, grid, dim3(256), 0, st, bpos, n, box, G, coef, out32);
+    else
+        hipLaunchKernelGGL(k_le_apply<double4>, grid, dim3(256), 0, st, bpos, n, box, G, coef, out64);
+    return hipGetLastError();
+}
+
+}  // namespace pgrav

@@ -1,0 +1,17 @@
+// Launcher of the far-field pass of periodic = 2 plans (pgrav_ewald_le.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pgrav {
+
+#define LE_MAX_BLOCKS 4096  // workgroups of the moment pass (partial sums: 165 doubles each)
+
+// Adds the far images and background of the explicit-image scheme, times G, to
+// the result of the sorted bodies bpos[0..n) (exactly one of out32 / out64).
+// W: LE_NMON x LE_NMON (le_matrix); partial: nblocks x LE_NMON; coef: LE_NMON.
+hipError_t launch_ewald_le(const double4 *bpos, int64_t n, double box, double G, const double *W,
+                           double *partial, int nblocks, double *coef, float4 *out32,
+                           double4 *out64, hipStream_t st);
+
+}  // namespace pgrav

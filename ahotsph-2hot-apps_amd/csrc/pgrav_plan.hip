@@ -18,6 +18,8 @@
 #include "pgrav_batch.h"
 #include "pgrav_device.h"
 #include "pgrav_host.h"
+#include "pgrav_le.h"
+#include "pgrav_le_launch.h"
 #include "pgrav_moments.h"
 #include "pgrav_internal.h"
 
@@ -54,6 +56,9 @@ struct Plan {
     uint2 *a_tlist = nullptr;
     int *a_tcells = nullptr;
     int a_ntcells = 0;
+    // far-field pass of the explicit-image scheme (periodic = 2)
+    double *le_W = nullptr, *le_partial = nullptr, *le_coef = nullptr;
+    int le_blocks = 0;
 
     template <typename P>
     P *dalloc(size_t bytes) {
@@ -141,6 +146,17 @@ static hipStream_t stream_of(Plan *P, void *stream) {
     return stream ? (hipStream_t)stream : P->stream;
 }
 
+// the far images and background of periodic = 2 plans, added to the result
+static int far_field(Plan *P, int variant, hipStream_t st) {
+    if (P->prm.periodic != 2)
+        return 0;
+    const bool f64 = variant == VAR_F64 || variant == VAR_PLAIN64;
+    return check(launch_ewald_le(P->v.bpos, P->tree.n, P->prm.box, P->prm.G, P->le_W, P->le_partial,
+                                 P->le_blocks, P->le_coef, f64 ? nullptr : P->v.out32,
+                                 f64 ? P->v.out64 : nullptr, st), "far-field kernels")
+               ? 0 : -1;
+}
+
 static int evaluate(Plan *P, int variant, hipStream_t st) {
     if (variant == VAR_PLAIN64) {
         if (!P->v.mom64) {
@@ -161,7 +177,7 @@ static int evaluate(Plan *P, int variant, hipStream_t st) {
                 !check(hipStreamWaitEvent(st, P->ev1, 0), "hipStreamWaitEvent"))
                 return -1;
         }
-        return 0;
+        return far_field(P, variant, st);
     }
     if (variant == VAR_F64 && !P->v.mom64) {
         fail("the FP64 tables were not built (pgrav_params.fp64 = 0)");
@@ -174,7 +190,9 @@ static int evaluate(Plan *P, int variant, hipStream_t st) {
     int pb = P->prm.persistent_blocks;
     if (pb < 0)
         pb = auto_blocks();
-    return check(launch_batch(P->v, variant, pb, st), "batched kernel") ? 0 : -1;
+    if (!check(launch_batch(P->v, variant, pb, st), "batched kernel"))
+        return -1;
+    return far_field(P, variant, st);
 }
 
 }  // namespace pgrav
@@ -201,6 +219,7 @@ void pgrav_params_default(pgrav_params *p) {
     p->fp64 = 1;
     p->dispatch = 1;
     p->mixed_order = 0;
+    p->nimage = 2;  // nln/main.c:246
 }
 
 void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const pgrav_params *p) {
@@ -218,9 +237,10 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         return nullptr;
     }
     if ((p->soft_type != SOFT_NONE && !(p->eps > 0)) || !(p->theta > 0) || p->group_max < 1 ||
-        p->group_max > 64 || (p->periodic && !(p->box > 0))) {
-        fail("pgrav_plan_create: bad parameters (eps %g theta %g group_max %d box %g)", p->eps,
-             p->theta, p->group_max, p->box);
+        p->group_max > 64 || (p->periodic && !(p->box > 0)) || p->periodic < 0 ||
+        p->periodic > 2 || (p->periodic == 2 && (p->nimage < 0 || p->nimage > MAX_NIMAGE))) {
+        fail("pgrav_plan_create: bad parameters (eps %g theta %g group_max %d periodic %d box %g "
+             "nimage %d)", p->eps, p->theta, p->group_max, p->periodic, p->box, p->nimage);
         return nullptr;
     }
     if (n > (int64_t)UINT32_MAX) {
@@ -243,6 +263,7 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         P->tp.order = p->order;
         P->tp.pcut = p->pcut;
         P->tp.periodic = p->periodic;
+        P->tp.nimage = p->nimage;
         P->tp.box = p->box;
         P->tp.nthreads = p->nthreads;
         P->tp.merge_groups = p->merge_groups;
@@ -315,6 +336,19 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
                 std::vector<uint32_t>().swap(L.idx[k]);  // the device copy is the one used
             }
             P->v.idx = idx;
+            if (p->periodic == 2) {
+                uint8_t *img = P->dalloc<uint8_t>(std::max<size_t>(1, total));
+                at = 0;
+                for (int k = 0; k < NLIST; k++) {
+                    const size_t m = L.img[k].size();
+                    if (m && !check(hipMemcpy(img + at, L.img[k].data(), m, hipMemcpyHostToDevice),
+                                    "hipMemcpy"))
+                        throw std::runtime_error("upload failed");
+                    at += m;
+                    std::vector<uint8_t>().swap(L.img[k]);
+                }
+                P->v.img = img;
+            }
         }
         P->v.out32 = P->dalloc<float4>(sizeof(float4) * (size_t)n);
         if (p->fp64)
@@ -326,7 +360,16 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         P->v.eps = p->eps;
         P->v.G = p->G;
         P->v.periodic = p->periodic;
+        P->v.nimage = p->nimage;
         P->v.box = p->box;
+        if (p->periodic == 2) {
+            std::vector<double> W;
+            le_matrix(p->nimage, W);
+            P->le_W = P->upload(W.data(), W.size());
+            P->le_blocks = (int)std::min<int64_t>(1024, (n + 127) / 128);
+            P->le_partial = P->dalloc<double>(sizeof(double) * LE_NMON * P->le_blocks);
+            P->le_coef = P->dalloc<double>(sizeof(double) * LE_NMON);
+        }
         if (p->periodic) {
             const size_t ne = 4 * (size_t)EWALD_N * EWALD_N * EWALD_N;  // value + 3 derivatives
             double4 *t64 = P->dalloc<double4>(ne * sizeof(double4));

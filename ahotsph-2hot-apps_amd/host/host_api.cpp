@@ -45,6 +45,7 @@ void *pgrav_host_plan(const double *pos3, const double *mass, int64_t n, const p
         H->tp.order = p->order;
         H->tp.pcut = p->pcut;
         H->tp.periodic = p->periodic;
+        H->tp.nimage = p->nimage;
         H->tp.box = p->box;
         H->tp.nthreads = p->nthreads;
         H->tp.merge_groups = p->merge_groups;
@@ -105,6 +106,22 @@ int64_t pgrav_host_list(void *h, int64_t g, int kind, uint32_t *out, int64_t cap
     const int64_t a = H->lists.off[kind][g], b = H->lists.off[kind][g + 1];
     if (out)
         for (int64_t i = 0; i < b - a && i < cap; i++) out[i] = H->lists.idx[kind][a + i];
+    return b - a;
+}
+
+int64_t pgrav_host_list_images(void *h, int64_t g, int kind, uint8_t *out, int64_t cap) {
+    HostPlan *H = (HostPlan *)h;
+    if (!H || g < 0 || g >= H->lists.ngroups || kind < 0 || kind >= NLIST) {
+        fail("bad group or list kind");
+        return -1;
+    }
+    if (H->lists.img[kind].empty() && H->lists.off[kind][H->lists.ngroups] > 0) {
+        fail("the lists carry no images (periodic != 2)");
+        return -1;
+    }
+    const int64_t a = H->lists.off[kind][g], b = H->lists.off[kind][g + 1];
+    if (out)
+        for (int64_t i = 0; i < b - a && i < cap; i++) out[i] = H->lists.img[kind][a + i];
     return b - a;
 }
 
@@ -193,6 +210,32 @@ int pgrav_host_sorted(void *h, int64_t *perm, double *pos3) {
         memcpy(perm, H->tree.perm.data(), 8 * (size_t)H->tree.n);
     if (pos3)
         memcpy(pos3, H->tree.pos.data(), 24 * (size_t)H->tree.n);
+    return 0;
+}
+
+int pgrav_host_ewald_le(const double *pos3, const double *mass, int64_t n, double box, int nimage,
+                        double G, double *acc4) {
+    clear_error();
+    if (!pos3 || !mass || !acc4 || n < 0 || !(box > 0) || nimage < 0 || nimage > MAX_NIMAGE) {
+        fail("pgrav_host_ewald_le: bad arguments");
+        return -1;
+    }
+    try {
+        le_evaluate_host(pos3, mass, n, box, nimage, G, acc4);
+    } catch (const std::exception &e) {
+        fail("pgrav_host_ewald_le: %s", e.what());
+        return -1;
+    }
+    return 0;
+}
+
+int pgrav_host_le_lattice(int k, int nimage, double *ck) {
+    clear_error();
+    if (!ck || k < 4 || k > 8 || (k & 1) || nimage < 0) {
+        fail("pgrav_host_le_lattice: k in {4, 6, 8}, nimage >= 0");
+        return -1;
+    }
+    *ck = le_lattice_coeff(k, nimage);
     return 0;
 }
 

@@ -84,7 +84,7 @@ struct Mac {
         kind = p.order >= 4 ? L_HEXA : (p.order >= 2 ? L_QUAD : L_MONO);
         inv_theta = 1.0 / p.theta;
         box = p.box;
-        periodic = p.periodic != 0;
+        periodic = p.periodic == 1;  // explicit images (2) shift the group instead
         // mixed order: distance (in bmax) beyond which order 2 / order 0 meets the
         // truncation bound theta^(order+1) that the top order meets at bmax/theta
         const int top = p.order >= 4 ? 4 : (p.order >= 2 ? 2 : 0);
@@ -326,10 +326,15 @@ void build_lists(const Tree &t, const TreeParams &p, Lists &l) {
     const Mac mac(t, p);
     struct Buf {
         std::vector<uint32_t> v[NLIST];
+        std::vector<uint8_t> im[NLIST];  // explicit images: the entry's image
         std::vector<int64_t> gid;
         std::vector<int64_t> start;  // NLIST per processed group
         int64_t pairs[NLIST] = {0, 0, 0, 0};
     };
+    const bool images = p.periodic == 2;
+    if (images && (p.nimage < 0 || p.nimage > MAX_NIMAGE))
+        throw std::runtime_error("nimage must be 0, 1 or 2 for explicit periodic images");
+    const int h = images ? p.nimage : 0, f = 2 * h + 1, nim = f * f * f;
     std::vector<Buf> bufs(nth);
     std::atomic<int64_t> next(0);
     std::atomic<bool> miscount(false);
@@ -345,12 +350,28 @@ void build_lists(const Tree &t, const TreeParams &p, Lists &l) {
             for (int64_t g = g0; g < std::min(G, g0 + chunk); g++) {
                 B.gid.push_back(g);
                 for (int k = 0; k < NLIST; k++) B.start.push_back((int64_t)B.v[k].size());
-                const int64_t covered = mac.walk(
-                    &l.gcenter[4 * g], stack, [&](int k, int32_t c) { B.v[k].push_back((uint32_t)c); },
-                    [&](int64_t f, int32_t n) {
-                        for (int64_t i = f; i < f + n; i++) B.v[L_BODY].push_back((uint32_t)i);
-                    });
-                if (covered != t.n)
+                // explicit images (nln/mac.c:91-111): the group walks the tree once per
+                // image offset o, as if the sources sat at x + o; entries carry the image
+                int64_t covered = 0;
+                for (int im = 0; im < nim; im++) {
+                    const double o[3] = {(im / (f * f) - h) * p.box, ((im / f) % f - h) * p.box,
+                                         (im % f - h) * p.box};
+                    const double gc[4] = {l.gcenter[4 * g] - o[0], l.gcenter[4 * g + 1] - o[1],
+                                          l.gcenter[4 * g + 2] - o[2], l.gcenter[4 * g + 3]};
+                    covered += mac.walk(
+                        gc, stack,
+                        [&](int k, int32_t c) {
+                            B.v[k].push_back((uint32_t)c);
+                            if (images)
+                                B.im[k].push_back((uint8_t)im);
+                        },
+                        [&](int64_t f0, int32_t n) {
+                            for (int64_t i = f0; i < f0 + n; i++) B.v[L_BODY].push_back((uint32_t)i);
+                            if (images)
+                                B.im[L_BODY].insert(B.im[L_BODY].end(), (size_t)n, (uint8_t)im);
+                        });
+                }
+                if (covered != t.n * nim)  // nln/mac.c:734: Nimage * N interactions per sink
                     miscount.store(true);
                 const int64_t *st = &B.start[B.start.size() - NLIST];
                 for (int k = 0; k < NLIST; k++)
@@ -366,7 +387,7 @@ void build_lists(const Tree &t, const TreeParams &p, Lists &l) {
         for (auto &x : th) x.join();
     }
     if (miscount.load())
-        throw std::runtime_error("a group's sources do not cover N bodies (nln/mac.c:734)");
+        throw std::runtime_error("a group's sources do not cover N bodies per image (nln/mac.c:734)");
     // assemble CSR in group order
     std::vector<int32_t> owner(G);
     std::vector<int64_t> slot(G);
@@ -393,6 +414,7 @@ void build_lists(const Tree &t, const TreeParams &p, Lists &l) {
         }
     for (int k = 0; k < NLIST; k++) {
         l.idx[k].resize(l.off[k][G]);
+        l.img[k].resize(images ? l.off[k][G] : 0);
         std::vector<std::thread> th;
         const int nt = std::min(nth, 16);
         for (int t0 = 0; t0 < nt; t0++)
@@ -402,6 +424,9 @@ void build_lists(const Tree &t, const TreeParams &p, Lists &l) {
                     seg(g, k, a, b);
                     std::copy(bufs[owner[g]].v[k].begin() + a, bufs[owner[g]].v[k].begin() + b,
                               l.idx[k].begin() + l.off[k][g]);
+                    if (images)
+                        std::copy(bufs[owner[g]].im[k].begin() + a, bufs[owner[g]].im[k].begin() + b,
+                                  l.img[k].begin() + l.off[k][g]);
                 }
             });
         for (auto &x : th) x.join();

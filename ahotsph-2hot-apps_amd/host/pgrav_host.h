@@ -15,7 +15,10 @@ struct TreeParams {
     int order = 4;         // highest multipole order used for cells (0, 2, 4)
     int pcut = 0;          // cells with fewer bodies than this go to the body list
                            // (0: automatic, the point where a cell costs less than its bodies)
-    int periodic = 0;
+    int periodic = 0;      // 1: nearest image (Ewald correction per pair in the kernel);
+                           // 2: the (2 nimage + 1)^3 images listed explicitly (far field
+                           //    by the local expansion, nln/mac.c:91-111 + nln/main.c:1811)
+    int nimage = 2;        // explicit images per side for periodic = 2 (nln/main.c:246)
     double box = 1.0;      // periodic box size (origin at 0)
     int nthreads = 0;      // 0: hardware concurrency
     int merge_groups = 1;  // 0: maximal cells <= group_max; 1: adjacent small siblings merged;
@@ -50,6 +53,10 @@ struct Tree {
 
 enum ListKind { L_HEXA = 0, L_QUAD = 1, L_MONO = 2, L_BODY = 3, NLIST = 4 };
 
+// periodic = 2: each list entry has an image, (i + n) (2n+1)^2 + (j + n) (2n+1) + (k + n)
+// for the offset (i, j, k) box (nln/mac.c:91-111 numbering), in a parallel u8 array
+const int MAX_NIMAGE = 2;  // (2*2+1)^3 = 125 images
+
 struct Lists {
     int64_t ngroups = 0;
     std::vector<int64_t> gfirst;
@@ -57,6 +64,7 @@ struct Lists {
     std::vector<double> gcenter;  // 4 per group: centre x, y, z and radius
     std::vector<int64_t> off[NLIST];   // ngroups + 1 offsets per list kind
     std::vector<uint32_t> idx[NLIST];  // node indices (cells) or sorted body slots
+    std::vector<uint8_t> img[NLIST];   // periodic = 2: image of each entry (else empty)
     int64_t pairs[NLIST] = {0, 0, 0, 0};  // (sink, source) pairs per kind
     std::vector<int32_t> lpt;     // dispatch order: NXCD slices, longest-first within each
     std::vector<int32_t> xoff;    // NXCD + 1 slice offsets into lpt
@@ -79,5 +87,13 @@ void prepare_cells(const Tree &t, int order, std::vector<float> &rec32, std::vec
 int default_threads(int requested);
 // estimated VALU instructions of one wave over a group's lists (entries per kind)
 double wave_cost(int64_t hexa, int64_t quad, int64_t mono, int64_t body);
+
+// Far field of the explicit-image scheme (host/ewald_le.cpp, csrc/pgrav_le.h):
+// lattice coefficient c_k, the moments-to-coefficients matrix (LE_NMON^2, row
+// major) and an FP64 evaluation at the bodies (acc4 per body, G applied).
+double le_lattice_coeff(int k, int nimage);
+void le_matrix(int nimage, std::vector<double> &W);
+void le_evaluate_host(const double *pos3, const double *mass, int64_t n, double box, int nimage,
+                      double G, double *acc4);
 
 }  // namespace pgrav

@@ -27,7 +27,13 @@ class Params(ctypes.Structure):
                 ("soft_type", ctypes.c_int), ("eps", ctypes.c_double), ("G", ctypes.c_double),
                 ("periodic", ctypes.c_int), ("box", ctypes.c_double), ("nthreads", ctypes.c_int),
                 ("merge_groups", ctypes.c_int), ("persistent_blocks", ctypes.c_int),
-                ("fp64", ctypes.c_int), ("dispatch", ctypes.c_int), ("mixed_order", ctypes.c_int)]
+                ("fp64", ctypes.c_int), ("dispatch", ctypes.c_int), ("mixed_order", ctypes.c_int),
+                ("nimage", ctypes.c_int)]
+
+# periodic boundary handling: none; nearest image + Ewald correction per pair
+# (nln/ewald.c); explicit (2 nimage + 1)^3 images + local expansion of the far
+# images (nln/mac.c:91-111, nln/main.c:1811-1862)
+PERIODIC = {False: 0, True: 1, "none": 0, "ewald": 1, "images": 2, 0: 0, 1: 1, 2: 2}
 
 
 class Stats(ctypes.Structure):
@@ -64,6 +70,7 @@ register({
     "pgrav_host_stats": (_i, [_vp, _S]),
     "pgrav_host_group": (_i, [_vp, _i64, _vp, _vp, _vp]),
     "pgrav_host_list": (_i64, [_vp, _i64, _i, _vp, _i64]),
+    "pgrav_host_list_images": (_i64, [_vp, _i64, _i, _vp, _i64]),
     "pgrav_host_groups": (_i, [_vp, _vp, _vp, _vp]),
     "pgrav_host_csr": (_i64, [_vp, _i, _vp, _vp]),
     "pgrav_host_cell": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -71,18 +78,23 @@ register({
     "pgrav_host_records": (_i, [_vp, _vp, _i64, _vp]),
     "pgrav_host_sorted": (_i, [_vp, _vp, _vp]),
     "pgrav_host_lpt": (_i, [_vp, _vp]),
+    "pgrav_host_ewald_le": (_i, [_vp, _vp, _i64, _d, _i, _d, _vp]),
+    "pgrav_host_le_lattice": (_i, [_i, _i, ctypes.POINTER(_d)]),
 })
 
 
 def make_params(theta=0.6, order=4, eps=1e-4, soft="plummer", G=1.0, group_max=64, leaf_max=16,
                 pcut=0, periodic=False, box=1.0, nthreads=0, merge_groups=True,
-                persistent_blocks=-1, fp64=True, dispatch=1, mixed_order=False):
+                persistent_blocks=-1, fp64=True, dispatch=1, mixed_order=False, nimage=2):
     p = Params()
     lib().pgrav_params_default(ctypes.byref(p))
     p.theta, p.order, p.eps, p.G = theta, order, eps, G
     p.soft_type = SOFT[soft] if isinstance(soft, str) else int(soft)
     p.group_max, p.leaf_max, p.pcut = group_max, leaf_max, pcut
-    p.periodic, p.box, p.nthreads = int(periodic), box, nthreads
+    if periodic not in PERIODIC:
+        raise ValueError("periodic must be one of %s" % sorted(map(str, PERIODIC)))
+    p.periodic, p.box, p.nthreads = PERIODIC[periodic], box, nthreads
+    p.nimage = nimage
     p.merge_groups, p.persistent_blocks, p.fp64 = int(merge_groups), persistent_blocks, int(fp64)
     p.dispatch = dispatch
     p.mixed_order = int(mixed_order)
@@ -261,6 +273,16 @@ class HostPlan:
         lib().pgrav_host_list(self.h, g, k, out.ctypes.data, n)
         return out
 
+    def list_images(self, g, kind):
+        """periodic = 2: the image of each entry of list(g, kind), numbered
+        (i + n)(2n + 1)^2 + (j + n)(2n + 1) + (k + n) for the offset (i, j, k) box."""
+        k = KINDS.index(kind) if isinstance(kind, str) else kind
+        n = lib().pgrav_host_list_images(self.h, g, k, None, 0)
+        check()
+        out = np.empty(n, np.uint8)
+        lib().pgrav_host_list_images(self.h, g, k, out.ctypes.data, n)
+        return out
+
     def cell(self, node):
         c = np.empty(3, np.float64)
         raw = np.empty(35, np.float64)
@@ -320,6 +342,29 @@ class HostPlan:
         lib().pgrav_host_lpt(self.h, o.ctypes.data)
         check()
         return o
+
+
+def ewald_le_host(pos, mass, box=1.0, nimage=2, G=1.0):
+    """Far images + background of the explicit-image scheme at each body, FP64 on
+    the host: (n, 4) = (a, phi), what FixCubeEwaldLE adds (nln/main.c:1836-1860)."""
+    pos, mass = _as_inputs(pos, mass)
+    out = np.zeros((pos.shape[0], 4))
+    rc = lib().pgrav_host_ewald_le(pos.ctypes.data, mass.ctypes.data, pos.shape[0], float(box),
+                                   int(nimage), float(G), out.ctypes.data)
+    check()
+    if rc:
+        raise PgravError("pgrav_host_ewald_le failed")
+    return out
+
+
+def le_lattice(k, nimage):
+    """x^k coefficient c_k of the far images' sum of 1/|r - n| (k = 4, 6, 8)."""
+    c = ctypes.c_double()
+    rc = lib().pgrav_host_le_lattice(int(k), int(nimage), ctypes.byref(c))
+    check()
+    if rc:
+        raise PgravError("pgrav_host_le_lattice failed")
+    return c.value
 
 
 def rel_errors(acc, ref):

@@ -2,6 +2,7 @@
 
     python -m pgrav.study c3 [--n 16777216] [--thetas 0.5 0.6 0.7]
     python -m pgrav.study c4 [--ng 256] [--theta 0.6 | --thetas 0.4 0.5 0.6]
+                             [--scheme ewald | images --nimage 2]
     python -m pgrav.study c5 [--n 1048576]
 
 c3  clustered bodies (pgrav.ics.clustered), order 4, Plummer eps 1e-4, for each
@@ -10,9 +11,11 @@ c3  clustered bodies (pgrav.ics.clustered), order 4, Plummer eps 1e-4, for each
     variants and of the batched FP64 variant, each with its force error against
     the FP64 direct sum on 8192 sampled bodies.
 c4  periodic unit box, ng^3 lattice displaced by a Gaussian random field
-    (pgrav.ics.lattice_grf), 64^3 Ewald table: eager and graph-replayed kernel
-    time, error against the periodic FP64 direct sum (nearest image + Ewald
-    table) on 1024 sampled bodies, net momentum.
+    (pgrav.ics.lattice_grf): eager and graph-replayed evaluation time, error
+    against the periodic FP64 direct sum (nearest image + 64^3 Ewald table) on
+    1024 sampled bodies, net momentum.  --scheme ewald: nearest image + the
+    Ewald correction per pair; images: (2 nimage + 1)^3 explicit images + the
+    far field by the local expansion (the reference's FixCubeEwaldLE scheme).
 c5  clustered N bodies, tree and lists fixed: forward and adjoint wall time, and
     the adjoint against central finite differences (h = 1e-5 and h/2, combined
     by Richardson extrapolation; FP64 forward) of L = sum_i w_i.a_i + v_i phi_i
@@ -82,8 +85,8 @@ def c4(args):
     n = pos.shape[0]
     dec = _decomposition(n, args.theta, "lattice")
     t0 = time.perf_counter()
-    plan = batch.Plan(pos, mass, periodic=True, box=1.0, theta=args.theta, order=4, eps=args.eps,
-                      soft="plummer", fp64=False, **dec)
+    plan = batch.Plan(pos, mass, periodic=args.scheme, nimage=args.nimage, box=1.0,
+                      theta=args.theta, order=4, eps=args.eps, soft="plummer", fp64=False, **dec)
     st = plan.stats()
     _log("c4 %d^3: plan %.1fs, %d groups, %.3e pairs" % (args.ng, time.perf_counter() - t0,
                                                          st["ngroups"], st["pairs_total"]))
@@ -106,7 +109,10 @@ def c4(args):
     rms = float(np.sqrt((ref[:, :3] ** 2).sum(1).mean()))
     net = np.abs((mass[:, None] * acc[:, :3]).sum(0)).max()
     typical = (mass[:, None] * np.abs(acc[:, :3])).sum()
-    row = {"config": "c4", "n": int(n), "theta": args.theta, "eps": args.eps, "decomposition": dec,
+    row = {"config": "c4", "n": int(n), "theta": args.theta, "eps": args.eps,
+           "scheme": args.scheme, "nimage": args.nimage if args.scheme == "images" else None,
+           "entries": {k: st["entries_" + k] for k in batch.KINDS},
+           "host_lists_s": st["t_lists"], "decomposition": dec,
            "groups": st["ngroups"], "pairs": st["pairs_total"], "kernel_ms_eager": eager,
            "ms_graph_replay_wall": graph, "interactions_per_s": st["pairs_total"] / (eager * 1e-3),
            "errors_vs_direct_ewald": {"median_over_rms": float(np.median(err) / rms),
@@ -188,6 +194,8 @@ def main(argv=None):
     ap.add_argument("--theta", type=float, default=0.6)
     ap.add_argument("--eps", type=float, default=1e-4)
     ap.add_argument("--thetas", type=float, nargs="+", default=None)
+    ap.add_argument("--scheme", choices=["ewald", "images"], default="ewald")
+    ap.add_argument("--nimage", type=int, default=2)
     a = ap.parse_args(argv)
     a.thetas_given = a.thetas is not None
     a.thetas = a.thetas or [0.5, 0.6, 0.7]

@@ -129,7 +129,10 @@ typedef struct {
     int soft_type;         /* 0 none, 1 Plummer, 4 Dehnen K1 (nln/mac.c:196-237) */
     double eps;            /* softening length */
     double G;              /* Newton's constant applied to the results (nln/mac.c:269-277) */
-    int periodic;          /* 1: unit-cell images + Ewald correction table */
+    int periodic;          /* 0 isolated; 1: nearest image + Ewald correction table per pair
+                              (nln/ewald.c); 2: the (2 nimage + 1)^3 images listed explicitly
+                              (nln/mac.c:91-111) and the far images by the cubic local
+                              expansion + background (nln/main.c:1811-1862, ewald_le2.c) */
     double box;            /* periodic box edge */
     int nthreads;          /* host threads for the build (0: all) */
     int merge_groups;      /* merge adjacent small sibling groups */
@@ -142,6 +145,8 @@ typedef struct {
                               opening-angle boundary, theta^(order+1): order 2 beyond
                               bmax * theta^(-(order+1)/3), order 0 beyond bmax * theta^-(order+1)
                               (the order-by-distance choice of nln/mac.c:1797-1810); 0: off */
+    int nimage;            /* periodic = 2: explicit images per side, 0..2 (default 2,
+                              nln/main.c:246) */
 } pgrav_params;
 
 typedef struct {
@@ -211,6 +216,9 @@ void pgrav_host_destroy(void *h);
 int pgrav_host_stats(void *h, pgrav_stats *st);
 int pgrav_host_group(void *h, int64_t g, int64_t *first, int *count, double *center4);
 int64_t pgrav_host_list(void *h, int64_t g, int kind, uint32_t *out, int64_t cap);
+/* periodic = 2: the image of each entry of that list, (i + n)(2n+1)^2 + (j + n)(2n+1)
+ * + (k + n) for the source offset (i, j, k) x box (nln/mac.c:91-111 numbering). */
+int64_t pgrav_host_list_images(void *h, int64_t g, int kind, uint8_t *out, int64_t cap);
 /* All groups at once: first body, count and FP64 centre + radius (arrays of ngroups). */
 int pgrav_host_groups(void *h, int64_t *first, int *count, double *center4);
 /* One list kind for all groups as CSR: off[ngroups+1], idx[off[ngroups]]; returns
@@ -222,6 +230,16 @@ int pgrav_host_cell(void *h, int64_t node, double *cofm3, double *bmax, double *
 int pgrav_host_record(void *h, int64_t node, double *rec29);
 int pgrav_host_records(void *h, const uint32_t *nodes, int64_t n, double *rec29);
 int pgrav_host_sorted(void *h, int64_t *perm, double *pos3);
+/* Far field of the explicit-image scheme on the host, in FP64 (the check for
+ * the device pass of periodic = 2 plans): for bodies in [0, box)^3, acc4[i] =
+ * the images beyond the (2 nimage + 1)^3 listed ones (cubic local expansion
+ * to degree 8) plus the uniform-background terms, times G -- what
+ * FixCubeEwaldLE adds (nln/main.c:1836-1860; ewald_le, ewald_background in
+ * nln/ewald_le2.c).  pgrav_host_le_lattice returns the lattice coefficient c_k
+ * (k = 4, 6, 8): the x^k coefficient of the far images' 1/r sum. */
+int pgrav_host_ewald_le(const double *pos3, const double *mass, int64_t n, double box, int nimage,
+                        double G, double *acc4);
+int pgrav_host_le_lattice(int k, int nimage, double *ck);
 int pgrav_host_lpt(void *h, int32_t *order);
 
 #ifdef __cplusplus
