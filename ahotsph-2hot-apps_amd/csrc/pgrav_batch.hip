@@ -136,8 +136,10 @@ __device__ __forceinline__ V4<T> source_offset(double4 c, double4 gc, double box
     return V4<T>{T(dx), T(dy), T(dz), T(c.w)};
 }
 
-template <typename T, bool REL, bool COMP, int SOFT, bool PERIODIC>
+// PER: 0 isolated, 1 nearest image + Ewald correction, 2 explicit images
+template <typename T, bool REL, bool COMP, int SOFT, int PER>
 __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds) {
+    constexpr bool PERIODIC = PER == 1;
     const int lane = threadIdx.x;
     const int64_t first = A.gfirst[g];
     const int cnt = A.gcount[g];
@@ -173,7 +175,7 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
             if (lane < nt) {
                 const uint32_t e = idx[t0 + lane];
                 double4 src = kind == 3 ? A.bpos[e] : A.cpos[e];
-                if (A.img) {
+                if (PER == 2) {
                     // image (i, j, k) in nln/mac.c:91-111 order: the source sits at
                     // x + (i, j, k) box, added in FP64
                     const int im = A.img[t0 + lane], f = A.img_f, h = f >> 1;
@@ -255,17 +257,17 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
 // Minimum waves per SIMD asked of the register allocator: left to it, the FP32
 // periodic kernels keep the Ewald records in flight with ~80 VGPRs (6 waves);
 // asked for 8 they fit in 64 (the K1 persistent variant spills 4 registers).
-template <typename T, bool PERIODIC>
+template <typename T, int PER>
 struct MinWaves {
-    static constexpr int value = (sizeof(T) == 4 && PERIODIC) ? 8 : 1;
+    static constexpr int value = (sizeof(T) == 4 && PER == 1) ? 8 : 1;
 };
 
 // one workgroup per group, groups taken in longest-first order
-template <typename T, bool REL, bool COMP, int SOFT, bool PERIODIC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PERIODIC>::value)))
+template <typename T, bool REL, bool COMP, int SOFT, int PER>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PER>::value)))
 void k_batch(BatchArgs A) {
     __shared__ V4<T> lds[64];
-    eval_group<T, REL, COMP, SOFT, PERIODIC>(A, A.order_np[blockIdx.x], lds);
+    eval_group<T, REL, COMP, SOFT, PER>(A, A.order_np[blockIdx.x], lds);
 }
 
 // persistent: a fixed grid drains per-XCD work counters.  Workgroup b runs on
@@ -280,8 +282,8 @@ void k_batch(BatchArgs A) {
 // a group's results and bumps the counters before it reads the next group's
 // lists and records, and only pointers known to be distinct let the compiler
 // keep those reads on wave-uniform scalar loads.
-template <typename T, bool REL, bool COMP, int SOFT, bool PERIODIC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PERIODIC>::value)))
+template <typename T, bool REL, bool COMP, int SOFT, int PER>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PER>::value)))
 void k_batch_persistent(
     BatchArgs A, const double4 *__restrict__ bpos, const double4 *__restrict__ cpos,
     const T *__restrict__ mom, const int64_t *__restrict__ gfirst, const int *__restrict__ gcount,
@@ -316,7 +318,7 @@ void k_batch_persistent(
         __syncthreads();
         if (item >= hi)
             break;
-        eval_group<T, REL, COMP, SOFT, PERIODIC>(A, A.lpt[item], lds);
+        eval_group<T, REL, COMP, SOFT, PER>(A, A.lpt[item], lds);
     }
     if (threadIdx.x == 0) {
         const int done = atomicAdd(&A.counter[1], 1);
@@ -328,10 +330,10 @@ void k_batch_persistent(
 }
 
 // the FP64 "plain" reference dispatch: one launch per group (nln/pgrav.cu:1078)
-template <int SOFT, bool PERIODIC>
+template <int SOFT, int PER>
 __global__ __launch_bounds__(64) void k_group64(BatchArgs A, int g) {
     __shared__ V4<double> lds[64];
-    eval_group<double, false, false, SOFT, PERIODIC>(A, g, lds);
+    eval_group<double, false, false, SOFT, PER>(A, g, lds);
 }
 
 __global__ void k_reset_counter(int *c) {
@@ -488,7 +490,7 @@ hipError_t launch_reset_counter(int *c, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <typename T, bool REL, bool COMP, int SOFT, bool PER>
+template <typename T, bool REL, bool COMP, int SOFT, int PER>
 static void launch_one(const BatchArgs &A, int persistent_blocks, hipStream_t st) {
     // every XCD needs workgroups to drain its slice: round up to a multiple of NXCD
     persistent_blocks = (persistent_blocks + NXCD - 1) / NXCD * NXCD;
@@ -501,20 +503,22 @@ static void launch_one(const BatchArgs &A, int persistent_blocks, hipStream_t st
         hipLaunchKernelGGL((k_batch<T, REL, COMP, SOFT, PER>), dim3(A.ngroups), dim3(64), 0, st, A);
 }
 
+template <typename T, bool REL, bool COMP, int SOFT>
+static void launch_per(const BatchArgs &A, int per, int pb, hipStream_t st) {
+    if (per == 1)
+        launch_one<T, REL, COMP, SOFT, 1>(A, pb, st);
+    else if (per == 2)
+        launch_one<T, REL, COMP, SOFT, 2>(A, pb, st);
+    else
+        launch_one<T, REL, COMP, SOFT, 0>(A, pb, st);
+}
+
 template <typename T, bool REL, bool COMP>
-static void launch_soft(const BatchArgs &A, bool periodic, int pb, hipStream_t st) {
+static void launch_soft(const BatchArgs &A, int per, int pb, hipStream_t st) {
     switch (A.soft) {
-    case SOFT_K1:
-        if (periodic) launch_one<T, REL, COMP, SOFT_K1, true>(A, pb, st);
-        else launch_one<T, REL, COMP, SOFT_K1, false>(A, pb, st);
-        break;
-    case SOFT_PLUMMER:
-        if (periodic) launch_one<T, REL, COMP, SOFT_PLUMMER, true>(A, pb, st);
-        else launch_one<T, REL, COMP, SOFT_PLUMMER, false>(A, pb, st);
-        break;
-    default:
-        if (periodic) launch_one<T, REL, COMP, SOFT_NONE, true>(A, pb, st);
-        else launch_one<T, REL, COMP, SOFT_NONE, false>(A, pb, st);
+    case SOFT_K1: launch_per<T, REL, COMP, SOFT_K1>(A, per, pb, st); break;
+    case SOFT_PLUMMER: launch_per<T, REL, COMP, SOFT_PLUMMER>(A, per, pb, st); break;
+    default: launch_per<T, REL, COMP, SOFT_NONE>(A, per, pb, st);
     }
 }
 
@@ -539,12 +543,13 @@ hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, 
     A.eps = v.eps;
     A.G = v.G;
     A.box = v.box;
-    A.img = v.periodic == 2 ? v.img : nullptr;
+    A.img = v.img;
     A.img_f = 2 * v.nimage + 1;
     if (A.ngroups <= 0)
         return hipSuccess;
-    // explicit images run the isolated kernels (the offsets come with the entries)
-    const bool per = v.periodic == 1;
+    if (v.periodic < 0 || v.periodic > 2 || (v.periodic == 2 && !v.img))
+        return hipErrorInvalidValue;
+    const int per = v.periodic;
     switch (variant) {
     case VAR_REL32C: launch_soft<float, true, true>(A, per, persistent_blocks, st); break;
     case VAR_REL32: launch_soft<float, true, false>(A, per, persistent_blocks, st); break;
@@ -576,12 +581,15 @@ hipError_t launch_group64(const BatchView &v, int g, hipStream_t st) {
     A.eps = v.eps;
     A.G = v.G;
     A.box = v.box;
-    A.img = v.periodic == 2 ? v.img : nullptr;
+    A.img = v.img;
     A.img_f = 2 * v.nimage + 1;
-    const bool per = v.periodic == 1;
+    if (v.periodic < 0 || v.periodic > 2 || (v.periodic == 2 && !v.img))
+        return hipErrorInvalidValue;
+    const int per = v.periodic;
 #define G64(S) \
-    do { if (per) hipLaunchKernelGGL((k_group64<S, true>), dim3(1), dim3(64), 0, st, A, g); \
-         else hipLaunchKernelGGL((k_group64<S, false>), dim3(1), dim3(64), 0, st, A, g); } while (0)
+    do { if (per == 1) hipLaunchKernelGGL((k_group64<S, 1>), dim3(1), dim3(64), 0, st, A, g); \
+         else if (per == 2) hipLaunchKernelGGL((k_group64<S, 2>), dim3(1), dim3(64), 0, st, A, g); \
+         else hipLaunchKernelGGL((k_group64<S, 0>), dim3(1), dim3(64), 0, st, A, g); } while (0)
     switch (v.soft) {
     case SOFT_K1: G64(SOFT_K1); break;
     case SOFT_PLUMMER: G64(SOFT_PLUMMER); break;
