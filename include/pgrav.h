@@ -242,6 +242,42 @@ int pgrav_host_ewald_le(const double *pos3, const double *mass, int64_t n, doubl
 int pgrav_host_le_lattice(int k, int nimage, double *ck);
 int pgrav_host_lpt(void *h, int32_t *order);
 
+/* ------------------------------------------------------------------ part 4
+ * The step either side of the force evaluation: the leapfrog kick-drift of the
+ * body table.  Fields are float arrays with a common byte stride (the
+ * reference passes &btab[0].mass, &btab[0].pos[0], ... and sizeof(body)); in
+ * and out may be the same table.  Arithmetic is the reference's: FP64 steps,
+ * results rounded to float, no fused multiply-adds. */
+/* nln/integrate.h:5-20 (Integrate), same arguments, host pointers: the tables'
+ * spans are copied to the GPU and back.  vel += dtv acc; pos += dt vel;
+ * ke = 1/2 sum m |vel + (tpos - tvel) acc|^2, pe = 1/2 sum m phi; tvel += dtv,
+ * tpos += dt. */
+int pgrav_integrate(const float *inmass, const float *inpos, const float *invel, const float *inacc,
+                    const float *inphi, int instride, float *outpos, float *outvel, int outstride,
+                    int n, double dt, double dtv, double *tpos, double *tvel, double *kep,
+                    double *pep);
+/* The same on device pointers (a table resident in HBM), on `stream` (NULL: the
+ * null stream); returns after kep / pep are known. */
+int pgrav_integrate_device(const float *inmass, const float *inpos, const float *invel,
+                           const float *inacc, const float *inphi, int instride, float *outpos,
+                           float *outvel, int outstride, int n, double dt, double dtv, double *tpos,
+                           double *tvel, double *kep, double *pep, void *stream);
+/* nln/integrate.c:82-135 (CosmoIntegrate) on device pointers, with the
+ * cosmology's factors passed in (the reference computes them from its
+ * cosmology object, libcosmo, which is not part of this path):
+ * a0 = a(tpos), a1 = a(tpos + dt), dt_kick_topos = a0^2 kick(tvel, tpos),
+ * dt_kick = a0^2 kick(tvel, tvel + dtv), dt_drift = drift(tpos, tpos + dt).
+ * The caller advances tpos and tvel. */
+int pgrav_cosmo_integrate_device(const float *inmass, const float *inpos, const float *invel,
+                                 const float *inacc, const float *inphi, int instride,
+                                 float *outpos, float *outvel, int outstride, int n, double a0,
+                                 double a1, double dt_kick_topos, double dt_kick, double dt_drift,
+                                 double *kep, double *pep, void *stream);
+/* nln/integrate.c:62-80 (ConvertV) on a device table: vel /= scale
+ * (to_physical) or vel *= scale. */
+int pgrav_convert_v_device(float *vel, int stride, int n, double scale, int to_physical,
+                           void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -166,6 +166,37 @@ def direct_ewald(sink_pos, src_pos, src_mass, L=1.0, nthreads=None):
     return out
 
 
+def _fields(table, names):
+    base = table.ctypes.data
+    return [ctypes.c_void_p(base + table.dtype.fields[f][1]) for f in names]
+
+
+def integrate(table, dt, dtv, tpos, tvel, out=None, cosmo=None):
+    """nln/integrate.c Integrate (cosmo None) or CosmoIntegrate (cosmo = (a0, a1,
+    dt_kick_topos, dt_kick, dt_drift)) on a numpy structured table with fields
+    mass, pos, vel, acc, phi; `out` (fields pos, vel) defaults to the table
+    itself.  Returns (tpos, tvel, ke, pe)."""
+    out = table if out is None else out
+    L = lib()
+    ke, pe = _d(0.0), _d(0.0)
+    fi = _fields(table, ("mass", "pos", "vel", "acc", "phi"))
+    fo = _fields(out, ("pos", "vel"))
+    if cosmo is None:
+        tp, tv = _d(tpos), _d(tvel)
+        L.orc_integrate(*fi, table.dtype.itemsize, *fo, out.dtype.itemsize, table.shape[0],
+                        _d(dt), _d(dtv), ctypes.byref(tp), ctypes.byref(tv), ctypes.byref(ke),
+                        ctypes.byref(pe))
+        return tp.value, tv.value, ke.value, pe.value
+    L.orc_cosmo_integrate(*fi, table.dtype.itemsize, *fo, out.dtype.itemsize, table.shape[0],
+                          *[_d(c) for c in cosmo], ctypes.byref(ke), ctypes.byref(pe))
+    return tpos, tvel, ke.value, pe.value
+
+
+def convert_v(table, scale, to_physical):
+    lib().orc_convert_v(*_fields(table, ("vel",)), table.dtype.itemsize, table.shape[0],
+                        _d(scale), int(to_physical))
+
+
 def cell_record(pos, mass, center, bmax):
     rec = np.zeros(HSZ, np.float64)
     p = _c(pos, np.float64).reshape(-1)

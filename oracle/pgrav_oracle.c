@@ -544,3 +544,81 @@ void orc_cell_record(const double *pos3, const double *mass, int64_t n, const do
     rec[ORC_QXYYZ] = (u[XY2Z] - txz) * r4;
     rec[ORC_QYYYZ] = (u[Y3Z] - 3.0 * tyz) * r4;
 }
+
+/* ------------------------------------------------------------ integrator */
+#define FLD(p, i, stride) ((const float *)((const char *)(p) + (size_t)(i) * (size_t)(stride)))
+#define OFLD(p, i, stride) ((float *)((char *)(p) + (size_t)(i) * (size_t)(stride)))
+
+/* nln/integrate.c:16-59 */
+void orc_integrate(const float *inmass, const float *inpos, const float *invel, const float *inacc,
+                   const float *inphi, int instride, float *outpos, float *outvel, int outstride,
+                   int n, double dt, double dtv, double *tpos, double *tvel, double *kep,
+                   double *pep) {
+    const double dtsync = *tpos - *tvel; /* :32 */
+    double ke = 0.0, pe = 0.0;
+    for (int i = 0; i < n; i++) {
+        const float m = *FLD(inmass, i, instride), phi = *FLD(inphi, i, instride);
+        const float *x = FLD(inpos, i, instride), *v = FLD(invel, i, instride),
+                    *a = FLD(inacc, i, instride);
+        float *ox = OFLD(outpos, i, outstride), *ov = OFLD(outvel, i, outstride);
+        double vc[3];
+        float nv[3], nx[3];
+        for (int k = 0; k < 3; k++) vc[k] = v[k] + dtsync * a[k];    /* :40 */
+        ke += m * (vc[0] * vc[0] + vc[1] * vc[1] + vc[2] * vc[2]);  /* :41 */
+        pe += m * phi;                                              /* :42, a float product */
+        for (int k = 0; k < 3; k++) nv[k] = (float)(v[k] + dtv * a[k]);   /* :43 */
+        for (int k = 0; k < 3; k++) nx[k] = (float)(x[k] + dt * nv[k]);   /* :44 */
+        for (int k = 0; k < 3; k++) {
+            ov[k] = nv[k];
+            ox[k] = nx[k];
+        }
+    }
+    *tvel += dtv; /* :54-57 */
+    *tpos += dt;
+    *kep = 0.5 * ke;
+    *pep = 0.5 * pe;
+}
+
+/* nln/integrate.c:82-135, the cosmology's factors given */
+void orc_cosmo_integrate(const float *inmass, const float *inpos, const float *invel,
+                         const float *inacc, const float *inphi, int instride, float *outpos,
+                         float *outvel, int outstride, int n, double a0, double a1,
+                         double dt_kick_topos, double dt_kick, double dt_drift, double *kep,
+                         double *pep) {
+    double ke = 0.0, pe = 0.0;
+    for (int i = 0; i < n; i++) {
+        const float m = *FLD(inmass, i, instride), phi = *FLD(inphi, i, instride);
+        const float *x = FLD(inpos, i, instride), *v = FLD(invel, i, instride),
+                    *a = FLD(inacc, i, instride);
+        float *ox = OFLD(outpos, i, outstride), *ov = OFLD(outvel, i, outstride);
+        double vc[3];
+        float nv[3], nx[3];
+        for (int k = 0; k < 3; k++) vc[k] = v[k] + dt_kick_topos * a[k];              /* :115 */
+        ke += m * (vc[0] * vc[0] + vc[1] * vc[1] + vc[2] * vc[2]) / (a0 * a0);       /* :116 */
+        pe += m * phi;                                                              /* :117 */
+        for (int k = 0; k < 3; k++) nv[k] = (float)(v[k] + dt_kick * a[k]);           /* :118 */
+        for (int k = 0; k < 3; k++) nx[k] = (float)((1.0 / a0) * x[k]);               /* :119 */
+        for (int k = 0; k < 3; k++) nx[k] = (float)(nx[k] + dt_drift * nv[k]);        /* :120 */
+        for (int k = 0; k < 3; k++) nx[k] = (float)(a1 * nx[k]);                      /* :121 */
+        for (int k = 0; k < 3; k++) {
+            ov[k] = nv[k];
+            ox[k] = nx[k];
+        }
+    }
+    *kep = 0.5 * ke;
+    *pep = 0.5 * pe;
+}
+
+/* nln/integrate.c:62-80 */
+void orc_convert_v(float *vel, int stride, int n, double scale, int to_physical) {
+    const float one_on_scale = (float)1.0 / scale;
+    for (int i = 0; i < n; i++) {
+        float *v = OFLD(vel, i, stride);
+        for (int k = 0; k < 3; k++) {
+            if (to_physical)
+                v[k] *= one_on_scale;
+            else
+                v[k] = (float)(v[k] * scale);
+        }
+    }
+}

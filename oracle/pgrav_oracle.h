@@ -101,6 +101,21 @@ void orc_cell_record(const double *pos3, const double *mass, int64_t n, const do
 void orc_raw_moments(const double *pos3, const double *mass, int64_t n, const double center[3],
                      double raw[35]);
 
+/* nln/integrate.c:16-59 (Integrate): leapfrog kick-drift of strided float
+ * fields, FP64 steps rounded to float; returns ke, pe and advances tpos, tvel. */
+void orc_integrate(const float *inmass, const float *inpos, const float *invel, const float *inacc,
+                   const float *inphi, int instride, float *outpos, float *outvel, int outstride,
+                   int n, double dt, double dtv, double *tpos, double *tvel, double *kep,
+                   double *pep);
+/* nln/integrate.c:82-135 (CosmoIntegrate) with the cosmology's factors given. */
+void orc_cosmo_integrate(const float *inmass, const float *inpos, const float *invel,
+                         const float *inacc, const float *inphi, int instride, float *outpos,
+                         float *outvel, int outstride, int n, double a0, double a1,
+                         double dt_kick_topos, double dt_kick, double dt_drift, double *kep,
+                         double *pep);
+/* nln/integrate.c:62-80 (ConvertV). */
+void orc_convert_v(float *vel, int stride, int n, double scale, int to_physical);
+
 #ifdef __cplusplus
 }
 #endif
