@@ -278,6 +278,41 @@ int pgrav_cosmo_integrate_device(const float *inmass, const float *inpos, const 
 int pgrav_convert_v_device(float *vel, int stride, int n, double scale, int to_physical,
                            void *stream);
 
+/* ------------------------------------------------------------------ part 5
+ * SDF snapshots of the body table (the format nln/output.c:270-423 writes
+ * through libSDF's SDFwrite64, laid out as sph+nln/wind.c:82-135 writes it):
+ * an ASCII header of typed declarations and a struct description, then the
+ * packed records.  Host-only. */
+#define PGRAV_SDF_INT 0
+#define PGRAV_SDF_INT64 1
+#define PGRAV_SDF_FLOAT 2
+#define PGRAV_SDF_DOUBLE 3
+#define PGRAV_SDF_STRING 4
+#define PGRAV_SDF_CHAR 5  /* struct members only */
+#define PGRAV_SDF_SHORT 6 /* struct members only */
+typedef struct {
+    const char *name;
+    int type;        /* PGRAV_SDF_INT .. PGRAV_SDF_STRING */
+    int64_t i;       /* INT, INT64 */
+    double d;        /* FLOAT, DOUBLE */
+    const char *s;   /* STRING */
+} pgrav_sdf_value;
+/* Writes the header values in order, `desc` ("struct {...}") with [nrec], and
+ * nrec records of recsize bytes. */
+int pgrav_sdf_write(const char *path, const pgrav_sdf_value *vals, int nvals, const char *desc,
+                    const void *records, int64_t nrec, int64_t recsize);
+/* Parses the header; NULL on error.  Strings returned stay valid until close. */
+void *pgrav_sdf_open(const char *path);
+void pgrav_sdf_close(void *h);
+int pgrav_sdf_nvalues(void *h);
+int pgrav_sdf_value_at(void *h, int k, pgrav_sdf_value *out);
+int64_t pgrav_sdf_nrecords(void *h);
+int64_t pgrav_sdf_recsize(void *h);
+int pgrav_sdf_nfields(void *h);
+/* Member k of the record struct (arrays are expanded to name[i]): byte offset, type. */
+int pgrav_sdf_field_at(void *h, int k, const char **name, int *type, int64_t *offset);
+int pgrav_sdf_read(void *h, int64_t first, int64_t count, void *out);
+
 #ifdef __cplusplus
 }
 #endif
