@@ -257,9 +257,12 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
 // Minimum waves per SIMD asked of the register allocator: left to it, the FP32
 // periodic kernels keep the Ewald records in flight with ~80 VGPRs (6 waves);
 // asked for 8 they fit in 64 (the K1 persistent variant spills 4 registers).
+#ifndef PGRAV_MINWAVES_FP32
+#define PGRAV_MINWAVES_FP32 1  // the FP32 kernels without the Ewald table
+#endif
 template <typename T, int PER>
 struct MinWaves {
-    static constexpr int value = (sizeof(T) == 4 && PER == 1) ? 8 : 1;
+    static constexpr int value = (sizeof(T) == 4 && PER == 1) ? 8 : (sizeof(T) == 4 ? PGRAV_MINWAVES_FP32 : 1);
 };
 
 // one workgroup per group, groups taken in longest-first order

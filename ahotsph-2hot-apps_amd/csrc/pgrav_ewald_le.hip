@@ -1,13 +1,15 @@
 // Far field of the explicit-image periodic scheme on the GPU, in FP64: the
 // pass nln/main.c:1811-1862 (FixCubeEwaldLE) runs on the host after the walk.
-// Three launches, all graph-capturable (no host round trip, no memset):
+// Four launches, all graph-capturable (no host round trip, no memset):
 //
 //   k_le_partials  global moments mu_gamma = sum_j m_j x_j^gamma (165 monomials,
 //                  degree <= 8, box units about the box centre): each workgroup
 //                  sums a contiguous range of the sorted bodies, 128 at a time
 //                  staged in LDS as m x^a, y^b, z^c; one thread per monomial
-//   k_le_coeffs    fixed-order sum of the partials, then C = W mu (W: the
-//                  lattice expansion and background, built on the host)
+//   k_le_reduce    fixed-order sum of the workgroups' partials, one workgroup
+//                  per monomial
+//   k_le_coeffs    C = W mu (W: the lattice expansion and background, built on
+//                  the host)
 //   k_le_apply     per body: the degree-8 polynomial sum C_beta x^beta and its
 //                  gradient, added to the force result with the G / L and
 //                  G / L^2 scales and the self term -xi m_i (nln/ewald_le2.c:44)
@@ -25,7 +27,6 @@ namespace pgrav {
 namespace {
 
 constexpr int LE_TILE = 128;
-
 __global__ __launch_bounds__(256) void k_le_partials(const double4 *__restrict__ bpos, int64_t n,
                                                      double box, int64_t chunk,
                                                      double *__restrict__ partial) {
@@ -70,20 +71,36 @@ __global__ __launch_bounds__(256) void k_le_partials(const double4 *__restrict__
         partial[(size_t)blockIdx.x * LE_NMON + t] = acc;
 }
 
-__global__ __launch_bounds__(256) void k_le_coeffs(const double *__restrict__ partial, int nblocks,
-                                                   const double *__restrict__ W,
-                                                   double *__restrict__ coef) {
-    __shared__ double mu[LE_NMON];
-    const int t = threadIdx.x;
-    if (t < LE_NMON) {
-        double s = 0.0;
-        for (int b = 0; b < nblocks; b++) s += partial[(size_t)b * LE_NMON + t];
-        mu[t] = s;
+// mu_k = sum over the workgroups' partials, one workgroup per monomial (fixed order)
+__global__ __launch_bounds__(256) void k_le_reduce(const double *__restrict__ partial, int nblocks,
+                                                   double *__restrict__ mu) {
+    __shared__ double r[256];
+    const int k = blockIdx.x, t = threadIdx.x;
+    double s = 0.0;
+    for (int b = t; b < nblocks; b += blockDim.x) s += partial[(size_t)b * LE_NMON + k];
+    r[t] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h)
+            r[t] += r[t + h];
+        __syncthreads();
     }
+    if (t == 0)
+        mu[k] = r[0];
+}
+
+// C = W mu; Wt is W transposed (Wt[g][beta]) so that the threads read along a row
+__global__ __launch_bounds__(256) void k_le_coeffs(const double *__restrict__ mu,
+                                                   const double *__restrict__ Wt,
+                                                   double *__restrict__ coef) {
+    __shared__ double m[LE_NMON];
+    const int t = threadIdx.x;
+    if (t < LE_NMON)
+        m[t] = mu[t];
     __syncthreads();
     if (t < LE_NMON) {
         double s = 0.0;
-        for (int g = 0; g < LE_NMON; g++) s += W[(size_t)t * LE_NMON + g] * mu[g];
+        for (int g = 0; g < LE_NMON; g++) s += Wt[(size_t)g * LE_NMON + t] * m[g];
         coef[t] = s;
     }
 }
@@ -134,7 +151,7 @@ __global__ __launch_bounds__(256) void k_le_apply(const double4 *__restrict__ bp
 
 }  // namespace
 
-hipError_t launch_ewald_le(const double4 *bpos, int64_t n, double box, double G, const double *W,
+hipError_t launch_ewald_le(const double4 *bpos, int64_t n, double box, double G, const double *Wt,
                            double *partial, int nblocks, double *coef, float4 *out32,
                            double4 *out64, hipStream_t st) {
     if (n <= 0)
@@ -142,8 +159,10 @@ hipError_t launch_ewald_le(const double4 *bpos, int64_t n, double box, double G,
     if (nblocks < 1 || nblocks > LE_MAX_BLOCKS || (out32 == nullptr) == (out64 == nullptr))
         return hipErrorInvalidValue;
     const int64_t chunk = (n + nblocks - 1) / nblocks;
+    double *mu = partial + (size_t)nblocks * LE_NMON;
     hipLaunchKernelGGL(k_le_partials, dim3(nblocks), dim3(256), 0, st, bpos, n, box, chunk, partial);
-    hipLaunchKernelGGL(k_le_coeffs, dim3(1), dim3(256), 0, st, partial, nblocks, W, coef);
+    hipLaunchKernelGGL(k_le_reduce, dim3(LE_NMON), dim3(256), 0, st, partial, nblocks, mu);
+    hipLaunchKernelGGL(k_le_coeffs, dim3(1), dim3(256), 0, st, mu, Wt, coef);
     const dim3 grid((unsigned)((n + 255) / 256));
     if (out32)
         hipLaunchKernelGGL(k_le_apply<float4>, grid, dim3(256), 0, st, bpos, n, box, G, coef, out32);

@@ -9,8 +9,9 @@ namespace pgrav {
 
 // Adds the far images and background of the explicit-image scheme, times G, to
 // the result of the sorted bodies bpos[0..n) (exactly one of out32 / out64).
-// W: LE_NMON x LE_NMON (le_matrix); partial: nblocks x LE_NMON; coef: LE_NMON.
-hipError_t launch_ewald_le(const double4 *bpos, int64_t n, double box, double G, const double *W,
+// Wt: the LE_NMON x LE_NMON matrix of le_matrix, transposed; partial: (nblocks + 1) x
+// LE_NMON (the last row receives the global moments); coef: LE_NMON.
+hipError_t launch_ewald_le(const double4 *bpos, int64_t n, double box, double G, const double *Wt,
                            double *partial, int nblocks, double *coef, float4 *out32,
                            double4 *out64, hipStream_t st);
 

@@ -363,11 +363,13 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         P->v.nimage = p->nimage;
         P->v.box = p->box;
         if (p->periodic == 2) {
-            std::vector<double> W;
+            std::vector<double> W, Wt((size_t)LE_NMON * LE_NMON);
             le_matrix(p->nimage, W);
-            P->le_W = P->upload(W.data(), W.size());
-            P->le_blocks = (int)std::min<int64_t>(1024, (n + 127) / 128);
-            P->le_partial = P->dalloc<double>(sizeof(double) * LE_NMON * P->le_blocks);
+            for (int b = 0; b < LE_NMON; b++)
+                for (int g = 0; g < LE_NMON; g++) Wt[(size_t)g * LE_NMON + b] = W[(size_t)b * LE_NMON + g];
+            P->le_W = P->upload(Wt.data(), Wt.size());
+            P->le_blocks = (int)std::min<int64_t>(2048, (n + 511) / 512);
+            P->le_partial = P->dalloc<double>(sizeof(double) * LE_NMON * (P->le_blocks + 1));
             P->le_coef = P->dalloc<double>(sizeof(double) * LE_NMON);
         }
         if (p->periodic) {
