@@ -118,6 +118,8 @@ static const int NQ = 32;  // nln/pgrav.cu:811
 struct Slot {
     bool inuse = false;
     hipStream_t stream = nullptr;
+    hipEvent_t staged = nullptr;  // recorded after the last copy out of `host`
+    bool staged_pending = false;
     HostBuf host;
     DevBuf dev;   // raw copy of the caller's list
     DevBuf pos4;  // prepared source positions
@@ -130,6 +132,7 @@ struct State {
     Slot slot[NQ];
     float *btab = nullptr;
     int64_t nobj = 0;
+    int seg_mismatch = 0;  // pMM_sK1 calls whose segments did not sum to source_n
     int stride = 0;
     DevBuf devpos, devacc, errflag;
     HostBuf staging;
@@ -156,7 +159,9 @@ static bool ensure_init() {
         return false;
     for (int i = 0; i < NQ; i++)
         if (!check(hipStreamCreateWithFlags(&S.slot[i].stream, hipStreamNonBlocking),
-                   "hipStreamCreate"))
+                   "hipStreamCreate") ||
+            !check(hipEventCreateWithFlags(&S.slot[i].staged, hipEventDisableTiming),
+                   "hipEventCreate"))
             return false;
     if (!S.errflag.ensure(sizeof(int)) || !check(hipMemset(S.errflag.p, 0, sizeof(int)), "hipMemset"))
         return false;
@@ -171,10 +176,41 @@ static bool slot_ok(int q) {
         fail("bad queue slot %d", q);
         return false;
     }
-    Slot &s = S.slot[q];
-    if (s.inuse && !check(hipStreamSynchronize(s.stream), "hipStreamSynchronize"))
+    // work already queued on the slot stays queued: the stream orders it before
+    // this call's, and stage() guards the host-side buffers
+    S.slot[q].inuse = true;
+    return true;
+}
+
+// Before a call rewrites a slot's pinned staging buffer or grows its buffers.
+// A slot can carry several calls before qallocCUDA sees it idle (the cell and
+// body lists of one sink group): the pinned buffer is rewritten once the copy
+// out of it has run (an event after that copy), and buffers grow only once the
+// slot's stream has drained.
+static bool stage(Slot &s, size_t host_bytes, size_t dev_bytes, size_t pos4_bytes = 0,
+                  size_t mom_bytes = 0) {
+    const bool grow = host_bytes > s.host.cap || dev_bytes > s.dev.cap ||
+                      pos4_bytes > s.pos4.cap || mom_bytes > s.mom.cap;
+    if (grow) {
+        if (!check(hipStreamSynchronize(s.stream), "hipStreamSynchronize"))
+            return false;
+        s.staged_pending = false;
+    } else if (s.staged_pending) {
+        if (!check(hipEventSynchronize(s.staged), "hipEventSynchronize"))
+            return false;
+        s.staged_pending = false;
+    }
+    return s.host.ensure(host_bytes) && s.dev.ensure(dev_bytes) &&
+           (!pos4_bytes || s.pos4.ensure(pos4_bytes)) && (!mom_bytes || s.mom.ensure(mom_bytes));
+}
+
+// queue the copy of the staged bytes to the slot's device buffer
+static bool upload(Slot &s, size_t bytes) {
+    if (!check(hipMemcpyAsync(s.dev.p, s.host.p, bytes, hipMemcpyHostToDevice, s.stream),
+               "hipMemcpyAsync") ||
+        !check(hipEventRecord(s.staged, s.stream), "hipEventRecord"))
         return false;
-    s.inuse = true;
+    s.staged_pending = true;
     return true;
 }
 
@@ -265,7 +301,11 @@ void CUDA_Finalize(void) {
     for (int i = 0; i < NQ; i++) {
         if (S.slot[i].stream)
             (void)hipStreamDestroy(S.slot[i].stream);
+        if (S.slot[i].staged)
+            (void)hipEventDestroy(S.slot[i].staged);
         S.slot[i].stream = nullptr;
+        S.slot[i].staged = nullptr;
+        S.slot[i].staged_pending = false;
         S.slot[i].inuse = false;
     }
     S.btab = nullptr;
@@ -315,6 +355,7 @@ void WalkInitSinkCUDA(float *btab, int stride, int64_t nobj) {
     wait_all();
     S.btab = btab;
     S.nobj = nobj;
+    S.seg_mismatch = 0;
     S.stride = stride;
     size_t pos_bytes = 3 * sizeof(float) * (size_t)(nobj ? nobj : 1);
     size_t acc_bytes = 4 * sizeof(float) * (size_t)(nobj ? nobj : 1);
@@ -329,8 +370,10 @@ void WalkInitSinkCUDA(float *btab, int stride, int64_t nobj) {
     if (!check(hipMemcpy(S.devpos.p, h, 3 * sizeof(float) * nobj, hipMemcpyHostToDevice),
                "hipMemcpy"))
         return;
-    check(hipMemset(S.devacc.p, 0, acc_bytes), "hipMemset");
-    check(hipMemset(S.errflag.p, 0, sizeof(int)), "hipMemset");
+    // cleared before any slot stream may add into the table
+    check(hipMemset(S.devacc.p, 0, acc_bytes), "hipMemset") &&
+        check(hipMemset(S.errflag.p, 0, sizeof(int)), "hipMemset") &&
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
 }
 
 void WalkTerminateSinkCUDA(float *btab, int stride, int64_t nobj) {
@@ -351,6 +394,8 @@ void WalkTerminateSinkCUDA(float *btab, int stride, int64_t nobj) {
     int err = 0;
     if (!check(hipMemcpy(&err, S.errflag.p, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy"))
         return;
+    err += S.seg_mismatch;
+    S.seg_mismatch = 0;
     for (int64_t i = 0; i < nobj; i++) {
         btab[i * stride + 7] += h[4 * i + 0];
         btab[i * stride + 8] += h[4 * i + 1];
@@ -375,6 +420,7 @@ void WalkInitSrcCUDA(float *qtab, int stride, int64_t nobj) {
         fail("WalkInitSrcCUDA: stride %d bytes is not a hexacell/quadcell record", stride);
         return;
     }
+    wait_all();  // queued calls may still read the previous cell table
     size_t bytes = (size_t)stride * (size_t)nobj;
     if (!S.qraw.ensure(bytes ? bytes : 16) || !S.qpos4.ensure(16 * (size_t)(nobj ? nobj : 1)) ||
         !S.qmom.ensure(PREC2 * 4 * (size_t)(nobj ? nobj : 1)))
@@ -426,13 +472,11 @@ static void mn_common(int order, int soft, const float *p, int m, int stride, co
     Slot &s = S.slot[q];
     size_t bytes = (size_t)source_n * sz * sizeof(float);
     const int mstride = order == 4 ? PREC4 : PREC2;
-    if (!s.host.ensure(bytes + 16) || !s.dev.ensure(bytes + 16) ||
-        !s.pos4.ensure(16 * (size_t)(source_n + 1)) ||
-        !s.mom.ensure(4 * (size_t)mstride * (source_n + 1)))
+    if (!stage(s, bytes + 16, bytes + 16, 16 * (size_t)(source_n + 1),
+               4 * (size_t)mstride * (source_n + 1)))
         return;
     memcpy(s.host.p, f, bytes);  // the caller may reuse f on return (nln/pgrav.cu:1053)
-    if (!check(hipMemcpyAsync(s.dev.p, s.host.p, bytes, hipMemcpyHostToDevice, s.stream),
-               "hipMemcpyAsync"))
+    if (!upload(s, bytes))
         return;
     if (!check(launch_prep_soa8((const float *)s.dev.p, source_n, sz, order, (float4 *)s.pos4.p,
                                 (float *)s.mom.p, s.stream),
@@ -496,22 +540,29 @@ void grav_mns_CUDA(const char *routine, const int base, const int m, const segme
     }
     if (!slot_ok(q) || !sinks_ready(base, m))
         return;
-    for (int g = 0; g < seg_n; g++)
+    int64_t total = 0;
+    for (int g = 0; g < seg_n; g++) {
         if ((int64_t)seg[g].base + seg[g].length > S.nobj) {
             fail("segment %d [%u,+%u) outside the body table", g, seg[g].base, seg[g].length);
             return;
         }
+        total += seg[g].length;
+    }
+    if (total != source_n)
+        S.seg_mismatch++;  // assert(sum_n == source_n), nln/pgrav.cu:636; reported at terminate
+    // the segments expanded into body indices: one coalesced load per 64 bodies
     Slot &s = S.slot[q];
-    size_t bytes = (size_t)seg_n * sizeof(segment);
-    if (!s.host.ensure(bytes + 16) || !s.dev.ensure(bytes + 16))
+    size_t bytes = (size_t)total * sizeof(uint32_t);
+    if (!stage(s, bytes + 16, bytes + 16))
         return;
-    memcpy(s.host.p, seg, bytes);
-    if (!check(hipMemcpyAsync(s.dev.p, s.host.p, bytes, hipMemcpyHostToDevice, s.stream),
-               "hipMemcpyAsync"))
+    uint32_t *h = (uint32_t *)s.host.p;
+    for (int g = 0; g < seg_n; g++)
+        for (uint32_t j = 0; j < seg[g].length; j++) *h++ = seg[g].base + j;
+    if (total && !upload(s, bytes))
         return;
     check(launch_mns((const float *)S.devpos.p + 3 * (size_t)base,
-                     (float *)S.devacc.p + 4 * (size_t)base, m, (const uint32_t *)s.dev.p, seg_n,
-                     (const float *)S.devpos.p, mmass, e, (int *)S.errflag.p, source_n, s.stream),
+                     (float *)S.devacc.p + 4 * (size_t)base, m, (const uint32_t *)s.dev.p,
+                     (int)total, (const float *)S.devpos.p, mmass, e, s.stream),
           "pMM_sK1 kernel");
 }
 
@@ -542,15 +593,14 @@ void grav_mnss_CUDA(const char *routine, const int sink_base, const int m,
     size_t o_seg = align128(o_srcn + (size_t)ss_len * 4);
     size_t total = o_seg + (size_t)seg_len * 8;
     Slot &s = S.slot[q];
-    if (!s.host.ensure(total + 16) || !s.dev.ensure(total + 16))
+    if (!stage(s, total + 16, total + 16))
         return;
     char *h = (char *)s.host.p;
     memcpy(h, ss_index, (size_t)m * 2);
     memcpy(h + o_ss_seg, ss_seg, (size_t)ss_len * 8);
     memcpy(h + o_srcn, source_n, (size_t)ss_len * 4);
     memcpy(h + o_seg, seg, (size_t)seg_len * 8);
-    if (!check(hipMemcpyAsync(s.dev.p, s.host.p, total, hipMemcpyHostToDevice, s.stream),
-               "hipMemcpyAsync"))
+    if (!upload(s, total))
         return;
     char *d = (char *)s.dev.p;
     check(launch_mnss((const float *)S.devpos.p + 3 * (size_t)sink_base,
@@ -581,11 +631,10 @@ void grav_qns_CUDA(const char *routine, const int base, const int m, const int *
         }
     Slot &s = S.slot[q];
     size_t bytes = (size_t)source_n * sizeof(int);
-    if (!s.host.ensure(bytes + 16) || !s.dev.ensure(bytes + 16))
+    if (!stage(s, bytes + 16, bytes + 16))
         return;
     memcpy(s.host.p, source_list, bytes);
-    if (!check(hipMemcpyAsync(s.dev.p, s.host.p, bytes, hipMemcpyHostToDevice, s.stream),
-               "hipMemcpyAsync"))
+    if (!upload(s, bytes))
         return;
     check(launch_qns(quad_phi_sign(), (const float *)S.devpos.p + 3 * (size_t)base,
                      (float *)S.devacc.p + 4 * (size_t)base, m, (const int *)s.dev.p, source_n,
@@ -615,14 +664,12 @@ void grav_qnss_CUDA(const char *routine, const int base, const int m, const segm
     size_t bytes = (size_t)source_n * 13 * sizeof(float);
     size_t total = o_src + bytes;
     Slot &s = S.slot[q];
-    if (!s.host.ensure(total + 16) || !s.dev.ensure(total + 16) ||
-        !s.pos4.ensure(16 * (size_t)(source_n + 1)) ||
-        !s.mom.ensure(4 * (size_t)PREC2 * (source_n + 1)))
+    if (!stage(s, total + 16, total + 16, 16 * (size_t)(source_n + 1),
+               4 * (size_t)PREC2 * (source_n + 1)))
         return;
     memcpy(s.host.p, seg, (size_t)m * 8);
     memcpy((char *)s.host.p + o_src, source, bytes);
-    if (!check(hipMemcpyAsync(s.dev.p, s.host.p, total, hipMemcpyHostToDevice, s.stream),
-               "hipMemcpyAsync"))
+    if (!upload(s, total))
         return;
     if (!check(launch_prep_soa8((const float *)((char *)s.dev.p + o_src), source_n, 13, 2,
                                 (float4 *)s.pos4.p, (float *)s.mom.p, s.stream),

@@ -24,25 +24,31 @@ __device__ __forceinline__ float soa8(const float *__restrict__ f, int i, int k,
 // records) are parked at 1e8 so they contribute exactly zero even for a sink
 // at the origin (where the reference would form 0 * inf).  1e8 keeps x^4 finite
 // in FP32, so no inf - inf can appear in the order-4 basis.
-__global__ void k_prep_soa8(const float *__restrict__ f, int source_n, int sz, int order,
-                            int dipole, float4 *__restrict__ pos4, float *__restrict__ mom,
-                            int mstride) {
+// the order is a template parameter so that the record stays in registers
+template <int ORDER>
+__global__ __launch_bounds__(128) void k_prep_soa8(const float *__restrict__ f, int source_n,
+                                                   int sz, int dipole, float4 *__restrict__ pos4,
+                                                   float *__restrict__ mom) {
+    constexpr int NF = ORDER == 0 ? 4 : (ORDER == 2 ? 13 : 29);
+    constexpr int MSTRIDE = ORDER == 4 ? PREC4 : PREC2;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= source_n)
         return;
     float q[29];
-    const int nf = order == 0 ? 4 : (order == 2 ? 13 : 29);
-    for (int k = 0; k < 29; k++) q[k] = k < nf ? soa8(f, i, k, sz) : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 29; k++) q[k] = k < NF ? soa8(f, i, k, sz) : 0.0f;
     float4 p = make_float4(q[1], q[2], q[3], q[0]);
     bool zero = q[0] == 0.0f;
-    for (int k = 4; k < nf; k++) zero = zero && q[k] == 0.0f;
+#pragma unroll
+    for (int k = 4; k < NF; k++) zero = zero && q[k] == 0.0f;
     if (zero)
         p = make_float4(1e8f, 1e8f, 1e8f, 0.0f);
     pos4[i] = p;
-    if (order > 0) {
+    if (ORDER > 0) {
         float rec[PREC4];
-        prep_record(q, order, dipole, rec);
-        for (int k = 0; k < mstride; k++) mom[(size_t)i * mstride + k] = rec[k];
+        prep_record(q, ORDER, dipole, rec);
+#pragma unroll
+        for (int k = 0; k < MSTRIDE; k++) mom[(size_t)i * MSTRIDE + k] = rec[k];
     }
 }
 
@@ -105,32 +111,39 @@ __global__ __launch_bounds__(64) void k_mn(const float *__restrict__ sinkpos,
 }
 
 // pMM_sK1 (nln/pgrav.cu:575-637): every sink against the bodies of a
-// segment list into the sink table.  The segment list is shared by all lanes.
+// segment list into the sink table.  The host expands the segments into body
+// indices; grid.y splits that list so that a call runs on several waves, and
+// each wave stages its bodies through LDS 64 at a time (one coalesced load per
+// tile, however short the segments are).
 __global__ __launch_bounds__(64) void k_mns(const float *__restrict__ sinkpos,
                                             float *__restrict__ acc, int m,
-                                            const uint32_t *__restrict__ seg, int seg_n,
+                                            const uint32_t *__restrict__ idx, int nidx, int chunk,
                                             const float *__restrict__ srcpos, float mmass,
-                                            float eps_inv, int *__restrict__ errflag,
-                                            int source_n) {
+                                            float eps_inv) {
+    __shared__ float3 tile[64];
     const int s = blockIdx.x * 64 + threadIdx.x;
     const bool live = s < m;
     const int sc = live ? s : 0;
     const float px = sinkpos[3 * sc], py = sinkpos[3 * sc + 1], pz = sinkpos[3 * sc + 2];
     const float eps2 = 1.0f / (eps_inv * eps_inv);
     Acc4 a = {0.f, 0.f, 0.f, 0.f};
-    int count = 0;
-    for (int g = 0; g < seg_n; g++) {
-        const uint32_t base = seg[2 * g], len = seg[2 * g + 1];
-        count += (int)len;
-        for (uint32_t j = 0; j < len; j++) {
-            const float *q = srcpos + 3 * (size_t)(base + j);
-            k1_interact(mmass, px - q[0], py - q[1], pz - q[2], eps_inv, eps2, a);
+    const int j0 = blockIdx.y * chunk, j1 = min(nidx, j0 + chunk);
+    for (int t0 = j0; t0 < j1; t0 += 64) {
+        const int nt = min(64, j1 - t0);
+        if ((int)threadIdx.x < nt) {
+            const float *q = srcpos + 3 * (size_t)idx[t0 + threadIdx.x];
+            tile[threadIdx.x] = make_float3(q[0], q[1], q[2]);
         }
+        __syncthreads();
+#pragma unroll 4
+        for (int j = 0; j < nt; j++) {
+            const float3 q = tile[j];
+            k1_interact(mmass, px - q.x, py - q.y, pz - q.z, eps_inv, eps2, a);
+        }
+        __syncthreads();
     }
-    if (live)
+    if (live && j0 < j1)
         flush_acc(acc + 4 * s, a);
-    if (count != source_n && threadIdx.x == 0 && blockIdx.x == 0)
-        atomicAdd(errflag, 1);  // assert(sum_n == source_n), nln/pgrav.cu:636
 }
 
 // pMM1_mnss_sK1 (nln/pgrav.cu:713-790): sink i uses segment run ss_seg[ss_index[i]].
@@ -220,9 +233,13 @@ hipError_t launch_prep_soa8(const float *f, int source_n, int sz, int order, flo
                             float *mom, hipStream_t st) {
     if (source_n <= 0)
         return hipSuccess;
-    const int mstride = order == 4 ? PREC4 : PREC2;
-    hipLaunchKernelGGL(k_prep_soa8, dim3((source_n + 127) / 128), dim3(128), 0, st, f, source_n,
-                       sz, order, 1, pos4, mom, mstride);
+    const dim3 grid((source_n + 127) / 128);
+    if (order == 4)
+        hipLaunchKernelGGL(k_prep_soa8<4>, grid, dim3(128), 0, st, f, source_n, sz, 1, pos4, mom);
+    else if (order == 2)
+        hipLaunchKernelGGL(k_prep_soa8<2>, grid, dim3(128), 0, st, f, source_n, sz, 1, pos4, mom);
+    else
+        hipLaunchKernelGGL(k_prep_soa8<0>, grid, dim3(128), 0, st, f, source_n, sz, 1, pos4, mom);
     return hipGetLastError();
 }
 
@@ -241,9 +258,11 @@ hipError_t launch_mn(int order, int soft, int qsign, const float *sinkpos, float
         return hipSuccess;
     const int sink_blocks = (m + 63) / 64;
     // Split the source list so that one call launches >= ~1024 waves when the
-    // list is long; each wave keeps >= 128 sources to amortise the atomics.
+    // list is long; each wave keeps >= 32 sources to amortise the atomics (a
+    // call of one sink group otherwise runs on a handful of waves for tens of
+    // microseconds while the rest of the GPU idles).
     int split = 1;
-    while (sink_blocks * split < 1024 && n / (split * 2) >= 128) split *= 2;
+    while (sink_blocks * split < 1024 && n / (split * 2) >= 32) split *= 2;
     const int chunk = (n + split - 1) / split;
     dim3 grid(sink_blocks, split), block(64);
 #define MN(O, Q, S) hipLaunchKernelGGL((k_mn<O, Q, S>), grid, block, 0, st, sinkpos, acc, m, pos4, mom, n, chunk, e)
@@ -261,13 +280,15 @@ hipError_t launch_mn(int order, int soft, int qsign, const float *sinkpos, float
     return hipGetLastError();
 }
 
-hipError_t launch_mns(const float *sinkpos, float *acc, int m, const uint32_t *seg, int seg_n,
-                      const float *srcpos, float mmass, float eps_inv, int *errflag, int source_n,
-                      hipStream_t st) {
-    if (m <= 0)
+hipError_t launch_mns(const float *sinkpos, float *acc, int m, const uint32_t *idx, int nidx,
+                      const float *srcpos, float mmass, float eps_inv, hipStream_t st) {
+    if (m <= 0 || nidx <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_mns, dim3((m + 63) / 64), dim3(64), 0, st, sinkpos, acc, m, seg, seg_n,
-                       srcpos, mmass, eps_inv, errflag, source_n);
+    // >= 256 bodies per wave, up to 64 waves per call
+    const int split = std::max(1, std::min(64, nidx / 256));
+    const int chunk = (nidx + split - 1) / split;
+    hipLaunchKernelGGL(k_mns, dim3((m + 63) / 64, split), dim3(64), 0, st, sinkpos, acc, m, idx,
+                       nidx, chunk, srcpos, mmass, eps_inv);
     return hipGetLastError();
 }
 

@@ -20,9 +20,8 @@ hipError_t launch_prep_hexacell(const float *tab, int stride_floats, int64_t n, 
                                 float *mom, hipStream_t st);
 hipError_t launch_mn(int order, int soft, int qsign, const float *sinkpos, float *acc, int m,
                      const float4 *pos4, const float *mom, int n, float e, hipStream_t st);
-hipError_t launch_mns(const float *sinkpos, float *acc, int m, const uint32_t *seg, int seg_n,
-                      const float *srcpos, float mmass, float eps_inv, int *errflag,
-                      int source_n, hipStream_t st);
+hipError_t launch_mns(const float *sinkpos, float *acc, int m, const uint32_t *idx, int nidx,
+                      const float *srcpos, float mmass, float eps_inv, hipStream_t st);
 hipError_t launch_mnss(const float *sinkpos, float *acc, int m, const uint16_t *ss_index,
                        const uint32_t *ss_seg, int ss_len, const uint32_t *seg, int seg_len,
                        const float *srcpos, const int *source_n, float mmass, float eps_inv,

@@ -4,6 +4,7 @@
     python -m pgrav.study c4 [--ng 256] [--theta 0.6 | --thetas 0.4 0.5 0.6]
                              [--scheme ewald | images --nimage 2]
     python -m pgrav.study c5 [--n 1048576]
+    python -m pgrav.study dropin [--n 262144]
 
 c3  clustered bodies (pgrav.ics.clustered), order 4, Plummer eps 1e-4, for each
     opening angle: kernel time of the FP32 group-relative compensated (rel32c),
@@ -20,6 +21,14 @@ c5  clustered N bodies, tree and lists fixed: forward and adjoint wall time, and
     the adjoint against central finite differences (h = 1e-5 and h/2, combined
     by Richardson extrapolation; FP64 forward) of L = sum_i w_i.a_i + v_i phi_i
     on 64 sampled coordinates.
+dropin  the reference's own GPU dispatch through the drop-in ABI (nln/gravcuda.h):
+    per sink group one slot (qallocCUDA), grav_mn_CUDA("pH") on the group's
+    order-4 cells as SoA-8 Hvec blocks and grav_mns_CUDA("pMM_sK1") on its body
+    segments (the calls of nln/mac.c:1218 and the leaf path), results added
+    into the body table by WalkTerminateSinkCUDA; timed from the first call to
+    the end of WalkTerminateSinkCUDA with the lists already built, against the
+    batched kernel on the same lists (K1 bodies, unsoftened cells, as
+    pgrav.cu).
 
 Each configuration prints one JSON line on stdout; progress goes to stderr.
 """
@@ -186,9 +195,70 @@ def c5(args):
     print(json.dumps(row), flush=True)
 
 
+def dropin(args):
+    import pgrav
+    from . import reflists
+    pos, mass = ics.clustered(args.n, seed=3)
+    n = pos.shape[0]
+    eps = 1e-3
+    kw = dict(theta=0.6, order=4, eps=eps, soft="k1", G=1.0)
+    H = batch.HostPlan(pos, mass, **kw)
+    perm, spos = H.sorted()
+    st = H.stats()
+    G = st["ngroups"]
+    t0 = time.perf_counter()
+    work = []
+    for g in range(G):
+        first, cnt, _ = H.group(g)
+        soa, source_n = reflists.group_soa(H, g, "hexa")
+        seg, nb = reflists.group_segments(H, g)
+        work.append((first, cnt, soa, source_n, np.ascontiguousarray(seg.reshape(-1)), nb))
+    _log("dropin: %d groups, lists in reference layouts %.1fs" % (G, time.perf_counter() - t0))
+    btab = np.zeros((n, 14), np.float32)
+    btab[:, 0] = mass[perm]
+    btab[:, 1:4] = spos
+    flat = btab.reshape(-1)
+    gc = pgrav.GravCuda()
+    best = None
+    for rep in range(3):
+        btab[:, 7:11] = 0
+        t1 = time.perf_counter()
+        gc.walk_init_sink(flat, 14)
+        for first, cnt, soa, source_n, seg, nb in work:
+            q = -1
+            while q < 0:
+                q, _ = gc.qalloc()
+            if source_n:
+                gc.grav_mn("pH", first, cnt, soa, source_n, 29, 0.0, q)
+            if nb:
+                gc.grav_mns("pMM_sK1", first, cnt, seg, nb, float(mass[0]), 1.0 / eps, q)
+        gc.walk_terminate_sink(flat, 14)
+        ms = (time.perf_counter() - t1) * 1e3
+        best = ms if best is None else min(best, ms)
+        _log("dropin rep %d: %.1f ms" % (rep, ms))
+    got = btab[:, 7:11].astype(np.float64)
+    plan = batch.Plan(pos, mass, fp64=True, **kw)
+    plan.evaluate("abs32")
+    kern = plan.time("abs32", reps=3)
+    plan.evaluate("f64")
+    ref = plan.result("f64")[perm]
+    plan.close()
+    da = np.linalg.norm(got[:, :3] - ref[:, :3], axis=1) / np.linalg.norm(ref[:, :3], axis=1)
+    pairs = st["pairs_hexa"] + st["pairs_body"]
+    row = {"config": "dropin", "n": int(n), "groups": G, "pairs_hexa": st["pairs_hexa"],
+           "pairs_body": st["pairs_body"], "dropin_ms": best,
+           "dropin_interactions_per_s": pairs / (best * 1e-3), "calls": int(sum(
+               (w[3] > 0) + (w[5] > 0) for w in work)),
+           "batched_abs32_kernel_ms": kern, "batched_interactions_per_s": pairs / (kern * 1e-3),
+           "median_rel_err_vs_fp64": float(np.median(da)), "max_rel_err_vs_fp64": float(da.max())}
+    _log("dropin: %.1f ms (%.3g interactions/s) vs batched %.3f ms; median err %.2e" %
+         (best, row["dropin_interactions_per_s"], kern, row["median_rel_err_vs_fp64"]))
+    print(json.dumps(row), flush=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", choices=["c3", "c4", "c5"])
+    ap.add_argument("config", choices=["c3", "c4", "c5", "dropin"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--ng", type=int, default=256)
     ap.add_argument("--theta", type=float, default=0.6)
@@ -207,6 +277,9 @@ def main(argv=None):
         for theta in (a.thetas if a.thetas_given else [a.theta]):
             a.theta = theta
             c4(a)
+    elif a.config == "dropin":
+        a.n = a.n or 262144
+        dropin(a)
     else:
         a.n = a.n or 1048576
         c5(a)
