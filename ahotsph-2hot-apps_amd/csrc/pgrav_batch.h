@@ -24,6 +24,8 @@ struct BatchView {
     const double *mom64 = nullptr;
     const int64_t *gfirst = nullptr;
     const int *gcount = nullptr;
+    const int64_t *gslot_off = nullptr;  // bisection groups (merge_groups 5): sinks of group g
+    const uint32_t *gslot = nullptr;     // are gslot[gslot_off[g] .. gslot_off[g + 1])
     const double4 *gcen = nullptr;
     const int *lpt = nullptr;   // dispatch order
     const int *xoff = nullptr;  // NXCD + 1 slice offsets into lpt

@@ -4,7 +4,8 @@
 //   bpos   double4[N]      Morton-sorted bodies: FP64 x, y, z and mass
 //   cpos   double4[Nnode]  cell centre of mass (FP64) and mass
 //   mom32  float[Nnode*48] prepared multipole records (pgrav_device.h); mom64 FP64 twin
-//   gfirst/gcount/gcen     group body range and FP64 centre + radius
+//   gfirst/gcount/gcen     group body range and FP64 centre + radius (or, for
+//                          bisection groups, gslot_off/gslot: the group's body slots)
 //   off[k]/idx[k]          CSR lists per group: order-4 cells, order-2 cells,
 //                          order-0 cells, bodies (u32 node / sorted-body slots)
 //   lpt    int[G]          groups by decreasing cost (longest first)
@@ -39,6 +40,10 @@ struct BatchArgs {
     const void *mom;  // float* or double* prepared records
     const int64_t *gfirst;
     const int *gcount;
+    // bisection groups: the sinks of group g are the slots gslot[gslot_off[g] ..]
+    // (null: the run gfirst[g] .. gfirst[g] + gcount[g])
+    const int64_t *gslot_off;
+    const uint32_t *gslot;
     const double4 *gcen;
     const int *lpt;   // dispatch order of the groups
     const int *xoff;  // NXCD + 1 slice offsets into lpt (persistent launch)
@@ -141,11 +146,15 @@ template <typename T, bool REL, bool COMP, int SOFT, int PER>
 __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds) {
     constexpr bool PERIODIC = PER == 1;
     const int lane = threadIdx.x;
-    const int64_t first = A.gfirst[g];
     const int cnt = A.gcount[g];
     const double4 gc = A.gcen[g];
     const bool live = lane < cnt;
-    const double4 sp = A.bpos[first + (live ? lane : 0)];
+    int64_t slot;  // this lane's sink
+    if (A.gslot)
+        slot = A.gslot[A.gslot_off[g] + (live ? lane : 0)];
+    else
+        slot = A.gfirst[g] + (live ? lane : 0);
+    const double4 sp = A.bpos[slot];
     T sx, sy, sz;
     if (REL) {
         sx = T(sp.x - gc.x);
@@ -250,7 +259,7 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
     if (live) {
         const T Gn = T(A.G);  // nln/mac.c:269-277: results scaled by G at the end
         V4<T> r = {Gn * (s.x + c.x), Gn * (s.y + c.y), Gn * (s.z + c.z), Gn * (s.w + c.w)};
-        ((V4<T> *)A.out)[first + lane] = r;
+        ((V4<T> *)A.out)[slot] = r;
     }
 }
 
@@ -532,6 +541,8 @@ hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, 
     A.mom = variant == VAR_F64 ? (const void *)v.mom64 : (const void *)v.mom32;
     A.gfirst = v.gfirst;
     A.gcount = v.gcount;
+    A.gslot_off = v.gslot_off;
+    A.gslot = v.gslot;
     A.gcen = v.gcen;
     A.lpt = v.lpt;
     A.xoff = v.xoff;
@@ -570,6 +581,8 @@ hipError_t launch_group64(const BatchView &v, int g, hipStream_t st) {
     A.mom = v.mom64;
     A.gfirst = v.gfirst;
     A.gcount = v.gcount;
+    A.gslot_off = v.gslot_off;
+    A.gslot = v.gslot;
     A.gcen = v.gcen;
     A.lpt = v.lpt;
     A.xoff = v.xoff;

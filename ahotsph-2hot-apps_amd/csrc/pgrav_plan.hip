@@ -220,6 +220,7 @@ void pgrav_params_default(pgrav_params *p) {
     p->dispatch = 1;
     p->mixed_order = 0;
     p->nimage = 2;  // nln/main.c:246
+    p->rcb_cell = 32768;
 }
 
 void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const pgrav_params *p) {
@@ -264,6 +265,8 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         P->tp.pcut = p->pcut;
         P->tp.periodic = p->periodic;
         P->tp.nimage = p->nimage;
+        if (p->rcb_cell > 0)
+            P->tp.rcb_cell = p->rcb_cell;
         P->tp.box = p->box;
         P->tp.nthreads = p->nthreads;
         P->tp.merge_groups = p->merge_groups;
@@ -311,6 +314,11 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         P->v.ngroups = (int)L.ngroups;
         P->v.gfirst = P->upload(L.gfirst.data(), L.gfirst.size());
         P->v.gcount = P->upload(L.gcount.data(), L.gcount.size());
+        if (!L.gslots.empty()) {
+            std::vector<uint32_t> slots(L.gslots.begin(), L.gslots.end());
+            P->v.gslot_off = P->upload(L.gslot_off.data(), L.gslot_off.size());
+            P->v.gslot = P->upload(slots.data(), slots.size());
+        }
         P->v.gcen = (const double4 *)P->upload(L.gcenter.data(), L.gcenter.size());
         P->v.lpt = P->upload(L.lpt.data(), L.lpt.size());
         P->v.xoff = P->upload(L.xoff.data(), L.xoff.size());
@@ -453,6 +461,10 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
     }
     if (P->prm.periodic) {
         fail("the adjoint covers the isolated kernels only");
+        return -1;
+    }
+    if (P->v.gslot) {
+        fail("the adjoint needs sink groups that are runs of bodies (merge_groups 0-4)");
         return -1;
     }
     try {

@@ -46,6 +46,8 @@ void *pgrav_host_plan(const double *pos3, const double *mass, int64_t n, const p
         H->tp.pcut = p->pcut;
         H->tp.periodic = p->periodic;
         H->tp.nimage = p->nimage;
+        if (p->rcb_cell > 0)
+            H->tp.rcb_cell = p->rcb_cell;
         H->tp.box = p->box;
         H->tp.nthreads = p->nthreads;
         H->tp.merge_groups = p->merge_groups;
@@ -107,6 +109,37 @@ int64_t pgrav_host_list(void *h, int64_t g, int kind, uint32_t *out, int64_t cap
     if (out)
         for (int64_t i = 0; i < b - a && i < cap; i++) out[i] = H->lists.idx[kind][a + i];
     return b - a;
+}
+
+int64_t pgrav_host_group_slots(void *h, int64_t g, int64_t *slots) {
+    HostPlan *H = (HostPlan *)h;
+    if (!H || g < 0 || g >= H->lists.ngroups) {
+        fail("bad group");
+        return -1;
+    }
+    const Lists &L = H->lists;
+    const int64_t n = L.gcount[g];
+    if (slots)
+        for (int64_t i = 0; i < n; i++)
+            slots[i] = L.gfirst[g] >= 0 ? L.gfirst[g] + i : L.gslots[L.gslot_off[g] + i];
+    return n;
+}
+
+int pgrav_host_group_slot_table(void *h, int64_t *off, int64_t *slots) {
+    HostPlan *H = (HostPlan *)h;
+    if (!H || !off || !slots) {
+        fail("pgrav_host_group_slot_table: null argument");
+        return -1;
+    }
+    const Lists &L = H->lists;
+    off[0] = 0;
+    for (int64_t g = 0; g < L.ngroups; g++) {
+        const int64_t n = L.gcount[g];
+        for (int64_t i = 0; i < n; i++)
+            slots[off[g] + i] = L.gfirst[g] >= 0 ? L.gfirst[g] + i : L.gslots[L.gslot_off[g] + i];
+        off[g + 1] = off[g] + n;
+    }
+    return 0;
 }
 
 int64_t pgrav_host_list_images(void *h, int64_t g, int kind, uint8_t *out, int64_t cap) {

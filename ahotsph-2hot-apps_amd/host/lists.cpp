@@ -162,6 +162,24 @@ static void group_sphere(const Tree &t, int64_t f, int64_t n, double *out4) {
     out4[3] = sqrt(r2);
 }
 
+// bounding-box centre and enclosing radius of a point set (x, y, z triples)
+static void sphere_of(const std::vector<double> &p, double *out4) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (size_t i = 0; i < p.size(); i += 3)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], p[i + a]);
+            hi[a] = std::max(hi[a], p[i + a]);
+        }
+    double r2 = 0;
+    for (int a = 0; a < 3; a++) out4[a] = 0.5 * (lo[a] + hi[a]);
+    for (size_t i = 0; i < p.size(); i += 3) {
+        double d2 = 0;
+        for (int a = 0; a < 3; a++) d2 += (p[i + a] - out4[a]) * (p[i + a] - out4[a]);
+        r2 = std::max(r2, d2);
+    }
+    out4[3] = sqrt(r2);
+}
+
 static void run_threads(int nth, const std::function<void(int)> &fn) {
     if (nth <= 1) {
         fn(0);
@@ -243,6 +261,79 @@ static void merge_by_cost(const Tree &t, const TreeParams &p, Lists &l) {
     }
 }
 
+// Balanced recursive coordinate bisection (mode 5): the bodies of a cell are cut
+// into m groups of near-equal size; each cut halves the longest extent of the
+// members' bounding box at the count that leaves m1 and m - m1 groups on the
+// two sides.  Groups come out in the order of the bisection (spatially
+// coherent, like a k-d tree walk).
+struct RcbOut {
+    std::vector<int64_t> slots;
+    std::vector<int32_t> count;
+};
+
+static void rcb_split(const Tree &t, std::vector<int64_t> &s, size_t a, size_t b, int m, RcbOut &o) {
+    if (m <= 1) {
+        o.slots.insert(o.slots.end(), s.begin() + a, s.begin() + b);
+        o.count.push_back((int32_t)(b - a));
+        return;
+    }
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (size_t i = a; i < b; i++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], t.pos[3 * s[i] + k]);
+            hi[k] = std::max(hi[k], t.pos[3 * s[i] + k]);
+        }
+    int ax = 0;
+    for (int k = 1; k < 3; k++)
+        if (hi[k] - lo[k] > hi[ax] - lo[ax])
+            ax = k;
+    const int m1 = m / 2;
+    const size_t n1 = a + (size_t)llround((double)(b - a) * m1 / m);
+    std::nth_element(s.begin() + a, s.begin() + n1, s.begin() + b, [&](int64_t x, int64_t y) {
+        return t.pos[3 * x + ax] < t.pos[3 * y + ax] || (t.pos[3 * x + ax] == t.pos[3 * y + ax] && x < y);
+    });
+    rcb_split(t, s, a, n1, m1, o);
+    rcb_split(t, s, n1, b, m - m1, o);
+}
+
+// cells of at most kmax bodies (or leaves), in tree order
+static void rcb_cells(const Tree &t, int32_t nd, int64_t kmax, std::vector<int32_t> &out) {
+    if (t.count[nd] > kmax && t.child[nd] >= 0) {
+        for (int j = 0; j < t.nchild[nd]; j++) rcb_cells(t, t.child[nd] + j, kmax, out);
+        return;
+    }
+    out.push_back(nd);
+}
+
+static void rcb_groups(const Tree &t, const TreeParams &p, Lists &l) {
+    std::vector<int32_t> cells;
+    rcb_cells(t, 0, std::max<int64_t>(p.group_max, p.rcb_cell), cells);
+    std::vector<RcbOut> part(cells.size());
+    std::atomic<int64_t> next(0);
+    run_threads(default_threads(p.nthreads), [&](int) {
+        std::vector<int64_t> s;
+        for (;;) {
+            const int64_t c = next.fetch_add(1);
+            if (c >= (int64_t)cells.size())
+                break;
+            const int32_t nd = cells[c];
+            const int64_t n = t.count[nd];
+            s.resize(n);
+            for (int64_t i = 0; i < n; i++) s[i] = t.first[nd] + i;
+            rcb_split(t, s, 0, (size_t)n, (int)((n + p.group_max - 1) / p.group_max), part[c]);
+        }
+    });
+    l.gslot_off.assign(1, 0);
+    for (const RcbOut &o : part) {
+        for (int32_t c : o.count) {
+            l.gfirst.push_back(-1);
+            l.gcount.push_back(c);
+            l.gslot_off.push_back(l.gslot_off.back() + c);
+        }
+        l.gslots.insert(l.gslots.end(), o.slots.begin(), o.slots.end());
+    }
+}
+
 void build_groups(const Tree &t, const TreeParams &p, Lists &l) {
     if (p.group_max < 1 || p.group_max > 64)
         throw std::runtime_error("group_max must be in [1, 64] (one wave64 lane per sink)");
@@ -307,6 +398,10 @@ void build_groups(const Tree &t, const TreeParams &p, Lists &l) {
         }
         l.gfirst.push_back(cf);
         l.gcount.push_back((int32_t)cc);
+    } else if (p.merge_groups == 5) {
+        // cells of <= rcb_cell bodies cut into ceil(n / group_max) balanced groups by
+        // coordinate bisection; groups list their bodies (gslots), not a run
+        rcb_groups(t, p, l);
     } else if (p.merge_groups == 4) {
         TreeParams q = p;
         q.merge_groups = 1;
@@ -317,7 +412,16 @@ void build_groups(const Tree &t, const TreeParams &p, Lists &l) {
     }
     l.ngroups = (int64_t)l.gfirst.size();
     l.gcenter.assign(4 * l.ngroups, 0.0);
-    for (int64_t g = 0; g < l.ngroups; g++) group_sphere(t, l.gfirst[g], l.gcount[g], &l.gcenter[4 * g]);
+    for (int64_t g = 0; g < l.ngroups; g++) {
+        if (l.gfirst[g] >= 0) {
+            group_sphere(t, l.gfirst[g], l.gcount[g], &l.gcenter[4 * g]);
+        } else {
+            std::vector<double> pts;
+            for (int64_t i = l.gslot_off[g]; i < l.gslot_off[g + 1]; i++)
+                for (int k = 0; k < 3; k++) pts.push_back(t.pos[3 * l.gslots[i] + k]);
+            sphere_of(pts, &l.gcenter[4 * g]);
+        }
+    }
 }
 
 void build_lists(const Tree &t, const TreeParams &p, Lists &l) {

@@ -23,8 +23,11 @@ struct TreeParams {
     int nthreads = 0;      // 0: hardware concurrency
     int merge_groups = 1;  // 0: maximal cells <= group_max; 1: adjacent small siblings merged;
                            // 2: fixed Morton runs; 3: box-limited Morton merge; 4: mode 1, then
-                           // Morton-adjacent groups merged where the joint list costs less
+                           // Morton-adjacent groups merged where the joint list costs less;
+                           // 5: cells of <= rcb_cell bodies cut into balanced groups by
+                           // coordinate bisection (groups are body sets, not runs)
     int dispatch = 1;      // 0: global longest-first; 1: per-XCD Morton slices, longest-first inside
+    int rcb_cell = 32768;  // mode 5: cells of at most this many bodies are cut into groups
     int mixed_order = 0;   // 1: an accepted cell far enough for a lower order to meet the same
                            // truncation bound is listed at that order (order-2 / order-0 lists)
 };
@@ -62,6 +65,8 @@ struct Lists {
     std::vector<int64_t> gfirst;
     std::vector<int32_t> gcount;
     std::vector<double> gcenter;  // 4 per group: centre x, y, z and radius
+    // mode 5: a group's bodies are gslots[gslot_off[g] .. gslot_off[g+1]) (gfirst = -1)
+    std::vector<int64_t> gslot_off, gslots;
     std::vector<int64_t> off[NLIST];   // ngroups + 1 offsets per list kind
     std::vector<uint32_t> idx[NLIST];  // node indices (cells) or sorted body slots
     std::vector<uint8_t> img[NLIST];   // periodic = 2: image of each entry (else empty)

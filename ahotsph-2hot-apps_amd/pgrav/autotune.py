@@ -2,7 +2,8 @@
 
 Searches the parameters that change the work decomposition but not the
 physics: how sink groups are formed (tree cells with siblings merged, then
-optionally merged further where the joint list costs less), the group size
+optionally merged further where the joint list costs less, or cells cut into
+balanced groups by coordinate bisection), the group size
 (lanes per wave used), the leaf size, and the launch shape (one workgroup per group in longest-first order,
 or a persistent grid of k single-wave workgroups per CU draining a work
 counter).  The opening angle, multipole order and softening are inputs, so the
@@ -25,7 +26,7 @@ import numpy as np
 from . import batch
 
 DEFAULT_SPACE = {
-    "merge_groups": [1, 4],
+    "merge_groups": [1, 4, 5],
     "group_max": [64],
     "leaf_max": [16, 32, 48, 64],
 }

@@ -28,7 +28,7 @@ class Params(ctypes.Structure):
                 ("periodic", ctypes.c_int), ("box", ctypes.c_double), ("nthreads", ctypes.c_int),
                 ("merge_groups", ctypes.c_int), ("persistent_blocks", ctypes.c_int),
                 ("fp64", ctypes.c_int), ("dispatch", ctypes.c_int), ("mixed_order", ctypes.c_int),
-                ("nimage", ctypes.c_int)]
+                ("nimage", ctypes.c_int), ("rcb_cell", ctypes.c_int)]
 
 # periodic boundary handling: none; nearest image + Ewald correction per pair
 # (nln/ewald.c); explicit (2 nimage + 1)^3 images + local expansion of the far
@@ -71,6 +71,8 @@ register({
     "pgrav_host_group": (_i, [_vp, _i64, _vp, _vp, _vp]),
     "pgrav_host_list": (_i64, [_vp, _i64, _i, _vp, _i64]),
     "pgrav_host_list_images": (_i64, [_vp, _i64, _i, _vp, _i64]),
+    "pgrav_host_group_slots": (_i64, [_vp, _i64, _vp]),
+    "pgrav_host_group_slot_table": (_i, [_vp, _vp, _vp]),
     "pgrav_host_groups": (_i, [_vp, _vp, _vp, _vp]),
     "pgrav_host_csr": (_i64, [_vp, _i, _vp, _vp]),
     "pgrav_host_cell": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -85,7 +87,8 @@ register({
 
 def make_params(theta=0.6, order=4, eps=1e-4, soft="plummer", G=1.0, group_max=64, leaf_max=16,
                 pcut=0, periodic=False, box=1.0, nthreads=0, merge_groups=True,
-                persistent_blocks=-1, fp64=True, dispatch=1, mixed_order=False, nimage=2):
+                persistent_blocks=-1, fp64=True, dispatch=1, mixed_order=False, nimage=2,
+                rcb_cell=32768):
     p = Params()
     lib().pgrav_params_default(ctypes.byref(p))
     p.theta, p.order, p.eps, p.G = theta, order, eps, G
@@ -95,6 +98,7 @@ def make_params(theta=0.6, order=4, eps=1e-4, soft="plummer", G=1.0, group_max=6
         raise ValueError("periodic must be one of %s" % sorted(map(str, PERIODIC)))
     p.periodic, p.box, p.nthreads = PERIODIC[periodic], box, nthreads
     p.nimage = nimage
+    p.rcb_cell = rcb_cell
     p.merge_groups, p.persistent_blocks, p.fp64 = int(merge_groups), persistent_blocks, int(fp64)
     p.dispatch = dispatch
     p.mixed_order = int(mixed_order)
@@ -272,6 +276,23 @@ class HostPlan:
         out = np.empty(n, np.uint32)
         lib().pgrav_host_list(self.h, g, k, out.ctypes.data, n)
         return out
+
+    def group_slots(self, g):
+        """Sorted body slots of group g (a run, or the body set of a bisection group)."""
+        n = lib().pgrav_host_group_slots(self.h, g, None)
+        check()
+        out = np.empty(n, np.int64)
+        lib().pgrav_host_group_slots(self.h, g, out.ctypes.data)
+        return out
+
+    def group_slot_table(self):
+        """(off[G+1], slots[N]): every group's sink slots, in group order."""
+        st = self.stats()
+        off = np.empty(st["ngroups"] + 1, np.int64)
+        slots = np.empty(st["n"], np.int64)
+        lib().pgrav_host_group_slot_table(self.h, off.ctypes.data, slots.ctypes.data)
+        check()
+        return off, slots
 
     def list_images(self, g, kind):
         """periodic = 2: the image of each entry of list(g, kind), numbered

@@ -69,6 +69,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--variant", default="rel32c")
     ap.add_argument("--group-mode", type=int, default=1)
+    ap.add_argument("--leaf", type=int, default=16, help="leaf size without tuned.json")
     ap.add_argument("--persistent", type=int, default=-1)
     ap.add_argument("--sample", type=int, default=8192)
     ap.add_argument("--no-plain64", action="store_true")
@@ -112,7 +113,7 @@ def main():
     key = autotune.workload_key(pos.shape[0], args.theta, args.order, "plummer", args.eps,
                                 args.workload)
     tuned = None if args.no_tuned else autotune.load(key)
-    dec = {"merge_groups": args.group_mode, "group_max": 64, "leaf_max": 16,
+    dec = {"merge_groups": args.group_mode, "group_max": 64, "leaf_max": args.leaf,
            "persistent_blocks": args.persistent}
     if tuned:
         dec.update({k: tuned["best"][k] for k in dec})

@@ -135,7 +135,10 @@ typedef struct {
                               expansion + background (nln/main.c:1811-1862, ewald_le2.c) */
     double box;            /* periodic box edge */
     int nthreads;          /* host threads for the build (0: all) */
-    int merge_groups;      /* merge adjacent small sibling groups */
+    int merge_groups;      /* sink groups: 0 maximal cells; 1 (default) small siblings merged;
+                              4 then Morton neighbours merged where cheaper; 5 balanced
+                              coordinate bisection of cells into groups of ~group_max bodies
+                              (groups are body sets; not supported by the adjoint) */
     int persistent_blocks; /* 0: one workgroup per group; >0: persistent grid of that many
                               single-wave workgroups; -1 (default): 32 per CU */
     int fp64;              /* also build the FP64 tables (variants 3 and 4) */
@@ -147,6 +150,8 @@ typedef struct {
                               (the order-by-distance choice of nln/mac.c:1797-1810); 0: off */
     int nimage;            /* periodic = 2: explicit images per side, 0..2 (default 2,
                               nln/main.c:246) */
+    int rcb_cell;          /* merge_groups = 5: tree cells of at most this many bodies are
+                              cut into balanced groups by coordinate bisection (default 32768) */
 } pgrav_params;
 
 typedef struct {
@@ -219,6 +224,12 @@ int64_t pgrav_host_list(void *h, int64_t g, int kind, uint32_t *out, int64_t cap
 /* periodic = 2: the image of each entry of that list, (i + n)(2n+1)^2 + (j + n)(2n+1)
  * + (k + n) for the source offset (i, j, k) x box (nln/mac.c:91-111 numbering). */
 int64_t pgrav_host_list_images(void *h, int64_t g, int kind, uint8_t *out, int64_t cap);
+/* The sorted body slots of group g (count of them returned; `slots` may be NULL):
+ * the run first .. first + count, or for merge_groups = 5 the group's body set. */
+int64_t pgrav_host_group_slots(void *h, int64_t g, int64_t *slots);
+/* All groups at once: off[ngroups + 1] and slots[n], group g's sinks being
+ * slots[off[g] .. off[g + 1]). */
+int pgrav_host_group_slot_table(void *h, int64_t *off, int64_t *slots);
 /* All groups at once: first body, count and FP64 centre + radius (arrays of ngroups). */
 int pgrav_host_groups(void *h, int64_t *first, int *count, double *center4);
 /* One list kind for all groups as CSR: off[ngroups+1], idx[off[ngroups]]; returns

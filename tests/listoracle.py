@@ -28,7 +28,8 @@ def group_reference(H, g, pos_sorted, mass_sorted, soft="plummer", eps=1e-4, G=1
                     quad_phi_sign=1, images=None):
     """images = (box, nimage) for lists of explicit periodic images."""
     first, cnt, _ = H.group(g)
-    sinks = pos_sorted[first:first + cnt]
+    slots = H.group_slots(g)  # a run, or a bisection group's body set
+    sinks = pos_sorted[slots]
     acc = np.zeros((cnt, 4))
     eps2 = eps * eps if soft == "plummer" else 0.0
     for kind, order in ORDER.items():
@@ -57,7 +58,7 @@ def group_reference(H, g, pos_sorted, mass_sorted, soft="plummer", eps=1e-4, G=1
         st = {"none": 0, "plummer": 1, "k1": 4}[soft]
         acc += oracle.direct(sinks, pos_sorted[bodies] + offs, mass_sorted[bodies], soft_type=st,
                              eps=eps)
-    return first, G * acc
+    return (first if first >= 0 else slots), G * acc
 
 
 def interactions_cover_all(H, g, n):

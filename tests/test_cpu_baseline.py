@@ -14,22 +14,29 @@ pytestmark = pytest.mark.skipif(not oracle.ref_available(),
                                 reason="oracle/_ref not built (needs the reference sources)")
 
 
+@pytest.mark.parametrize("mode", [1, 5])
 @pytest.mark.parametrize("order,kind,cell,csz", [(4, "hexa", "ph", 29), (2, "quad", "pq", 13)])
 @pytest.mark.parametrize("arch", ["sse4", "avx8"])
-def test_harness_results_match_list_oracle(arch, order, kind, cell, csz):
+def test_harness_results_match_list_oracle(arch, order, kind, cell, csz, mode):
     pos, mass = ics.clustered(6000, seed=5)
     H = batch.HostPlan(pos, mass, theta=0.6, order=order, eps=1e-3, soft="plummer",
-                       merge_groups=1)
+                       merge_groups=mode, rcb_cell=1024)
     st = H.stats()
     perm, spos = H.sorted()
     smass = mass[perm]
-    first, count, _ = H.groups()
+    # the harness takes every group as a run of sinks: the sinks go in group order
+    # (bisection groups, mode 5, are body sets of the sorted order)
+    goff, gslots = H.group_slot_table()
+    assert np.array_equal(np.sort(gslots), np.arange(st["n"]))
+    first, count = goff[:-1].copy(), np.diff(goff).astype(np.int32)
+    if mode == 1:
+        assert np.array_equal(first, H.groups()[0])
     hoff, hidx = H.csr(kind)
     boff, bidx = H.csr("body")
     assert hoff[-1] == st["entries_" + kind] and boff[-1] == st["entries_body"]
     rec = H.records(np.arange(st["nnodes"], dtype=np.uint32))[:, :csz]
     D = {"rec": np.ascontiguousarray(rec, np.float32),
-         "sinks": np.ascontiguousarray(spos, np.float32),
+         "sinks": np.ascontiguousarray(spos[gslots], np.float32),
          "bodies": np.ascontiguousarray(np.concatenate([smass[:, None], spos], 1), np.float32),
          "first": first, "count": count, "hoff": hoff, "hidx": hidx, "boff": boff, "bidx": bidx}
     groups = np.arange(0, first.size, 7, dtype=np.int32)
@@ -39,9 +46,9 @@ def test_harness_results_match_list_oracle(arch, order, kind, cell, csz):
     # convention); do_gravph's is the physical one
     qsign = -1 if order == 2 else 1
     for g in groups[:12]:
-        f, ref = group_reference(H, int(g), spos, smass, soft="plummer", eps=1e-3,
+        _, ref = group_reference(H, int(g), spos, smass, soft="plummer", eps=1e-3,
                                  quad_phi_sign=qsign)
-        got = out[f:f + count[g]]
+        got = out[first[g]:first[g] + count[g]]
         rel = np.linalg.norm(got[:, :3] - ref[:, :3], axis=1) / np.linalg.norm(ref[:, :3], axis=1)
         # FP32 absolute coordinates in the reference kernels: ~1e-5 relative
         assert np.median(rel) < 1e-4, (g, np.median(rel))

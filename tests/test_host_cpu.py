@@ -190,3 +190,33 @@ def test_mixed_order_lists_keep_the_truncation_bound():
         e2.append(np.linalg.norm(r2[:, :3] - ex[:, :3], axis=1) / n)
     m1, m2 = np.median(np.concatenate(e1)), np.median(np.concatenate(e2))
     assert m2 < 3 * m1 and m2 < 1e-3, (m1, m2)
+
+
+@pytest.mark.parametrize("rcb_cell", [512, 1 << 20])
+def test_bisection_groups_partition_bodies_in_balanced_sets(rcb_cell):
+    """merge_groups 5: tree cells of <= rcb_cell bodies cut into ceil(n/64)
+    groups of near-equal size by coordinate bisection; the groups partition the
+    bodies, every sink's lists still cover N once, and the group sphere holds
+    every member."""
+    pos, mass = ics.clustered(20000, seed=12, nhalo=16)
+    H = batch.HostPlan(pos, mass, theta=0.6, order=4, eps=1e-4, merge_groups=5, leaf_max=32,
+                       rcb_cell=rcb_cell)
+    perm, spos = H.sorted()
+    st = H.stats()
+    seen = np.zeros(pos.shape[0], np.int64)
+    sizes = []
+    for g in range(st["ngroups"]):
+        s = H.group_slots(g)
+        seen[s] += 1
+        sizes.append(s.size)
+        _, cnt, c = H.group(g)
+        assert cnt == s.size <= 64
+        assert (np.linalg.norm(spos[s] - c[:3], axis=1) <= c[3] * (1 + 1e-12)).all()
+    assert (seen == 1).all()
+    sizes = np.array(sizes)
+    if rcb_cell > pos.shape[0]:  # one cell: every group within one body of the others
+        assert sizes.max() - sizes.min() <= 1 and sizes.min() >= 32
+        assert sizes.mean() > batch.HostPlan(pos, mass, theta=0.6, order=4, eps=1e-4,
+                                             merge_groups=4, leaf_max=32).stats()["mean_group"]
+    for g in range(0, st["ngroups"], max(1, st["ngroups"] // 20)):
+        assert listoracle.interactions_cover_all(H, g, pos.shape[0])
