@@ -221,6 +221,9 @@ void pgrav_params_default(pgrav_params *p) {
     p->mixed_order = 0;
     p->nimage = 2;  // nln/main.c:246
     p->rcb_cell = 32768;
+    p->mac = 0;
+    p->quad_ncut = 7;   // nln/main.c:540
+    p->hexa_ncut = 20;  // nln/main.c:541
 }
 
 void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const pgrav_params *p) {
@@ -229,19 +232,18 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         fail("pgrav_plan_create: bad arguments");
         return nullptr;
     }
-    if (p->order != 0 && p->order != 2 && p->order != 4) {
-        fail("order must be 0, 2 or 4");
-        return nullptr;
-    }
     if (p->soft_type != SOFT_NONE && p->soft_type != SOFT_PLUMMER && p->soft_type != SOFT_K1) {
         fail("soft_type must be 0 (none), 1 (Plummer) or 4 (K1), nln/mac.c:196-237");
         return nullptr;
     }
-    if ((p->soft_type != SOFT_NONE && !(p->eps > 0)) || !(p->theta > 0) || p->group_max < 1 ||
-        p->group_max > 64 || (p->periodic && !(p->box > 0)) || p->periodic < 0 ||
-        p->periodic > 2 || (p->periodic == 2 && (p->nimage < 0 || p->nimage > MAX_NIMAGE))) {
-        fail("pgrav_plan_create: bad parameters (eps %g theta %g group_max %d periodic %d box %g "
-             "nimage %d)", p->eps, p->theta, p->group_max, p->periodic, p->box, p->nimage);
+    if (p->soft_type != SOFT_NONE && !(p->eps > 0)) {
+        fail("pgrav_plan_create: softening needs eps > 0 (got %g)", p->eps);
+        return nullptr;
+    }
+    TreeParams tp;
+    const std::string bad = tree_params_from(p, tp);
+    if (!bad.empty()) {
+        fail("pgrav_plan_create: %s", bad.c_str());
         return nullptr;
     }
     if (n > (int64_t)UINT32_MAX) {
@@ -258,20 +260,7 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
     }
     try {
         P->prm = *p;
-        P->tp.theta = p->theta;
-        P->tp.group_max = p->group_max;
-        P->tp.leaf_max = p->leaf_max > 0 ? p->leaf_max : 16;
-        P->tp.order = p->order;
-        P->tp.pcut = p->pcut;
-        P->tp.periodic = p->periodic;
-        P->tp.nimage = p->nimage;
-        if (p->rcb_cell > 0)
-            P->tp.rcb_cell = p->rcb_cell;
-        P->tp.box = p->box;
-        P->tp.nthreads = p->nthreads;
-        P->tp.merge_groups = p->merge_groups;
-        P->tp.dispatch = p->dispatch;
-        P->tp.mixed_order = p->mixed_order;
+        P->tp = tp;
         double t0 = now();
         build_tree(pos3, mass, n, P->tp, P->tree);
         double t1 = now();

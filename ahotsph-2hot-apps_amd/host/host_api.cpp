@@ -1,5 +1,6 @@
 // Host-only view of the tree and interaction lists (no device needed): lets a
 // caller or a test inspect exactly what the batched kernel will evaluate.
+#include <stdio.h>
 #include <string.h>
 
 #include <new>
@@ -12,6 +13,55 @@
 namespace pgrav {
 void fail(const char *fmt, ...);
 void clear_error();
+
+std::string tree_params_from(const pgrav_params *p, TreeParams &tp) {
+    char msg[256];
+    msg[0] = 0;
+    if (p->order != 0 && p->order != 2 && p->order != 4)
+        snprintf(msg, sizeof msg, "order must be 0, 2 or 4 (got %d)", p->order);
+    else if (p->group_max < 1 || p->group_max > 64)
+        snprintf(msg, sizeof msg, "group_max must be 1..64 (got %d)", p->group_max);
+    else if (p->mac != 0 && p->mac != 1)
+        snprintf(msg, sizeof msg, "mac must be 0 (opening angle) or 1 (error tolerance)");
+    else if (p->mac == 0 && !(p->theta > 0))
+        snprintf(msg, sizeof msg, "theta must be > 0 (got %g)", p->theta);
+    else if (p->mac == 1 && (!(p->errtol > 0) || !(p->frac_tol >= 0) || !(p->frac_tol0 >= 0) ||
+                             !(p->ptol_boost >= 0) || p->quad_ncut < 0 || p->hexa_ncut < 0))
+        snprintf(msg, sizeof msg,
+                 "mac 1 needs errtol > 0 and frac_tol, frac_tol0, ptol_boost, quad_ncut, "
+                 "hexa_ncut >= 0 (errtol %g frac_tol %g frac_tol0 %g)",
+                 p->errtol, p->frac_tol, p->frac_tol0);
+    else if (p->periodic < 0 || p->periodic > 2 || (p->periodic && !(p->box > 0)))
+        snprintf(msg, sizeof msg, "periodic must be 0..2 with box > 0 (periodic %d box %g)",
+                 p->periodic, p->box);
+    else if (p->periodic == 2 && (p->nimage < 0 || p->nimage > MAX_NIMAGE))
+        snprintf(msg, sizeof msg, "nimage must be 0..%d (got %d)", MAX_NIMAGE, p->nimage);
+    if (msg[0])
+        return msg;
+    tp.theta = p->theta;
+    tp.group_max = p->group_max;
+    tp.leaf_max = p->leaf_max > 0 ? p->leaf_max : 16;
+    tp.order = p->order;
+    tp.pcut = p->pcut;
+    tp.periodic = p->periodic;
+    tp.nimage = p->nimage;
+    if (p->rcb_cell > 0)
+        tp.rcb_cell = p->rcb_cell;
+    tp.box = p->box;
+    tp.nthreads = p->nthreads;
+    tp.merge_groups = p->merge_groups;
+    tp.dispatch = p->dispatch;
+    tp.mixed_order = p->mixed_order;
+    tp.mac = p->mac;
+    tp.errtol = p->errtol;
+    tp.frac_tol = p->frac_tol;
+    tp.frac_tol0 = p->frac_tol0;
+    tp.ptol_boost = p->ptol_boost;
+    tp.stol_max = p->stol_max;
+    tp.quad_ncut = p->quad_ncut;
+    tp.hexa_ncut = p->hexa_ncut;
+    return std::string();
+}
 }  // namespace pgrav
 
 using namespace pgrav;
@@ -28,8 +78,7 @@ extern "C" {
 
 void *pgrav_host_plan(const double *pos3, const double *mass, int64_t n, const pgrav_params *p) {
     clear_error();
-    if (!pos3 || !mass || n <= 0 || !p || !(p->theta > 0) || p->group_max < 1 ||
-        p->group_max > 64 || (p->order != 0 && p->order != 2 && p->order != 4)) {
+    if (!pos3 || !mass || n <= 0 || !p) {
         fail("pgrav_host_plan: bad arguments");
         return nullptr;
     }
@@ -38,21 +87,13 @@ void *pgrav_host_plan(const double *pos3, const double *mass, int64_t n, const p
         fail("out of host memory");
         return nullptr;
     }
+    const std::string bad = tree_params_from(p, H->tp);
+    if (!bad.empty()) {
+        fail("pgrav_host_plan: %s", bad.c_str());
+        delete H;
+        return nullptr;
+    }
     try {
-        H->tp.theta = p->theta;
-        H->tp.group_max = p->group_max;
-        H->tp.leaf_max = p->leaf_max > 0 ? p->leaf_max : 16;
-        H->tp.order = p->order;
-        H->tp.pcut = p->pcut;
-        H->tp.periodic = p->periodic;
-        H->tp.nimage = p->nimage;
-        if (p->rcb_cell > 0)
-            H->tp.rcb_cell = p->rcb_cell;
-        H->tp.box = p->box;
-        H->tp.nthreads = p->nthreads;
-        H->tp.merge_groups = p->merge_groups;
-        H->tp.dispatch = p->dispatch;
-        H->tp.mixed_order = p->mixed_order;
         build_tree(pos3, mass, n, H->tp, H->tree);
         build_groups(H->tree, H->tp, H->lists);
         build_lists(H->tree, H->tp, H->lists);
@@ -210,6 +251,39 @@ int pgrav_host_cell(void *h, int64_t node, double *cofm3, double *bmax, double *
         *child = t.child[node];
     if (nchild)
         *nchild = t.nchild[node];
+    return 0;
+}
+
+int pgrav_host_rcrit(void *h, double *rcrit3) {
+    HostPlan *H = (HostPlan *)h;
+    if (!H || !rcrit3) {
+        fail("pgrav_host_rcrit: null argument");
+        return -1;
+    }
+    if (H->tp.mac != 1) {
+        fail("pgrav_host_rcrit: the plan uses the opening-angle MAC (no critical radii)");
+        return -1;
+    }
+    memcpy(rcrit3, H->tree.rcrit.data(), sizeof(double) * H->tree.rcrit.size());
+    return 0;
+}
+
+int pgrav_host_arel_rcrit(double m, double bmax, double B2, double B4, double B6, int64_t count,
+                          double r0, const pgrav_params *p, double *rcrit3) {
+    clear_error();
+    if (!p || !rcrit3 || !(r0 > 0)) {
+        fail("pgrav_host_arel_rcrit: bad arguments");
+        return -1;
+    }
+    TreeParams tp;
+    pgrav_params q = *p;
+    q.mac = 1;
+    const std::string bad = tree_params_from(&q, tp);
+    if (!bad.empty()) {
+        fail("pgrav_host_arel_rcrit: %s", bad.c_str());
+        return -1;
+    }
+    arel_rcrit(m, bmax, B2, B4, B6, count, tp, r0, rcrit3);
     return 0;
 }
 

@@ -78,7 +78,8 @@ struct Mac {
     const Tree &t;
     int pcut, kind;
     double inv_theta, box, r_quad, r_mono;
-    bool periodic, mixed;
+    bool periodic, mixed, arel;
+    int top, quad_ncut, hexa_ncut;
     Mac(const Tree &tree, const TreeParams &p) : t(tree) {
         pcut = p.pcut > 0 ? p.pcut : auto_pcut(p.order);
         kind = p.order >= 4 ? L_HEXA : (p.order >= 2 ? L_QUAD : L_MONO);
@@ -87,8 +88,13 @@ struct Mac {
         periodic = p.periodic == 1;  // explicit images (2) shift the group instead
         // mixed order: distance (in bmax) beyond which order 2 / order 0 meets the
         // truncation bound theta^(order+1) that the top order meets at bmax/theta
-        const int top = p.order >= 4 ? 4 : (p.order >= 2 ? 2 : 0);
-        mixed = p.mixed_order && top > 0;
+        top = p.order >= 4 ? 4 : (p.order >= 2 ? 2 : 0);
+        arel = p.mac == 1;
+        if (arel && t.rcrit.size() != 3 * (size_t)t.nnodes())
+            throw std::runtime_error("tolerance MAC without critical radii");
+        quad_ncut = p.quad_ncut;
+        hexa_ncut = p.hexa_ncut;
+        mixed = p.mixed_order && top > 0 && !arel;
         r_quad = top == 4 ? pow(p.theta, -5.0 / 3.0) : 1e300;
         r_mono = pow(p.theta, -(double)(top + 1));
     }
@@ -111,20 +117,40 @@ struct Mac {
             }
             const double d = sqrt(dx * dx + dy * dy + dz * dz);
             const int32_t cnt = t.count[c];
-            bool accept = cnt >= pcut && d - rg > t.bmax[c] * inv_theta;
-            // in a periodic box only cells well inside half a box have an
-            // unambiguous nearest image
-            if (accept && periodic && t.bmax[c] * inv_theta + rg + t.bmax[c] > 0.5 * box)
-                accept = false;
-            if (accept) {
-                int k = kind;
+            const double b = t.bmax[c];
+            int k = -1;  // the list kind the cell is taken at; -1: not taken
+            double reach = 0.0;
+            if (arel) {
+                // nln/mac.c:1784-1800 for the group's nearest possible sink: order 0 if
+                // it clears rcrit + bmax, else order 2, else order 4, each by its radius
+                const double D = d - rg, *rc = &t.rcrit[3 * (size_t)c];
+                if (D > rc[0] + b) {
+                    k = L_MONO;
+                    reach = rc[0] + b;
+                } else if (top >= 2 && quad_ncut > 0 && cnt >= quad_ncut && D > rc[1] + b) {
+                    k = L_QUAD;
+                    reach = rc[1] + b;
+                } else if (top >= 4 && hexa_ncut > 0 && cnt >= hexa_ncut && D > rc[2] + b) {
+                    k = L_HEXA;
+                    reach = rc[2] + b;
+                }
+            } else if (cnt >= pcut && d - rg > b * inv_theta) {
+                k = kind;
+                reach = b * inv_theta;
                 if (mixed) {
-                    const double s = (d - rg) / t.bmax[c];
+                    const double s = (d - rg) / b;
                     k = s > r_mono ? L_MONO : (s > r_quad ? L_QUAD : kind);
                 }
+            }
+            // in a periodic box only cells well inside half a box have an
+            // unambiguous nearest image
+            if (k >= 0 && periodic && reach + rg + b > 0.5 * box)
+                k = -1;
+            if (k >= 0) {
                 cell(k, c);
                 covered += cnt;
-            } else if (t.child[c] < 0 || cnt < pcut) {
+            } else if (t.child[c] < 0 || (!arel && cnt < pcut)) {
+                // a leaf that is not taken lists its bodies (nln/mac.c:1816-1832)
                 bodies(t.first[c], cnt);
                 covered += cnt;
             } else {

@@ -4,7 +4,11 @@
 #pragma once
 #include <stdint.h>
 
+#include <functional>
+#include <string>
 #include <vector>
+
+#include "pgrav.h"
 
 namespace pgrav {
 
@@ -30,6 +34,16 @@ struct TreeParams {
     int rcb_cell = 32768;  // mode 5: cells of at most this many bodies are cut into groups
     int mixed_order = 0;   // 1: an accepted cell far enough for a lower order to meet the same
                            // truncation bound is listed at that order (order-2 / order-0 lists)
+    // acceptance: 0 opening angle theta; 1 error tolerance (nln/cofm.c:276-565 critical
+    // radii, nln/mac.c:1776-1800 lowest order first), the parameters of nln/main.c:533-544
+    int mac = 0;
+    double errtol = 0.0;      // absolute acceleration error per interaction (G = 1 units)
+    double frac_tol = 0.0;    // relative to the cell's m / r^2 (0: off)
+    double frac_tol0 = 0.0;   // relative floor (0: off)
+    double ptol_boost = 0.0;  // errtol / (1 + ptol_boost bmax / r0)
+    double stol_max = 0.0;    // > 1: errtol scaled by sigma(m) up to stol_max
+    int quad_ncut = 7;        // cells of >= this many bodies have order-2 moments
+    int hexa_ncut = 20;       // ... order-4 moments
 };
 
 const int HOST_NXCD = 8;  // MI355X XCDs (one L2 each); workgroup b runs on XCD b % 8
@@ -49,6 +63,8 @@ struct Tree {
     std::vector<double> cofm;     // 3 per node, FP64
     std::vector<double> bmax;     // FP64
     std::vector<double> raw;      // 35 per node: raw moments about cofm (nln/cofm.c:660-720 order)
+    std::vector<double> rcrit;    // mac 1: 3 per node, critical radii of orders 0, 2, 4
+                                  // (NaN where the cell has too few bodies for that order)
     double x0[3] = {0, 0, 0};
     double size = 1.0;            // root cube edge
     int64_t nnodes() const { return (int64_t)count.size(); }
@@ -90,6 +106,18 @@ void build_lists(const Tree &t, const TreeParams &p, Lists &l);
 void prepare_cells(const Tree &t, int order, std::vector<float> &rec32, std::vector<double> *rec64,
                    std::vector<double> &cpos4);
 int default_threads(int requested);
+// pgrav_params -> TreeParams; an empty string, or what is wrong with the parameters
+std::string tree_params_from(const pgrav_params *p, TreeParams &tp);
+void parallel_for(int nthreads, int64_t n, const std::function<void(int64_t, int64_t)> &fn);
+// index of the raw moment x^a y^b z^c in a node's 35
+int raw_index(int a, int b, int c);
+// Error-tolerance MAC (host/mac_arel.cpp): the absolute tolerance of a cell, its
+// three critical radii from (m, bmax, B2, B4, B6), and every node's (Tree::rcrit).
+// bmax0: the system half-extent (the reference's r0) that ptol_boost divides by.
+double arel_ptol(double m, double bmax, const TreeParams &p, double bmax0);
+void arel_rcrit(double m, double bmax, double B2, double B4, double B6, int64_t count,
+                const TreeParams &p, double bmax0, double out[3]);
+void compute_rcrit(Tree &t, const TreeParams &p, int nthreads);
 // estimated VALU instructions of one wave over a group's lists (entries per kind)
 double wave_cost(int64_t hexa, int64_t quad, int64_t mono, int64_t body);
 

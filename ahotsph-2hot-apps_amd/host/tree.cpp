@@ -32,7 +32,7 @@ int default_threads(int requested) {
 }
 
 // static split of [0, n) over nthreads
-static void parallel_for(int nthreads, int64_t n, const std::function<void(int64_t, int64_t)> &fn) {
+void parallel_for(int nthreads, int64_t n, const std::function<void(int64_t, int64_t)> &fn) {
     if (nthreads <= 1 || n < 4096) {
         fn(0, n);
         return;
@@ -145,6 +145,14 @@ struct MomentTable {
 static const MomentTable &mtab() {
     static MomentTable t;
     return t;
+}
+
+int raw_index(int a, int b, int c) {
+    const MomentTable &T = mtab();
+    for (int k = 0; k < 35; k++)
+        if (T.e[k][0] == a && T.e[k][1] == b && T.e[k][2] == c)
+            return k;
+    throw std::runtime_error("raw_index: no such moment");
 }
 
 // dst += moments `src` (about c_src) translated to c_dst; d = c_src - c_dst
@@ -298,6 +306,8 @@ void build_tree(const double *pos, const double *mass, int64_t n, const TreePara
         for (int c = nc - 1; c >= 0; c--) stack.push_back({c0 + c});
     }
     compute_moments(t, nth, &key);
+    if (p.mac == 1)
+        compute_rcrit(t, p, nth);
 }
 
 // Moments, centres of mass and bmax of every node, deepest level first (every

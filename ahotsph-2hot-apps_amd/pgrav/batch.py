@@ -28,7 +28,15 @@ class Params(ctypes.Structure):
                 ("periodic", ctypes.c_int), ("box", ctypes.c_double), ("nthreads", ctypes.c_int),
                 ("merge_groups", ctypes.c_int), ("persistent_blocks", ctypes.c_int),
                 ("fp64", ctypes.c_int), ("dispatch", ctypes.c_int), ("mixed_order", ctypes.c_int),
-                ("nimage", ctypes.c_int), ("rcb_cell", ctypes.c_int)]
+                ("nimage", ctypes.c_int), ("rcb_cell", ctypes.c_int), ("mac", ctypes.c_int),
+                ("errtol", ctypes.c_double), ("frac_tol", ctypes.c_double),
+                ("frac_tol0", ctypes.c_double), ("ptol_boost", ctypes.c_double),
+                ("stol_max", ctypes.c_double), ("quad_ncut", ctypes.c_int),
+                ("hexa_ncut", ctypes.c_int)]
+
+# acceptance tests: opening angle theta; error tolerance (the reference's AREL_MAC
+# critical radii, nln/cofm.c:276-565, taken lowest order first, nln/mac.c:1776-1800)
+MAC = {"theta": 0, "tolerance": 1, "arel": 1, 0: 0, 1: 1}
 
 # periodic boundary handling: none; nearest image + Ewald correction per pair
 # (nln/ewald.c); explicit (2 nimage + 1)^3 images + local expansion of the far
@@ -76,6 +84,8 @@ register({
     "pgrav_host_groups": (_i, [_vp, _vp, _vp, _vp]),
     "pgrav_host_csr": (_i64, [_vp, _i, _vp, _vp]),
     "pgrav_host_cell": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pgrav_host_rcrit": (_i, [_vp, _vp]),
+    "pgrav_host_arel_rcrit": (_i, [_d, _d, _d, _d, _d, _i64, _d, _vp, _vp]),
     "pgrav_host_record": (_i, [_vp, _i64, _vp]),
     "pgrav_host_records": (_i, [_vp, _vp, _i64, _vp]),
     "pgrav_host_sorted": (_i, [_vp, _vp, _vp]),
@@ -88,7 +98,13 @@ register({
 def make_params(theta=0.6, order=4, eps=1e-4, soft="plummer", G=1.0, group_max=64, leaf_max=16,
                 pcut=0, periodic=False, box=1.0, nthreads=0, merge_groups=True,
                 persistent_blocks=-1, fp64=True, dispatch=1, mixed_order=False, nimage=2,
-                rcb_cell=32768):
+                rcb_cell=32768, mac="theta", errtol=0.0, frac_tol=0.0, frac_tol0=0.0,
+                ptol_boost=0.0, stol_max=0.0, quad_ncut=7, hexa_ncut=20):
+    """pgrav_params; mac="tolerance" selects the error-tolerance acceptance, with
+    errtol the absolute acceleration error per interaction (G = 1 units), frac_tol
+    and frac_tol0 the relative bound and floor, quad_ncut / hexa_ncut the body
+    counts from which a cell may be taken at order 2 / 4 (theta, pcut and
+    mixed_order are then unused)."""
     p = Params()
     lib().pgrav_params_default(ctypes.byref(p))
     p.theta, p.order, p.eps, p.G = theta, order, eps, G
@@ -102,7 +118,22 @@ def make_params(theta=0.6, order=4, eps=1e-4, soft="plummer", G=1.0, group_max=6
     p.merge_groups, p.persistent_blocks, p.fp64 = int(merge_groups), persistent_blocks, int(fp64)
     p.dispatch = dispatch
     p.mixed_order = int(mixed_order)
+    if mac not in MAC:
+        raise ValueError("mac must be one of %s" % sorted(map(str, MAC)))
+    p.mac = MAC[mac]
+    p.errtol, p.frac_tol, p.frac_tol0 = errtol, frac_tol, frac_tol0
+    p.ptol_boost, p.stol_max = ptol_boost, stol_max
+    p.quad_ncut, p.hexa_ncut = quad_ncut, hexa_ncut
     return p
+
+
+def arel_rcrit(m, bmax, B2, B4, B6, count, r0=1.0, **kw):
+    """(rcrit, rcrit_q, rcrit_h) of one cell under the tolerances of make_params(**kw)."""
+    out = np.empty(3, np.float64)
+    p = make_params(**kw)
+    lib().pgrav_host_arel_rcrit(m, bmax, B2, B4, B6, count, r0, ctypes.byref(p), out.ctypes.data)
+    check()
+    return out
 
 
 def _as_inputs(pos, mass):
@@ -315,6 +346,13 @@ class HostPlan:
         check()
         return {"cofm": c, "bmax": bmax.value, "raw": raw, "first": first.value,
                 "count": cnt.value, "child": child.value, "nchild": nch.value}
+
+    def rcrit(self):
+        """(nnodes, 3) critical radii of orders 0, 2, 4 (tolerance-MAC plans)."""
+        out = np.empty((self.stats()["nnodes"], 3), np.float64)
+        lib().pgrav_host_rcrit(self.h, out.ctypes.data)
+        check()
+        return out
 
     def record(self, node):
         rec = np.empty(29, np.float64)

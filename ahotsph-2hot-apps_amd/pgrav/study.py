@@ -5,6 +5,7 @@
                              [--scheme ewald | images --nimage 2]
     python -m pgrav.study c5 [--n 1048576]
     python -m pgrav.study dropin [--n 262144]
+    python -m pgrav.study mac [--n 16777216] [--errtols 1e-2 1e-3 1e-4] [--leaves 16 64]
 
 c3  clustered bodies (pgrav.ics.clustered), order 4, Plummer eps 1e-4, for each
     opening angle: kernel time of the FP32 group-relative compensated (rel32c),
@@ -29,6 +30,12 @@ dropin  the reference's own GPU dispatch through the drop-in ABI (nln/gravcuda.h
     the end of WalkTerminateSinkCUDA with the lists already built, against the
     batched kernel on the same lists (K1 bodies, unsoftened cells, as
     pgrav.cu).
+mac  clustered bodies as c3, the error-tolerance acceptance (mac = 1, the
+    reference's AREL_MAC radii taken lowest order first) against the opening
+    angle: errtol = each of --errtols times the median |a| of the sampled bodies,
+    frac_tol 0, bisection groups, each leaf size of --leaves; kernel time of
+    rel32c, list composition and the error against the FP64 direct sum on 8192
+    sampled bodies; first a theta = 0.6 row with the tuned decomposition.
 
 Each configuration prints one JSON line on stdout; progress goes to stderr.
 """
@@ -256,9 +263,47 @@ def dropin(args):
     print(json.dumps(row), flush=True)
 
 
+def mac(args):
+    pos, mass = ics.clustered(args.n, seed=2)
+    rng = np.random.default_rng(12345)
+    sample = rng.choice(pos.shape[0], min(8192, pos.shape[0]), replace=False)
+    runs = [("theta", dict(theta=0.6, **_decomposition(pos.shape[0], 0.6, "clustered")))]
+    ref = amed = None
+    for et in args.errtols:
+        for leaf in args.leaves:
+            runs.append(("tolerance", dict(mac="tolerance", errtol=et, merge_groups=5, group_max=64,
+                                           leaf_max=leaf, persistent_blocks=-1)))
+    for kind, kw in runs:
+        if kind == "tolerance":
+            kw = dict(kw, errtol=kw["errtol"] * amed)
+        t0 = time.perf_counter()
+        plan = batch.Plan(pos, mass, order=4, eps=1e-4, soft="plummer", fp64=False, **kw)
+        st = plan.stats()
+        build = time.perf_counter() - t0
+        if ref is None:
+            ref = plan.direct(sample)
+            amed = float(np.median(np.linalg.norm(ref[:, :3], axis=1)))
+        plan.evaluate("rel32c")
+        ms = plan.time("rel32c", reps=3)
+        e = _errors(plan.result("rel32c")[sample], ref)
+        plan.close()
+        row = {"config": "mac", "n": int(pos.shape[0]), "mac": kind, "params": kw,
+               "median_abs_acc": amed, "groups": st["ngroups"], "build_s": build,
+               "pairs": {k: st["pairs_" + k] for k in ("hexa", "quad", "mono", "body")},
+               "pairs_total": st["pairs_total"], "kernel_ms": ms,
+               "interactions_per_s": st["pairs_total"] / (ms * 1e-3), "errors": e}
+        _log("mac %-9s %s: %.3e pairs (h %.2e q %.2e m %.2e b %.2e) %8.2f ms  median %.2e p99 %.2e"
+             % (kind, {k: v for k, v in kw.items() if k in ("theta", "errtol", "leaf_max")},
+                st["pairs_total"], st["pairs_hexa"], st["pairs_quad"], st["pairs_mono"],
+                st["pairs_body"], ms, e["median"], e["p99"]))
+        print(json.dumps(row), flush=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", choices=["c3", "c4", "c5", "dropin"])
+    ap.add_argument("config", choices=["c3", "c4", "c5", "dropin", "mac"])
+    ap.add_argument("--errtols", type=float, nargs="+", default=[1e-2, 1e-3, 1e-4])
+    ap.add_argument("--leaves", type=int, nargs="+", default=[16, 64])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--ng", type=int, default=256)
     ap.add_argument("--theta", type=float, default=0.6)
@@ -280,6 +325,9 @@ def main(argv=None):
     elif a.config == "dropin":
         a.n = a.n or 262144
         dropin(a)
+    elif a.config == "mac":
+        a.n = a.n or 16777216
+        mac(a)
     else:
         a.n = a.n or 1048576
         c5(a)

@@ -152,6 +152,21 @@ typedef struct {
                               nln/main.c:246) */
     int rcb_cell;          /* merge_groups = 5: tree cells of at most this many bodies are
                               cut into balanced groups by coordinate bisection (default 32768) */
+    int mac;               /* acceptance test: 0 (default) opening angle theta; 1 error
+                              tolerance, the reference's AREL_MAC: per cell, critical radii of
+                              orders 0, 2, 4 from the Salmon-Warren bound (nln/cofm.c:276-565),
+                              a cell taken at the lowest order whose radius + bmax the group's
+                              nearest sink clears (DLRcritMAC, nln/mac.c:1776-1800); theta,
+                              pcut and mixed_order are then unused */
+    double errtol;         /* mac 1: absolute acceleration error per interaction, G = 1 units
+                              (the reference's this_tol, nln/main.c:844-858); > 0 */
+    double frac_tol;       /* mac 1: error relative to the cell's own m / r^2 (0: off) */
+    double frac_tol0;      /* mac 1: relative floor: no radius beyond this one's (0: off) */
+    double ptol_boost;     /* mac 1: errtol / (1 + ptol_boost bmax / r0), r0 the half-extent */
+    double stol_max;       /* mac 1: > 1 scales errtol by the reference's sigma(m) table */
+    int quad_ncut;         /* mac 1: cells of at least this many bodies may be taken at order 2
+                              (default 7, nln/main.c:540) */
+    int hexa_ncut;         /* mac 1: ... at order 4 (default 20, nln/main.c:541) */
 } pgrav_params;
 
 typedef struct {
@@ -237,6 +252,15 @@ int pgrav_host_groups(void *h, int64_t *first, int *count, double *center4);
 int64_t pgrav_host_csr(void *h, int kind, int64_t *off, uint32_t *idx);
 int pgrav_host_cell(void *h, int64_t node, double *cofm3, double *bmax, double *raw35,
                     int64_t *first, int *count, int *child, int *nchild);
+/* mac = 1 plans: the critical radii of orders 0, 2, 4 of every node (3 per node,
+ * NaN where the node has too few bodies for that order); -1 for mac = 0 plans. */
+int pgrav_host_rcrit(void *h, double *rcrit3);
+/* The critical radii of one cell from its mass, extent bmax, absolute moments
+ * B2, B4, B6 (sum m |x - c|^k about the centre of mass) and body count, under the
+ * tolerances of p (mac, errtol, frac_tol, frac_tol0, ptol_boost, stol_max, quad_ncut,
+ * hexa_ncut); r0: the system half-extent ptol_boost divides by.  nln/cofm.c:276-565. */
+int pgrav_host_arel_rcrit(double m, double bmax, double B2, double B4, double B6, int64_t count,
+                          double r0, const pgrav_params *p, double *rcrit3);
 /* 29-field record of a node (nln/cofm.c:398-562 normalisation, R = bmax). */
 int pgrav_host_record(void *h, int64_t node, double *rec29);
 int pgrav_host_records(void *h, const uint32_t *nodes, int64_t n, double *rec29);

@@ -68,6 +68,49 @@ def test_batched_variants_match_oracle_on_same_lists(setup, soft, eps, mode):
     #                             group-relative path removes
 
 
+@pytest.mark.parametrize("mode", [1, 5])
+def test_tolerance_mac_lists_match_oracle(setup, mode):
+    """Lists of the error-tolerance acceptance (cells at orders 0, 2 and 4 mixed
+    by their critical radii, nln/mac.c:1776-1800): every variant against the
+    oracle's FP64 evaluation of the same lists."""
+    batch, ics, pos, mass = setup
+    kw = dict(order=4, eps=1e-3, soft="plummer", G=1.0, merge_groups=mode, leaf_max=32,
+              rcb_cell=4096, mac="tolerance", errtol=0.1, frac_tol=0.02)
+    st = batch.HostPlan(pos, mass, **kw).stats()
+    assert min(st["pairs_hexa"], st["pairs_quad"], st["pairs_mono"]) > 0, st
+    e = _group_errors(batch, pos, mass, kw, ["rel32c", "f64"])
+    assert e["f64"] < 1e-10 and e["rel32c"] < 1e-5, e
+
+
+def test_tolerance_mac_error_bound_on_gpu(setup):
+    """Each sink's error against the FP64 direct sum stays below (cells listed) x
+    errtol: every accepted cell's expansion error is within errtol."""
+    batch, ics, pos, mass = setup
+    errtol = 0.05
+    kw = dict(order=4, eps=1e-3, soft="plummer", leaf_max=32, mac="tolerance", errtol=errtol,
+              merge_groups=5, rcb_cell=4096)
+    plan = batch.Plan(pos, mass, **kw)
+    H = batch.HostPlan(pos, mass, **kw)
+    plan.evaluate("rel32c")
+    got = plan.result("rel32c")
+    sample = np.random.default_rng(8).choice(pos.shape[0], 512, replace=False)
+    ref = plan.direct(sample)
+    plan.close()
+    # cells listed for each sampled body's group
+    perm, _ = H.sorted()
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    off, slots = H.group_slot_table()
+    group_of = np.empty(perm.size, np.int64)
+    group_of[slots] = np.repeat(np.arange(off.size - 1), np.diff(off))
+    ncell = sum(np.diff(H.csr(k)[0]) for k in ("hexa", "quad", "mono"))
+    n_s = ncell[group_of[inv[sample]]]
+    err = np.linalg.norm(got[sample, :3] - ref[:, :3], axis=1)
+    # FP32 rounding adds ~1e-6 of |a| on top of the truncation bound
+    assert np.all(err <= n_s * errtol + 1e-5 * np.linalg.norm(ref[:, :3], axis=1)), \
+        (err / (n_s * errtol)).max()
+
+
 @pytest.mark.parametrize("order", [2, 0])
 def test_lower_orders_match_oracle(setup, order):
     batch, ics, pos, mass = setup
