@@ -31,6 +31,8 @@ std::string tree_params_from(const pgrav_params *p, TreeParams &tp) {
                  "mac 1 needs errtol > 0 and frac_tol, frac_tol0, ptol_boost, quad_ncut, "
                  "hexa_ncut >= 0 (errtol %g frac_tol %g frac_tol0 %g)",
                  p->errtol, p->frac_tol, p->frac_tol0);
+    else if (p->group_box != 0 && p->group_box != 1)
+        snprintf(msg, sizeof msg, "group_box must be 0 (sphere) or 1 (bounding box)");
     else if (p->periodic < 0 || p->periodic > 2 || (p->periodic && !(p->box > 0)))
         snprintf(msg, sizeof msg, "periodic must be 0..2 with box > 0 (periodic %d box %g)",
                  p->periodic, p->box);
@@ -60,6 +62,7 @@ std::string tree_params_from(const pgrav_params *p, TreeParams &tp) {
     tp.stol_max = p->stol_max;
     tp.quad_ncut = p->quad_ncut;
     tp.hexa_ncut = p->hexa_ncut;
+    tp.group_box = p->group_box;
     return std::string();
 }
 }  // namespace pgrav

@@ -78,7 +78,7 @@ struct Mac {
     const Tree &t;
     int pcut, kind;
     double inv_theta, box, r_quad, r_mono;
-    bool periodic, mixed, arel;
+    bool periodic, mixed, arel, boxd;
     int top, quad_ncut, hexa_ncut;
     Mac(const Tree &tree, const TreeParams &p) : t(tree) {
         pcut = p.pcut > 0 ? p.pcut : auto_pcut(p.order);
@@ -90,6 +90,7 @@ struct Mac {
         // truncation bound theta^(order+1) that the top order meets at bmax/theta
         top = p.order >= 4 ? 4 : (p.order >= 2 ? 2 : 0);
         arel = p.mac == 1;
+        boxd = p.group_box == 1;
         if (arel && t.rcrit.size() != 3 * (size_t)t.nnodes())
             throw std::runtime_error("tolerance MAC without critical radii");
         quad_ncut = p.quad_ncut;
@@ -99,9 +100,12 @@ struct Mac {
         r_mono = pow(p.theta, -(double)(top + 1));
     }
     // cell(kind, node) for every accepted cell, bodies(first, count) for every run of
-    // bodies listed directly; returns the number of bodies covered (== N when whole)
+    // bodies listed directly; returns the number of bodies covered (== N when whole).
+    // gc: the group's box centre and enclosing radius; gh: the box half-extents, or
+    // null to measure from the enclosing sphere
     template <class C, class B>
-    int64_t walk(const double *gc, std::vector<int32_t> &stack, C cell, B bodies) const {
+    int64_t walk(const double *gc, const double *gh, std::vector<int32_t> &stack, C cell,
+                 B bodies) const {
         const double gx = gc[0], gy = gc[1], gz = gc[2], rg = gc[3];
         int64_t covered = 0;
         stack.clear();
@@ -115,7 +119,16 @@ struct Mac {
                 dy -= box * nearbyint(dy / box);
                 dz -= box * nearbyint(dz / box);
             }
-            const double d = sqrt(dx * dx + dy * dy + dz * dz);
+            // D: a lower bound on the distance from the cell's centre of mass to every
+            // sink of the group
+            double D;
+            if (gh) {
+                const double ex = std::max(fabs(dx) - gh[0], 0.0), ey = std::max(fabs(dy) - gh[1], 0.0),
+                             ez = std::max(fabs(dz) - gh[2], 0.0);
+                D = sqrt(ex * ex + ey * ey + ez * ez);
+            } else {
+                D = sqrt(dx * dx + dy * dy + dz * dz) - rg;
+            }
             const int32_t cnt = t.count[c];
             const double b = t.bmax[c];
             int k = -1;  // the list kind the cell is taken at; -1: not taken
@@ -123,7 +136,7 @@ struct Mac {
             if (arel) {
                 // nln/mac.c:1784-1800 for the group's nearest possible sink: order 0 if
                 // it clears rcrit + bmax, else order 2, else order 4, each by its radius
-                const double D = d - rg, *rc = &t.rcrit[3 * (size_t)c];
+                const double *rc = &t.rcrit[3 * (size_t)c];
                 if (D > rc[0] + b) {
                     k = L_MONO;
                     reach = rc[0] + b;
@@ -134,11 +147,11 @@ struct Mac {
                     k = L_HEXA;
                     reach = rc[2] + b;
                 }
-            } else if (cnt >= pcut && d - rg > b * inv_theta) {
+            } else if (cnt >= pcut && D > b * inv_theta) {
                 k = kind;
                 reach = b * inv_theta;
                 if (mixed) {
-                    const double s = (d - rg) / b;
+                    const double s = D / b;
                     k = s > r_mono ? L_MONO : (s > r_quad ? L_QUAD : kind);
                 }
             }
@@ -170,8 +183,9 @@ double wave_cost(int64_t hexa, int64_t quad, int64_t mono, int64_t body) {
            40.0 * (tiles(hexa) + tiles(quad) + tiles(mono) + tiles(body)) + 60.0;
 }
 
-// bounding-box centre and enclosing radius of a run of sorted bodies
-static void group_sphere(const Tree &t, int64_t f, int64_t n, double *out4) {
+// bounding-box centre and enclosing radius of a run of sorted bodies (and, with
+// half3, the box's half-extents)
+static void group_sphere(const Tree &t, int64_t f, int64_t n, double *out4, double *half3 = nullptr) {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int64_t i = f; i < f + n; i++)
         for (int a = 0; a < 3; a++) {
@@ -180,6 +194,8 @@ static void group_sphere(const Tree &t, int64_t f, int64_t n, double *out4) {
         }
     double r2 = 0;
     for (int a = 0; a < 3; a++) out4[a] = 0.5 * (lo[a] + hi[a]);
+    if (half3)
+        for (int a = 0; a < 3; a++) half3[a] = 0.5 * (hi[a] - lo[a]);
     for (int64_t i = f; i < f + n; i++) {
         double d2 = 0;
         for (int a = 0; a < 3; a++) d2 += (t.pos[3 * i + a] - out4[a]) * (t.pos[3 * i + a] - out4[a]);
@@ -189,7 +205,7 @@ static void group_sphere(const Tree &t, int64_t f, int64_t n, double *out4) {
 }
 
 // bounding-box centre and enclosing radius of a point set (x, y, z triples)
-static void sphere_of(const std::vector<double> &p, double *out4) {
+static void sphere_of(const std::vector<double> &p, double *out4, double *half3) {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (size_t i = 0; i < p.size(); i += 3)
         for (int a = 0; a < 3; a++) {
@@ -198,6 +214,7 @@ static void sphere_of(const std::vector<double> &p, double *out4) {
         }
     double r2 = 0;
     for (int a = 0; a < 3; a++) out4[a] = 0.5 * (lo[a] + hi[a]);
+    for (int a = 0; a < 3; a++) half3[a] = 0.5 * (hi[a] - lo[a]);
     for (size_t i = 0; i < p.size(); i += 3) {
         double d2 = 0;
         for (int a = 0; a < 3; a++) d2 += (p[i + a] - out4[a]) * (p[i + a] - out4[a]);
@@ -224,10 +241,10 @@ static void merge_by_cost(const Tree &t, const TreeParams &p, Lists &l) {
     const Mac mac(t, p);
     const int nth = default_threads(p.nthreads);
     auto cost_of = [&](int64_t f, int64_t n, std::vector<int32_t> &stack) {
-        double gc[4];
-        group_sphere(t, f, n, gc);
+        double gc[4], gh[3];
+        group_sphere(t, f, n, gc, gh);
         int64_t cnt[NLIST] = {0, 0, 0, 0};
-        mac.walk(gc, stack, [&](int k, int32_t) { cnt[k]++; },
+        mac.walk(gc, mac.boxd ? gh : nullptr, stack, [&](int k, int32_t) { cnt[k]++; },
                  [&](int64_t, int32_t m) { cnt[L_BODY] += m; });
         return wave_cost(cnt[L_HEXA], cnt[L_QUAD], cnt[L_MONO], cnt[L_BODY]);
     };
@@ -438,14 +455,15 @@ void build_groups(const Tree &t, const TreeParams &p, Lists &l) {
     }
     l.ngroups = (int64_t)l.gfirst.size();
     l.gcenter.assign(4 * l.ngroups, 0.0);
+    l.ghalf.assign(3 * l.ngroups, 0.0);
     for (int64_t g = 0; g < l.ngroups; g++) {
         if (l.gfirst[g] >= 0) {
-            group_sphere(t, l.gfirst[g], l.gcount[g], &l.gcenter[4 * g]);
+            group_sphere(t, l.gfirst[g], l.gcount[g], &l.gcenter[4 * g], &l.ghalf[3 * g]);
         } else {
             std::vector<double> pts;
             for (int64_t i = l.gslot_off[g]; i < l.gslot_off[g + 1]; i++)
                 for (int k = 0; k < 3; k++) pts.push_back(t.pos[3 * l.gslots[i] + k]);
-            sphere_of(pts, &l.gcenter[4 * g]);
+            sphere_of(pts, &l.gcenter[4 * g], &l.ghalf[3 * g]);
         }
     }
 }
@@ -489,7 +507,7 @@ void build_lists(const Tree &t, const TreeParams &p, Lists &l) {
                     const double gc[4] = {l.gcenter[4 * g] - o[0], l.gcenter[4 * g + 1] - o[1],
                                           l.gcenter[4 * g + 2] - o[2], l.gcenter[4 * g + 3]};
                     covered += mac.walk(
-                        gc, stack,
+                        gc, mac.boxd ? &l.ghalf[3 * g] : nullptr, stack,
                         [&](int k, int32_t c) {
                             B.v[k].push_back((uint32_t)c);
                             if (images)

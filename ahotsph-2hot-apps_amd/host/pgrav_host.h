@@ -44,6 +44,8 @@ struct TreeParams {
     double stol_max = 0.0;    // > 1: errtol scaled by sigma(m) up to stol_max
     int quad_ncut = 7;        // cells of >= this many bodies have order-2 moments
     int hexa_ncut = 20;       // ... order-4 moments
+    int group_box = 0;        // 1: the acceptance distance is from the group's bounding box
+                              // (every sink is inside it) instead of its enclosing sphere
 };
 
 const int HOST_NXCD = 8;  // MI355X XCDs (one L2 each); workgroup b runs on XCD b % 8
@@ -80,7 +82,8 @@ struct Lists {
     int64_t ngroups = 0;
     std::vector<int64_t> gfirst;
     std::vector<int32_t> gcount;
-    std::vector<double> gcenter;  // 4 per group: centre x, y, z and radius
+    std::vector<double> gcenter;  // 4 per group: bounding-box centre x, y, z and enclosing radius
+    std::vector<double> ghalf;    // 3 per group: bounding-box half-extents
     // mode 5: a group's bodies are gslots[gslot_off[g] .. gslot_off[g+1]) (gfirst = -1)
     std::vector<int64_t> gslot_off, gslots;
     std::vector<int64_t> off[NLIST];   // ngroups + 1 offsets per list kind

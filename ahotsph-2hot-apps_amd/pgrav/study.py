@@ -1,6 +1,7 @@
 """Measurements of the path on the configurations it is specified on (one GPU).
 
-    python -m pgrav.study c3 [--n 16777216] [--thetas 0.5 0.6 0.7]
+    python -m pgrav.study c3 [--n 16777216] [--thetas 0.5 0.6 0.7] [--group-boxes 0 1]
+                             [--variants rel32c rel32 abs32 f64]
     python -m pgrav.study c4 [--ng 256] [--theta 0.6 | --thetas 0.4 0.5 0.6]
                              [--scheme ewald | images --nimage 2]
     python -m pgrav.study c5 [--n 1048576]
@@ -11,7 +12,8 @@ c3  clustered bodies (pgrav.ics.clustered), order 4, Plummer eps 1e-4, for each
     opening angle: kernel time of the FP32 group-relative compensated (rel32c),
     plain-sum (rel32) and absolute-coordinate (abs32, the reference's arithmetic)
     variants and of the batched FP64 variant, each with its force error against
-    the FP64 direct sum on 8192 sampled bodies.
+    the FP64 direct sum on 8192 sampled bodies; --group-boxes 1 measures the
+    acceptance distance from each group's bounding box instead of its sphere.
 c4  periodic unit box, ng^3 lattice displaced by a Gaussian random field
     (pgrav.ics.lattice_grf): eager and graph-replayed evaluation time, error
     against the periodic FP64 direct sum (nearest image + 64^3 Ewald table) on
@@ -72,20 +74,20 @@ def c3(args):
     rng = np.random.default_rng(12345)
     sample = rng.choice(pos.shape[0], min(8192, pos.shape[0]), replace=False)
     ref = None
-    for theta in args.thetas:
+    for theta, gbox in [(t, b) for t in args.thetas for b in args.group_boxes]:
         dec = _decomposition(pos.shape[0], theta, "clustered")
         t0 = time.perf_counter()
-        plan = batch.Plan(pos, mass, theta=theta, order=4, eps=1e-4, soft="plummer", fp64=True,
-                          **dec)
+        plan = batch.Plan(pos, mass, theta=theta, order=4, eps=1e-4, soft="plummer",
+                          fp64="f64" in args.variants, group_box=gbox, **dec)
         st = plan.stats()
-        _log("c3 theta %.2f: plan %.1fs, %d groups, %.3e pairs" %
-             (theta, time.perf_counter() - t0, st["ngroups"], st["pairs_total"]))
+        _log("c3 theta %.2f group_box %d: plan %.1fs, %d groups, %.3e pairs" %
+             (theta, gbox, time.perf_counter() - t0, st["ngroups"], st["pairs_total"]))
         if ref is None:  # the direct sum does not depend on theta
             ref = plan.direct(sample)
-        row = {"config": "c3", "n": int(pos.shape[0]), "theta": theta, "decomposition": dec,
-               "groups": st["ngroups"], "pairs": st["pairs_total"],
+        row = {"config": "c3", "n": int(pos.shape[0]), "theta": theta, "group_box": gbox,
+               "decomposition": dec, "groups": st["ngroups"], "pairs": st["pairs_total"],
                "pairs_hexa": st["pairs_hexa"], "pairs_body": st["pairs_body"], "variants": {}}
-        for var in ("rel32c", "rel32", "abs32", "f64"):
+        for var in args.variants:
             plan.evaluate(var)
             ms = plan.time(var, reps=3)
             e = _errors(plan.result(var)[sample], ref)
@@ -304,6 +306,8 @@ def main(argv=None):
     ap.add_argument("config", choices=["c3", "c4", "c5", "dropin", "mac"])
     ap.add_argument("--errtols", type=float, nargs="+", default=[1e-2, 1e-3, 1e-4])
     ap.add_argument("--leaves", type=int, nargs="+", default=[16, 64])
+    ap.add_argument("--group-boxes", type=int, nargs="+", default=[0])
+    ap.add_argument("--variants", nargs="+", default=["rel32c", "rel32", "abs32", "f64"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--ng", type=int, default=256)
     ap.add_argument("--theta", type=float, default=0.6)

@@ -167,6 +167,10 @@ typedef struct {
     int quad_ncut;         /* mac 1: cells of at least this many bodies may be taken at order 2
                               (default 7, nln/main.c:540) */
     int hexa_ncut;         /* mac 1: ... at order 4 (default 20, nln/main.c:541) */
+    int group_box;         /* 0 (default): a cell's distance to a group is measured to the
+                              group's enclosing sphere; 1: to its bounding box (every sink
+                              is inside it: a tighter bound for every sink, fewer cells
+                              opened, larger errors at the same theta) */
 } pgrav_params;
 
 typedef struct {
