@@ -116,6 +116,14 @@ __device__ __forceinline__ void cell_beta(float x, float y, float z, float eps2,
     }
 }
 
+// Sorted-body slot of sink `lane` of group g: a run from gfirst, or for bisection
+// groups (body sets) the group's entry in the slot table.  `ref` is what the
+// transposed cell lists store for a group: its first sink, or its index when the
+// groups are body sets.
+__device__ __forceinline__ int64_t sink_slot(const BatchView &V, int64_t ref, int lane) {
+    return V.gslot ? (int64_t)V.gslot[V.gslot_off[ref] + lane] : ref + lane;
+}
+
 template <int SOFT>
 __global__ __launch_bounds__(64) void k_adjoint(BatchView V, const float4 *__restrict__ wv,
                                                 float4 *__restrict__ gsink,
@@ -124,11 +132,10 @@ __global__ __launch_bounds__(64) void k_adjoint(BatchView V, const float4 *__res
     __shared__ float4 lds[64];
     const int g = blockIdx.x;
     const int lane = threadIdx.x;
-    const int64_t first = V.gfirst[g];
     const int cnt = V.gcount[g];
     const double4 gc = V.gcen[g];
     const bool live = lane < cnt;
-    const int64_t me = first + (live ? lane : 0);
+    const int64_t me = sink_slot(V, V.gslot ? g : V.gfirst[g], live ? lane : 0);
     const double4 sp = V.bpos[me];
     const float sx = (float)(sp.x - gc.x), sy = (float)(sp.y - gc.y), sz = (float)(sp.z - gc.z);
     const float4 w = live ? wv[me] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -209,13 +216,13 @@ __global__ __launch_bounds__(64) void k_adjoint_bodysrc(BatchView V, const float
     __shared__ float4 sk[64], sw[64];
     const int g = blockIdx.x;
     const int lane = threadIdx.x;
-    const int64_t first = V.gfirst[g];
     const int cnt = V.gcount[g];
     const double4 gc = V.gcen[g];
     if (lane < cnt) {
-        const double4 sp = V.bpos[first + lane];
+        const int64_t me = sink_slot(V, V.gslot ? g : V.gfirst[g], lane);
+        const double4 sp = V.bpos[me];
         sk[lane] = make_float4((float)(sp.x - gc.x), (float)(sp.y - gc.y), (float)(sp.z - gc.z), 0.f);
-        sw[lane] = wv[first + lane];
+        sw[lane] = wv[me];
     }
     __syncthreads();
     const float eps = (float)V.eps, eps2 = eps * eps;
@@ -256,7 +263,8 @@ __global__ __launch_bounds__(64) void k_adjoint_bodysrc(BatchView V, const float
 }
 
 // ---- transposed cell lists (built by the host): for every cell, the groups
-// that list it, entry = (first sink, sink count | list kind << 8), so that a
+// that list it, entry = (first sink -- the group's index for body-set groups --,
+// sink count | list kind << 8), so that a
 // cell's record gradient is summed by one wave alone.
 
 // One wave per listed cell: lanes take a listing group's sinks (coalesced), the
@@ -296,8 +304,9 @@ __global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, const float4 
         double4 sp = make_double4(0, 0, 0, 0);
         float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
         if (lane < cnt) {
-            sp = V.bpos[f + lane];
-            w = wv[f + lane];
+            const int64_t me = sink_slot(V, f, lane);
+            sp = V.bpos[me];
+            w = wv[me];
         }
         for (int k = 0; k < nt; k++) {
             uint32_t fn = 0;
@@ -307,8 +316,9 @@ __global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, const float4 
             if (k + 1 < nt) {
                 entry(k + 1, fn, cn, kn);
                 if (lane < cn) {
-                    spn = V.bpos[fn + lane];
-                    wn = wv[fn + lane];
+                    const int64_t me = sink_slot(V, fn, lane);
+                    spn = V.bpos[me];
+                    wn = wv[me];
                 }
             }
             if (lane < cnt) {

@@ -452,10 +452,6 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
         fail("the adjoint covers the isolated kernels only");
         return -1;
     }
-    if (P->v.gslot) {
-        fail("the adjoint needs sink groups that are runs of bodies (merge_groups 0-4)");
-        return -1;
-    }
     try {
         const int64_t n = P->tree.n, nn = P->nnodes;
         const Tree &t = P->tree;
@@ -516,7 +512,8 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
             int64_t base = 0;
             for (int k = 0; k < L_BODY; k++) {
                 for (int64_t g = 0; g < G; g++) {
-                    const uint2 ent = make_uint2((uint32_t)L.gfirst[g],
+                    // a run's first sink, or a body-set group's index (its slot table)
+                    const uint2 ent = make_uint2((uint32_t)(P->v.gslot ? g : L.gfirst[g]),
                                                  (uint32_t)L.gcount[g] | ((uint32_t)k << 8));
                     for (int64_t e = base + L.off[k][g]; e < base + L.off[k][g + 1]; e++)
                         tl[cursor[idx[e]]++] = ent;

@@ -138,7 +138,7 @@ typedef struct {
     int merge_groups;      /* sink groups: 0 maximal cells; 1 (default) small siblings merged;
                               4 then Morton neighbours merged where cheaper; 5 balanced
                               coordinate bisection of cells into groups of ~group_max bodies
-                              (groups are body sets; not supported by the adjoint) */
+                              (groups are body sets, read through a slot table) */
     int persistent_blocks; /* 0: one workgroup per group; >0: persistent grid of that many
                               single-wave workgroups; -1 (default): 32 per CU */
     int fp64;              /* also build the FP64 tables (variants 3 and 4) */

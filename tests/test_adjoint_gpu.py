@@ -20,12 +20,16 @@ def _loss(plan, w4, variant="f64"):
     return float((plan.result(variant) * w4).sum())
 
 
-def test_adjoint_matches_finite_differences(gpu_lib):
+@pytest.mark.parametrize("mode", [1, 5])
+def test_adjoint_matches_finite_differences(gpu_lib, mode):
+    """mode 5: bisection groups, whose sinks the adjoint kernels read and write
+    through the slot table."""
     from pgrav import batch, ics
     pos, mass = ics.clustered(6000, seed=31, nhalo=8)
     rng = np.random.default_rng(5)
     w4 = rng.normal(size=(pos.shape[0], 4))
-    plan = batch.Plan(pos, mass, theta=0.6, order=4, eps=1e-3, soft="plummer")
+    plan = batch.Plan(pos, mass, theta=0.6, order=4, eps=1e-3, soft="plummer", merge_groups=mode,
+                      rcb_cell=1024)
     grad = plan.adjoint(w4)
     h = 1e-5
     picks = rng.choice(pos.shape[0], 16, replace=False)

@@ -24,13 +24,14 @@ def _lattice(ng, seed, rms=0.3):
     return ics.lattice_grf(ng, seed=seed, rms=rms)
 
 
-@pytest.mark.parametrize("nimage", [1, 2])
-def test_fp64_images_match_oracle_lists(gpu_lib, nimage):
+@pytest.mark.parametrize("nimage,mode", [(1, 1), (2, 1), (1, 5)])
+def test_fp64_images_match_oracle_lists(gpu_lib, nimage, mode):
+    """mode 5: bisection groups (body sets) under the explicit images."""
     from pgrav import batch
     pos, mass = _lattice(10, seed=21)
     kw = dict(periodic="images", nimage=nimage, box=1.0, theta=0.5, order=4, soft="none",
-              eps=0.0, G=1.3)
-    print("images nimage=%d n=%d" % (nimage, pos.shape[0]), flush=True)
+              eps=0.0, G=1.3, merge_groups=mode, rcb_cell=256)
+    print("images nimage=%d mode=%d n=%d" % (nimage, mode, pos.shape[0]), flush=True)
     plan = batch.Plan(pos, mass, **kw)
     plan.evaluate("f64")
     got = plan.result("f64")
@@ -41,9 +42,9 @@ def test_fp64_images_match_oracle_lists(gpu_lib, nimage):
     st = H.stats()
     scale = np.abs(far[:, :3]).max() + np.abs(got[:, :3]).max()
     for g in range(0, st["ngroups"], max(1, st["ngroups"] // 8)):
-        first, ref = listoracle.group_reference(H, g, spos, mass[perm], soft="none", eps=0.0,
-                                                G=1.3, images=(1.0, nimage))
-        idx = perm[first:first + ref.shape[0]]
+        _, ref = listoracle.group_reference(H, g, spos, mass[perm], soft="none", eps=0.0,
+                                            G=1.3, images=(1.0, nimage))
+        idx = perm[H.group_slots(g)]
         ref = ref + far[idx]
         np.testing.assert_allclose(got[idx, :3], ref[:, :3], rtol=0, atol=1e-10 * scale)
         np.testing.assert_allclose(got[idx, 3], ref[:, 3], rtol=0, atol=1e-10 * mass.sum() * 1.3)

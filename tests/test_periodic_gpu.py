@@ -34,10 +34,13 @@ def test_perfect_lattice_has_no_force(gpu_lib):
     plan.close()
 
 
-def test_periodic_matches_ewald_direct_sum(gpu_lib):
+@pytest.mark.parametrize("mode", [1, 5])
+def test_periodic_matches_ewald_direct_sum(gpu_lib, mode):
+    """mode 5: bisection groups (body sets) with the nearest image + Ewald table."""
     from pgrav import batch, ics
     pos, mass = ics.lattice_grf(16, seed=8, rms=0.2)
-    plan = batch.Plan(pos, mass, periodic=True, box=1.0, theta=0.4, order=4, soft="none", eps=0.0)
+    plan = batch.Plan(pos, mass, periodic=True, box=1.0, theta=0.4, order=4, soft="none", eps=0.0,
+                      merge_groups=mode, rcb_cell=512)
     plan.evaluate("rel32c")
     acc = plan.result("rel32c")
     sample = np.random.default_rng(2).choice(pos.shape[0], 48, replace=False)
