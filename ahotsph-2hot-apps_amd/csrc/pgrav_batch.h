@@ -41,6 +41,7 @@ struct BatchView {
     int *counter = nullptr;
     const float4 *ewald32 = nullptr;
     const double4 *ewald64 = nullptr;  // FP64 table (direct-sum check)
+    int unroll = 4;  // order-0 / body loop unroll of the rel32c kernels: 2, 4 or 8
     int ngroups = 0;
     int soft = 0;
     double eps = 0;

@@ -55,6 +55,7 @@ struct BatchArgs {
     const float4 *ewald;  // 64^3 correction table (periodic)
     const uint8_t *img;  // explicit images: the image of each entry, parallel to idx (or null)
     int img_f;           // 2 nimage + 1 images per side
+    int unroll;          // order-0 / body loop unroll of the rel32c kernels: 2, 4 or 8
     int ngroups;
     int soft;
     double eps;
@@ -142,7 +143,9 @@ __device__ __forceinline__ V4<T> source_offset(double4 c, double4 gc, double box
 }
 
 // PER: 0 isolated, 1 nearest image + Ewald correction, 2 explicit images
-template <typename T, bool REL, bool COMP, int SOFT, int PER>
+// BU: unroll factor of the order-0 and body loops (2, 4 or 8; the autotuner picks it
+// for the group-relative compensated FP32 kernels, the others use 4)
+template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU>
 __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds) {
     constexpr bool PERIODIC = PER == 1;
     const int lane = threadIdx.x;
@@ -219,7 +222,7 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                         ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
                 }
             } else if (kind == 2) {
-#pragma unroll 4
+#pragma unroll BU
                 for (int k = 0; k < nt; k++) {
                     const V4<T> r = lds[k];
                     const T x = pbc<PERIODIC>(sx - r.x, box, ibox);
@@ -231,7 +234,7 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                 }
             } else {
                 // bodies: ~18 VALU per pair, so unroll to keep several LDS reads in flight
-#pragma unroll 4
+#pragma unroll BU
                 for (int k = 0; k < nt; k++) {
                     const V4<T> r = lds[k];
                     const T x = pbc<PERIODIC>(sx - r.x, box, ibox);
@@ -275,11 +278,11 @@ struct MinWaves {
 };
 
 // one workgroup per group, groups taken in longest-first order
-template <typename T, bool REL, bool COMP, int SOFT, int PER>
+template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PER>::value)))
 void k_batch(BatchArgs A) {
     __shared__ V4<T> lds[64];
-    eval_group<T, REL, COMP, SOFT, PER>(A, A.order_np[blockIdx.x], lds);
+    eval_group<T, REL, COMP, SOFT, PER, BU>(A, A.order_np[blockIdx.x], lds);
 }
 
 // persistent: a fixed grid drains per-XCD work counters.  Workgroup b runs on
@@ -294,7 +297,7 @@ void k_batch(BatchArgs A) {
 // a group's results and bumps the counters before it reads the next group's
 // lists and records, and only pointers known to be distinct let the compiler
 // keep those reads on wave-uniform scalar loads.
-template <typename T, bool REL, bool COMP, int SOFT, int PER>
+template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PER>::value)))
 void k_batch_persistent(
     BatchArgs A, const double4 *__restrict__ bpos, const double4 *__restrict__ cpos,
@@ -330,7 +333,7 @@ void k_batch_persistent(
         __syncthreads();
         if (item >= hi)
             break;
-        eval_group<T, REL, COMP, SOFT, PER>(A, A.lpt[item], lds);
+        eval_group<T, REL, COMP, SOFT, PER, BU>(A, A.lpt[item], lds);
     }
     if (threadIdx.x == 0) {
         const int done = atomicAdd(&A.counter[1], 1);
@@ -345,7 +348,7 @@ void k_batch_persistent(
 template <int SOFT, int PER>
 __global__ __launch_bounds__(64) void k_group64(BatchArgs A, int g) {
     __shared__ V4<double> lds[64];
-    eval_group<double, false, false, SOFT, PER>(A, g, lds);
+    eval_group<double, false, false, SOFT, PER, 4>(A, g, lds);
 }
 
 __global__ void k_reset_counter(int *c) {
@@ -502,27 +505,39 @@ hipError_t launch_reset_counter(int *c, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <typename T, bool REL, bool COMP, int SOFT, int PER>
+template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU>
 static void launch_one(const BatchArgs &A, int persistent_blocks, hipStream_t st) {
     // every XCD needs workgroups to drain its slice: round up to a multiple of NXCD
     persistent_blocks = (persistent_blocks + NXCD - 1) / NXCD * NXCD;
     if (persistent_blocks > 0)
-        hipLaunchKernelGGL((k_batch_persistent<T, REL, COMP, SOFT, PER>), dim3(persistent_blocks),
+        hipLaunchKernelGGL((k_batch_persistent<T, REL, COMP, SOFT, PER, BU>), dim3(persistent_blocks),
                            dim3(64), 0, st, A, A.bpos, A.cpos, (const T *)A.mom, A.gfirst,
                            A.gcount, A.gcen, A.lpt, A.xoff, A.off, A.idx, (V4<T> *)A.out,
                            A.counter, A.ewald);
     else
-        hipLaunchKernelGGL((k_batch<T, REL, COMP, SOFT, PER>), dim3(A.ngroups), dim3(64), 0, st, A);
+        hipLaunchKernelGGL((k_batch<T, REL, COMP, SOFT, PER, BU>), dim3(A.ngroups), dim3(64), 0, st, A);
+}
+
+// the unroll factor: a choice for the group-relative compensated FP32 kernels only
+template <typename T, bool REL, bool COMP, int SOFT, int PER>
+static void launch_bu(const BatchArgs &A, int pb, hipStream_t st) {
+    if constexpr (REL && COMP && sizeof(T) == 4) {
+        if (A.unroll == 2)
+            return launch_one<T, REL, COMP, SOFT, PER, 2>(A, pb, st);
+        if (A.unroll == 8)
+            return launch_one<T, REL, COMP, SOFT, PER, 8>(A, pb, st);
+    }
+    launch_one<T, REL, COMP, SOFT, PER, 4>(A, pb, st);
 }
 
 template <typename T, bool REL, bool COMP, int SOFT>
 static void launch_per(const BatchArgs &A, int per, int pb, hipStream_t st) {
     if (per == 1)
-        launch_one<T, REL, COMP, SOFT, 1>(A, pb, st);
+        launch_bu<T, REL, COMP, SOFT, 1>(A, pb, st);
     else if (per == 2)
-        launch_one<T, REL, COMP, SOFT, 2>(A, pb, st);
+        launch_bu<T, REL, COMP, SOFT, 2>(A, pb, st);
     else
-        launch_one<T, REL, COMP, SOFT, 0>(A, pb, st);
+        launch_bu<T, REL, COMP, SOFT, 0>(A, pb, st);
 }
 
 template <typename T, bool REL, bool COMP>
@@ -559,6 +574,7 @@ hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, 
     A.box = v.box;
     A.img = v.img;
     A.img_f = 2 * v.nimage + 1;
+    A.unroll = v.unroll;
     if (A.ngroups <= 0)
         return hipSuccess;
     if (v.periodic < 0 || v.periodic > 2 || (v.periodic == 2 && !v.img))
@@ -599,6 +615,7 @@ hipError_t launch_group64(const BatchView &v, int g, hipStream_t st) {
     A.box = v.box;
     A.img = v.img;
     A.img_f = 2 * v.nimage + 1;
+    A.unroll = v.unroll;
     if (v.periodic < 0 || v.periodic > 2 || (v.periodic == 2 && !v.img))
         return hipErrorInvalidValue;
     const int per = v.periodic;

@@ -236,6 +236,10 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         fail("soft_type must be 0 (none), 1 (Plummer) or 4 (K1), nln/mac.c:196-237");
         return nullptr;
     }
+    if (p->body_unroll != 0 && p->body_unroll != 2 && p->body_unroll != 4 && p->body_unroll != 8) {
+        fail("pgrav_plan_create: body_unroll must be 0, 2, 4 or 8 (got %d)", p->body_unroll);
+        return nullptr;
+    }
     if (p->soft_type != SOFT_NONE && !(p->eps > 0)) {
         fail("pgrav_plan_create: softening needs eps > 0 (got %g)", p->eps);
         return nullptr;
@@ -354,6 +358,7 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         if (!check(hipMemset(P->v.counter, 0, 64), "hipMemset"))
             throw std::runtime_error("memset");
         P->v.soft = p->soft_type;
+        P->v.unroll = p->body_unroll ? p->body_unroll : 4;
         P->v.eps = p->eps;
         P->v.G = p->G;
         P->v.periodic = p->periodic;
@@ -604,6 +609,17 @@ int pgrav_plan_set_launch(void *plan, int persistent_blocks) {
         return -1;
     }
     P->prm.persistent_blocks = persistent_blocks;
+    return 0;
+}
+
+int pgrav_plan_set_unroll(void *plan, int unroll) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P || (unroll != 2 && unroll != 4 && unroll != 8)) {
+        fail("pgrav_plan_set_unroll: the unroll factor must be 2, 4 or 8");
+        return -1;
+    }
+    P->v.unroll = unroll;
     return 0;
 }
 

@@ -4,9 +4,10 @@ Searches the parameters that change the work decomposition but not the
 physics: how sink groups are formed (tree cells with siblings merged, then
 optionally merged further where the joint list costs less, or cells cut into
 balanced groups by coordinate bisection), the group size
-(lanes per wave used), the leaf size, and the launch shape (one workgroup per group in longest-first order,
-or a persistent grid of k single-wave workgroups per CU draining a work
-counter).  The opening angle, multipole order and softening are inputs, so the
+(lanes per wave used), the leaf size, the launch shape (one workgroup per group in
+longest-first order, or a persistent grid of k single-wave workgroups per CU
+draining a work counter) and the unroll factor of the kernel's order-0 and body
+loops (compiled variants).  The opening angle, multipole order and softening are inputs, so the
 accuracy target is held fixed.  Each candidate is timed with HIP events over
 back-to-back launches on the plan's stream; the fastest wins.
 
@@ -27,9 +28,12 @@ from . import batch
 
 DEFAULT_SPACE = {
     "merge_groups": [1, 4, 5],
-    "group_max": [64],
+    "group_max": [48, 64],
     "leaf_max": [16, 32, 48, 64],
 }
+# unroll factors of the order-0 / body loops of the rel32c kernels (compiled
+# variants, chosen per plan without rebuilding it)
+UNROLL = [2, 4, 8]
 # launch shapes: 0 = one workgroup per group; -1 = persistent grid of 32
 # single-wave workgroups per CU; k > 0 = persistent grid of k per CU.  The
 # table and tuned.json store the resulting workgroup count (what
@@ -56,15 +60,17 @@ def tune(pos, mass, base, space=None, reps=3, variant="rel32c", log=print):
         build = time.perf_counter() - t0
         st = plan.stats()
         plan.evaluate(variant)  # warm-up
-        for pb in PERSISTENT:
+        for pb, bu in itertools.product(PERSISTENT, UNROLL if variant == "rel32c" else [4]):
             blocks = pb * _num_cus() if pb > 0 else pb
+            plan.set_unroll(bu)
             ms = _time_with_blocks(plan, variant, reps, blocks)
-            row = dict(kw, persistent_blocks=blocks, ms=ms, pairs=st["pairs_total"],
-                       ngroups=st["ngroups"], mean_group=st["mean_group"], build_s=build)
+            row = dict(kw, persistent_blocks=blocks, body_unroll=bu, ms=ms,
+                       pairs=st["pairs_total"], ngroups=st["ngroups"],
+                       mean_group=st["mean_group"], build_s=build)
             table.append(row)
             log("tune %s -> %.3f ms (groups %d, mean %.1f, pairs %.3e)" %
-                ({k: row[k] for k in keys + ["persistent_blocks"]}, ms, st["ngroups"],
-                 st["mean_group"], st["pairs_total"]))
+                ({k: row[k] for k in keys + ["persistent_blocks", "body_unroll"]}, ms,
+                 st["ngroups"], st["mean_group"], st["pairs_total"]))
             if best is None or ms < best["ms"]:
                 best = row
         plan.close()
@@ -102,7 +108,7 @@ def save(key, best, table, path=TUNED):
             data = json.load(f)
     except (OSError, ValueError):
         pass
-    keep = ("merge_groups", "group_max", "leaf_max", "persistent_blocks", "ms")
+    keep = ("merge_groups", "group_max", "leaf_max", "persistent_blocks", "body_unroll", "ms")
     data[key] = {"best": {k: best[k] for k in keep},
                  "table": [{k: r[k] for k in keep + ("pairs", "mean_group")} for r in table]}
     with open(path, "w") as f:
@@ -134,7 +140,7 @@ def main(argv=None):
     save(key, best, table, a.out)
     print(json.dumps({"key": key, "best": {k: best[k] for k in
                                            ("merge_groups", "group_max", "leaf_max",
-                                            "persistent_blocks", "ms")}}))
+                                            "persistent_blocks", "body_unroll", "ms")}}))
 
 
 if __name__ == "__main__":

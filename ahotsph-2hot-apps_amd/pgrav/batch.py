@@ -32,7 +32,8 @@ class Params(ctypes.Structure):
                 ("errtol", ctypes.c_double), ("frac_tol", ctypes.c_double),
                 ("frac_tol0", ctypes.c_double), ("ptol_boost", ctypes.c_double),
                 ("stol_max", ctypes.c_double), ("quad_ncut", ctypes.c_int),
-                ("hexa_ncut", ctypes.c_int), ("group_box", ctypes.c_int)]
+                ("hexa_ncut", ctypes.c_int), ("body_unroll", ctypes.c_int),
+                ("group_box", ctypes.c_int)]
 
 # acceptance tests: opening angle theta; error tolerance (the reference's AREL_MAC
 # critical radii, nln/cofm.c:276-565, taken lowest order first, nln/mac.c:1776-1800)
@@ -63,6 +64,7 @@ register({
     "pgrav_plan_stats": (_i, [_vp, _S]),
     "pgrav_plan_evaluate": (_i, [_vp, _i, _vp]),
     "pgrav_plan_set_launch": (_i, [_vp, _i]),
+    "pgrav_plan_set_unroll": (_i, [_vp, _i]),
     "pgrav_plan_set_dispatch": (_i, [_vp, _i]),
     "pgrav_plan_cells": (_i, [_vp, _i64, _i64, _vp, _vp, _vp]),
     "pgrav_plan_time": (_i, [_vp, _i, _i, _vp, ctypes.POINTER(_d)]),
@@ -99,7 +101,8 @@ def make_params(theta=0.6, order=4, eps=1e-4, soft="plummer", G=1.0, group_max=6
                 pcut=0, periodic=False, box=1.0, nthreads=0, merge_groups=True,
                 persistent_blocks=-1, fp64=True, dispatch=1, mixed_order=False, nimage=2,
                 rcb_cell=32768, mac="theta", errtol=0.0, frac_tol=0.0, frac_tol0=0.0,
-                ptol_boost=0.0, stol_max=0.0, quad_ncut=7, hexa_ncut=20, group_box=0):
+                ptol_boost=0.0, stol_max=0.0, quad_ncut=7, hexa_ncut=20, group_box=0,
+                body_unroll=0):
     """pgrav_params; mac="tolerance" selects the error-tolerance acceptance, with
     errtol the absolute acceleration error per interaction (G = 1 units), frac_tol
     and frac_tol0 the relative bound and floor, quad_ncut / hexa_ncut the body
@@ -125,6 +128,7 @@ def make_params(theta=0.6, order=4, eps=1e-4, soft="plummer", G=1.0, group_max=6
     p.ptol_boost, p.stol_max = ptol_boost, stol_max
     p.quad_ncut, p.hexa_ncut = quad_ncut, hexa_ncut
     p.group_box = int(group_box)
+    p.body_unroll = int(body_unroll)
     return p
 
 
@@ -205,6 +209,10 @@ class Plan:
 
     def set_launch(self, persistent_blocks):
         lib().pgrav_plan_set_launch(self.h, int(persistent_blocks))
+        check()
+
+    def set_unroll(self, unroll):
+        lib().pgrav_plan_set_unroll(self.h, int(unroll))
         check()
 
     def set_dispatch(self, mode):

@@ -53,6 +53,8 @@ def workload_name(args, n, variant=None, dec=None):
         name += " | %s groups %d/%d/%d persistent %d" % (
             variant, dec["merge_groups"], dec["group_max"], dec["leaf_max"],
             dec["persistent_blocks"])
+        if dec.get("body_unroll", 4) != 4:
+            name += " unroll %d" % dec["body_unroll"]
     return name
 
 
@@ -114,15 +116,16 @@ def main():
                                 args.workload)
     tuned = None if args.no_tuned else autotune.load(key)
     dec = {"merge_groups": args.group_mode, "group_max": 64, "leaf_max": args.leaf,
-           "persistent_blocks": args.persistent}
+           "persistent_blocks": args.persistent, "body_unroll": 4}
     if tuned:
-        dec.update({k: tuned["best"][k] for k in dec})
+        dec.update({k: tuned["best"][k] for k in dec if k in tuned["best"]})
         log("using tuned decomposition %s" % dec)
     t0 = time.perf_counter()
     plan = batch.Plan(pos, mass, theta=args.theta, order=args.order, eps=args.eps,
                       soft="plummer", merge_groups=dec["merge_groups"],
                       group_max=dec["group_max"], leaf_max=dec["leaf_max"],
-                      persistent_blocks=dec["persistent_blocks"], fp64=True)
+                      persistent_blocks=dec["persistent_blocks"], body_unroll=dec["body_unroll"],
+                      fp64=True)
     st = plan.stats()
     log("plan: %.1fs (tree %.1fs lists %.1fs upload %.1fs) groups %d mean %.1f pairs %.3e" %
         (time.perf_counter() - t0, st["t_tree"], st["t_lists"], st["t_upload"], st["ngroups"],

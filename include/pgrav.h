@@ -167,6 +167,8 @@ typedef struct {
     int quad_ncut;         /* mac 1: cells of at least this many bodies may be taken at order 2
                               (default 7, nln/main.c:540) */
     int hexa_ncut;         /* mac 1: ... at order 4 (default 20, nln/main.c:541) */
+    int body_unroll;       /* rel32c kernels: unroll of the order-0 and body loops, 2, 4 or 8
+                              (0: 4) -- a performance choice, same results */
     int group_box;         /* 0 (default): a cell's distance to a group is measured to the
                               group's enclosing sphere; 1: to its bounding box (every sink
                               is inside it: a tighter bound for every sink, fewer cells
@@ -206,6 +208,9 @@ int pgrav_plan_stats(void *plan, pgrav_stats *st);
  * persistent grid, k > 0 a persistent grid of k workgroups (a captured graph
  * keeps the shape it was captured with). */
 int pgrav_plan_set_launch(void *plan, int persistent_blocks);
+/* Unroll factor of the order-0 and body loops of the rel32c kernels for later
+ * evaluations: 2, 4 (default) or 8 (an autotuner choice; same results). */
+int pgrav_plan_set_unroll(void *plan, int unroll);
 /* Group dispatch order: 0 global longest-first, 1 per-XCD Morton slices. */
 int pgrav_plan_set_dispatch(void *plan, int mode);
 /* Enqueues one force evaluation on `stream` (NULL: the plan's stream). */

@@ -171,6 +171,37 @@ def test_graph_persistent_and_plain_dispatch_reproduce_eager(setup):
     pers.close()
 
 
+@pytest.mark.parametrize("mode", [1, 5])
+def test_unroll_variants_give_identical_results(setup, mode):
+    """The autotuner's unroll choices (order-0 and body loops of rel32c) are the
+    same arithmetic in the same order: bit-identical results, eager and from a
+    graph, persistent and one workgroup per group."""
+    batch, ics, pos, mass = setup
+    plan = batch.Plan(pos, mass, theta=0.6, order=4, eps=1e-4, merge_groups=mode, rcb_cell=4096,
+                      mac="tolerance", errtol=0.1)  # lists with order-0, 2, 4 cells and bodies
+    st = plan.stats()
+    assert min(st["pairs_mono"], st["pairs_body"]) > 0
+    plan.evaluate("rel32c")
+    want = plan.result("rel32c")
+    for bu in (2, 8, 4):
+        plan.set_unroll(bu)
+        for pb in (0, -1):
+            plan.set_launch(pb)
+            plan.evaluate("rel32c")
+            np.testing.assert_array_equal(plan.result("rel32c"), want)
+    plan.set_unroll(8)
+    plan.graph_capture("rel32c")
+    for rep in range(2):
+        plan.evaluate("f64")
+        plan.synchronize()
+        plan.graph_launch()
+        np.testing.assert_array_equal(plan.result("rel32c"), want)
+    from pgrav._lib import PgravError
+    with pytest.raises(PgravError):
+        plan.set_unroll(3)
+    plan.close()
+
+
 def test_plan_rejects_bad_input(gpu_lib):
     from pgrav import batch
     from pgrav._lib import PgravError
