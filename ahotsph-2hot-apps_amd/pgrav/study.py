@@ -1,5 +1,6 @@
 """Measurements of the path on the configurations it is specified on (one GPU).
 
+    python -m pgrav.study c2 [--n 2097152]
     python -m pgrav.study c3 [--n 16777216] [--thetas 0.5 0.6 0.7] [--group-boxes 0 1]
                              [--variants rel32c rel32 abs32 f64]
     python -m pgrav.study c4 [--ng 256] [--theta 0.6 | --thetas 0.4 0.5 0.6]
@@ -8,6 +9,11 @@
     python -m pgrav.study dropin [--n 262144]
     python -m pgrav.study mac [--n 16777216] [--errtols 1e-2 1e-3 1e-4] [--leaves 16 64]
 
+c2  the plain method on the GPU at scale: clustered bodies (pgrav.ics.clustered),
+    order 4, theta 0.6, Plummer eps 1e-4, FP64 with one launch per sink group
+    of <= 64 bodies (the reference's dispatch, nln/pgrav.cu:1078), beside the
+    batched FP64 kernel and the FP32 group-relative compensated kernel on the
+    same lists; errors against the FP64 direct sum on 8192 sampled bodies.
 c3  clustered bodies (pgrav.ics.clustered), order 4, Plummer eps 1e-4, for each
     opening angle: kernel time of the FP32 group-relative compensated (rel32c),
     plain-sum (rel32) and absolute-coordinate (abs32, the reference's arithmetic)
@@ -68,6 +74,32 @@ def _decomposition(n, theta, workload, eps=1e-4):
     if t:
         dec.update({k: t["best"][k] for k in dec if k in t["best"]})
     return dec
+
+
+def c2(args):
+    pos, mass = ics.clustered(args.n, seed=2)
+    rng = np.random.default_rng(12345)
+    sample = rng.choice(pos.shape[0], min(8192, pos.shape[0]), replace=False)
+    dec = _decomposition(pos.shape[0], 0.6, "clustered")
+    t0 = time.perf_counter()
+    plan = batch.Plan(pos, mass, theta=0.6, order=4, eps=1e-4, soft="plummer", fp64=True, **dec)
+    st = plan.stats()
+    _log("c2: plan %.1fs, %d groups (mean %.1f), %.3e pairs" %
+         (time.perf_counter() - t0, st["ngroups"], st["mean_group"], st["pairs_total"]))
+    ref = plan.direct(sample)
+    row = {"config": "c2", "n": int(pos.shape[0]), "theta": 0.6, "decomposition": dec,
+           "groups": st["ngroups"], "mean_group": st["mean_group"], "pairs": st["pairs_total"],
+           "variants": {}}
+    for var, reps in (("plain64", 1), ("f64", 3), ("rel32c", 3)):
+        plan.evaluate(var)
+        ms = plan.time(var, reps=reps)
+        e = _errors(plan.result(var)[sample], ref)
+        row["variants"][var] = {"ms": ms, "interactions_per_s": st["pairs_total"] / (ms * 1e-3),
+                                "errors": e}
+        _log("   %-8s %10.3f ms  %.3e interactions/s  median %.2e  p99 %.2e"
+             % (var, ms, st["pairs_total"] / (ms * 1e-3), e["median"], e["p99"]))
+    plan.close()
+    print(json.dumps(row), flush=True)
 
 
 def c3(args):
@@ -304,7 +336,7 @@ def mac(args):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", choices=["c3", "c4", "c5", "dropin", "mac"])
+    ap.add_argument("config", choices=["c2", "c3", "c4", "c5", "dropin", "mac"])
     ap.add_argument("--errtols", type=float, nargs="+", default=[1e-2, 1e-3, 1e-4])
     ap.add_argument("--leaves", type=int, nargs="+", default=[16, 64])
     ap.add_argument("--group-boxes", type=int, nargs="+", default=[0])
@@ -320,7 +352,10 @@ def main(argv=None):
     a.thetas_given = a.thetas is not None
     a.thetas = a.thetas or [0.5, 0.6, 0.7]
     import torch  # noqa: F401  (one HIP runtime in the process: torch's)
-    if a.config == "c3":
+    if a.config == "c2":
+        a.n = a.n or 2097152
+        c2(a)
+    elif a.config == "c3":
         a.n = a.n or 16777216
         c3(a)
     elif a.config == "c4":
