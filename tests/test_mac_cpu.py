@@ -117,6 +117,11 @@ def test_bad_tolerances_raise():
             batch.HostPlan(pos, mass, mac="tolerance", **kw)
     with pytest.raises(ValueError):
         batch.make_params(mac="bogus")
+    # the other decomposition parameters are checked on the same path
+    for kw in (dict(merge_groups=6), dict(dispatch=2), dict(leaf_max=-1), dict(mixed_order=2),
+               dict(group_box=3), dict(group_max=65), dict(order=3)):
+        with pytest.raises(batch.PgravError):
+            batch.HostPlan(pos, mass, **kw)
 
 
 @pytest.fixture(scope="module")
@@ -176,3 +181,26 @@ def test_tighter_tolerance_lists_more():
                            mac="tolerance", errtol=et)
         sizes.append(H.stats()["pairs_total"])
     assert sizes[0] < sizes[1] < sizes[2], sizes
+
+
+@pytest.mark.parametrize("group_box", [0, 1])
+def test_opening_angle_holds_for_every_sink(group_box):
+    """theta MAC, measured from the group's enclosing sphere (0) or its bounding
+    box (1): every sink of a group is farther than bmax / theta from each listed
+    cell's centre of mass; the box lists fewer pairs."""
+    pos, mass = ics.clustered(12000, seed=21, nhalo=12)
+    kw = dict(theta=0.6, order=4, eps=1e-3, soft="plummer", merge_groups=5, rcb_cell=1024,
+              leaf_max=32)
+    H = batch.HostPlan(pos, mass, group_box=group_box, **kw)
+    perm, spos = H.sorted()
+    st = H.stats()
+    rng = np.random.default_rng(6)
+    for g in rng.choice(st["ngroups"], 40, replace=False):
+        sinks = spos[H.group_slots(int(g))]
+        for nd in H.list(int(g), "hexa"):
+            c = H.cell(int(nd))
+            d = np.linalg.norm(sinks - c["cofm"], axis=1)
+            assert d.min() > c["bmax"] / 0.6
+    if group_box:
+        sphere = batch.HostPlan(pos, mass, group_box=0, **kw).stats()["pairs_total"]
+        assert st["pairs_total"] < sphere
