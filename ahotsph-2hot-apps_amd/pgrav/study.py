@@ -5,6 +5,7 @@
                              [--variants rel32c rel32 abs32 f64]
     python -m pgrav.study c4 [--ng 256] [--theta 0.6 | --thetas 0.4 0.5 0.6]
                              [--scheme ewald | images --nimage 2]
+                             [--mac tolerance --errtols 1e-2 1e-3]
     python -m pgrav.study c5 [--n 1048576]
     python -m pgrav.study dropin [--n 262144]
     python -m pgrav.study mac [--n 16777216] [--errtols 1e-2 1e-3 1e-4] [--leaves 16 64]
@@ -26,6 +27,8 @@ c4  periodic unit box, ng^3 lattice displaced by a Gaussian random field
     1024 sampled bodies, net momentum.  --scheme ewald: nearest image + the
     Ewald correction per pair; images: (2 nimage + 1)^3 explicit images + the
     far field by the local expansion (the reference's FixCubeEwaldLE scheme).
+    --mac tolerance: after the theta row, the error-tolerance acceptance with
+    errtol = each of --errtols times the rms force of the sampled bodies.
 c5  clustered N bodies, tree and lists fixed: forward and adjoint wall time, and
     the adjoint against central finite differences (h = 1e-5 and h/2, combined
     by Richardson extrapolation; FP64 forward) of L = sum_i w_i.a_i + v_i phi_i
@@ -131,16 +134,14 @@ def c3(args):
         print(json.dumps(row), flush=True)
 
 
-def c4(args):
-    pos, mass = ics.lattice_grf(args.ng, seed=4)
+def _c4_run(args, pos, mass, label, sample, ref, **kw):
     n = pos.shape[0]
-    dec = _decomposition(n, args.theta, "lattice")
     t0 = time.perf_counter()
-    plan = batch.Plan(pos, mass, periodic=args.scheme, nimage=args.nimage, box=1.0,
-                      theta=args.theta, order=4, eps=args.eps, soft="plummer", fp64=False, **dec)
+    plan = batch.Plan(pos, mass, periodic=args.scheme, nimage=args.nimage, box=1.0, order=4,
+                      eps=args.eps, soft="plummer", fp64=False, **kw)
     st = plan.stats()
-    _log("c4 %d^3: plan %.1fs, %d groups, %.3e pairs" % (args.ng, time.perf_counter() - t0,
-                                                         st["ngroups"], st["pairs_total"]))
+    _log("c4 %s %d^3: plan %.1fs, %d groups, %.3e pairs" % (label, args.ng, time.perf_counter() - t0,
+                                                            st["ngroups"], st["pairs_total"]))
     plan.evaluate("rel32c")
     eager = plan.time("rel32c", reps=3)
     plan.graph_capture("rel32c")
@@ -153,27 +154,44 @@ def c4(args):
     plan.synchronize()
     graph = (time.perf_counter() - t0) / reps * 1e3
     acc = plan.result("rel32c")
-    rng = np.random.default_rng(7)
-    sample = rng.choice(n, 1024, replace=False)
-    ref = plan.direct(sample)
+    if ref is None:
+        ref = plan.direct(sample)
     err = np.linalg.norm(acc[sample, :3] - ref[:, :3], axis=1)
     rms = float(np.sqrt((ref[:, :3] ** 2).sum(1).mean()))
     net = np.abs((mass[:, None] * acc[:, :3]).sum(0)).max()
     typical = (mass[:, None] * np.abs(acc[:, :3])).sum()
-    row = {"config": "c4", "n": int(n), "theta": args.theta, "eps": args.eps,
+    row = {"config": "c4", "n": int(n), "params": kw, "eps": args.eps,
            "scheme": args.scheme, "nimage": args.nimage if args.scheme == "images" else None,
-           "entries": {k: st["entries_" + k] for k in batch.KINDS},
-           "host_lists_s": st["t_lists"], "decomposition": dec,
-           "groups": st["ngroups"], "pairs": st["pairs_total"], "kernel_ms_eager": eager,
-           "ms_graph_replay_wall": graph, "interactions_per_s": st["pairs_total"] / (eager * 1e-3),
+           "rms_force": rms, "entries": {k: st["entries_" + k] for k in batch.KINDS},
+           "pairs_by_kind": {k: st["pairs_" + k] for k in batch.KINDS},
+           "host_lists_s": st["t_lists"], "groups": st["ngroups"], "pairs": st["pairs_total"],
+           "kernel_ms_eager": eager, "ms_graph_replay_wall": graph,
+           "interactions_per_s": st["pairs_total"] / (eager * 1e-3),
            "errors_vs_direct_ewald": {"median_over_rms": float(np.median(err) / rms),
                                       "p99_over_rms": float(np.percentile(err, 99) / rms),
                                       "median_rel": _errors(acc[sample], ref)["median"]},
            "net_force_over_sum_abs": float(net / typical)}
-    _log("c4: eager %.3f ms, graph %.3f ms, median |da|/rms %.2e" %
-         (eager, graph, row["errors_vs_direct_ewald"]["median_over_rms"]))
+    _log("c4 %s: eager %.3f ms, graph %.3f ms, median |da|/rms %.2e, p99 %.2e" %
+         (label, eager, graph, row["errors_vs_direct_ewald"]["median_over_rms"],
+          row["errors_vs_direct_ewald"]["p99_over_rms"]))
     plan.close()
     print(json.dumps(row), flush=True)
+    return ref, rms
+
+
+def c4(args):
+    pos, mass = ics.lattice_grf(args.ng, seed=4)
+    n = pos.shape[0]
+    sample = np.random.default_rng(7).choice(n, 1024, replace=False)
+    dec = _decomposition(n, args.theta, "lattice")
+    ref, rms = _c4_run(args, pos, mass, "theta %.2f" % args.theta, sample, None,
+                       theta=args.theta, **dec)
+    if args.mac == "tolerance":
+        # errtol as a fraction of the rms force of the sampled bodies
+        for f in args.errtols:
+            _c4_run(args, pos, mass, "errtol %.0e rms" % f, sample, ref, mac="tolerance",
+                    errtol=f * rms, merge_groups=dec["merge_groups"], leaf_max=dec["leaf_max"],
+                    persistent_blocks=dec["persistent_blocks"])
 
 
 def c5(args):
@@ -340,6 +358,7 @@ def main(argv=None):
     ap.add_argument("--errtols", type=float, nargs="+", default=[1e-2, 1e-3, 1e-4])
     ap.add_argument("--leaves", type=int, nargs="+", default=[16, 64])
     ap.add_argument("--group-boxes", type=int, nargs="+", default=[0])
+    ap.add_argument("--mac", choices=["theta", "tolerance"], default="theta")
     ap.add_argument("--variants", nargs="+", default=["rel32c", "rel32", "abs32", "f64"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--ng", type=int, default=256)
