@@ -7,10 +7,11 @@
 // of the cell's bodies.  Three kernels:
 //
 //  k_adjoint        one wave per sink group, the forward's tiling (lane = sink).
-//                   Per pair: the sink-position gradient by forward-mode dual
-//                   numbers through the very same cell_interact/mono_interact
-//                   code; for cells also the pair's share of dL/dX_c, summed
-//                   over the wave's sinks and added atomically.
+//                   Per pair: the sink-position gradient by one forward-mode
+//                   derivative (along the sink's weight w) through the very same
+//                   cell_interact/mono_interact code; for cells also the pair's
+//                   share of dL/dX_c, summed over the wave's sinks and added
+//                   atomically.
 //  k_adjoint_bodysrc one wave per sink group, lane = body of a body-list tile,
 //                   looping over the group's sinks: dL/dx_j and dL/dm_j of the
 //                   source bodies, lane-private sums, one atomic add per term
@@ -32,30 +33,33 @@
 namespace pgrav {
 
 // ---------------------------------------------------------------- dual numbers
-struct D3 {
-    float v, x, y, z;  // value and gradient with respect to the sink position
-    __device__ D3() = default;
-    __device__ D3(float a) : v(a), x(0.f), y(0.f), z(0.f) {}
-    __device__ D3(float a, float b, float c, float d) : v(a), x(b), y(c), z(d) {}
+// The sink-position gradient of one pair's w.a + v phi needs J^T w + v grad(phi),
+// J the Jacobian of a.  Every interaction here is a conservative field, a = -grad(phi)
+// (the order-2/4 expansions with the physical quadrupole sign, the monopole, Plummer
+// and K1 bodies), so J is the negated Hessian of phi, symmetric: J^T w = J w, the
+// derivative of a along w, and grad(phi) = -a.  One forward-mode derivative along w
+// therefore gives the whole gradient:  J w - v a.
+struct D1 {
+    float v, d;  // value and derivative along the seeded direction
+    __device__ D1() = default;
+    __device__ D1(float a) : v(a), d(0.f) {}
+    __device__ D1(float a, float b) : v(a), d(b) {}
 };
-__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return D3(a.v + b.v, a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return D3(a.v - b.v, a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ D3 operator-(D3 a) { return D3(-a.v, -a.x, -a.y, -a.z); }
-__device__ __forceinline__ D3 operator*(D3 a, D3 b) {
-    return D3(a.v * b.v, a.x * b.v + a.v * b.x, a.y * b.v + a.v * b.y, a.z * b.v + a.v * b.z);
-}
-__device__ __forceinline__ D3 operator*(float s, D3 b) { return D3(s * b.v, s * b.x, s * b.y, s * b.z); }
-__device__ __forceinline__ D3 operator*(D3 b, float s) { return s * b; }
-__device__ __forceinline__ D3 fma(D3 a, D3 b, D3 c) { return a * b + c; }
-__device__ __forceinline__ D3 &operator+=(D3 &a, D3 b) { return a = a + b; }
-__device__ __forceinline__ D3 &operator-=(D3 &a, D3 b) { return a = a - b; }
-__device__ __forceinline__ bool operator>(D3 a, D3 b) { return a.v > b.v; }
-__device__ __forceinline__ bool operator==(D3 a, D3 b) { return a.v == b.v; }
+__device__ __forceinline__ D1 operator+(D1 a, D1 b) { return D1(a.v + b.v, a.d + b.d); }
+__device__ __forceinline__ D1 operator-(D1 a, D1 b) { return D1(a.v - b.v, a.d - b.d); }
+__device__ __forceinline__ D1 operator-(D1 a) { return D1(-a.v, -a.d); }
+__device__ __forceinline__ D1 operator*(D1 a, D1 b) { return D1(a.v * b.v, a.d * b.v + a.v * b.d); }
+__device__ __forceinline__ D1 operator*(float s, D1 b) { return D1(s * b.v, s * b.d); }
+__device__ __forceinline__ D1 operator*(D1 b, float s) { return s * b; }
+__device__ __forceinline__ D1 fma(D1 a, D1 b, D1 c) { return a * b + c; }
+__device__ __forceinline__ D1 &operator+=(D1 &a, D1 b) { return a = a + b; }
+__device__ __forceinline__ D1 &operator-=(D1 &a, D1 b) { return a = a - b; }
+__device__ __forceinline__ bool operator>(D1 a, D1 b) { return a.v > b.v; }
+__device__ __forceinline__ bool operator==(D1 a, D1 b) { return a.v == b.v; }
 // -1/sqrt(s): derivative 0.5 s^-3/2 = -0.5 f^3
-__device__ __forceinline__ D3 nrsq(D3 s) {
+__device__ __forceinline__ D1 nrsq(D1 s) {
     const float f = nrsq(s.v);
-    const float d = -0.5f * f * f * f;
-    return D3(f, d * s.x, d * s.y, d * s.z);
+    return D1(f, -0.5f * f * f * f * s.d);
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -161,27 +165,28 @@ __global__ __launch_bounds__(64) void k_adjoint(BatchView V, const float4 *__res
                 const uint32_t e = idx[t0 + k];
                 const float4 r = lds[k];
                 const float X = sx - r.x, Y = sy - r.y, Z = sz - r.z;
-                const D3 dx(X, 1.f, 0.f, 0.f), dy(Y, 0.f, 1.f, 0.f), dz(Z, 0.f, 0.f, 1.f);
-                Acc4T<D3> a = {D3(0.f), D3(0.f), D3(0.f), D3(0.f)};
+                // the sink offset, differentiated along this sink's weight w
+                const D1 dx(X, w.x), dy(Y, w.y), dz(Z, w.z);
+                Acc4T<D1> a = {D1(0.f), D1(0.f), D1(0.f), D1(0.f)};
                 if (kind == 0)
-                    cell_interact<4, false, 1, D3, float>(V.mom32 + (size_t)e * PREC4, D3(r.w), dx, dy, dz, D3(eps2c), a);
+                    cell_interact<4, false, 1, D1, float>(V.mom32 + (size_t)e * PREC4, D1(r.w), dx, dy, dz, D1(eps2c), a);
                 else if (kind == 1)
-                    cell_interact<2, false, 1, D3, float>(V.mom32 + (size_t)e * PREC4, D3(r.w), dx, dy, dz, D3(eps2c), a);
+                    cell_interact<2, false, 1, D1, float>(V.mom32 + (size_t)e * PREC4, D1(r.w), dx, dy, dz, D1(eps2c), a);
                 else if (kind == 2)
-                    mono_interact<D3>(D3(r.w), dx, dy, dz, D3(eps2c), a);
+                    mono_interact<D1>(D1(r.w), dx, dy, dz, D1(eps2c), a);
                 else {
                     const bool self = X == 0.f && Y == 0.f && Z == 0.f;
                     if (SOFT == SOFT_K1)
-                        k1_interact<D3>(D3(r.w), dx, dy, dz, D3(eps_inv), D3(eps2), a);
+                        k1_interact<D1>(D1(r.w), dx, dy, dz, D1(eps_inv), D1(eps2), a);
                     else if (SOFT == SOFT_PLUMMER)
-                        mono_interact<D3>(D3(r.w), dx, dy, dz, D3(eps2), a);
+                        mono_interact<D1>(D1(r.w), dx, dy, dz, D1(eps2), a);
                     else if (!self)
-                        mono_interact<D3>(D3(r.w), dx, dy, dz, D3(0.f), a);
+                        mono_interact<D1>(D1(r.w), dx, dy, dz, D1(0.f), a);
                 }
-                // d(w.a + v phi)/d(sink position)
-                const float px = w.x * a.x.x + w.y * a.y.x + w.z * a.z.x + w.w * a.w.x;
-                const float py = w.x * a.x.y + w.y * a.y.y + w.z * a.z.y + w.w * a.w.y;
-                const float pz = w.x * a.x.z + w.y * a.y.z + w.z * a.z.z + w.w * a.w.z;
+                // d(w.a + v phi)/d(sink position) = J w - v a
+                const float px = a.x.d - w.w * a.x.v;
+                const float py = a.y.d - w.w * a.y.v;
+                const float pz = a.z.d - w.w * a.z.v;
                 gx += px;
                 gy += py;
                 gz += pz;
@@ -238,18 +243,20 @@ __global__ __launch_bounds__(64) void k_adjoint_bodysrc(BatchView V, const float
         for (int i = 0; i < cnt; i++) {
             const float4 x = sk[i], w = sw[i];
             const float X = x.x - rx, Y = x.y - ry, Z = x.z - rz;
-            const D3 dx(X, 1.f, 0.f, 0.f), dy(Y, 0.f, 1.f, 0.f), dz(Z, 0.f, 0.f, 1.f);
-            Acc4T<D3> a = {D3(0.f), D3(0.f), D3(0.f), D3(0.f)};
+            // the pair offset differentiated along sink i's weight; the source's
+            // gradient is minus the sink's, J w - v a
+            const D1 dx(X, w.x), dy(Y, w.y), dz(Z, w.z);
+            Acc4T<D1> a = {D1(0.f), D1(0.f), D1(0.f), D1(0.f)};
             const bool self = X == 0.f && Y == 0.f && Z == 0.f;
             if (SOFT == SOFT_K1)
-                k1_interact<D3>(D3(rm), dx, dy, dz, D3(eps_inv), D3(eps2), a);
+                k1_interact<D1>(D1(rm), dx, dy, dz, D1(eps_inv), D1(eps2), a);
             else if (SOFT == SOFT_PLUMMER)
-                mono_interact<D3>(D3(rm), dx, dy, dz, D3(eps2), a);
+                mono_interact<D1>(D1(rm), dx, dy, dz, D1(eps2), a);
             else if (!self)
-                mono_interact<D3>(D3(rm), dx, dy, dz, D3(0.f), a);
-            bx -= w.x * a.x.x + w.y * a.y.x + w.z * a.z.x + w.w * a.w.x;
-            by -= w.x * a.x.y + w.y * a.y.y + w.z * a.z.y + w.w * a.w.y;
-            bz -= w.x * a.x.z + w.y * a.y.z + w.z * a.z.z + w.w * a.w.z;
+                mono_interact<D1>(D1(rm), dx, dy, dz, D1(0.f), a);
+            bx -= a.x.d - w.w * a.x.v;
+            by -= a.y.d - w.w * a.y.v;
+            bz -= a.z.d - w.w * a.z.v;
             // the pair is linear in the source mass
             if (rm != 0.f)
                 bm += (w.x * a.x.v + w.y * a.y.v + w.z * a.z.v + w.w * a.w.v) / rm;
