@@ -440,24 +440,3 @@ def rel_errors(acc, ref):
     da = np.linalg.norm(acc[:, :3] - ref[:, :3], axis=1) / np.linalg.norm(ref[:, :3], axis=1)
     dp = np.abs(acc[:, 3] - ref[:, 3]) / np.abs(ref[:, 3])
     return da, dp
-
-
-def smoke_check(rng, verbose=False):
-    """One small clustered evaluation, FP32 mixed vs the library's own FP64
-    direct sum (the oracle is checked against that direct sum in tests/)."""
-    from . import ics
-    pos, mass = ics.clustered(20000, seed=int(rng.integers(1 << 30)), nhalo=16)
-    plan = Plan(pos, mass, theta=0.5, order=4, eps=1e-3)
-    plan.evaluate(REL32C)
-    acc = plan.result(REL32C)
-    sample = rng.choice(pos.shape[0], 256, replace=False)
-    ref = plan.direct(sample)
-    da, dp = rel_errors(acc[sample], ref)
-    st = plan.stats()
-    if verbose:
-        print("smoke: batched REL32C N=%d groups=%d pairs=%d median |da|/|a| %.2e (99%% %.2e)"
-              % (pos.shape[0], st["ngroups"], st["pairs_total"], np.median(da),
-                 np.percentile(da, 99)))
-    plan.close()
-    assert np.median(da) < 1e-4, np.median(da)
-    return float(np.median(da))
