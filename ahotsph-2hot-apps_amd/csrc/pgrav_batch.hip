@@ -287,11 +287,13 @@ void k_batch(BatchArgs A) {
 
 // persistent: a fixed grid drains per-XCD work counters.  Workgroup b runs on
 // XCD b % NXCD (the dispatcher deals workgroups to the XCDs round-robin) and
-// takes groups only from that XCD's slice [xoff[x], xoff[x+1]) of the dispatch
-// order: the host cuts the Morton-ordered groups into NXCD contiguous slices of
+// takes groups from that XCD's slice [xoff[x], xoff[x+1]) of the dispatch order
+// first: the host cuts the Morton-ordered groups into NXCD contiguous slices of
 // equal estimated cost, so neighbouring groups -- which share most of their
-// cells -- run under one XCD's L2.  The last workgroup to leave resets the
-// counters so the launch can be replayed (e.g. from a graph).
+// cells -- run under one XCD's L2.  When its slice is drained it takes the
+// remaining groups of the other slices, so an XCD whose estimate was low does
+// not idle through the tail.  The last workgroup to leave resets the counters so
+// the launch can be replayed (e.g. from a graph).
 //
 // Every buffer also arrives as its own __restrict__ argument: the loop stores
 // a group's results and bumps the counters before it reads the next group's
@@ -321,17 +323,29 @@ void k_batch_persistent(
     __shared__ V4<T> lds[64];
     __shared__ int w;
     const int x = blockIdx.x % NXCD;
-    const int lo = A.xoff[x], hi = A.xoff[x + 1];
+    // own slice first; once it is drained, the next group of the other XCDs' slices
+    // in turn (each counter only grows, so a drained slice stays drained)
+    int from = 0;
     for (;;) {
-        if (threadIdx.x == 0)
-            w = lo + atomicAdd(&A.counter[2 + x], 1);
+        if (threadIdx.x == 0) {
+            int it = -1;
+            for (; from < NXCD; from++) {
+                const int xx = (x + from) % NXCD;
+                const int v = A.xoff[xx] + atomicAdd(&A.counter[2 + xx], 1);
+                if (v < A.xoff[xx + 1]) {
+                    it = v;
+                    break;
+                }
+            }
+            w = it;
+        }
         __syncthreads();
         // the LDS value is the same in every lane: say so, so that the group's
         // scalars (list offsets, record addresses) stay in SGPRs and the
         // records keep coming through scalar loads
         const int item = __builtin_amdgcn_readfirstlane(w);
         __syncthreads();
-        if (item >= hi)
+        if (item < 0)
             break;
         eval_group<T, REL, COMP, SOFT, PER, BU>(A, A.lpt[item], lds);
     }
