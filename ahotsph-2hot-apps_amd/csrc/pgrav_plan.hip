@@ -217,7 +217,7 @@ void pgrav_params_default(pgrav_params *p) {
     p->merge_groups = 1;
     p->persistent_blocks = -1;  // persistent grid, 32 workgroups per CU
     p->fp64 = 1;
-    p->dispatch = 1;
+    p->dispatch = 2;
     p->mixed_order = 0;
     p->nimage = 2;  // nln/main.c:246
     p->rcb_cell = 32768;
@@ -583,7 +583,7 @@ int pgrav_plan_stats(void *plan, pgrav_stats *st) {
 int pgrav_plan_set_dispatch(void *plan, int mode) {
     Plan *P = (Plan *)plan;
     clear_error();
-    if (!P || (mode != 0 && mode != 1)) {
+    if (!P || mode < 0 || mode > 3) {
         fail("pgrav_plan_set_dispatch: bad arguments");
         return -1;
     }

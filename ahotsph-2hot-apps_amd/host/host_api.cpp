@@ -33,8 +33,8 @@ std::string tree_params_from(const pgrav_params *p, TreeParams &tp) {
                  p->errtol, p->frac_tol, p->frac_tol0);
     else if (p->merge_groups < 0 || p->merge_groups > 5)
         snprintf(msg, sizeof msg, "merge_groups must be 0..5 (got %d)", p->merge_groups);
-    else if (p->dispatch != 0 && p->dispatch != 1)
-        snprintf(msg, sizeof msg, "dispatch must be 0 or 1 (got %d)", p->dispatch);
+    else if (p->dispatch < 0 || p->dispatch > 3)
+        snprintf(msg, sizeof msg, "dispatch must be 0..3 (got %d)", p->dispatch);
     else if (p->leaf_max < 0 || p->pcut < 0 || p->rcb_cell < 0 || p->nthreads < 0)
         snprintf(msg, sizeof msg, "leaf_max, pcut, rcb_cell and nthreads must be >= 0");
     else if (p->mixed_order != 0 && p->mixed_order != 1)

@@ -611,8 +611,25 @@ void make_dispatch(const std::vector<double> &cost, int mode, Lists &l) {
             slice[s].push_back((int32_t)g);
             acc += cost[g];
         }
-        for (auto &v : slice)
-            std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+        if (mode == 1) {
+            for (auto &v : slice)
+                std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+        } else if (mode == 3) {
+            // the costliest 2 % of each slice first, the rest in Morton order
+            for (auto &v : slice) {
+                std::vector<int32_t> byc(v);
+                std::stable_sort(byc.begin(), byc.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+                const size_t k = byc.size() / 50;
+                std::vector<char> head(G, 0);
+                std::vector<int32_t> out(byc.begin(), byc.begin() + k);
+                for (size_t i = 0; i < k; i++) head[byc[i]] = 1;
+                for (int32_t g : v)
+                    if (!head[g])
+                        out.push_back(g);
+                v.swap(out);
+            }
+        }
+        // mode 2: Morton order inside each slice
     }
     l.lpt.clear();
     l.xoff.assign(X + 1, 0);

@@ -30,7 +30,8 @@ struct TreeParams {
                            // Morton-adjacent groups merged where the joint list costs less;
                            // 5: cells of <= rcb_cell bodies cut into balanced groups by
                            // coordinate bisection (groups are body sets, not runs)
-    int dispatch = 1;      // 0: global longest-first; 1: per-XCD Morton slices, longest-first inside
+    int dispatch = 2;      // 0: global longest-first; per-XCD Morton slices of equal cost with,
+                           // inside, 1 longest-first, 2 Morton order, 3 costliest 2 % then Morton
     int rcb_cell = 32768;  // mode 5: cells of at most this many bodies are cut into groups
     int mixed_order = 0;   // 1: an accepted cell far enough for a lower order to meet the same
                            // truncation bound is listed at that order (order-2 / order-0 lists)

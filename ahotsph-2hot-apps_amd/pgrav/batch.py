@@ -99,7 +99,7 @@ register({
 
 def make_params(theta=0.6, order=4, eps=1e-4, soft="plummer", G=1.0, group_max=64, leaf_max=16,
                 pcut=0, periodic=False, box=1.0, nthreads=0, merge_groups=True,
-                persistent_blocks=-1, fp64=True, dispatch=1, mixed_order=False, nimage=2,
+                persistent_blocks=-1, fp64=True, dispatch=2, mixed_order=False, nimage=2,
                 rcb_cell=32768, mac="theta", errtol=0.0, frac_tol=0.0, frac_tol0=0.0,
                 ptol_boost=0.0, stol_max=0.0, quad_ncut=7, hexa_ncut=20, group_box=0,
                 body_unroll=0):

@@ -142,7 +142,9 @@ typedef struct {
     int persistent_blocks; /* 0: one workgroup per group; >0: persistent grid of that many
                               single-wave workgroups; -1 (default): 32 per CU */
     int fp64;              /* also build the FP64 tables (variants 3 and 4) */
-    int dispatch;          /* 0: global longest-first; 1: per-XCD Morton slices (default) */
+    int dispatch;          /* 0: global longest-first; per-XCD Morton slices of equal estimated
+                              cost, ordered inside 1: longest-first, 2 (default): Morton order,
+                              3: the costliest 2 % first, then Morton order */
     int mixed_order;       /* 1: list an accepted cell at the lowest order whose truncation
                               bound (bmax/d)^(q+1) is within the order-`order` bound at the
                               opening-angle boundary, theta^(order+1): order 2 beyond
@@ -211,7 +213,7 @@ int pgrav_plan_set_launch(void *plan, int persistent_blocks);
 /* Unroll factor of the order-0 and body loops of the rel32c kernels for later
  * evaluations: 2, 4 (default) or 8 (an autotuner choice; same results). */
 int pgrav_plan_set_unroll(void *plan, int unroll);
-/* Group dispatch order: 0 global longest-first, 1 per-XCD Morton slices. */
+/* Group dispatch order (pgrav_params.dispatch, 0..3) for later evaluations. */
 int pgrav_plan_set_dispatch(void *plan, int mode);
 /* Enqueues one force evaluation on `stream` (NULL: the plan's stream). */
 int pgrav_plan_evaluate(void *plan, int variant, void *stream);

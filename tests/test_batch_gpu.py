@@ -173,9 +173,10 @@ def test_graph_persistent_and_plain_dispatch_reproduce_eager(setup):
 
 @pytest.mark.parametrize("mode", [1, 5])
 def test_unroll_variants_give_identical_results(setup, mode):
-    """The autotuner's unroll choices (order-0 and body loops of rel32c) are the
-    same arithmetic in the same order: bit-identical results, eager and from a
-    graph, persistent and one workgroup per group."""
+    """The autotuner's unroll choices (order-0 and body loops of rel32c) and the
+    dispatch orders are the same arithmetic in the same order per group:
+    bit-identical results, eager and from a graph, persistent (with its work
+    stealing across XCD slices) and one workgroup per group."""
     batch, ics, pos, mass = setup
     plan = batch.Plan(pos, mass, theta=0.6, order=4, eps=1e-4, merge_groups=mode, rcb_cell=4096,
                       mac="tolerance", errtol=0.1)  # lists with order-0, 2, 4 cells and bodies
@@ -189,6 +190,12 @@ def test_unroll_variants_give_identical_results(setup, mode):
             plan.set_launch(pb)
             plan.evaluate("rel32c")
             np.testing.assert_array_equal(plan.result("rel32c"), want)
+    # the group orders (global longest-first; XCD slices ordered longest-first,
+    # Morton, costliest-first then Morton) change only who evaluates a group when
+    for mode in (0, 1, 3, 2):
+        plan.set_dispatch(mode)
+        plan.evaluate("rel32c")
+        np.testing.assert_array_equal(plan.result("rel32c"), want)
     plan.set_unroll(8)
     plan.graph_capture("rel32c")
     for rep in range(2):

@@ -118,7 +118,7 @@ def test_bad_tolerances_raise():
     with pytest.raises(ValueError):
         batch.make_params(mac="bogus")
     # the other decomposition parameters are checked on the same path
-    for kw in (dict(merge_groups=6), dict(dispatch=2), dict(leaf_max=-1), dict(mixed_order=2),
+    for kw in (dict(merge_groups=6), dict(dispatch=4), dict(leaf_max=-1), dict(mixed_order=2),
                dict(group_box=3), dict(group_max=65), dict(order=3)):
         with pytest.raises(batch.PgravError):
             batch.HostPlan(pos, mass, **kw)
