@@ -6,8 +6,8 @@ optionally merged further where the joint list costs less, or cells cut into
 balanced groups by coordinate bisection), the group size
 (lanes per wave used), the leaf size, the launch shape (one workgroup per group in
 longest-first order, or a persistent grid of k single-wave workgroups per CU
-draining a work counter) and the unroll factor of the kernel's order-0 and body
-loops (compiled variants).  The opening angle, multipole order and softening are inputs, so the
+draining a work counter), the group order inside the per-XCD slices, and the
+unroll factor of the kernel's order-0 and body loops (compiled variants).  The opening angle, multipole order and softening are inputs, so the
 accuracy target is held fixed.  Each candidate is timed with HIP events over
 back-to-back launches on the plan's stream; the fastest wins.
 
@@ -34,6 +34,9 @@ DEFAULT_SPACE = {
 # unroll factors of the order-0 / body loops of the rel32c kernels (compiled
 # variants, chosen per plan without rebuilding it)
 UNROLL = [2, 4, 8]
+# group order inside the per-XCD slices: 1 longest-first, 2 Morton order (a
+# plan-level choice, no rebuild)
+DISPATCH = [1, 2]
 # launch shapes: 0 = one workgroup per group; -1 = persistent grid of 32
 # single-wave workgroups per CU; k > 0 = persistent grid of k per CU.  The
 # table and tuned.json store the resulting workgroup count (what
@@ -60,16 +63,18 @@ def tune(pos, mass, base, space=None, reps=3, variant="rel32c", log=print):
         build = time.perf_counter() - t0
         st = plan.stats()
         plan.evaluate(variant)  # warm-up
-        for pb, bu in itertools.product(PERSISTENT, UNROLL if variant == "rel32c" else [4]):
+        for pb, bu, dsp in itertools.product(PERSISTENT, UNROLL if variant == "rel32c" else [4],
+                                             DISPATCH):
             blocks = pb * _num_cus() if pb > 0 else pb
             plan.set_unroll(bu)
+            plan.set_dispatch(dsp)
             ms = _time_with_blocks(plan, variant, reps, blocks)
-            row = dict(kw, persistent_blocks=blocks, body_unroll=bu, ms=ms,
+            row = dict(kw, persistent_blocks=blocks, body_unroll=bu, dispatch=dsp, ms=ms,
                        pairs=st["pairs_total"], ngroups=st["ngroups"],
                        mean_group=st["mean_group"], build_s=build)
             table.append(row)
             log("tune %s -> %.3f ms (groups %d, mean %.1f, pairs %.3e)" %
-                ({k: row[k] for k in keys + ["persistent_blocks", "body_unroll"]}, ms,
+                ({k: row[k] for k in keys + ["persistent_blocks", "body_unroll", "dispatch"]}, ms,
                  st["ngroups"], st["mean_group"], st["pairs_total"]))
             if best is None or ms < best["ms"]:
                 best = row
@@ -108,7 +113,8 @@ def save(key, best, table, path=TUNED):
             data = json.load(f)
     except (OSError, ValueError):
         pass
-    keep = ("merge_groups", "group_max", "leaf_max", "persistent_blocks", "body_unroll", "ms")
+    keep = ("merge_groups", "group_max", "leaf_max", "persistent_blocks", "body_unroll", "dispatch",
+            "ms")
     data[key] = {"best": {k: best[k] for k in keep},
                  "table": [{k: r[k] for k in keep + ("pairs", "mean_group")} for r in table]}
     with open(path, "w") as f:
@@ -140,7 +146,8 @@ def main(argv=None):
     save(key, best, table, a.out)
     print(json.dumps({"key": key, "best": {k: best[k] for k in
                                            ("merge_groups", "group_max", "leaf_max",
-                                            "persistent_blocks", "body_unroll", "ms")}}))
+                                            "persistent_blocks", "body_unroll", "dispatch",
+                                            "ms")}}))
 
 
 if __name__ == "__main__":

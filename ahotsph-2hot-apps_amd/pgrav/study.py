@@ -72,7 +72,7 @@ def _errors(acc, ref):
 
 def _decomposition(n, theta, workload, eps=1e-4):
     dec = {"merge_groups": 1, "group_max": 64, "leaf_max": 16, "persistent_blocks": -1,
-           "body_unroll": 4}
+           "body_unroll": 4, "dispatch": 2}
     t = autotune.load(autotune.workload_key(n, theta, 4, "plummer", eps, workload))
     if t:
         dec.update({k: t["best"][k] for k in dec if k in t["best"]})
