@@ -211,7 +211,8 @@ int pgrav_plan_stats(void *plan, pgrav_stats *st);
  * keeps the shape it was captured with). */
 int pgrav_plan_set_launch(void *plan, int persistent_blocks);
 /* Unroll factor of the order-0 and body loops of the rel32c kernels for later
- * evaluations: 2, 4 (default) or 8 (an autotuner choice; same results). */
+ * evaluations: 2, 4 (default) or 8 (an autotuner choice; same results; a captured
+ * graph keeps the kernel it was captured with). */
 int pgrav_plan_set_unroll(void *plan, int unroll);
 /* Group dispatch order (pgrav_params.dispatch, 0..3) for later evaluations. */
 int pgrav_plan_set_dispatch(void *plan, int mode);
