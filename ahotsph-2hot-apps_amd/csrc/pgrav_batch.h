@@ -42,6 +42,7 @@ struct BatchView {
     const float4 *ewald32 = nullptr;
     const double4 *ewald64 = nullptr;  // FP64 table (direct-sum check)
     int unroll = 4;  // order-0 / body loop unroll of the rel32c kernels: 2, 4 or 8
+    int ewgroup = 0;  // periodic 1: cells' Ewald corrections expanded about the group centre
     int ngroups = 0;
     int soft = 0;
     double eps = 0;

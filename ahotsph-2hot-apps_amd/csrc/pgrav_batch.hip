@@ -56,6 +56,7 @@ struct BatchArgs {
     const uint8_t *img;  // explicit images: the image of each entry, parallel to idx (or null)
     int img_f;           // 2 nimage + 1 images per side
     int unroll;          // order-0 / body loop unroll of the rel32c kernels: 2, 4 or 8
+    int ewgroup;         // periodic 1: cells' Ewald corrections expanded about the group centre
     int ngroups;
     int soft;
     double eps;
@@ -105,6 +106,67 @@ __device__ __forceinline__ void ewald_corr(const TV *__restrict__ tab, T m, T x,
     a.w += m * cp;
 }
 
+// The same lookup at the group centre for a cell list entry, as a first-order
+// expansion in the sink's offset s from that centre: out[0] = (C, psi) at x0 =
+// centre - source, out[1..3] = row i of the Jacobian d(C_i, psi)/dx:
+// (dC_i/dx, dC_i/dy, dC_i/dz, dpsi/dx_i).  With the table's octant symmetry (C_i
+// odd in x_i, even in the others; psi even) d(sgn_i C_i)/dx_j = sgn_i sgn_j dC_i/du_j.
+// A sink then takes C(x0 + s) = C + J s: 12 FMAs on values the tile loader left in
+// LDS, instead of a 64-byte gather per pair.  Cells are accepted only well inside
+// half a box, so every sink of the group sees the cell in the same image as the
+// centre.
+template <typename T, typename TV>
+__device__ __forceinline__ void ewald_taylor(const TV *__restrict__ tab, T x, T y, T z, T box,
+                                             V4<T> *out) {
+    const T h = T(0.5) * box / T(EWALD_N - 1);
+    const T inv_h = T(1) / h;
+    const T sx = x < 0 ? T(-1) : T(1), sy = y < 0 ? T(-1) : T(1), sz = z < 0 ? T(-1) : T(1);
+    const T ax = fabs(x), ay = fabs(y), az = fabs(z);
+    const int i = max(0, min((int)rint(ax * inv_h), EWALD_N - 1));
+    const int j = max(0, min((int)rint(ay * inv_h), EWALD_N - 1));
+    const int k = max(0, min((int)rint(az * inv_h), EWALD_N - 1));
+    const T du = ax - T(i) * h, dv = ay - T(j) * h, dw = az - T(k) * h;
+    const TV *p = tab + 4 * (((size_t)i * EWALD_N + j) * EWALD_N + k);
+    const TV t0 = p[0], gu = p[1], gv = p[2], gw = p[3];
+    V4<T> c;
+    c.x = sx * (T(t0.x) + du * T(gu.x) + dv * T(gv.x) + dw * T(gw.x));
+    c.y = sy * (T(t0.y) + du * T(gu.y) + dv * T(gv.y) + dw * T(gw.y));
+    c.z = sz * (T(t0.z) + du * T(gu.z) + dv * T(gv.z) + dw * T(gw.z));
+    c.w = T(t0.w) + du * T(gu.w) + dv * T(gv.w) + dw * T(gw.w);
+    out[0] = c;
+    V4<T> r;
+    r.x = T(gu.x);
+    r.y = sx * sy * T(gv.x);
+    r.z = sx * sz * T(gw.x);
+    r.w = sx * T(gu.w);
+    out[1] = r;
+    r.x = sy * sx * T(gu.y);
+    r.y = T(gv.y);
+    r.z = sy * sz * T(gw.y);
+    r.w = sy * T(gv.w);
+    out[2] = r;
+    r.x = sz * sx * T(gu.z);
+    r.y = sz * sy * T(gv.z);
+    r.z = T(gw.z);
+    r.w = sz * T(gw.w);
+    out[3] = r;
+}
+
+// A cell's Ewald correction for one sink from the group-centre expansion in LDS:
+// the pull is m C at x = sink - source (as ewald_corr applies it).
+template <typename T>
+__device__ __forceinline__ void ewald_apply(const V4<T> *e, T m, T sx, T sy, T sz, Acc4T<T> &a) {
+    const V4<T> c = e[0], jx = e[1], jy = e[2], jz = e[3];
+    const T cx = c.x + jx.x * sx + jx.y * sy + jx.z * sz;
+    const T cy = c.y + jy.x * sx + jy.y * sy + jy.z * sz;
+    const T cz = c.z + jz.x * sx + jz.y * sy + jz.z * sz;
+    const T cp = c.w + jx.w * sx + jy.w * sy + jz.w * sz;
+    a.x -= m * cx;
+    a.y -= m * cy;
+    a.z -= m * cz;
+    a.w += m * cp;
+}
+
 template <typename T>
 __device__ __forceinline__ void fold(Acc4T<T> &s, Acc4T<T> &c, const Acc4T<T> &p, bool comp) {
     if (comp) {
@@ -146,8 +208,13 @@ __device__ __forceinline__ V4<T> source_offset(double4 c, double4 gc, double box
 // BU: unroll factor of the order-0 and body loops (2, 4 or 8; the autotuner picks it
 // for the group-relative compensated FP32 kernels, the others use 4)
 template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU>
-__device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds) {
+__device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds, V4<T> *ewl) {
     constexpr bool PERIODIC = PER == 1;
+    // group-relative periodic with A.ewgroup: the cells' Ewald corrections come
+    // expanded about the group centre (ewl: 4 V4 per tile entry, then one flag per
+    // entry saying the expansion is used and holds for every sink); bodies keep the
+    // per-pair lookup
+    constexpr bool EW_GROUP = PERIODIC && REL;
     const int lane = threadIdx.x;
     const int cnt = A.gcount[g];
     const double4 gc = A.gcen[g];
@@ -196,6 +263,19 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                     src.z += (double)(im % f - h) * A.box;
                 }
                 lds[lane] = source_offset<T, REL, PERIODIC>(src, gc, A.box);
+                if (EW_GROUP && kind < 3 && A.ewgroup) {
+                    const V4<T> o = lds[lane];
+                    const T x0 = pbc<PERIODIC>(-o.x, box, ibox), y0 = pbc<PERIODIC>(-o.y, box, ibox),
+                            z0 = pbc<PERIODIC>(-o.z, box, ibox);
+                    ewald_taylor<T>(A.ewald, x0, y0, z0, box, ewl + 4 * lane);
+                    // the expansion holds only if no sink of the group sees another image
+                    // of the cell: every sink within the group radius of the centre stays
+                    // on the centre's side of the half-box boundary in each axis
+                    const T lim = T(0.5) * box - T(gc.w);
+                    ewl[4 * 64 + lane].x = (fabs(x0) < lim && fabs(y0) < lim && fabs(z0) < lim) ? T(1) : T(0);
+                } else if (EW_GROUP) {
+                    ewl[4 * 64 + lane].x = T(0);  // per-pair lookups for this tile
+                }
             }
             __syncthreads();
             Acc4T<T> p = {0, 0, 0, 0};
@@ -207,7 +287,9 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                     const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
                     const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
                     cell_interact<4, false, 1, T>(mom + (size_t)e * PREC4, r.w, x, y, z, eps2_cell, p);
-                    if (PERIODIC)
+                    if (EW_GROUP && ewl[4 * 64 + k].x != T(0))
+                        ewald_apply<T>(ewl + 4 * k, r.w, sx, sy, sz, p);
+                    else if (PERIODIC)
                         ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
                 }
             } else if (kind == 1) {
@@ -218,7 +300,9 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                     const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
                     const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
                     cell_interact<2, false, 1, T>(mom + (size_t)e * PREC4, r.w, x, y, z, eps2_cell, p);
-                    if (PERIODIC)
+                    if (EW_GROUP && ewl[4 * 64 + k].x != T(0))
+                        ewald_apply<T>(ewl + 4 * k, r.w, sx, sy, sz, p);
+                    else if (PERIODIC)
                         ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
                 }
             } else if (kind == 2) {
@@ -229,7 +313,9 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                     const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
                     const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
                     mono_interact<T>(r.w, x, y, z, eps2_cell, p);
-                    if (PERIODIC)
+                    if (EW_GROUP && ewl[4 * 64 + k].x != T(0))
+                        ewald_apply<T>(ewl + 4 * k, r.w, sx, sy, sz, p);
+                    else if (PERIODIC)
                         ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
                 }
             } else {
@@ -282,7 +368,8 @@ template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PER>::value)))
 void k_batch(BatchArgs A) {
     __shared__ V4<T> lds[64];
-    eval_group<T, REL, COMP, SOFT, PER, BU>(A, A.order_np[blockIdx.x], lds);
+    __shared__ V4<T> ewl[PER == 1 && REL ? 5 * 64 : 1];
+    eval_group<T, REL, COMP, SOFT, PER, BU>(A, A.order_np[blockIdx.x], lds, ewl);
 }
 
 // persistent: a fixed grid drains per-XCD work counters.  Workgroup b runs on
@@ -321,6 +408,7 @@ void k_batch_persistent(
     A.counter = counter;
     A.ewald = ewald;
     __shared__ V4<T> lds[64];
+    __shared__ V4<T> ewl[PER == 1 && REL ? 5 * 64 : 1];
     __shared__ int w;
     const int x = blockIdx.x % NXCD;
     // own slice first; once it is drained, the next group of the other XCDs' slices
@@ -347,7 +435,7 @@ void k_batch_persistent(
         __syncthreads();
         if (item < 0)
             break;
-        eval_group<T, REL, COMP, SOFT, PER, BU>(A, A.lpt[item], lds);
+        eval_group<T, REL, COMP, SOFT, PER, BU>(A, A.lpt[item], lds, ewl);
     }
     if (threadIdx.x == 0) {
         const int done = atomicAdd(&A.counter[1], 1);
@@ -362,7 +450,7 @@ void k_batch_persistent(
 template <int SOFT, int PER>
 __global__ __launch_bounds__(64) void k_group64(BatchArgs A, int g) {
     __shared__ V4<double> lds[64];
-    eval_group<double, false, false, SOFT, PER, 4>(A, g, lds);
+    eval_group<double, false, false, SOFT, PER, 4>(A, g, lds, nullptr);
 }
 
 __global__ void k_reset_counter(int *c) {
@@ -589,6 +677,7 @@ hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, 
     A.img = v.img;
     A.img_f = 2 * v.nimage + 1;
     A.unroll = v.unroll;
+    A.ewgroup = v.ewgroup;
     if (A.ngroups <= 0)
         return hipSuccess;
     if (v.periodic < 0 || v.periodic > 2 || (v.periodic == 2 && !v.img))
@@ -630,6 +719,7 @@ hipError_t launch_group64(const BatchView &v, int g, hipStream_t st) {
     A.img = v.img;
     A.img_f = 2 * v.nimage + 1;
     A.unroll = v.unroll;
+    A.ewgroup = v.ewgroup;
     if (v.periodic < 0 || v.periodic > 2 || (v.periodic == 2 && !v.img))
         return hipErrorInvalidValue;
     const int per = v.periodic;

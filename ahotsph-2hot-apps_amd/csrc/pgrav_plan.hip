@@ -359,6 +359,7 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
             throw std::runtime_error("memset");
         P->v.soft = p->soft_type;
         P->v.unroll = p->body_unroll ? p->body_unroll : 4;
+        P->v.ewgroup = p->ewald_expand ? 1 : 0;
         P->v.eps = p->eps;
         P->v.G = p->G;
         P->v.periodic = p->periodic;

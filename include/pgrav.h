@@ -171,6 +171,13 @@ typedef struct {
     int hexa_ncut;         /* mac 1: ... at order 4 (default 20, nln/main.c:541) */
     int body_unroll;       /* rel32c kernels: unroll of the order-0 and body loops, 2, 4 or 8
                               (0: 4) -- a performance choice, same results */
+    int ewald_expand;      /* periodic 1, group-relative variants: 1 = a cell's Ewald
+                              correction is looked up once per (group, cell) at the group
+                              centre and expanded to first order in each sink's offset,
+                              instead of one table lookup per pair (cells near an image
+                              boundary of the group keep the per-pair lookup); faster, with
+                              an error ~ (group radius)^2 |C''| that shows at tight theta;
+                              0 (default): per pair */
     int group_box;         /* 0 (default): a cell's distance to a group is measured to the
                               group's enclosing sphere; 1: to its bounding box (every sink
                               is inside it: a tighter bound for every sink, fewer cells

@@ -16,14 +16,16 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 
-def test_perfect_lattice_has_no_force(gpu_lib):
+@pytest.mark.parametrize("expand", [False, True])
+def test_perfect_lattice_has_no_force(gpu_lib, expand):
     from pgrav import batch
     ng = 12
     g = (np.arange(ng) + 0.5) / ng
     gx, gy, gz = np.meshgrid(g, g, g, indexing="ij")
     pos = np.stack([gx, gy, gz], -1).reshape(-1, 3)
     mass = np.full(pos.shape[0], 1.0 / pos.shape[0])
-    plan = batch.Plan(pos, mass, periodic=True, box=1.0, theta=0.5, order=4, soft="none", eps=0.0)
+    plan = batch.Plan(pos, mass, periodic=True, box=1.0, theta=0.5, order=4, soft="none", eps=0.0,
+                      ewald_expand=expand)
     plan.evaluate("rel32c")
     acc = plan.result("rel32c")
     pair = mass[0] * ng ** 2  # one neighbour's pull at one lattice spacing
@@ -34,21 +36,25 @@ def test_perfect_lattice_has_no_force(gpu_lib):
     plan.close()
 
 
-@pytest.mark.parametrize("mode", [1, 5])
-def test_periodic_matches_ewald_direct_sum(gpu_lib, mode):
-    """mode 5: bisection groups (body sets) with the nearest image + Ewald table."""
+@pytest.mark.parametrize("mode,expand", [(1, False), (5, False), (1, True)])
+def test_periodic_matches_ewald_direct_sum(gpu_lib, mode, expand):
+    """mode 5: bisection groups (body sets) with the nearest image + Ewald table;
+    expand: the cells' corrections expanded about the group centre, whose
+    first-order error in the sink offset is large for groups this big relative to
+    the box (radius ~0.1 L): about 2e-2 of the rms force here."""
     from pgrav import batch, ics
     pos, mass = ics.lattice_grf(16, seed=8, rms=0.2)
     plan = batch.Plan(pos, mass, periodic=True, box=1.0, theta=0.4, order=4, soft="none", eps=0.0,
-                      merge_groups=mode, rcb_cell=512)
+                      merge_groups=mode, rcb_cell=512, ewald_expand=expand)
     plan.evaluate("rel32c")
     acc = plan.result("rel32c")
     sample = np.random.default_rng(2).choice(pos.shape[0], 48, replace=False)
     ref = oracle.direct_ewald(pos[sample], pos, mass, L=1.0)
     err = np.linalg.norm(acc[sample, :3] - ref[:, :3], axis=1)
     rms = np.sqrt((ref[:, :3] ** 2).sum(1).mean())
-    assert np.median(err) / rms < 1e-2, np.median(err) / rms
-    assert err.max() / rms < 5e-2, err.max() / rms
+    tol = 4e-2 if expand else 1e-2
+    assert np.median(err) / rms < tol, np.median(err) / rms
+    assert err.max() / rms < 5 * tol, err.max() / rms
     # FP64 batched variant agrees with the FP32 one to FP32 precision
     plan.evaluate("f64")
     f64 = plan.result("f64")
