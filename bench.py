@@ -87,7 +87,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); rerun with --gpus 1", file=sys.stderr)
+              "(the path runs on one GPU's share of the bodies); rerun with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.quick:
         args.n = 1 << 21
