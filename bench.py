@@ -86,8 +86,8 @@ def main():
                          "FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path "
-              "(the path runs on one GPU's share of the bodies); rerun with --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU: there is no multi-GPU path; rerun with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     if args.quick:
         args.n = 1 << 21
