@@ -1,14 +1,14 @@
 """Measurements of the path on the configurations it is specified on (one GPU).
 
-    python -m pgrav.study c2 [--n 2097152]
-    python -m pgrav.study c3 [--n 16777216] [--thetas 0.5 0.6 0.7] [--group-boxes 0 1]
+    python studies.py c2 [--n 2097152]
+    python studies.py c3 [--n 16777216] [--thetas 0.5 0.6 0.7] [--group-boxes 0 1]
                              [--variants rel32c rel32 abs32 f64]
-    python -m pgrav.study c4 [--ng 256] [--theta 0.6 | --thetas 0.4 0.5 0.6]
+    python studies.py c4 [--ng 256] [--theta 0.6 | --thetas 0.4 0.5 0.6]
                              [--scheme ewald | images --nimage 2]
                              [--mac tolerance --errtols 1e-2 1e-3] [--ewald-expand]
-    python -m pgrav.study c5 [--n 1048576]
-    python -m pgrav.study dropin [--n 262144]
-    python -m pgrav.study mac [--n 16777216] [--errtols 1e-2 1e-3 1e-4] [--leaves 16 64]
+    python studies.py c5 [--n 1048576]
+    python studies.py dropin [--n 262144]
+    python studies.py mac [--n 16777216] [--errtols 1e-2 1e-3 1e-4] [--leaves 16 64]
 
 c2  the plain method on the GPU at scale: clustered bodies (pgrav.ics.clustered),
     order 4, theta 0.6, Plummer eps 1e-4, FP64 with one launch per sink group
@@ -55,9 +55,17 @@ import json
 import sys
 import time
 
-import numpy as np
+import os
 
-from . import autotune, batch, ics
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "ahotsph-2hot-apps_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+from pgrav import autotune, batch, ics  # noqa: E402
 
 
 def _log(*a):
@@ -258,7 +266,7 @@ def c5(args):
 
 def dropin(args):
     import pgrav
-    from . import reflists
+    from pgrav import reflists
     pos, mass = ics.clustered(args.n, seed=3)
     n = pos.shape[0]
     eps = 1e-3
