@@ -296,6 +296,25 @@ int pgrav_host_ewald_le(const double *pos3, const double *mass, int64_t n, doubl
 int pgrav_host_le_lattice(int k, int nimage, double *ck);
 int pgrav_host_lpt(void *h, int32_t *order);
 
+/* Keys of the reference's tree (host-only).  A key is 1 followed by 31 bits per
+ * axis interleaved x y z from the top bit down (94 bits), returned as two words
+ * key2 = (low 64 bits, high bits), the reference's two-word Key_t.
+ * pgrav_host_fix_rsize: the key cube from the bounding box, nln/physics_generic.c:77-102
+ * (FixRsize, exact = 0) or :104-128 (FixRsizeExact, exact = 1).
+ * pgrav_host_keys: kind 0 GetKey (Morton, :146-167; bodies outside the cube are an
+ * error, as IntPos :130-144 makes them), 1 GetKeyFast (:172-223, same keys for bodies
+ * inside the cube, no check), 2 GetKeyPH (Peano-Hilbert, :226-233 over the automaton
+ * of :340-482).  Positions are float, as in the reference's body.
+ * pgrav_host_cell_corner: the corner of the cell a key names, CellCorner (:236-258,
+ * size = the cell's edge) or, ph_curve = 1, CellCornerPH (:260-272, size = the finest
+ * cell's edge, as there; the corner from the exact inverse of the PH key). */
+int pgrav_host_fix_rsize(const float *rmin, const float *rmax, int exact, float *out_rmin,
+                         float *rsize, float *keyfactor);
+int pgrav_host_keys(const float *pos3, int64_t n, const float *rmin, float rsize, float keyfactor,
+                    int kind, uint64_t *key2);
+int pgrav_host_cell_corner(const uint64_t *key2, int ph_curve, const float *rmin, float rsize,
+                           float *corner3, float *size);
+
 /* ------------------------------------------------------------------ part 4
  * The step either side of the force evaluation: the leapfrog kick-drift of the
  * body table.  Fields are float arrays with a common byte stride (the
