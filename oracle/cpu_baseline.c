@@ -11,8 +11,10 @@
  * reference runs one single-threaded process per core; threads over disjoint
  * groups are the same work distribution).
  */
+#define _GNU_SOURCE
 #include <dlfcn.h>
 #include <pthread.h>
+#include <sched.h>
 #include <stdatomic.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -41,7 +43,7 @@ typedef struct {
     float eps2;
     float *out4;
     atomic_int next;
-    double ksec[256];
+    double ksec[256][2];  /* per thread: seconds in cell calls, in body calls */
 } ljob_t;
 
 typedef struct {
@@ -67,7 +69,7 @@ static void *lworker(void *arg) {
     const int ns = J->nsse, csz = J->csz;
     float *cb = NULL, *bb = NULL;
     size_t ccap = 0, bcap = 0;
-    double ks = 0;
+    double kc = 0, kb = 0;
     for (;;) {
         const int k = atomic_fetch_add(&J->next, 1);
         if (k >= J->ng)
@@ -90,37 +92,55 @@ static void *lworker(void *arg) {
             float *d = bb + (i / ns) * ns * 4 + (i % ns);
             for (int f = 0; f < 4; f++) d[f * ns] = r[f];
         }
-        struct timespec t0, t1;
+        /* the cell kernel for every sink of the group, then the body kernel, each
+         * timed as one block (the reference's per-kind timers, nln/main.c:1259-1275) */
+        const int64_t s0 = J->gfirst[g], s1 = s0 + J->gcount[g];
+        struct timespec t0, t1, t2;
         clock_gettime(CLOCK_MONOTONIC, &t0);
-        for (int64_t s = J->gfirst[g]; s < J->gfirst[g] + J->gcount[g]; s++) {
-            float mass0 = 0, acc[3] = {0, 0, 0}, phi = 0, acc2[3] = {0, 0, 0}, phi2 = 0;
+        for (int64_t s = s0; s < s1; s++) {
+            float mass0 = 0, acc[3] = {0, 0, 0}, phi = 0;
             int ncut = 0;
             if (nhb)
                 J->cell(cb, cb + nhb * ns * csz, J->sinks3 + 3 * s, &mass0, acc, &phi, &J->eps2, &ncut);
-            if (nbb)
-                J->body(bb, bb + nbb * ns * 4, J->sinks3 + 3 * s, &mass0, acc2, &phi2, &J->eps2, &ncut);
-            J->out4[4 * s + 0] = acc[0] + acc2[0];
-            J->out4[4 * s + 1] = acc[1] + acc2[1];
-            J->out4[4 * s + 2] = acc[2] + acc2[2];
-            J->out4[4 * s + 3] = phi + phi2;
+            J->out4[4 * s + 0] = acc[0];
+            J->out4[4 * s + 1] = acc[1];
+            J->out4[4 * s + 2] = acc[2];
+            J->out4[4 * s + 3] = phi;
         }
         clock_gettime(CLOCK_MONOTONIC, &t1);
-        ks += (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+        for (int64_t s = s0; s < s1; s++) {
+            float mass0 = 0, acc2[3] = {0, 0, 0}, phi2 = 0;
+            int ncut = 0;
+            if (nbb)
+                J->body(bb, bb + nbb * ns * 4, J->sinks3 + 3 * s, &mass0, acc2, &phi2, &J->eps2, &ncut);
+            J->out4[4 * s + 0] += acc2[0];
+            J->out4[4 * s + 1] += acc2[1];
+            J->out4[4 * s + 2] += acc2[2];
+            J->out4[4 * s + 3] += phi2;
+        }
+        clock_gettime(CLOCK_MONOTONIC, &t2);
+        kc += (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+        kb += (t2.tv_sec - t1.tv_sec) + 1e-9 * (t2.tv_nsec - t1.tv_nsec);
     }
-    J->ksec[tid] = ks;
+    J->ksec[tid][0] = kc;
+    J->ksec[tid][1] = kb;
     free(cb);
     free(bb);
     return NULL;
 }
 
 /* Returns wall seconds of the parallel region (gathers included), or -1 on
- * error; *kernel_seconds = kernel-call seconds summed over the threads. */
+ * error; kernel_seconds[0], [1] = seconds in cell-kernel and body-kernel calls,
+ * each summed over the threads.  cpus (may be NULL): thread t runs pinned to
+ * logical CPU cpus[t] (one per physical core, as one reference process per
+ * core would be). */
 double orc_cpu_ref_run_lists(const char *lib_path, const char *cell_fn, const char *body_fn,
                              int nsse, int cell_sz, const float *rec, const float *sinks3,
                              const float *bodies4, const int64_t *gfirst, const int *gcount,
                              const int64_t *hoff, const uint32_t *hidx, const int64_t *boff,
                              const uint32_t *bidx, const int *groups, int ng, float eps2,
-                             int nthreads, float *out4, double *kernel_seconds) {
+                             int nthreads, const int *cpus, float *out4,
+                             double *kernel_seconds) {
     void *h = dlopen(lib_path, RTLD_NOW | RTLD_LOCAL);
     if (!h)
         return -1;
@@ -155,16 +175,36 @@ double orc_cpu_ref_run_lists(const char *lib_path, const char *cell_fn, const ch
     larg_t args[256];
     struct timespec t0, t1;
     clock_gettime(CLOCK_MONOTONIC, &t0);
+    int started = 0;
     for (int t = 0; t < nthreads; t++) {
         args[t].J = &J;
         args[t].tid = t;
-        pthread_create(&th[t], NULL, lworker, &args[t]);
+        pthread_attr_t at;
+        pthread_attr_init(&at);
+        if (cpus) {
+            cpu_set_t set;
+            CPU_ZERO(&set);
+            CPU_SET(cpus[t], &set);
+            pthread_attr_setaffinity_np(&at, sizeof(set), &set);
+        }
+        if (pthread_create(&th[t], &at, lworker, &args[t]) == 0)
+            started++;
+        pthread_attr_destroy(&at);
+        if (started != t + 1)
+            break;
     }
-    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    for (int t = 0; t < started; t++) pthread_join(th[t], NULL);
+    if (started != nthreads)
+        return -1;
     clock_gettime(CLOCK_MONOTONIC, &t1);
-    double ks = 0;
-    for (int t = 0; t < nthreads; t++) ks += J.ksec[t];
-    if (kernel_seconds)
-        *kernel_seconds = ks;
+    double kc = 0, kb = 0;
+    for (int t = 0; t < nthreads; t++) {
+        kc += J.ksec[t][0];
+        kb += J.ksec[t][1];
+    }
+    if (kernel_seconds) {
+        kernel_seconds[0] = kc;
+        kernel_seconds[1] = kb;
+    }
     return (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
 }

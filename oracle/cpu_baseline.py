@@ -12,6 +12,7 @@ of nln/mac.c:477-630 does.  Positions are absolute FP32, as in the reference.
 import ctypes
 import os
 import platform
+import subprocess
 import time
 
 import numpy as np
@@ -19,55 +20,129 @@ import numpy as np
 from . import REF_DIR, lib, ref_available
 
 
-def _cpu_model():
+def _lscpu():
+    """lscpu's key: value lines (empty when lscpu is absent)."""
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                             text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return {}
+    return {k.strip(): v.strip() for k, v in
+            (line.split(":", 1) for line in out.splitlines() if ":" in line)}
+
+
+def _cgroup_cpus():
+    """CPUs the cgroup quota grants this process (cgroup v2 cpu.max), or None."""
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        return None if q == "max" else max(1, int(int(q) // int(per)))
+    except (OSError, ValueError):
+        return None
+
+
+def host_cpu():
+    """The host's processor as the reference reports it (nln/main.c:1447-1448: model,
+    cores, clock), plus the CPUs this process may use: one logical CPU per physical
+    core of the affinity set, capped by the cgroup quota."""
+    ls = _lscpu()
+    model = ls.get("Model name", "")
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if line.startswith("model name") and len(line.split(":", 1)[1].strip()) > len(model):
+                model = line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return platform.processor() or "unknown"
+    allowed = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else \
+        list(range(os.cpu_count() or 1))
+    seen, per_core, cores_host = set(), [], set()
+    for c in range(os.cpu_count() or 1):
+        try:
+            base = "/sys/devices/system/cpu/cpu%d/topology/" % c
+            key = (int(open(base + "physical_package_id").read()), int(open(base + "core_id").read()))
+        except (OSError, ValueError):
+            key = (0, c)
+        cores_host.add(key)
+        if c in allowed and key not in seen:
+            seen.add(key)
+            per_core.append(c)
+    quota = _cgroup_cpus()
+    pin = per_core[:quota] if quota else per_core
+    mhz = None
+    for k in ("CPU max MHz", "CPU MHz"):
+        try:
+            mhz = float(ls[k])
+            break
+        except (KeyError, ValueError):
+            pass
+    if mhz is None:
+        try:
+            v = [float(line.split(":")[1]) for line in open("/proc/cpuinfo")
+                 if line.startswith("cpu MHz")]
+            mhz = sum(v) / len(v) if v else None
+        except (OSError, ValueError):
+            mhz = None
+    return {"cpu_model": model or platform.processor() or "unknown",
+            "vendor": ls.get("Vendor ID"), "sockets": ls.get("Socket(s)"),
+            "cores_per_socket": ls.get("Core(s) per socket"),
+            "threads_per_core": ls.get("Thread(s) per core"),
+            "physical_cores": len(cores_host), "nproc": os.cpu_count(),
+            "affinity_cpus": len(allowed), "cgroup_cpu_quota": quota,
+            "clock_mhz": mhz, "pin_cpus": pin}
 
 
-def _run_lists(arch, D, groups, eps2, nthreads, cell="ph", csz=29):
+def _run_lists(arch, D, groups, eps2, nthreads, cell="ph", csz=29, cpus=None):
+    """Wall seconds, (cell, body) kernel seconds summed over threads, results."""
     L = lib()
     fn = L.orc_cpu_ref_run_lists
     fn.restype = ctypes.c_double
     fn.argtypes = ([ctypes.c_char_p] * 3 + [ctypes.c_int] * 2 + [ctypes.c_void_p] * 10 +
-                   [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
+                   [ctypes.c_int, ctypes.c_float, ctypes.c_int] + [ctypes.c_void_p] * 3)
     nsse = 4 if arch == "sse4" else 8
     out = np.zeros((D["sinks"].shape[0], 4), np.float32)
-    ks = ctypes.c_double(0)
+    ks = (ctypes.c_double * 2)()
+    pin = None
+    if cpus is not None:
+        pin = np.ascontiguousarray(cpus[:nthreads], np.int32)
     groups = np.ascontiguousarray(groups, np.int32)
     path = os.path.join(REF_DIR, "libref_%s.so" % arch).encode()
     wall = fn(path, ("do_grav%s_%s" % (cell, arch)).encode(), ("do_gravp_" + arch).encode(), nsse, csz,
               D["rec"].ctypes.data, D["sinks"].ctypes.data, D["bodies"].ctypes.data,
               D["first"].ctypes.data, D["count"].ctypes.data, D["hoff"].ctypes.data,
               D["hidx"].ctypes.data, D["boff"].ctypes.data, D["bidx"].ctypes.data,
-              groups.ctypes.data, groups.size, eps2, nthreads, out.ctypes.data, ctypes.byref(ks))
+              groups.ctypes.data, groups.size, eps2, nthreads,
+              pin.ctypes.data if pin is not None else None, out.ctypes.data, ks)
     if wall < 0:
-        raise RuntimeError("reference CPU kernels could not be loaded from %s" % path)
-    return wall, ks.value, out
+        raise RuntimeError("reference CPU kernels could not be loaded from %s, or a thread "
+                           "could not be started" % path)
+    return wall, (ks[0], ks[1]), out
 
 
-def _interactions(D, groups):
+def _interactions(D, groups, kind=None):
+    """(sink, source) pairs of the groups: cells, bodies, or both (kind None)."""
     c = D["count"][groups].astype(np.int64)
     nh = D["hoff"][groups + 1] - D["hoff"][groups]
     nb = D["boff"][groups + 1] - D["boff"][groups]
-    return int((c * (nh + nb)).sum())
+    return int((c * {"cell": nh, "body": nb, None: nh + nb}[kind]).sum())
 
 
 def run_cpu_baseline(pos, mass, args, acc_gpu=None, target_s=15.0, nthreads=None, dec=None):
     """Returns the bench.py cpu_baseline object (interactions/s on the host cores).
 
-    The sample is every group when the two timed passes fit in about
-    `target_s` seconds of wall time, else a seeded random subset sized to fit.
-    The rate is interactions / (kernel-call seconds summed over threads /
-    threads): the gathers into SoA blocks are not counted against the CPU."""
+    One thread per physical core this process may use (the reference runs one
+    single-threaded process per core), each pinned to its core.  The sample is
+    every group when the two timed passes fit in about `target_s` seconds of wall
+    time, else a seeded random subset sized to fit.  The rate is interactions /
+    (kernel-call seconds summed over threads / threads): the gathers into SoA
+    blocks are not counted against the CPU.  Cycles per interaction per kind are
+    the reference's own figure (nln/main.c:1259-1275: Hz x kernel time / count;
+    its counters hold ten times that)."""
     if not ref_available():
         raise RuntimeError("oracle/_ref is not built (make -C oracle ref)")
     from pgrav import batch
-    nthreads = nthreads or min(16, os.cpu_count() or 1)
+    cpu = host_cpu()
+    cpus = cpu["pin_cpus"]
+    nthreads = nthreads or len(cpus)
+    cpus = cpus if nthreads <= len(cpus) else None
     t0 = time.perf_counter()
     dec = dec or {"merge_groups": args.group_mode, "group_max": 64, "leaf_max": 16}
     H = batch.HostPlan(pos, mass, theta=args.theta, order=args.order, eps=args.eps,
@@ -98,10 +173,11 @@ def run_cpu_baseline(pos, mass, args, acc_gpu=None, target_s=15.0, nthreads=None
     rng = np.random.default_rng(777)
     eps2 = float(args.eps) ** 2
     result = {"unit": "interactions/s", "cores": nthreads, "kind": "reference",
-              "cpu_model": _cpu_model()}
+              "pinned": cpus is not None}
+    result.update({k: v for k, v in cpu.items() if k != "pin_cpus"})
     # calibrate on a small sample, then size the timed sample to ~target_s seconds
     cal = rng.choice(ng, min(ng, 64 * nthreads), replace=False).astype(np.int32)
-    wall, _, _ = _run_lists("sse4", D, cal, eps2, nthreads, cell, csz)
+    wall, _, _ = _run_lists("sse4", D, cal, eps2, nthreads, cell, csz, cpus)
     per_int = wall / max(1, _interactions(D, cal))
     total = _interactions(D, np.arange(ng))
     # SSE4 then AVX (about half the time): ~1.5 SSE4 passes in the budget
@@ -109,16 +185,24 @@ def run_cpu_baseline(pos, mass, args, acc_gpu=None, target_s=15.0, nthreads=None
     groups = (np.arange(ng) if frac >= 1.0 else
               np.sort(rng.choice(ng, max(len(cal), int(frac * ng)), replace=False))).astype(np.int32)
     inter = _interactions(D, groups)
+    n_cell, n_body = _interactions(D, groups, "cell"), _interactions(D, groups, "body")
+    hz = cpu["clock_mhz"] * 1e6 if cpu["clock_mhz"] else None
     for arch in ("sse4", "avx8"):
         # a workload that finishes in milliseconds is repeated (best pass kept)
-        wall, ks, out = _run_lists(arch, D, groups, eps2, nthreads, cell, csz)
+        wall, ks, out = _run_lists(arch, D, groups, eps2, nthreads, cell, csz, cpus)
         for _ in range(min(50, int(1.0 / max(wall, 1e-4)))):
-            w2, k2, _ = _run_lists(arch, D, groups, eps2, nthreads, cell, csz)
-            if k2 < ks:
+            w2, k2, _ = _run_lists(arch, D, groups, eps2, nthreads, cell, csz, cpus)
+            if sum(k2) < sum(ks):
                 wall, ks = w2, k2
-        result[arch] = inter / (ks / nthreads)
+        result[arch] = inter / (sum(ks) / nthreads)
         result[arch + "_wall_seconds"] = wall
-        result[arch + "_kernel_seconds_per_thread"] = ks / nthreads
+        result[arch + "_kernel_seconds_per_thread"] = sum(ks) / nthreads
+        if hz:
+            # kernel seconds summed over threads = core-seconds, so Hz x that / pairs is
+            # the cycles one core spends per pair
+            result[arch + "_cycles_per_interaction"] = {
+                "order%d_cell" % (4 if cell == "ph" else 2): hz * ks[0] / max(1, n_cell),
+                "body": hz * ks[1] / max(1, n_body)}
         if acc_gpu is not None and arch == "avx8":
             sidx = np.concatenate([np.arange(first[g], first[g] + count[g]) for g in groups[:4096]])
             ref = np.asarray(acc_gpu)[perm[gslots[sidx]]]
@@ -126,10 +210,14 @@ def run_cpu_baseline(pos, mass, args, acc_gpu=None, target_s=15.0, nthreads=None
                  np.linalg.norm(ref[:, :3], axis=1))
             result["median_rel_diff_vs_gpu"] = float(np.median(d))
     result["value"] = max(result["sse4"], result["avx8"])
+    result["cycles_per_interaction"] = result.get(
+        ("avx8" if result["avx8"] >= result["sse4"] else "sse4") + "_cycles_per_interaction")
+    result["interactions_by_kind"] = {"cell": n_cell, "body": n_body}
     result["sample"] = ("%s %d of %d sink groups (%d sinks, %.3e interactions) of the benchmark "
                         "workload: reference do_grav%s+do_gravp (SSE4 and AVX builds) per sink "
-                        "over the same lists, %d threads, kernel calls timed"
+                        "over the same lists, %d threads%s, kernel calls timed per kind"
                         % ("all" if groups.size == ng else "random", groups.size, ng,
-                           int(count[groups].sum()), inter, cell, nthreads))
+                           int(count[groups].sum()), inter, cell, nthreads,
+                           ", one pinned per physical core" if cpus is not None else ""))
     result["prep_seconds"] = time.perf_counter() - t0
     return result

@@ -40,8 +40,10 @@ def test_harness_results_match_list_oracle(arch, order, kind, cell, csz, mode):
          "bodies": np.ascontiguousarray(np.concatenate([smass[:, None], spos], 1), np.float32),
          "first": first, "count": count, "hoff": hoff, "hidx": hidx, "boff": boff, "bidx": bidx}
     groups = np.arange(0, first.size, 7, dtype=np.int32)
-    wall, ks, out = cpu_baseline._run_lists(arch, D, groups, 1e-6, 4, cell, csz)
-    assert wall > 0 and ks > 0
+    pin = cpu_baseline.host_cpu()["pin_cpus"]
+    wall, ks, out = cpu_baseline._run_lists(arch, D, groups, 1e-6, 4, cell, csz,
+                                            pin if len(pin) >= 4 else None)
+    assert wall > 0 and ks[0] > 0 and ks[1] > 0  # cell and body kernel seconds
     # do_gravpq's quadrupole potential carries the sign pgrav.cu has (the -1
     # convention); do_gravph's is the physical one
     qsign = -1 if order == 2 else 1
@@ -58,3 +60,14 @@ def test_harness_results_match_list_oracle(arch, order, kind, cell, csz, mode):
     for g in groups:
         mask[first[g]:first[g] + count[g]] = False
     assert not out[mask].any()
+
+
+def test_host_cpu_topology():
+    """One pinned CPU per physical core of the affinity set, within the quota."""
+    c = cpu_baseline.host_cpu()
+    assert 1 <= len(c["pin_cpus"]) <= c["affinity_cpus"] <= c["nproc"]
+    assert len(set(c["pin_cpus"])) == len(c["pin_cpus"])
+    assert c["physical_cores"] >= len(c["pin_cpus"])
+    if c["cgroup_cpu_quota"]:
+        assert len(c["pin_cpus"]) <= c["cgroup_cpu_quota"]
+    assert c["cpu_model"]
