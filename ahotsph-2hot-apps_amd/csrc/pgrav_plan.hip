@@ -35,7 +35,8 @@ struct Plan {
     std::vector<int64_t> inv;  // original index -> sorted slot
     hipStream_t stream = nullptr;
     hipStream_t qs[NQ_PLAIN] = {};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t qev[NQ_PLAIN] = {};  // each slot's last launch
+    hipEvent_t ev0 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::vector<void *> allocs;
@@ -90,8 +91,9 @@ struct Plan {
             (void)hipStreamDestroy(stream);
         if (ev0)
             (void)hipEventDestroy(ev0);
-        if (ev1)
-            (void)hipEventDestroy(ev1);
+        for (auto &e : qev)
+            if (e)
+                (void)hipEventDestroy(e);
     }
 };
 
@@ -163,20 +165,26 @@ static int evaluate(Plan *P, int variant, hipStream_t st) {
             fail("the FP64 tables were not built (pgrav_params.fp64 = 0)");
             return -1;
         }
-        // one launch per group, spread over 32 streams as the reference's slot queue
+        // one launch per group over 32 slot streams, as the reference's slot queue
+        // (nln/pgrav.cu:811-851, qallocCUDA): a slot takes its next group only once its
+        // previous launch has completed, so at most NQ_PLAIN launches are in flight, as
+        // there; the host waits on the oldest slot where the reference spins over all 32
         if (!check(hipEventRecord(P->ev0, st), "hipEventRecord"))
             return -1;
         for (int q = 0; q < NQ_PLAIN; q++)
             if (!check(hipStreamWaitEvent(P->qs[q], P->ev0, 0), "hipStreamWaitEvent"))
                 return -1;
-        for (int g = 0; g < P->v.ngroups; g++)
-            if (!check(launch_group64(P->v, g, P->qs[g % NQ_PLAIN]), "group kernel"))
+        for (int g = 0; g < P->v.ngroups; g++) {
+            const int q = g % NQ_PLAIN;
+            if (g >= NQ_PLAIN && !check(hipEventSynchronize(P->qev[q]), "hipEventSynchronize"))
                 return -1;
-        for (int q = 0; q < NQ_PLAIN; q++) {
-            if (!check(hipEventRecord(P->ev1, P->qs[q]), "hipEventRecord") ||
-                !check(hipStreamWaitEvent(st, P->ev1, 0), "hipStreamWaitEvent"))
+            if (!check(launch_group64(P->v, g, P->qs[q]), "group kernel") ||
+                !check(hipEventRecord(P->qev[q], P->qs[q]), "hipEventRecord"))
                 return -1;
         }
+        for (int q = 0; q < NQ_PLAIN && q < P->v.ngroups; q++)
+            if (!check(hipStreamWaitEvent(st, P->qev[q], 0), "hipStreamWaitEvent"))
+                return -1;
         return far_field(P, variant, st);
     }
     if (variant == VAR_F64 && !P->v.mom64) {
@@ -278,11 +286,11 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         for (int64_t i = 0; i < n; i++) P->inv[P->tree.perm[i]] = i;
 
         if (!check(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking), "hipStreamCreate") ||
-            !check(hipEventCreate(&P->ev0), "hipEventCreate") ||
-            !check(hipEventCreate(&P->ev1), "hipEventCreate"))
+            !check(hipEventCreate(&P->ev0), "hipEventCreate"))
             throw std::runtime_error("stream");
         for (int q = 0; q < NQ_PLAIN; q++)
-            if (!check(hipStreamCreateWithFlags(&P->qs[q], hipStreamNonBlocking), "hipStreamCreate"))
+            if (!check(hipStreamCreateWithFlags(&P->qs[q], hipStreamNonBlocking), "hipStreamCreate") ||
+                !check(hipEventCreateWithFlags(&P->qev[q], hipEventDisableTiming), "hipEventCreate"))
                 throw std::runtime_error("stream");
         // bodies
         std::vector<double> b4(4 * (size_t)n);
