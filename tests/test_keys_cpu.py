@@ -2,8 +2,6 @@
 the oracle's restatement, and the oracle pinned by the reference's own table,
 its two Morton routines agreeing, and the Peano-Hilbert curve's defining
 properties."""
-import ctypes
-
 import numpy as np
 import pytest
 
@@ -11,34 +9,9 @@ from oracle import keys as K
 
 
 @pytest.fixture(scope="module")
-def L():
-    import pgrav
-    lib = pgrav.lib()
-    u64p, f32p = ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_float)
-    lib.pgrav_host_fix_rsize.argtypes = [f32p, f32p, ctypes.c_int, f32p, f32p, f32p]
-    lib.pgrav_host_keys.argtypes = [f32p, ctypes.c_int64, f32p, ctypes.c_float, ctypes.c_float,
-                                    ctypes.c_int, u64p]
-    lib.pgrav_host_cell_corner.argtypes = [u64p, ctypes.c_int, f32p, ctypes.c_float, f32p, f32p]
-    return lib
-
-
-def _fp(a, t=ctypes.c_float):
-    return a.ctypes.data_as(ctypes.POINTER(t))
-
-
-def _host_rsize(L, rmin, rmax, exact):
-    o, rs, kf = np.zeros(3, np.float32), np.zeros(1, np.float32), np.zeros(1, np.float32)
-    assert L.pgrav_host_fix_rsize(_fp(np.float32(rmin)), _fp(np.float32(rmax)), exact, _fp(o),
-                                  _fp(rs), _fp(kf)) == 0
-    return o, rs[0], kf[0]
-
-
-def _host_keys(L, pos, rmin, rsize, kf, kind):
-    pos = np.ascontiguousarray(pos, np.float32)
-    out = np.zeros((pos.shape[0], 2), np.uint64)
-    rc = L.pgrav_host_keys(_fp(pos), pos.shape[0], _fp(np.float32(rmin)), rsize, kf, kind,
-                           _fp(out, ctypes.c_uint64))
-    return rc, [int(lo) | (int(hi) << 64) for lo, hi in out]
+def P():
+    from pgrav import keys
+    return keys
 
 
 def test_morton_table_matches_reference_entries():
@@ -51,17 +24,17 @@ def test_morton_table_matches_reference_entries():
 
 
 @pytest.mark.parametrize("exact", [0, 1])
-def test_fix_rsize_matches_oracle(L, exact):
+def test_fix_rsize_matches_oracle(P, exact):
     rng = np.random.default_rng(3)
     for _ in range(20):
         lo = rng.normal(size=3).astype(np.float32)
         hi = (lo + rng.uniform(0.1, 3, 3)).astype(np.float32)
-        o, rs, kf = _host_rsize(L, lo, hi, exact)
+        o, rs, kf = P.fix_rsize(lo, hi, exact)
         Rmin, Rsize, keyfactor = K.fix_rsize(lo, hi, exact=bool(exact))
         assert np.array_equal(o, np.float32(Rmin)) and rs == Rsize and kf == keyfactor
 
 
-def test_keys_match_oracle_and_both_morton_routines_agree(L):
+def test_keys_match_oracle_and_both_morton_routines_agree(P):
     rng = np.random.default_rng(5)
     pos = rng.uniform(-1, 1, (400, 3)).astype(np.float32)
     pos[:4] = [[-1, -1, -1], [1, 1, 1], [-1, 1, -1], [0, 0, 0]]  # corners of the box
@@ -69,8 +42,7 @@ def test_keys_match_oracle_and_both_morton_routines_agree(L):
     Rmin, Rsize, kf = K.fix_rsize(lo, hi)
     rmin = np.float32(Rmin)
     for kind in (0, 1, 2):
-        rc, got = _host_keys(L, pos, rmin, Rsize, kf, kind)
-        assert rc == 0
+        got = P.keys(pos, rmin, Rsize, kf, kind)
         for p, k in zip(pos, got):
             xp = K.int_pos(p, Rmin, Rsize, kf)
             want = {0: K.get_key, 1: K.get_key_fast, 2: K.get_key_ph}[kind](xp)
@@ -80,10 +52,11 @@ def test_keys_match_oracle_and_both_morton_routines_agree(L):
                 assert want >> (3 * K.BITS) == 1  # the leading bit
 
 
-def test_keys_outside_cube_are_an_error(L):
+def test_keys_outside_cube_are_an_error(P):
+    from pgrav import PgravError
     Rmin, Rsize, kf = K.fix_rsize([0, 0, 0], [1, 1, 1])
-    rc, _ = _host_keys(L, np.array([[0.5, 0.5, 2.0]]), np.float32(Rmin), Rsize, kf, 0)
-    assert rc != 0
+    with pytest.raises(PgravError):
+        P.keys(np.array([[0.5, 0.5, 2.0]]), np.float32(Rmin), Rsize, kf, 0)
 
 
 @pytest.mark.parametrize("depth", [1, 2, 3, 4])
@@ -117,23 +90,19 @@ def test_ph_key_prefix_is_the_parent_cell():
 
 
 @pytest.mark.parametrize("ph", [0, 1])
-def test_cell_corner_holds_the_body(L, ph):
+def test_cell_corner_holds_the_body(P, ph):
     rng = np.random.default_rng(11)
     pos = rng.uniform(0, 1, (100, 3)).astype(np.float32)
     Rmin, Rsize, kf = K.fix_rsize([0, 0, 0], [1, 1, 1])
     rmin = np.float32(Rmin)
-    _, keys = _host_keys(L, pos, rmin, Rsize, kf, 2 if ph else 0)
+    keys = P.keys(pos, rmin, Rsize, kf, 2 if ph else 0)
     for p, k in zip(pos, keys):
         for d in (1, 4, 9):
-            kd = np.array([(k >> (3 * (K.BITS - d))) & (2 ** 64 - 1),
-                           (k >> (3 * (K.BITS - d))) >> 64], np.uint64)
-            c, sz = np.zeros(3, np.float32), np.zeros(1, np.float32)
-            assert L.pgrav_host_cell_corner(_fp(kd, ctypes.c_uint64), ph, _fp(rmin), Rsize,
-                                            _fp(c), _fp(sz)) == 0
+            c, sz = P.cell_corner(k >> (3 * (K.BITS - d)), rmin, Rsize, ph)
             edge = Rsize / (1 << d)
             assert np.all(c <= p + 1e-6) and np.all(p < c + edge + 1e-6), (p, c, edge)
             if not ph:
                 want, fac = K.cell_corner(int(k >> (3 * (K.BITS - d))), Rmin, Rsize)
-                assert np.array_equal(c, np.float32(want)) and sz[0] == fac
+                assert np.array_equal(c, np.float32(want)) and sz == fac
             else:
-                assert sz[0] == np.float32(Rsize / np.float32(2.0 ** K.BITS))
+                assert sz == np.float32(Rsize / np.float32(2.0 ** K.BITS))
