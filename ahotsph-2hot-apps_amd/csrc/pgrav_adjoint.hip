@@ -4,29 +4,32 @@
 // lists fixed, a_i and phi_i depend on the bodies through (1) the sink position
 // x_i, (2) each body-list source position x_j and mass m_j, and (3) each
 // accepted cell's centre of mass X_c and raw moments M_c, which are functions
-// of the cell's bodies.  Three kernels:
+// of the cell's bodies.
 //
-//  k_adjoint        one wave per sink group, the forward's tiling (lane = sink).
-//                   Per pair: the sink-position gradient by one forward-mode
-//                   derivative (along the sink's weight w) through the very same
-//                   cell_interact/mono_interact code; for cells also the pair's
-//                   share of dL/dX_c, summed over the wave's sinks and added
-//                   atomically.
-//  k_adjoint_bodysrc one wave per sink group, lane = body of a body-list tile,
-//                   looping over the group's sinks: dL/dx_j and dL/dm_j of the
-//                   source bodies, lane-private sums, one atomic add per term
-//                   and tile.
-//  k_adjoint_cells  one wave per listed cell (per chunk of 128 listing groups)
-//                   over the transposed lists: the 43 prepared-record
-//                   coefficients' gradient dL/dP (the interaction is linear in
-//                   P, so this is the hand-written transpose of the
-//                   evaluation), summed in registers and once across the wave.
-//  k_adjoint_nodes  per cell: dL/dP -> dL/d(raw moments) through the fixed
-//                   linear map raw -> record -> prepared record, and the total
-//                   dL/dX_c including the centre dependence of the moments.
-//  k_adjoint_bodies per body: walk up its ancestors, chaining dL/dM_c and dL/dX_c
-//                   to dL/dx_j and dL/dm_j (M_c[a] = sum_j m_j (x_j - X_c)^a,
-//                   X_c = sum_j m_j x_j / sum_j m_j).
+// Every sum has one writer and a fixed order -- no atomics -- so two calls give
+// bit-identical gradients:
+//
+//  k_adjoint         one wave per sink group (lane = sink), the forward's tiling
+//                    and XCD-aware group order.  Per pair one forward-mode
+//                    derivative along the sink's weight w through the very same
+//                    cell_interact / mono_interact code gives the sink's
+//                    gradient; the pair's source-side share (minus that, summed
+//                    over the group's sinks by a DPP wave reduction) is written
+//                    once per list entry: gpair (cells: dL/dX_c) and gentry
+//                    (bodies: dL/dx_j and dL/dm_j).
+//  k_adjoint_cells   one wave per listed cell and chunk of <= 128 listing groups
+//                    (the transposed lists): the 43 prepared-record
+//                    coefficients' gradient dL/dP (the interaction is linear in
+//                    P: the hand-written transpose of the evaluation) plus the
+//                    chunk's gpair entries, one output record per chunk.
+//  k_adjoint_bodygather  per body: its gentry entries in list order.
+//  k_adjoint_nodes   per cell: its chunks in order; dL/dP -> dL/d(raw moments)
+//                    through the fixed linear map raw -> record -> prepared
+//                    record, and the total dL/dX_c including the centre
+//                    dependence of the moments.
+//  k_adjoint_bodies  per body: walk up its ancestors, chaining dL/dM_c and dL/dX_c
+//                    to dL/dx_j and dL/dm_j (M_c[a] = sum_j m_j (x_j - X_c)^a,
+//                    X_c = sum_j m_j x_j / sum_j m_j).
 #include "pgrav_batch.h"
 #include "pgrav_device.h"
 
@@ -62,13 +65,28 @@ __device__ __forceinline__ D1 nrsq(D1 s) {
     return D1(f, -0.5f * f * f * f * s.d);
 }
 
+// ---------------------------------------------------------------- wave sums
+// The sum of a value over the 64 lanes of a wave by DPP row operations (VALU
+// only, no LDS): pairs and quads by quad permutes, 8 and 16 lanes by the half-
+// row and row mirrors, then rows 0+1 and 2+3 by the row-15 broadcast and all
+// four by the row-31 broadcast, which leaves the total in lane 63.  The lane
+// order of the additions is fixed, so the result is reproducible.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_add(float v) {
+    const int t = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWS, 0xf, false);
+    return v + __builtin_bit_cast(float, t);
+}
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
+    v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2, 3, 0, 1]
+    v = dpp_add<0x141, 0xf>(v);  // row_half_mirror
+    v = dpp_add<0x140, 0xf>(v);  // row_mirror
+    v = dpp_add<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_add<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
 }
 
-#define ADJ_CELL 52  // per node: 46 dL/dP, 3 dL/dX (pair part), pad
+#define ADJ_CELL 52  // per work item / node: 46 dL/dP, 3 dL/dX (pair part), pad
 #define L_BODY_KIND 3  // list kinds 0..2 are cells (orders 4, 2, 0), 3 bodies
 
 // dL/dP for one (sink, cell) pair: the evaluation in cell_interact is linear in
@@ -129,12 +147,12 @@ __device__ __forceinline__ int64_t sink_slot(const BatchView &V, int64_t ref, in
 }
 
 template <int SOFT>
-__global__ __launch_bounds__(64) void k_adjoint(BatchView V, const float4 *__restrict__ wv,
-                                                float4 *__restrict__ gsink,
-                                                float *__restrict__ gcell,
-                                                float4 *__restrict__ gbody) {
+__global__ __launch_bounds__(64) void k_adjoint(BatchView V, AdjointView A) {
     __shared__ float4 lds[64];
-    const int g = blockIdx.x;
+    // XCD-aware order: consecutive workgroups land on different XCDs, so the
+    // forward's per-XCD slices of Morton-neighbouring groups keep their cells in
+    // one XCD's L2 here too
+    const int g = V.order_np ? V.order_np[blockIdx.x] : (int)blockIdx.x;
     const int lane = threadIdx.x;
     const int cnt = V.gcount[g];
     const double4 gc = V.gcen[g];
@@ -142,7 +160,7 @@ __global__ __launch_bounds__(64) void k_adjoint(BatchView V, const float4 *__res
     const int64_t me = sink_slot(V, V.gslot ? g : V.gfirst[g], live ? lane : 0);
     const double4 sp = V.bpos[me];
     const float sx = (float)(sp.x - gc.x), sy = (float)(sp.y - gc.y), sz = (float)(sp.z - gc.z);
-    const float4 w = live ? wv[me] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 w = live ? A.wv[me] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float eps = (float)V.eps, eps2 = eps * eps;
     const float eps_inv = SOFT == SOFT_K1 ? 1.0f / eps : 0.0f;
     const float eps2c = SOFT == SOFT_PLUMMER ? eps2 : 0.0f;
@@ -162,142 +180,91 @@ __global__ __launch_bounds__(64) void k_adjoint(BatchView V, const float4 *__res
             }
             __syncthreads();
             for (int k = 0; k < nt; k++) {
-                const uint32_t e = idx[t0 + k];
                 const float4 r = lds[k];
                 const float X = sx - r.x, Y = sy - r.y, Z = sz - r.z;
                 // the sink offset, differentiated along this sink's weight w
                 const D1 dx(X, w.x), dy(Y, w.y), dz(Z, w.z);
                 Acc4T<D1> a = {D1(0.f), D1(0.f), D1(0.f), D1(0.f)};
-                if (kind == 0)
-                    cell_interact<4, false, 1, D1, float>(V.mom32 + (size_t)e * PREC4, D1(r.w), dx, dy, dz, D1(eps2c), a);
-                else if (kind == 1)
-                    cell_interact<2, false, 1, D1, float>(V.mom32 + (size_t)e * PREC4, D1(r.w), dx, dy, dz, D1(eps2c), a);
-                else if (kind == 2)
-                    mono_interact<D1>(D1(r.w), dx, dy, dz, D1(eps2c), a);
-                else {
+                if (kind < L_BODY_KIND) {
+                    const uint32_t e = idx[t0 + k];
+                    if (kind == 0)
+                        cell_interact<4, false, 1, D1, float>(V.mom32 + (size_t)e * PREC4, D1(r.w), dx, dy, dz, D1(eps2c), a);
+                    else if (kind == 1)
+                        cell_interact<2, false, 1, D1, float>(V.mom32 + (size_t)e * PREC4, D1(r.w), dx, dy, dz, D1(eps2c), a);
+                    else
+                        mono_interact<D1>(D1(r.w), dx, dy, dz, D1(eps2c), a);
+                    // d(w.a + v phi)/d(sink position) = J w - v a; the cell's centre
+                    // takes minus that, summed over the group's sinks
+                    const float px = a.x.d - w.w * a.x.v;
+                    const float py = a.y.d - w.w * a.y.v;
+                    const float pz = a.z.d - w.w * a.z.v;
+                    gx += px;
+                    gy += py;
+                    gz += pz;
+                    const float cx = wave_sum(-px), cy = wave_sum(-py), cz = wave_sum(-pz);
+                    if (lane == 0)
+                        A.gpair[t0 + k] = make_float4(cx, cy, cz, 0.f);
+                } else {
+                    // bodies at unit mass: the pair is linear in the source mass
                     const bool self = X == 0.f && Y == 0.f && Z == 0.f;
                     if (SOFT == SOFT_K1)
-                        k1_interact<D1>(D1(r.w), dx, dy, dz, D1(eps_inv), D1(eps2), a);
+                        k1_interact<D1>(D1(1.f), dx, dy, dz, D1(eps_inv), D1(eps2), a);
                     else if (SOFT == SOFT_PLUMMER)
-                        mono_interact<D1>(D1(r.w), dx, dy, dz, D1(eps2), a);
+                        mono_interact<D1>(D1(1.f), dx, dy, dz, D1(eps2), a);
                     else if (!self)
-                        mono_interact<D1>(D1(r.w), dx, dy, dz, D1(0.f), a);
-                }
-                // d(w.a + v phi)/d(sink position) = J w - v a
-                const float px = a.x.d - w.w * a.x.v;
-                const float py = a.y.d - w.w * a.y.v;
-                const float pz = a.z.d - w.w * a.z.v;
-                gx += px;
-                gy += py;
-                gz += pz;
-                if (kind < 3) {
-                    // the pair's share of dL/dX_c is minus the sink's; summed over
-                    // the wave's sinks (the record gradient dL/dP is accumulated
-                    // per cell by k_adjoint_cells over the transposed lists)
-                    const float sx_ = wave_sum(-px), sy_ = wave_sum(-py), sz_ = wave_sum(-pz);
-                    if (lane == 0) {
-                        float *gcp = gcell + (size_t)e * ADJ_CELL;
-                        atomicAdd(gcp + 46, sx_);
-                        atomicAdd(gcp + 47, sy_);
-                        atomicAdd(gcp + 48, sz_);
-                    }
+                        mono_interact<D1>(D1(1.f), dx, dy, dz, D1(0.f), a);
+                    const float px = r.w * (a.x.d - w.w * a.x.v);
+                    const float py = r.w * (a.y.d - w.w * a.y.v);
+                    const float pz = r.w * (a.z.d - w.w * a.z.v);
+                    const float pm = w.x * a.x.v + w.y * a.y.v + w.z * a.z.v + w.w * a.w.v;
+                    gx += px;
+                    gy += py;
+                    gz += pz;
+                    const float bx = wave_sum(-px), by = wave_sum(-py), bz = wave_sum(-pz),
+                                bm = wave_sum(pm);
+                    if (lane == 0)
+                        A.gentry[t0 + k - A.ncell_entries] = make_float4(bx, by, bz, bm);
                 }
             }
             __syncthreads();
         }
     }
     if (live)
-        gsink[me] = make_float4(gx, gy, gz, 0.f);
-}
-
-// Body sources, source-major: one wave per sink group, lane j = the j-th body
-// of a body-list tile, looping over the group's sinks (broadcast from LDS).
-// Each lane sums its body's pair position gradient (minus the sink-side one,
-// by the same dual numbers) and dL/dm_j over the group's sinks in registers
-// and adds them once per tile.
-template <int SOFT>
-__global__ __launch_bounds__(64) void k_adjoint_bodysrc(BatchView V, const float4 *__restrict__ wv,
-                                                        float4 *__restrict__ gbody) {
-    __shared__ float4 sk[64], sw[64];
-    const int g = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int cnt = V.gcount[g];
-    const double4 gc = V.gcen[g];
-    if (lane < cnt) {
-        const int64_t me = sink_slot(V, V.gslot ? g : V.gfirst[g], lane);
-        const double4 sp = V.bpos[me];
-        sk[lane] = make_float4((float)(sp.x - gc.x), (float)(sp.y - gc.y), (float)(sp.z - gc.z), 0.f);
-        sw[lane] = wv[me];
-    }
-    __syncthreads();
-    const float eps = (float)V.eps, eps2 = eps * eps;
-    const float eps_inv = SOFT == SOFT_K1 ? 1.0f / eps : 0.0f;
-    const int64_t *offk = V.off + (int64_t)L_BODY_KIND * (V.ngroups + 1);
-    const int64_t o0 = offk[g], o1 = offk[g + 1];
-    for (int64_t t0 = o0 + lane; t0 < o1; t0 += 64) {
-        const uint32_t e = V.idx[t0];
-        const double4 s = V.bpos[e];
-        const float rx = (float)(s.x - gc.x), ry = (float)(s.y - gc.y), rz = (float)(s.z - gc.z);
-        const float rm = (float)s.w;
-        float bx = 0.f, by = 0.f, bz = 0.f, bm = 0.f;
-        for (int i = 0; i < cnt; i++) {
-            const float4 x = sk[i], w = sw[i];
-            const float X = x.x - rx, Y = x.y - ry, Z = x.z - rz;
-            // the pair offset differentiated along sink i's weight; the source's
-            // gradient is minus the sink's, J w - v a
-            const D1 dx(X, w.x), dy(Y, w.y), dz(Z, w.z);
-            Acc4T<D1> a = {D1(0.f), D1(0.f), D1(0.f), D1(0.f)};
-            const bool self = X == 0.f && Y == 0.f && Z == 0.f;
-            if (SOFT == SOFT_K1)
-                k1_interact<D1>(D1(rm), dx, dy, dz, D1(eps_inv), D1(eps2), a);
-            else if (SOFT == SOFT_PLUMMER)
-                mono_interact<D1>(D1(rm), dx, dy, dz, D1(eps2), a);
-            else if (!self)
-                mono_interact<D1>(D1(rm), dx, dy, dz, D1(0.f), a);
-            bx -= a.x.d - w.w * a.x.v;
-            by -= a.y.d - w.w * a.y.v;
-            bz -= a.z.d - w.w * a.z.v;
-            // the pair is linear in the source mass
-            if (rm != 0.f)
-                bm += (w.x * a.x.v + w.y * a.y.v + w.z * a.z.v + w.w * a.w.v) / rm;
-        }
-        float *gb = (float *)(gbody + e);
-        atomicAdd(gb + 0, bx);
-        atomicAdd(gb + 1, by);
-        atomicAdd(gb + 2, bz);
-        atomicAdd(gb + 3, bm);
-    }
+        A.gsink[me] = make_float4(gx, gy, gz, 0.f);
 }
 
 // ---- transposed cell lists (built by the host): for every cell, the groups
 // that list it, entry = (first sink -- the group's index for body-set groups --,
-// sink count | list kind << 8), so that a
+// sink count | list kind << 8, the entry's position in the lists), so that a
 // cell's record gradient is summed by one wave alone.
 
-// One wave per listed cell: lanes take a listing group's sinks (coalesced), the
-// wave walks the cell's groups; every lane keeps its partial dL/dP in registers
-// across groups and the wave sums them once, at the end.  No atomics: the cell's
-// record gradient has one writer per chunk.  The cell's list entries arrive 64 at a time
-// in one vector load and are read back with v_readlane; the next group's sinks
-// load while the current group is evaluated.
+// One wave per (cell, chunk of <= 128 listing groups): lanes take a listing
+// group's sinks (coalesced), the wave walks the chunk's groups; every lane keeps
+// its partial dL/dP in registers across groups and the wave sums them once, at
+// the end, into the chunk's own output record.  The chunk's list entries arrive
+// 64 at a time in one vector load and are read back with v_readlane; the next
+// group's sinks load while the current group is evaluated.
 template <int SOFT>
-__global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, const float4 *__restrict__ wv,
-                                                      const int *__restrict__ tcells,
-                                                      const longlong2 *__restrict__ trange,
-                                                      const uint2 *__restrict__ tlist,
-                                                      float *__restrict__ gcell) {
-    // work item: a cell and one chunk [t0, t1) of its transposed list
-    const int c = tcells[blockIdx.x];
-    const int64_t t0 = trange[blockIdx.x].x, t1 = trange[blockIdx.x].y;
+__global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, AdjointView A) {
+    const int item = blockIdx.x;
+    const int c = A.tcells[item];
+    const int64_t t0 = A.trange[item].x, t1 = A.trange[item].y;
     const int lane = threadIdx.x;
     const double4 X = V.cpos[c];
     const float eps2c = SOFT == SOFT_PLUMMER ? (float)(V.eps * V.eps) : 0.0f;
     float acc[46];
 #pragma unroll
     for (int q = 0; q < 46; q++) acc[q] = 0.f;
+    float px = 0.f, py = 0.f, pz = 0.f;  // the chunk's gpair entries, one per lane
     for (int64_t tb = t0; tb < t1; tb += 64) {
         const int nt = (int)min((int64_t)64, t1 - tb);
-        const uint2 mine = lane < nt ? tlist[tb + lane] : make_uint2(0u, 0u);
+        const uint4 mine = lane < nt ? A.tlist[tb + lane] : make_uint4(0u, 0u, 0u, 0u);
+        if (lane < nt) {
+            const float4 gp = A.gpair[(int64_t)mine.z | ((int64_t)mine.w << 32)];
+            px += gp.x;
+            py += gp.y;
+            pz += gp.z;
+        }
         // entry k: first sink, count and kind, wave-uniform
         auto entry = [&](int k, uint32_t &first, int &cnt, int &kind) {
             first = (uint32_t)__builtin_amdgcn_readlane((int)mine.x, k);
@@ -313,7 +280,7 @@ __global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, const float4 
         if (lane < cnt) {
             const int64_t me = sink_slot(V, f, lane);
             sp = V.bpos[me];
-            w = wv[me];
+            w = A.wv[me];
         }
         for (int k = 0; k < nt; k++) {
             uint32_t fn = 0;
@@ -325,7 +292,7 @@ __global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, const float4 
                 if (lane < cn) {
                     const int64_t me = sink_slot(V, fn, lane);
                     spn = V.bpos[me];
-                    wn = wv[me];
+                    wn = A.wv[me];
                 }
             }
             if (lane < cnt) {
@@ -346,56 +313,67 @@ __global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, const float4 
             w = wn;
         }
     }
-    // a cell listed by many groups is split over several waves: each adds its sums
-    float *gcp = gcell + (size_t)c * ADJ_CELL;
+    float *out = A.gchunk + (size_t)item * ADJ_CELL;
 #pragma unroll
     for (int q = 3; q < 46; q++) {
         const float s = wave_sum(acc[q]);
         if (lane == 0)
-            atomicAdd(gcp + q, s);
+            out[q] = s;
+    }
+    const float sx = wave_sum(px), sy = wave_sum(py), sz = wave_sum(pz);
+    if (lane == 0) {
+        out[46] = sx;
+        out[47] = sy;
+        out[48] = sz;
     }
 }
 
-// per node: dL/d(raw moments) = Mat^T dL/dP and the total dL/dX_c.  Mat (46 x 35)
-// is the fixed linear map raw moments -> prepared record (R = 1).
-__global__ void k_adjoint_nodes(const float *__restrict__ gcell, const double *__restrict__ raw,
-                                const float *__restrict__ mat, int64_t nnodes,
-                                float *__restrict__ gnode /* 40 per node */) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nnodes)
+// per body: its body-list entries' source gradients, in list order
+__global__ void k_adjoint_bodygather(AdjointView A) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.n)
         return;
-    const float *gp = gcell + c * ADJ_CELL;
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int64_t e = A.boff[j]; e < A.boff[j + 1]; e++) {
+        const float4 g = A.gentry[A.blist[e]];
+        s.x += g.x;
+        s.y += g.y;
+        s.z += g.z;
+        s.w += g.w;
+    }
+    A.gbody[j] = s;
+}
+
+// per node: its chunks' sums in order, dL/d(raw moments) = Mat^T dL/dP and the
+// total dL/dX_c.  Mat (46 x 35) is the fixed linear map raw moments -> prepared
+// record (R = 1); A.rix holds each raw moment's exponents and the index of the
+// moment one order lower per axis.
+__global__ void k_adjoint_nodes(AdjointView A) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= A.nnodes)
+        return;
+    float gp[ADJ_CELL];
+    for (int q = 3; q < 49; q++) gp[q] = 0.f;
+    for (int64_t it = A.cchunk_off[c]; it < A.cchunk_off[c + 1]; it++) {
+        const float *p = A.gchunk + (size_t)it * ADJ_CELL;
+        for (int q = 3; q < 49; q++) gp[q] += p[q];
+    }
     float gr[35];
     for (int a = 0; a < 35; a++) {
         float s = 0.f;
-        for (int k = 3; k < 46; k++) s += mat[k * 35 + a] * gp[k];
+        for (int k = 3; k < 46; k++) s += A.mat[k * 35 + a] * gp[k];
         gr[a] = s;
     }
-    // exponents of the 35 raw moments (nln/cofm.c:660-720 order)
+    // dM_a/dX_q = -e_q M_{a - e_q}
     float gX[3] = {gp[46], gp[47], gp[48]};
-    int a = 0;
-    for (int o = 0; o <= 4; o++)
-        for (int ex = o; ex >= 0; ex--)
-            for (int ey = o - ex; ey >= 0; ey--, a++) {
-                const int ez = o - ex - ey;
-                // dM_a/dX = -(ex M_{a-x}, ey M_{a-y}, ez M_{a-z}); index of a lower
-                // moment found by scanning (tiny table)
-                const int e3[3] = {ex, ey, ez};
-                for (int q = 0; q < 3; q++) {
-                    if (!e3[q])
-                        continue;
-                    int f[3] = {ex, ey, ez};
-                    f[q]--;
-                    const int ol = o - 1;
-                    int b = 0;
-                    for (int oo = 0; oo < ol; oo++) b += (oo + 1) * (oo + 2) / 2;
-                    for (int fx = ol; fx >= 0; fx--)
-                        for (int fy = ol - fx; fy >= 0; fy--, b++)
-                            if (fx == f[0] && fy == f[1])
-                                gX[q] -= gr[a] * (float)e3[q] * (float)raw[c * 35 + b];
-                }
-            }
-    float *o = gnode + c * 40;
+    const double *raw = A.raw + (size_t)c * 35;
+    for (int a = 1; a < 35; a++)
+        for (int q = 0; q < 3; q++) {
+            const int e = A.rix.e[a][q];
+            if (e)
+                gX[q] -= gr[a] * (float)e * (float)raw[A.rix.lower[a][q]];
+        }
+    float *o = A.gnode + c * 40;
     for (int k = 0; k < 35; k++) o[k] = gr[k];
     o[35] = gX[0];
     o[36] = gX[1];
@@ -403,23 +381,19 @@ __global__ void k_adjoint_nodes(const float *__restrict__ gcell, const double *_
 }
 
 // per body: chain through every ancestor's moments and centre of mass
-__global__ void k_adjoint_bodies(const double4 *__restrict__ bpos, const double4 *__restrict__ cpos,
-                                 const int *__restrict__ leaf_of, const int *__restrict__ parent,
-                                 const float *__restrict__ gnode, const float4 *__restrict__ gsink,
-                                 const float4 *__restrict__ gbody, int64_t n, double G,
-                                 double4 *__restrict__ out /* dL/dx, dL/dm, sorted order */) {
+__global__ void k_adjoint_bodies(BatchView V, AdjointView A) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n)
+    if (j >= A.n)
         return;
-    const double4 p = bpos[j];
-    const float4 s = gsink[j], b = gbody[j];
+    const double4 p = V.bpos[j];
+    const float4 s = A.gsink[j], b = A.gbody[j];
     float g[3] = {s.x + b.x, s.y + b.y, s.z + b.z};
     float gm = b.w;
     const float mj = (float)p.w;
-    for (int c = leaf_of[j]; c >= 0; c = parent[c]) {
-        const double4 C = cpos[c];
+    for (int c = A.leaf_of[j]; c >= 0; c = A.parent[c]) {
+        const double4 C = V.cpos[c];
         const float d[3] = {(float)(p.x - C.x), (float)(p.y - C.y), (float)(p.z - C.z)};
-        const float *gn = gnode + (size_t)c * 40;
+        const float *gn = A.gnode + (size_t)c * 40;
         float pw[3][5];
         for (int q = 0; q < 3; q++) {
             pw[q][0] = 1.f;
@@ -442,22 +416,42 @@ __global__ void k_adjoint_bodies(const double4 *__restrict__ bpos, const double4
             gm += (gn[35] * d[0] + gn[36] * d[1] + gn[37] * d[2]) / Mc;
         }
     }
-    out[j] = make_double4(G * g[0], G * g[1], G * g[2], G * gm);
+    const double G = V.G;
+    A.out[j] = make_double4(G * g[0], G * g[1], G * g[2], G * gm);
 }
 
-hipError_t launch_adjoint(const BatchView &v, const float4 *wv, float4 *gsink, float *gcell,
-                          float4 *gbody, const double *raw, const float *mat, int64_t nnodes,
-                          float *gnode, const int *leaf_of, const int *parent, int64_t n,
-                          const int *tcells, int ntcells, const longlong2 *trange,
-                          const uint2 *tlist, double4 *out, hipStream_t st) {
+RawIndex raw_index() {
+    RawIndex r{};
+    int e[35][3], a = 0;
+    for (int o = 0; o <= 4; o++)
+        for (int ex = o; ex >= 0; ex--)
+            for (int ey = o - ex; ey >= 0; ey--, a++) {
+                e[a][0] = ex;
+                e[a][1] = ey;
+                e[a][2] = o - ex - ey;
+            }
+    for (int b = 0; b < 35; b++)
+        for (int q = 0; q < 3; q++) {
+            r.e[b][q] = (signed char)e[b][q];
+            r.lower[b][q] = 0;
+            if (!e[b][q])
+                continue;
+            int f[3] = {e[b][0], e[b][1], e[b][2]};
+            f[q]--;
+            for (int l = 0; l < 35; l++)
+                if (e[l][0] == f[0] && e[l][1] == f[1] && e[l][2] == f[2])
+                    r.lower[b][q] = (signed char)l;
+        }
+    return r;
+}
+
+hipError_t launch_adjoint(const BatchView &v, const AdjointView &A, hipStream_t st) {
     if (v.ngroups <= 0)
         return hipSuccess;
 #define ADJ(S)                                                                                     \
-    hipLaunchKernelGGL(k_adjoint<S>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gsink, gcell, gbody); \
-    hipLaunchKernelGGL(k_adjoint_bodysrc<S>, dim3(v.ngroups), dim3(64), 0, st, v, wv, gbody);      \
-    if (ntcells > 0)                                                                               \
-        hipLaunchKernelGGL(k_adjoint_cells<S>, dim3(ntcells), dim3(64), 0, st, v, wv, tcells, trange, \
-                           tlist, gcell);
+    hipLaunchKernelGGL(k_adjoint<S>, dim3(v.ngroups), dim3(64), 0, st, v, A);                      \
+    if (A.ntcells > 0)                                                                             \
+        hipLaunchKernelGGL(k_adjoint_cells<S>, dim3(A.ntcells), dim3(64), 0, st, v, A);
     if (v.soft == SOFT_K1) {
         ADJ(SOFT_K1)
     } else if (v.soft == SOFT_PLUMMER) {
@@ -466,10 +460,9 @@ hipError_t launch_adjoint(const BatchView &v, const float4 *wv, float4 *gsink, f
         ADJ(SOFT_NONE)
     }
 #undef ADJ
-    hipLaunchKernelGGL(k_adjoint_nodes, dim3((unsigned)((nnodes + 127) / 128)), dim3(128), 0, st,
-                       gcell, raw, mat, nnodes, gnode);
-    hipLaunchKernelGGL(k_adjoint_bodies, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st,
-                       v.bpos, v.cpos, leaf_of, parent, gnode, gsink, gbody, n, v.G, out);
+    hipLaunchKernelGGL(k_adjoint_bodygather, dim3((unsigned)((A.n + 127) / 128)), dim3(128), 0, st, A);
+    hipLaunchKernelGGL(k_adjoint_nodes, dim3((unsigned)((A.nnodes + 127) / 128)), dim3(128), 0, st, A);
+    hipLaunchKernelGGL(k_adjoint_bodies, dim3((unsigned)((A.n + 127) / 128)), dim3(128), 0, st, v, A);
     return hipGetLastError();
 }
 

@@ -56,11 +56,52 @@ hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, 
 hipError_t launch_group64(const BatchView &v, int g, hipStream_t st);
 hipError_t launch_reset_counter(int *c, hipStream_t st);
 hipError_t launch_ewald_table(double L, double4 *t64, float4 *t32, hipStream_t st);
-hipError_t launch_adjoint(const BatchView &v, const float4 *wv, float4 *gsink, float *gcell,
-                          float4 *gbody, const double *raw, const float *mat, int64_t nnodes,
-                          float *gnode, const int *leaf_of, const int *parent, int64_t n,
-                          const int *tcells, int ntcells, const longlong2 *trange,
-                          const uint2 *tlist, double4 *out, hipStream_t st);
+// raw moment a (nln/cofm.c:660-720 order): its exponents e[a][q] and the index
+// of the moment with e[a][q] - 1 (when e[a][q] > 0)
+struct RawIndex {
+    signed char e[35][3];
+    signed char lower[35][3];
+};
+RawIndex raw_index();
+
+// The adjoint's buffers (device; sorted body order).  Every sum has one writer.
+struct AdjointView {
+    const float4 *wv = nullptr;  // weights (w, v) per body
+    float4 *gsink = nullptr;     // sink-side position gradient per body
+    float4 *gpair = nullptr;     // per cell-list entry: the entry's dL/dX_c share
+    float4 *gentry = nullptr;    // per body-list entry: dL/d(source position, mass)
+    int64_t ncell_entries = 0;   // cell entries precede the body entries in idx
+    // transposed cell lists: work item i is node tcells[i] and entries
+    // trange[i] of tlist = (group ref, count | kind << 8, entry position lo, hi)
+    const int *tcells = nullptr;
+    int ntcells = 0;
+    const longlong2 *trange = nullptr;
+    const uint4 *tlist = nullptr;
+    float *gchunk = nullptr;              // 52 floats per work item
+    const int64_t *cchunk_off = nullptr;  // node c: work items [cchunk_off[c], cchunk_off[c+1])
+    // transposed body lists: body j's entries are blist[boff[j] .. boff[j+1]]
+    // (positions relative to ncell_entries, ascending)
+    const int64_t *boff = nullptr;
+    const uint32_t *blist = nullptr;
+    float4 *gbody = nullptr;
+    const double *raw = nullptr;  // 35 raw moments per node
+    const float *mat = nullptr;   // 46 x 35: raw moments -> prepared record
+    RawIndex rix;
+    int64_t nnodes = 0;
+    float *gnode = nullptr;  // 40 per node: dL/d(raw), dL/dX
+    const int *leaf_of = nullptr, *parent = nullptr;
+    int64_t n = 0;
+    double4 *out = nullptr;  // dL/dx, dL/dm (times G)
+};
+hipError_t launch_adjoint(const BatchView &v, const AdjointView &A, hipStream_t st);
+// caller order <-> sorted slots (slot i holds caller body perm[i]), pgrav_bodies.hip
+hipError_t launch_gather_bodies(const double *pos3, const double *mass, const int64_t *perm,
+                                int64_t n, int periodic, double box, const double4 *prev,
+                                double4 *out, unsigned long long *bad, hipStream_t st);
+hipError_t launch_scatter4(const void *src, int src_fp64, const int64_t *perm, int64_t n, void *dst,
+                           int dst_fp64, hipStream_t st);
+hipError_t launch_gather4(const void *src, int src_fp64, const int64_t *perm, int64_t n,
+                          float4 *dst, hipStream_t st);
 // ewald64 non-null: periodic (nearest image + Ewald correction, box edge `box`)
 hipError_t launch_direct64(const double4 *bpos, int64_t n, const int64_t *sample, int ns, int soft,
                            double eps, double G, double4 *part, int nsplit, double4 *out,

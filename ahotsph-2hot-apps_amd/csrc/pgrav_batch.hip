@@ -53,6 +53,7 @@ struct BatchArgs {
     void *out;  // float4* or double4*, sorted body order
     int *counter;
     const float4 *ewald;  // 64^3 correction table (periodic)
+    const double4 *ewald64;  // its FP64 original, read by the FP64 kernels
     const uint8_t *img;  // explicit images: the image of each entry, parallel to idx (or null)
     int img_f;           // 2 nimage + 1 images per side
     int unroll;          // order-0 / body loop unroll of the rel32c kernels: 2, 4 or 8
@@ -81,6 +82,16 @@ __device__ __forceinline__ T pbc(T x, T box, T ibox) {
 // interpolation over 8 nodes would have.  x = sink - source, already the
 // nearest image (|x_i| <= L/2), so the Newtonian term the caller added and
 // this correction refer to the same image.
+// The potential also takes the trace part of the second-order step: psi's
+// Hessian is C's Jacobian, whose trace is -4 pi / L^3 everywhere off the lattice
+// points, so (1/2) d.H.d = -(2 pi / 3) |d|^2 / L^3 plus a traceless rest two
+// orders smaller; the forces' second-order term has no trace part (div grad C =
+// grad div C = 0).
+template <typename T>
+__device__ __forceinline__ T psi_trace_coef(T box) {
+    return T(-2.0 * M_PI / 3.0) / (box * box * box);
+}
+
 template <typename T, typename TV>
 __device__ __forceinline__ void ewald_corr(const TV *__restrict__ tab, T m, T x, T y, T z,
                                            T box, Acc4T<T> &a) {
@@ -98,7 +109,8 @@ __device__ __forceinline__ void ewald_corr(const TV *__restrict__ tab, T m, T x,
     const T cx = T(t0.x) + du * T(gu.x) + dv * T(gv.x) + dw * T(gw.x);
     const T cy = T(t0.y) + du * T(gu.y) + dv * T(gv.y) + dw * T(gw.y);
     const T cz = T(t0.z) + du * T(gu.z) + dv * T(gv.z) + dw * T(gw.z);
-    const T cp = T(t0.w) + du * T(gu.w) + dv * T(gv.w) + dw * T(gw.w);
+    const T cp = T(t0.w) + du * T(gu.w) + dv * T(gv.w) + dw * T(gw.w) +
+                 psi_trace_coef(box) * (du * du + dv * dv + dw * dw);
     // C is odd per axis; the pull on the sink is m C(r) at r = source - sink = -x
     a.x -= m * sx * cx;
     a.y -= m * sy * cy;
@@ -132,7 +144,8 @@ __device__ __forceinline__ void ewald_taylor(const TV *__restrict__ tab, T x, T 
     c.x = sx * (T(t0.x) + du * T(gu.x) + dv * T(gv.x) + dw * T(gw.x));
     c.y = sy * (T(t0.y) + du * T(gu.y) + dv * T(gv.y) + dw * T(gw.y));
     c.z = sz * (T(t0.z) + du * T(gu.z) + dv * T(gv.z) + dw * T(gw.z));
-    c.w = T(t0.w) + du * T(gu.w) + dv * T(gv.w) + dw * T(gw.w);
+    c.w = T(t0.w) + du * T(gu.w) + dv * T(gv.w) + dw * T(gw.w) +
+          psi_trace_coef(box) * (du * du + dv * dv + dw * dw);
     out[0] = c;
     V4<T> r;
     r.x = T(gu.x);
@@ -153,18 +166,28 @@ __device__ __forceinline__ void ewald_taylor(const TV *__restrict__ tab, T x, T 
 }
 
 // A cell's Ewald correction for one sink from the group-centre expansion in LDS:
-// the pull is m C at x = sink - source (as ewald_corr applies it).
+// the pull is m C at x = sink - source (as ewald_corr applies it).  s2q is the
+// sink's own second-order potential term, psi_trace_coef |s|^2.
 template <typename T>
-__device__ __forceinline__ void ewald_apply(const V4<T> *e, T m, T sx, T sy, T sz, Acc4T<T> &a) {
+__device__ __forceinline__ void ewald_apply(const V4<T> *e, T m, T sx, T sy, T sz, T s2q,
+                                            Acc4T<T> &a) {
     const V4<T> c = e[0], jx = e[1], jy = e[2], jz = e[3];
     const T cx = c.x + jx.x * sx + jx.y * sy + jx.z * sz;
     const T cy = c.y + jy.x * sx + jy.y * sy + jy.z * sz;
     const T cz = c.z + jz.x * sx + jz.y * sy + jz.z * sz;
-    const T cp = c.w + jx.w * sx + jy.w * sy + jz.w * sz;
+    const T cp = c.w + jx.w * sx + jy.w * sy + jz.w * sz + s2q;
     a.x -= m * cx;
     a.y -= m * cy;
     a.z -= m * cz;
     a.w += m * cp;
+}
+
+// the table a kernel reads: FP32 arithmetic its FP32 copy, FP64 the FP64 original
+__device__ __forceinline__ const float4 *ewald_table(const BatchArgs &A, float) {
+    return A.ewald;
+}
+__device__ __forceinline__ const double4 *ewald_table(const BatchArgs &A, double) {
+    return A.ewald64;
 }
 
 template <typename T>
@@ -243,6 +266,10 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
     const T ibox = T(1) / box;
     const T *__restrict__ mom = (const T *)A.mom;
     Acc4T<T> s = {0, 0, 0, 0}, c = {0, 0, 0, 0};
+    // periodic: this lane's share of the listed cells' second-moment traces (the
+    // potential's second-order Ewald term, the same for every sink of the group)
+    double trace = 0.0;
+    const T s2q = EW_GROUP ? psi_trace_coef(box) * (sx * sx + sy * sy + sz * sz) : T(0);
 
 #pragma unroll 1
     for (int kind = 0; kind < 4; kind++) {
@@ -263,11 +290,13 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                     src.z += (double)(im % f - h) * A.box;
                 }
                 lds[lane] = source_offset<T, REL, PERIODIC>(src, gc, A.box);
+                if (PERIODIC && kind < 3)
+                    trace += (double)mom[(size_t)e * PREC4 + PREC_TRACE];
                 if (EW_GROUP && kind < 3 && A.ewgroup) {
                     const V4<T> o = lds[lane];
                     const T x0 = pbc<PERIODIC>(-o.x, box, ibox), y0 = pbc<PERIODIC>(-o.y, box, ibox),
                             z0 = pbc<PERIODIC>(-o.z, box, ibox);
-                    ewald_taylor<T>(A.ewald, x0, y0, z0, box, ewl + 4 * lane);
+                    ewald_taylor<T>(ewald_table(A, T(0)), x0, y0, z0, box, ewl + 4 * lane);
                     // the expansion holds only if no sink of the group sees another image
                     // of the cell: every sink within the group radius of the centre stays
                     // on the centre's side of the half-box boundary in each axis
@@ -288,9 +317,9 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                     const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
                     cell_interact<4, false, 1, T>(mom + (size_t)e * PREC4, r.w, x, y, z, eps2_cell, p);
                     if (EW_GROUP && ewl[4 * 64 + k].x != T(0))
-                        ewald_apply<T>(ewl + 4 * k, r.w, sx, sy, sz, p);
+                        ewald_apply<T>(ewl + 4 * k, r.w, sx, sy, sz, s2q, p);
                     else if (PERIODIC)
-                        ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
+                        ewald_corr<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
                 }
             } else if (kind == 1) {
                 for (int k = 0; k < nt; k++) {
@@ -301,9 +330,9 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                     const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
                     cell_interact<2, false, 1, T>(mom + (size_t)e * PREC4, r.w, x, y, z, eps2_cell, p);
                     if (EW_GROUP && ewl[4 * 64 + k].x != T(0))
-                        ewald_apply<T>(ewl + 4 * k, r.w, sx, sy, sz, p);
+                        ewald_apply<T>(ewl + 4 * k, r.w, sx, sy, sz, s2q, p);
                     else if (PERIODIC)
-                        ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
+                        ewald_corr<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
                 }
             } else if (kind == 2) {
 #pragma unroll BU
@@ -314,9 +343,9 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                     const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
                     mono_interact<T>(r.w, x, y, z, eps2_cell, p);
                     if (EW_GROUP && ewl[4 * 64 + k].x != T(0))
-                        ewald_apply<T>(ewl + 4 * k, r.w, sx, sy, sz, p);
+                        ewald_apply<T>(ewl + 4 * k, r.w, sx, sy, sz, s2q, p);
                     else if (PERIODIC)
-                        ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
+                        ewald_corr<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
                 }
             } else {
                 // bodies: ~18 VALU per pair, so unroll to keep several LDS reads in flight
@@ -334,16 +363,26 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                         const bool self = x == T(0) && y == T(0) && z == T(0);
                         mono_interact<T>(self ? T(0) : r.w, x, y, z, self ? T(1) : T(0), p);
                     }
-                    if (PERIODIC) {
-                        const bool self = x == T(0) && y == T(0) && z == T(0);
-                        if (!self)
-                            ewald_corr<T>(A.ewald, r.w, x, y, z, box, p);
-                    }
+                    // the self pair too: at r = 0 the correction is its limit, C = 0 and
+                    // psi(0) = 2.8373/L (the table's origin node), the body's potential
+                    // from its own images and the background
+                    if (PERIODIC)
+                        ewald_corr<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
                 }
             }
             __syncthreads();
             fold<T>(s, c, p, COMP);
         }
+    }
+    if (PERIODIC) {
+        // the cells' corrections were taken at their centres of mass; the second-
+        // order remainder of the potential is (1/2) H : M2 summed over the cells,
+        // with H the correction's Hessian (trace -4 pi / L^3 everywhere) and M2 the
+        // cell's second moments: its trace part, -(2 pi / 3) / L^3 sum m |x - cofm|^2,
+        // is added here (the traceless part is O(1e-2) of it)
+        for (int k = 32; k >= 1; k >>= 1) trace += __shfl_xor(trace, k, 64);
+        const double b3 = A.box * A.box * A.box;
+        fold<T>(s, c, Acc4T<T>{T(0), T(0), T(0), T(-2.0 * M_PI / 3.0 / b3 * trace)}, COMP);
     }
     if (live) {
         const T Gn = T(A.G);  // nln/mac.c:269-277: results scaled by G at the end
@@ -495,7 +534,7 @@ __global__ __launch_bounds__(64) void k_direct64(const double4 *__restrict__ bpo
             } else {
                 mono_interact<double>(self ? 0.0 : q.w, x, y, z, self ? 1.0 : 0.0, a);
             }
-            if (PERIODIC && !self)
+            if (PERIODIC)  // the self pair included: C(0) = 0, psi(0) = 2.8373/L
                 ewald_corr<double>(tab, q.w, x, y, z, box, a);
         }
         __syncthreads();
@@ -669,6 +708,7 @@ hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, 
     A.out = variant == VAR_F64 ? (void *)v.out64 : (void *)v.out32;
     A.counter = v.counter;
     A.ewald = v.ewald32;
+    A.ewald64 = v.ewald64;
     A.ngroups = v.ngroups;
     A.soft = v.soft;
     A.eps = v.eps;
@@ -711,6 +751,7 @@ hipError_t launch_group64(const BatchView &v, int g, hipStream_t st) {
     A.out = v.out64;
     A.counter = v.counter;
     A.ewald = v.ewald32;
+    A.ewald64 = v.ewald64;
     A.ngroups = v.ngroups;
     A.soft = v.soft;
     A.eps = v.eps;

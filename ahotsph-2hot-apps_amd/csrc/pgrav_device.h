@@ -18,7 +18,8 @@
 //   [25..45] 26.25 R^4 x 21 hexadecapole coefficients on the basis
 //            U1 = x(x^2-3z^2), U2 = y(y^2-3z^2), U3 = y(x^2-z^2), U4 = z(3x^2-z^2),
 //            U5 = x(y^2-z^2), U6 = xyz, U7 = z(3y^2-z^2)   (rows x,y,z), c4 = 1/4
-//   [46..47] pad
+//   [46]     sum m |x - cofm|^2 (the periodic kernels' potential term)
+//   [47]     pad
 // With Gk the order-k gradient vector and Pk = x.Gk (k = 1..4):
 //   S_phi = m + ri2 (P1 + ri2 (P2 + ri2 (P3 + ri2 P4)))
 //   T'    = P1 + 2ri2 (P2 + 1.5ri2 (P3 + 4/3 ri2 P4))

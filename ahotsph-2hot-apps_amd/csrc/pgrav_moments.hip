@@ -182,6 +182,7 @@ __global__ __launch_bounds__(128) void k_prepare_records(MomentsView T, int64_t 
     raw_to_record<double>(raw, c, 1.0, q);
     q[5] = q[6] = q[7] = 0.0;  // the dipole about the centre of mass is zero
     prep_record<double>(q, order, 0, out);
+    out[PREC_TRACE] = raw_trace<double>(raw);
     for (int k = 0; k < PREC4; k++) T.mom32[(size_t)i * PREC4 + k] = (float)out[k];
     if (T.mom64)
         for (int k = 0; k < PREC4; k++) T.mom64[(size_t)i * PREC4 + k] = out[k];

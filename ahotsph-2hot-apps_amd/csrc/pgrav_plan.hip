@@ -37,26 +37,30 @@ struct Plan {
     hipStream_t qs[NQ_PLAIN] = {};
     hipEvent_t qev[NQ_PLAIN] = {};  // each slot's last launch
     hipEvent_t ev0 = nullptr;
+    // the last evaluation, on whichever stream it ran: what writes to the bodies
+    // and dispatch tables wait for (evaluations of one plan must not overlap)
+    hipEvent_t ev_last = nullptr;
+    hipEvent_t ev_join = nullptr;  // orders the plan's stream after a caller's stream
+    // device-side body update: the permutation, staging for host arrays, the
+    // gathered slots and the first bad body
+    int64_t *d_perm = nullptr;
+    double *u_pos = nullptr, *u_mass = nullptr;
+    double *u_w = nullptr, *u_g = nullptr;  // the host-array adjoint's weights and gradient
+    double4 *u_b4 = nullptr;
+    unsigned long long *u_bad = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::vector<void *> allocs;
     BatchView v;
     double t_tree = 0, t_lists = 0, t_upload = 0;
     int64_t nnodes = 0;
-    // adjoint buffers, allocated on first use
-    float4 *a_wv = nullptr, *a_gsink = nullptr, *a_gbody = nullptr;
-    float *a_gcell = nullptr, *a_gnode = nullptr, *a_mat = nullptr;
-    double4 *a_out = nullptr;
+    // adjoint buffers and transposed lists, set up on the first adjoint
+    AdjointView av;
+    bool adj_ready = false;
     // FP64 upward pass on the device (pgrav_moments.hip), set up on first use
     MomentsView mv;
     std::vector<int> lev_off;  // nodes of level L: lev_nodes[lev_off[L] .. lev_off[L+1])
     void ensure_moments_view();
-    int *a_leaf = nullptr, *a_parent = nullptr;
-    // transposed cell lists (groups listing each cell), built on the first adjoint
-    longlong2 *a_trange = nullptr;  // per work item: [t0, t1) into a_tlist
-    uint2 *a_tlist = nullptr;
-    int *a_tcells = nullptr;
-    int a_ntcells = 0;
     // far-field pass of the explicit-image scheme (periodic = 2)
     double *le_W = nullptr, *le_partial = nullptr, *le_coef = nullptr;
     int le_blocks = 0;
@@ -91,6 +95,9 @@ struct Plan {
             (void)hipStreamDestroy(stream);
         if (ev0)
             (void)hipEventDestroy(ev0);
+        for (hipEvent_t e : {ev_last, ev_join})
+            if (e)
+                (void)hipEventDestroy(e);
         for (auto &e : qev)
             if (e)
                 (void)hipEventDestroy(e);
@@ -146,6 +153,55 @@ static int auto_blocks() {
 
 static hipStream_t stream_of(Plan *P, void *stream) {
     return stream ? (hipStream_t)stream : P->stream;
+}
+
+// wait until the last evaluation (on any stream) and the plan's own stream are idle
+static bool quiesce(Plan *P) {
+    return check(hipEventSynchronize(P->ev_last), "hipEventSynchronize") &&
+           check(hipStreamSynchronize(P->stream), "hipStreamSynchronize");
+}
+
+// work queued on `before` completes before what is queued on `after` next
+static bool order_streams(Plan *P, hipStream_t before, hipStream_t after) {
+    if (before == after)
+        return true;
+    return check(hipEventRecord(P->ev_join, before), "hipEventRecord") &&
+           check(hipStreamWaitEvent(after, P->ev_join, 0), "hipStreamWaitEvent");
+}
+
+static void ensure_perm(Plan *P) {
+    if (!P->d_perm)
+        P->d_perm = P->upload(P->tree.perm.data(), P->tree.perm.size());
+}
+
+// bodies (caller order, device arrays ready on stream cs) into the sorted slots,
+// then the FP64 moments and records recomputed on the GPU; returns when done
+static int update_device(Plan *P, const double *pos3, const double *mass, hipStream_t cs) {
+    const int64_t n = P->tree.n;
+    ensure_perm(P);
+    P->ensure_moments_view();
+    if (!P->u_b4) {
+        P->u_b4 = P->dalloc<double4>(sizeof(double4) * n);
+        P->u_bad = P->dalloc<unsigned long long>(sizeof(unsigned long long));
+    }
+    unsigned long long bad = 0;
+    if (!quiesce(P) || !order_streams(P, cs, P->stream) ||
+        !check(launch_gather_bodies(pos3, mass, P->d_perm, n, P->prm.periodic != 0, P->prm.box,
+                                    P->v.bpos, P->u_b4, P->u_bad, P->stream), "body gather kernel") ||
+        !check(hipMemcpyAsync(&bad, P->u_bad, sizeof(bad), hipMemcpyDeviceToHost, P->stream), "hipMemcpyAsync") ||
+        !check(hipStreamSynchronize(P->stream), "hipStreamSynchronize"))
+        return -1;
+    if (bad != ~0ull) {
+        fail("pgrav_plan_update: body %llu has a non-positive mass or a non-finite value", bad);
+        return -1;
+    }
+    if (!check(hipMemcpyAsync((void *)P->v.bpos, P->u_b4, sizeof(double4) * n,
+                              hipMemcpyDeviceToDevice, P->stream), "hipMemcpyAsync") ||
+        !check(launch_moments(P->mv, P->lev_off.data(), (int)P->lev_off.size() - 1, P->nnodes,
+                              P->prm.order, P->stream), "moment kernels") ||
+        !check(hipStreamSynchronize(P->stream), "hipStreamSynchronize"))
+        return -1;
+    return 0;
 }
 
 // the far images and background of periodic = 2 plans, added to the result
@@ -286,7 +342,10 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
         for (int64_t i = 0; i < n; i++) P->inv[P->tree.perm[i]] = i;
 
         if (!check(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking), "hipStreamCreate") ||
-            !check(hipEventCreate(&P->ev0), "hipEventCreate"))
+            !check(hipEventCreate(&P->ev0), "hipEventCreate") ||
+            !check(hipEventCreateWithFlags(&P->ev_last, hipEventDisableTiming), "hipEventCreate") ||
+            !check(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming), "hipEventCreate") ||
+            !check(hipEventRecord(P->ev_last, P->stream), "hipEventRecord"))
             throw std::runtime_error("stream");
         for (int q = 0; q < NQ_PLAIN; q++)
             if (!check(hipStreamCreateWithFlags(&P->qs[q], hipStreamNonBlocking), "hipStreamCreate") ||
@@ -414,44 +473,175 @@ int pgrav_plan_update(void *plan, const double *pos3, const double *mass) {
         return -1;
     }
     try {
-        // the bodies move into their sorted slots on the host; the moments, centres,
-        // bmax and records are recomputed from them on the GPU in FP64
-        Tree &t = P->tree;
-        const int64_t n = t.n;
-        std::vector<double> b4(4 * (size_t)n);
-        const bool per = P->prm.periodic != 0;
-        const double box = P->prm.box;
-        for (int64_t j = 0; j < n; j++) {
-            if (!(mass[j] > 0) || !std::isfinite(mass[j]) || !std::isfinite(pos3[3 * j]) ||
-                !std::isfinite(pos3[3 * j + 1]) || !std::isfinite(pos3[3 * j + 2])) {
-                fail("pgrav_plan_update: body %lld has a non-positive mass or a non-finite value",
-                     (long long)j);
-                return -1;
-            }
+        // the caller's arrays go to the device as they are; the bodies move into
+        // their sorted slots there and the moments, centres, bmax and records are
+        // recomputed from them in FP64
+        const int64_t n = P->tree.n;
+        if (!P->u_pos) {
+            P->u_pos = P->dalloc<double>(sizeof(double) * 3 * n);
+            P->u_mass = P->dalloc<double>(sizeof(double) * n);
         }
-        for (int64_t i = 0; i < n; i++) {
-            const int64_t j = t.perm[i];
-            for (int k = 0; k < 3; k++) {
-                double x = pos3[3 * j + k];
-                if (per)
-                    x -= box * floor(x / box);
-                t.pos[3 * i + k] = x;
-                b4[4 * i + k] = x;
-            }
-            t.mass[i] = mass[j];
-            b4[4 * i + 3] = mass[j];
-        }
-        P->ensure_moments_view();
-        if (!check(hipStreamSynchronize(P->stream), "hipStreamSynchronize") ||
-            !check(hipMemcpy((void *)P->v.bpos, b4.data(), b4.size() * 8, hipMemcpyHostToDevice), "hipMemcpy") ||
-            !check(launch_moments(P->mv, P->lev_off.data(), (int)P->lev_off.size() - 1, P->nnodes,
-                                  P->prm.order, P->stream), "moment kernels") ||
-            !check(hipStreamSynchronize(P->stream), "hipStreamSynchronize"))
+        if (!quiesce(P) ||
+            !check(hipMemcpyAsync(P->u_pos, pos3, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                                  P->stream), "hipMemcpyAsync") ||
+            !check(hipMemcpyAsync(P->u_mass, mass, sizeof(double) * n, hipMemcpyHostToDevice,
+                                  P->stream), "hipMemcpyAsync"))
             return -1;
+        return update_device(P, P->u_pos, P->u_mass, P->stream);
     } catch (const std::exception &e) {
         fail("pgrav_plan_update: %s", e.what());
         return -1;
     }
+}
+
+int pgrav_plan_update_device(void *plan, const double *pos3, const double *mass, void *stream) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P || !pos3 || !mass) {
+        fail("pgrav_plan_update_device: null argument");
+        return -1;
+    }
+    try {
+        return update_device(P, pos3, mass, stream_of(P, stream));
+    } catch (const std::exception &e) {
+        fail("pgrav_plan_update_device: %s", e.what());
+        return -1;
+    }
+}
+
+// The adjoint's buffers and the transposed lists, built once per plan from the
+// device's lists (the lists are fixed for the plan's lifetime).
+static int setup_adjoint(Plan *P) {
+    const int64_t n = P->tree.n, nn = P->nnodes;
+    const Tree &t = P->tree;
+    const Lists &L = P->lists;
+    const int64_t G = L.ngroups;
+    AdjointView &A = P->av;
+    int64_t ncell = 0;
+    for (int k = 0; k < L_BODY; k++) ncell += L.off[k][G];
+    const int64_t nbody = L.off[L_BODY][G];
+    if (ncell + nbody >= ((int64_t)1 << 32)) {
+        fail("pgrav_plan_adjoint: more than 2^32 list entries");
+        return -1;
+    }
+    std::vector<uint32_t> idx(std::max<int64_t>(1, ncell + nbody));
+    if (ncell + nbody &&
+        (!check(hipMemcpyAsync(idx.data(), P->v.idx, sizeof(uint32_t) * (ncell + nbody),
+                               hipMemcpyDeviceToHost, P->stream), "hipMemcpyAsync") ||
+         !check(hipStreamSynchronize(P->stream), "hipStreamSynchronize")))
+        return -1;
+    // raw moments -> prepared record is linear and centre-independent (R = 1)
+    std::vector<float> mat(46 * 35);
+    for (int a = 0; a < 35; a++) {
+        double raw[35] = {0}, c[3] = {0, 0, 0}, q[29], pr[PREC4];
+        raw[a] = 1.0;
+        raw_to_record<double>(raw, c, 1.0, q);
+        q[5] = q[6] = q[7] = 0.0;
+        prep_record<double>(q, 4, 0, pr);
+        for (int k = 0; k < 46; k++) mat[k * 35 + a] = (float)pr[k];
+    }
+    std::vector<int> leaf(n, -1), parent(t.parent.begin(), t.parent.end());
+    for (int64_t c = 0; c < nn; c++)
+        if (t.child[c] < 0)
+            for (int64_t i = t.first[c]; i < t.first[c] + t.count[c]; i++) leaf[i] = (int)c;
+    // transposed cell lists: for each cell the groups listing it, in group
+    // (Morton) order, so that the cells processed together walk neighbouring
+    // groups and every sum has a fixed order; work items of <= 128 groups
+    std::vector<int64_t> toff(nn + 1, 0);
+    for (int64_t e = 0; e < ncell; e++) toff[idx[e] + 1]++;
+    for (int64_t c = 0; c < nn; c++) toff[c + 1] += toff[c];
+    std::vector<int> tcells;
+    std::vector<longlong2> trange;
+    std::vector<int64_t> cchunk(nn + 1, 0);
+    for (int64_t c = 0; c < nn; c++) {
+        for (int64_t a = toff[c]; a < toff[c + 1]; a += 128) {
+            tcells.push_back((int)c);
+            trange.push_back(make_longlong2(a, std::min<int64_t>(toff[c + 1], a + 128)));
+        }
+        cchunk[c + 1] = (int64_t)tcells.size();
+    }
+    std::vector<int64_t> cursor(toff.begin(), toff.end() - 1);
+    std::vector<uint4> tl(std::max<int64_t>(1, ncell));
+    int64_t base = 0;
+    for (int k = 0; k < L_BODY; k++) {
+        for (int64_t g = 0; g < G; g++) {
+            // a run's first sink, or a body-set group's index (its slot table)
+            const uint32_t ref = (uint32_t)(P->v.gslot ? g : L.gfirst[g]);
+            const uint32_t ck = (uint32_t)L.gcount[g] | ((uint32_t)k << 8);
+            for (int64_t e = base + L.off[k][g]; e < base + L.off[k][g + 1]; e++)
+                tl[cursor[idx[e]]++] = make_uint4(ref, ck, (uint32_t)e, (uint32_t)((uint64_t)e >> 32));
+        }
+        base += L.off[k][G];
+    }
+    // group order within a cell across kinds (a cell has one kind per group)
+    for (int64_t c = 0; c < nn; c++)
+        if (toff[c + 1] - toff[c] > 1)
+            std::sort(tl.begin() + toff[c], tl.begin() + toff[c + 1],
+                      [](const uint4 &a, const uint4 &b) { return a.x != b.x ? a.x < b.x : a.z < b.z; });
+    // transposed body lists: each body's entries in ascending list position
+    std::vector<int64_t> boff(n + 1, 0);
+    for (int64_t e = 0; e < nbody; e++) boff[idx[ncell + e] + 1]++;
+    for (int64_t j = 0; j < n; j++) boff[j + 1] += boff[j];
+    std::vector<uint32_t> blist(std::max<int64_t>(1, nbody));
+    std::vector<int64_t> bcur(boff.begin(), boff.end() - 1);
+    for (int64_t e = 0; e < nbody; e++) blist[bcur[idx[ncell + e]]++] = (uint32_t)e;
+
+    A.wv = P->dalloc<float4>(sizeof(float4) * n);
+    A.gsink = P->dalloc<float4>(sizeof(float4) * n);
+    A.gbody = P->dalloc<float4>(sizeof(float4) * n);
+    A.gpair = P->dalloc<float4>(sizeof(float4) * std::max<int64_t>(1, ncell));
+    A.gentry = P->dalloc<float4>(sizeof(float4) * std::max<int64_t>(1, nbody));
+    A.ncell_entries = ncell;
+    A.tcells = P->upload(tcells.data(), tcells.size());
+    A.ntcells = (int)tcells.size();
+    A.trange = P->upload(trange.data(), trange.size());
+    A.tlist = P->upload(tl.data(), tl.size());
+    A.gchunk = P->dalloc<float>(sizeof(float) * 52 * std::max<size_t>(1, tcells.size()));
+    A.cchunk_off = P->upload(cchunk.data(), cchunk.size());
+    A.boff = P->upload(boff.data(), boff.size());
+    A.blist = P->upload(blist.data(), blist.size());
+    A.raw = P->mv.raw;
+    A.mat = P->upload(mat.data(), mat.size());
+    A.rix = raw_index();
+    A.nnodes = nn;
+    A.gnode = P->dalloc<float>(sizeof(float) * 40 * (size_t)nn);
+    A.leaf_of = P->upload(leaf.data(), leaf.size());
+    A.parent = P->upload(parent.data(), parent.size());
+    A.n = n;
+    A.out = P->dalloc<double4>(sizeof(double4) * n);
+    P->adj_ready = true;
+    return 0;
+}
+
+// the adjoint on device arrays in caller order: weights w4 (float or double
+// quadruples) ready on stream cs, gradient grad4 (double quadruples) queued on cs
+static bool aligned_quads(const void *p, int fp64, const char *what) {
+    if ((uintptr_t)p % (fp64 ? 32 : 16)) {
+        fail("%s: the array must be aligned to one quadruple (%d bytes)", what, fp64 ? 32 : 16);
+        return false;
+    }
+    return true;
+}
+
+static int adjoint_device(Plan *P, const void *w4, int w_fp64, double *grad4, hipStream_t cs) {
+    if (!aligned_quads(w4, w_fp64, "adjoint weights") || !aligned_quads(grad4, 1, "adjoint gradient"))
+        return -1;
+    if (P->prm.periodic) {
+        fail("the adjoint covers the isolated kernels only");
+        return -1;
+    }
+    P->ensure_moments_view();  // the device raw moments are the current ones
+    ensure_perm(P);
+    if (!P->adj_ready && setup_adjoint(P))
+        return -1;
+    AdjointView &A = P->av;
+    const int64_t n = P->tree.n;
+    if (!order_streams(P, cs, P->stream) ||
+        !check(launch_gather4(w4, w_fp64, P->d_perm, n, (float4 *)A.wv, P->stream), "weight gather") ||
+        !check(launch_adjoint(P->v, A, P->stream), "adjoint kernels") ||
+        !check(launch_scatter4(A.out, 1, P->d_perm, n, grad4, 1, P->stream), "gradient scatter") ||
+        !order_streams(P, P->stream, cs))
+        return -1;
     return 0;
 }
 
@@ -462,110 +652,39 @@ int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4) {
         fail("pgrav_plan_adjoint: null argument");
         return -1;
     }
-    if (P->prm.periodic) {
-        fail("the adjoint covers the isolated kernels only");
-        return -1;
-    }
     try {
-        const int64_t n = P->tree.n, nn = P->nnodes;
-        const Tree &t = P->tree;
-        P->ensure_moments_view();  // the device raw moments are the current ones
-        if (!P->a_wv) {
-            P->a_wv = P->dalloc<float4>(sizeof(float4) * n);
-            P->a_gsink = P->dalloc<float4>(sizeof(float4) * n);
-            P->a_gbody = P->dalloc<float4>(sizeof(float4) * n);
-            P->a_out = P->dalloc<double4>(sizeof(double4) * n);
-            P->a_gcell = P->dalloc<float>(sizeof(float) * 52 * (size_t)nn);
-            P->a_gnode = P->dalloc<float>(sizeof(float) * 40 * (size_t)nn);
-            // raw moments -> prepared record is linear and centre-independent (R = 1)
-            std::vector<float> mat(46 * 35);
-            for (int a = 0; a < 35; a++) {
-                double raw[35] = {0}, c[3] = {0, 0, 0}, q[29], pr[PREC4];
-                raw[a] = 1.0;
-                raw_to_record<double>(raw, c, 1.0, q);
-                q[5] = q[6] = q[7] = 0.0;
-                prep_record<double>(q, 4, 0, pr);
-                for (int k = 0; k < 46; k++) mat[k * 35 + a] = (float)pr[k];
-            }
-            P->a_mat = P->upload(mat.data(), mat.size());
-            std::vector<int> leaf(n, -1), parent(t.parent.begin(), t.parent.end());
-            for (int64_t c = 0; c < nn; c++)
-                if (t.child[c] < 0)
-                    for (int64_t i = t.first[c]; i < t.first[c] + t.count[c]; i++) leaf[i] = (int)c;
-            P->a_leaf = P->upload(leaf.data(), leaf.size());
-            P->a_parent = P->upload(parent.data(), parent.size());
-            // transposed cell lists, built on the host from the device's lists: for
-            // each cell, the groups listing it in group (Morton) order, so that the
-            // cells processed together walk neighbouring groups and the sums have
-            // a fixed order
-            const Lists &L = P->lists;
-            const int64_t G = L.ngroups;
-            int64_t ncell_entries = 0;
-            for (int k = 0; k < L_BODY; k++) ncell_entries += L.off[k][G];
-            std::vector<uint32_t> idx(std::max<int64_t>(1, ncell_entries));
-            if (ncell_entries &&
-                (!check(hipMemcpyAsync(idx.data(), P->v.idx, sizeof(uint32_t) * ncell_entries,
-                                       hipMemcpyDeviceToHost, P->stream), "hipMemcpyAsync") ||
-                 !check(hipStreamSynchronize(P->stream), "hipStreamSynchronize")))
-                return -1;
-            std::vector<int64_t> toff(nn + 1, 0);
-            for (int64_t e = 0; e < ncell_entries; e++) toff[idx[e] + 1]++;
-            // work items: a cell and a chunk of at most 128 of its listing groups
-            std::vector<int> tcells;
-            std::vector<longlong2> trange;
-            for (int64_t c = 0; c < nn; c++) {
-                const int64_t t0 = toff[c], m = toff[c + 1];
-                for (int64_t a = 0; a < m; a += 128) {
-                    tcells.push_back((int)c);
-                    trange.push_back(make_longlong2(t0 + a, t0 + std::min<int64_t>(m, a + 128)));
-                }
-                toff[c + 1] += toff[c];
-            }
-            std::vector<int64_t> cursor(toff.begin(), toff.end() - 1);
-            std::vector<uint2> tl(std::max<int64_t>(1, ncell_entries));
-            int64_t base = 0;
-            for (int k = 0; k < L_BODY; k++) {
-                for (int64_t g = 0; g < G; g++) {
-                    // a run's first sink, or a body-set group's index (its slot table)
-                    const uint2 ent = make_uint2((uint32_t)(P->v.gslot ? g : L.gfirst[g]),
-                                                 (uint32_t)L.gcount[g] | ((uint32_t)k << 8));
-                    for (int64_t e = base + L.off[k][g]; e < base + L.off[k][g + 1]; e++)
-                        tl[cursor[idx[e]]++] = ent;
-                }
-                base += L.off[k][G];
-            }
-            // group order within a cell across kinds: a cell has one kind per group
-            for (int64_t c = 0; c < nn; c++)
-                if (toff[c + 1] - toff[c] > 1)
-                    std::sort(tl.begin() + toff[c], tl.begin() + toff[c + 1],
-                              [](const uint2 &a, const uint2 &b) { return a.x < b.x; });
-            P->a_trange = P->upload(trange.data(), trange.size());
-            P->a_tlist = P->upload(tl.data(), tl.size());
-            P->a_tcells = P->upload(tcells.data(), tcells.size());
-            P->a_ntcells = (int)tcells.size();
+        const int64_t n = P->tree.n;
+        if (!P->u_w) {
+            P->u_w = P->dalloc<double>(sizeof(double) * 4 * n);
+            P->u_g = P->dalloc<double>(sizeof(double) * 4 * n);
         }
-        std::vector<float> wv(4 * (size_t)n);
-        for (int64_t i = 0; i < n; i++)
-            for (int k = 0; k < 4; k++) wv[4 * i + k] = (float)w4[4 * t.perm[i] + k];
-        if (!check(hipMemcpyAsync(P->a_wv, wv.data(), wv.size() * 4, hipMemcpyHostToDevice, P->stream), "hipMemcpyAsync") ||
-            !check(hipMemsetAsync(P->a_gcell, 0, sizeof(float) * 52 * (size_t)nn, P->stream), "hipMemsetAsync") ||
-            !check(hipMemsetAsync(P->a_gbody, 0, sizeof(float4) * n, P->stream), "hipMemsetAsync") ||
-            !check(launch_adjoint(P->v, P->a_wv, P->a_gsink, P->a_gcell, P->a_gbody, P->mv.raw,
-                                  P->a_mat, nn, P->a_gnode, P->a_leaf, P->a_parent, n,
-                                  P->a_tcells, P->a_ntcells, P->a_trange, P->a_tlist, P->a_out,
-                                  P->stream), "adjoint kernels"))
-            return -1;
-        std::vector<double> out(4 * (size_t)n);
-        if (!check(hipMemcpyAsync(out.data(), P->a_out, out.size() * 8, hipMemcpyDeviceToHost, P->stream), "hipMemcpyAsync") ||
+        if (!check(hipMemcpyAsync(P->u_w, w4, sizeof(double) * 4 * n, hipMemcpyHostToDevice,
+                                  P->stream), "hipMemcpyAsync") ||
+            adjoint_device(P, P->u_w, 1, P->u_g, P->stream) ||
+            !check(hipMemcpyAsync(grad4, P->u_g, sizeof(double) * 4 * n, hipMemcpyDeviceToHost,
+                                  P->stream), "hipMemcpyAsync") ||
             !check(hipStreamSynchronize(P->stream), "hipStreamSynchronize"))
             return -1;
-        for (int64_t i = 0; i < n; i++)
-            for (int k = 0; k < 4; k++) grad4[4 * t.perm[i] + k] = out[4 * i + k];
     } catch (const std::exception &e) {
         fail("pgrav_plan_adjoint: %s", e.what());
         return -1;
     }
     return 0;
+}
+
+int pgrav_plan_adjoint_device(void *plan, const void *w4, int w_fp64, double *grad4, void *stream) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P || !w4 || !grad4) {
+        fail("pgrav_plan_adjoint_device: null argument");
+        return -1;
+    }
+    try {
+        return adjoint_device(P, w4, w_fp64 ? 1 : 0, grad4, stream_of(P, stream));
+    } catch (const std::exception &e) {
+        fail("pgrav_plan_adjoint_device: %s", e.what());
+        return -1;
+    }
 }
 
 int pgrav_plan_stats(void *plan, pgrav_stats *st) {
@@ -604,7 +723,7 @@ int pgrav_plan_set_dispatch(void *plan, int mode) {
                             L.off[L_MONO][g + 1] - L.off[L_MONO][g], L.off[L_BODY][g + 1] - L.off[L_BODY][g]);
     make_dispatch(cost, mode, L);
     P->prm.dispatch = mode;
-    return (check(hipStreamSynchronize(P->stream), "hipStreamSynchronize") &&
+    return (quiesce(P) &&
             check(hipMemcpy((void *)P->v.lpt, L.lpt.data(), 4 * G, hipMemcpyHostToDevice), "hipMemcpy") &&
             check(hipMemcpy((void *)P->v.xoff, L.xoff.data(), 4 * L.xoff.size(), hipMemcpyHostToDevice), "hipMemcpy") &&
             check(hipMemcpy((void *)P->v.order_np, L.order_np.data(), 4 * G, hipMemcpyHostToDevice), "hipMemcpy"))
@@ -639,7 +758,10 @@ int pgrav_plan_evaluate(void *plan, int variant, void *stream) {
         fail("null plan");
         return -1;
     }
-    return evaluate(P, variant, stream_of(P, stream));
+    hipStream_t st = stream_of(P, stream);
+    if (evaluate(P, variant, st))
+        return -1;
+    return check(hipEventRecord(P->ev_last, st), "hipEventRecord") ? 0 : -1;
 }
 
 int pgrav_plan_time(void *plan, int variant, int reps, void *stream, double *ms_per_launch) {
@@ -655,6 +777,8 @@ int pgrav_plan_time(void *plan, int variant, int reps, void *stream, double *ms_
         return -1;
     int rc = check(hipEventRecord(a, st), "hipEventRecord") ? 0 : -1;
     for (int r = 0; r < reps && rc == 0; r++) rc = evaluate(P, variant, st);
+    if (rc == 0 && !check(hipEventRecord(P->ev_last, st), "hipEventRecord"))
+        rc = -1;
     if (rc == 0 && check(hipEventRecord(b, st), "hipEventRecord") &&
         check(hipEventSynchronize(b), "hipEventSynchronize")) {
         float ms = 0;
@@ -689,7 +813,7 @@ int pgrav_plan_graph_capture(void *plan, int variant) {
         (void)hipGraphDestroy(P->graph);
         P->graph = nullptr;
     }
-    if (!check(hipStreamSynchronize(P->stream), "hipStreamSynchronize") ||
+    if (!quiesce(P) ||
         !check(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
         return -1;
     int rc = evaluate(P, variant, P->stream);
@@ -708,7 +832,10 @@ int pgrav_plan_graph_launch(void *plan, void *stream) {
         fail("no captured graph");
         return -1;
     }
-    return check(hipGraphLaunch(P->gexec, stream_of(P, stream)), "hipGraphLaunch") ? 0 : -1;
+    hipStream_t st = stream_of(P, stream);
+    return check(hipGraphLaunch(P->gexec, st), "hipGraphLaunch") &&
+                   check(hipEventRecord(P->ev_last, st), "hipEventRecord")
+               ? 0 : -1;
 }
 
 int pgrav_plan_synchronize(void *plan) {
@@ -717,7 +844,7 @@ int pgrav_plan_synchronize(void *plan) {
         fail("null plan");
         return -1;
     }
-    return check(hipStreamSynchronize(P->stream), "hipStreamSynchronize") ? 0 : -1;
+    return quiesce(P) ? 0 : -1;
 }
 
 int pgrav_plan_result(void *plan, int variant, double *acc4) {
@@ -749,6 +876,35 @@ int pgrav_plan_result(void *plan, int variant, double *acc4) {
             for (int k = 0; k < 4; k++) acc4[4 * P->tree.perm[i] + k] = h[4 * i + k];
     }
     return 0;
+}
+
+int pgrav_plan_result_device(void *plan, int variant, void *acc4, int fp64, void *stream) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P || !acc4) {
+        fail("pgrav_plan_result_device: null argument");
+        return -1;
+    }
+    if (!aligned_quads(acc4, fp64, "pgrav_plan_result_device"))
+        return -1;
+    const bool f64 = variant == VAR_F64 || variant == VAR_PLAIN64;
+    const void *src = f64 ? (const void *)P->v.out64 : (const void *)P->v.out32;
+    if (!src) {
+        fail("no FP64 result buffer");
+        return -1;
+    }
+    try {
+        ensure_perm(P);
+        hipStream_t cs = stream_of(P, stream);
+        // after the last evaluation, on the caller's stream
+        return check(hipStreamWaitEvent(cs, P->ev_last, 0), "hipStreamWaitEvent") &&
+                       check(launch_scatter4(src, f64, P->d_perm, P->tree.n, acc4, fp64 ? 1 : 0, cs),
+                             "result scatter")
+                   ? 0 : -1;
+    } catch (const std::exception &e) {
+        fail("pgrav_plan_result_device: %s", e.what());
+        return -1;
+    }
 }
 
 int pgrav_plan_direct(void *plan, const int64_t *sample, int ns, double *out4) {

@@ -15,6 +15,7 @@
 
 #define PREC4 48
 #define PREC2 12
+#define PREC_TRACE 46  // prepared-record slot of sum m |x - cofm|^2 (periodic potentials)
 
 namespace pgrav {
 
@@ -65,6 +66,16 @@ PGRAV_HD inline void prep_record(const T *q, int order, int dipole, T *out /* PR
     h[14] = k4 * s6 * qxxxz; h[15] = k4 * s6 * qyyyz; h[16] = k4 * h2 * qxxyz;
     h[17] = -k4 * s6 * (qxxxx + qxxyy); h[18] = k4 * h2 * qxyyz; h[19] = -k4 * (qxyyy + qxxxy);
     h[20] = -k4 * s6 * (qyyyy + qxxyy);
+}
+
+// Trace of the raw second moments, sum m |x - cofm|^2: the periodic kernels add
+// the cells' Ewald correction at the centre of mass only, and the second-order
+// remainder of its potential is -(2 pi / 3) / L^3 times this trace (the trace of
+// the correction's Hessian is -4 pi / L^3 everywhere; the traceless rest of the
+// cell's second moments couples to the Hessian's small traceless part).
+template <typename T>
+PGRAV_HD inline T raw_trace(const T *u) {
+    return u[4] + u[7] + u[9];
 }
 
 // Raw moments about the cell centre (35 values in the order of

@@ -101,9 +101,6 @@ void make_dispatch(const std::vector<double> &cost, int mode, Lists &l);
 
 void build_tree(const double *pos, const double *mass, int64_t n, const TreeParams &p, Tree &t);
 void compute_moments(Tree &t, int nthreads, const std::vector<uint64_t> *key);
-// New positions/masses (caller order) for the same tree topology and lists.
-void update_bodies(Tree &t, const double *pos, const double *mass, int periodic, double box,
-                   int nthreads);
 void build_groups(const Tree &t, const TreeParams &p, Lists &l);
 void build_lists(const Tree &t, const TreeParams &p, Lists &l);
 // Prepared records for every node: FP32 (48 floats) and optionally FP64.

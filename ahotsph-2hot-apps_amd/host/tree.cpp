@@ -385,24 +385,6 @@ void compute_moments(Tree &t, int nth, const std::vector<uint64_t> *keyp) {
     }
 }
 
-void update_bodies(Tree &t, const double *pos, const double *mass, int periodic, double box,
-                   int nthreads) {
-    const int nth = default_threads(nthreads);
-    parallel_for(nth, t.n, [&](int64_t a, int64_t b) {
-        for (int64_t i = a; i < b; i++) {
-            const int64_t j = t.perm[i];
-            for (int k = 0; k < 3; k++) {
-                double x = pos[3 * j + k];
-                if (periodic)
-                    x -= box * floor(x / box);
-                t.pos[3 * i + k] = x;
-            }
-            t.mass[i] = mass[j];
-        }
-    });
-    compute_moments(t, nth, nullptr);
-}
-
 // ---------------------------------------------------------------- records
 void prepare_cells(const Tree &t, int order, std::vector<float> &rec32, std::vector<double> *rec64,
                    std::vector<double> &cpos4) {
@@ -419,6 +401,7 @@ void prepare_cells(const Tree &t, int order, std::vector<float> &rec32, std::vec
             raw_to_record<double>(raw, &t.cofm[3 * i], 1.0, q);
             q[5] = q[6] = q[7] = 0.0;  // the dipole about the centre of mass is zero
             prep_record<double>(q, order, 0, out);
+            out[PREC_TRACE] = raw_trace<double>(raw);
             for (int k = 0; k < PREC4; k++) rec32[(size_t)i * PREC4 + k] = (float)out[k];
             if (rec64)
                 for (int k = 0; k < PREC4; k++) (*rec64)[(size_t)i * PREC4 + k] = out[k];

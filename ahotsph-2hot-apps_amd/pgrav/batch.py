@@ -60,7 +60,10 @@ register({
     "pgrav_plan_create": (_vp, [_vp, _vp, _i64, _P]),
     "pgrav_plan_destroy": (None, [_vp]),
     "pgrav_plan_update": (_i, [_vp, _vp, _vp]),
+    "pgrav_plan_update_device": (_i, [_vp, _vp, _vp, _vp]),
     "pgrav_plan_adjoint": (_i, [_vp, _vp, _vp]),
+    "pgrav_plan_adjoint_device": (_i, [_vp, _vp, _i, _vp, _vp]),
+    "pgrav_plan_result_device": (_i, [_vp, _i, _vp, _i, _vp]),
     "pgrav_plan_stats": (_i, [_vp, _S]),
     "pgrav_plan_evaluate": (_i, [_vp, _i, _vp]),
     "pgrav_plan_set_launch": (_i, [_vp, _i]),
@@ -171,6 +174,7 @@ class Plan:
         check()
         if not self.h:
             raise PgravError("pgrav_plan_create failed")
+        self.version = 0  # bumped by every update: which bodies the device holds
 
     def close(self):
         if getattr(self, "h", None):
@@ -192,7 +196,27 @@ class Plan:
     def update(self, pos, mass):
         """Move the bodies, keeping the tree topology, groups and lists."""
         self.pos, self.mass = _as_inputs(pos, mass)
+        self.version += 1
         lib().pgrav_plan_update(self.h, self.pos.ctypes.data, self.mass.ctypes.data)
+        check()
+
+    def update_device(self, pos_ptr, mass_ptr, stream=None):
+        """update() from device arrays (caller order, FP64: n x 3 and n), ready on
+        `stream` (a hipStream_t handle; None: the plan's stream)."""
+        self.version += 1
+        lib().pgrav_plan_update_device(self.h, pos_ptr, mass_ptr, stream)
+        check()
+
+    def result_device(self, variant, ptr, fp64=False, stream=None):
+        """result() into a device array (n x 4 float32, or float64 with fp64),
+        queued on `stream` after the last evaluation."""
+        lib().pgrav_plan_result_device(self.h, _v(variant), ptr, int(fp64), stream)
+        check()
+
+    def adjoint_device(self, w_ptr, grad_ptr, w_fp64=False, stream=None):
+        """adjoint() on device arrays: weights n x 4 (float32, or float64 with
+        w_fp64), gradient n x 4 float64, both in caller order, on `stream`."""
+        lib().pgrav_plan_adjoint_device(self.h, w_ptr, int(w_fp64), grad_ptr, stream)
         check()
 
     def evaluate(self, variant=REL32C, stream=None):

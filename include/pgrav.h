@@ -205,13 +205,26 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
 void pgrav_plan_destroy(void *plan);
 /* New positions and masses (caller order) with the tree topology, sink groups
  * and lists held fixed: moments and centres are recomputed in FP64 and the
- * device tables refreshed (what the adjoint's "tree held fixed" means). */
+ * device tables refreshed (what the adjoint's "tree held fixed" means).
+ * Periodic plans keep each body at the image nearest its current position (a
+ * body that crosses the box boundary stays beside its cell).  Waits for the
+ * plan's last evaluation first; returns when the update is done.  Evaluations
+ * of one plan must not overlap one another or an update. */
 int pgrav_plan_update(void *plan, const double *pos3, const double *mass);
+/* The same from device arrays (caller order, FP64), ready on `stream` (NULL: the
+ * plan's stream). */
+int pgrav_plan_update_device(void *plan, const double *pos3, const double *mass, void *stream);
 /* Adjoint of the REL32C evaluation with the tree and lists held fixed: for
  * L = sum_i w_i . a_i + v_i phi_i (w4[i] = (w_i, v_i), caller order) returns
- * grad4[i] = (dL/dx_i, dL/dy_i, dL/dz_i, dL/dm_i).  No reference counterpart
- * (the reference has no derivatives). */
+ * grad4[i] = (dL/dx_i, dL/dy_i, dL/dz_i, dL/dm_i).  Fixed-order sums: two calls
+ * give identical gradients.  No reference counterpart (the reference has no
+ * derivatives). */
 int pgrav_plan_adjoint(void *plan, const double *w4, double *grad4);
+/* The same on device arrays in caller order: w4 float (w_fp64 = 0) or double
+ * quadruples, ready on `stream`; grad4 double quadruples, written in order on
+ * `stream` (NULL: the plan's stream) when this returns.  Arrays aligned to one
+ * quadruple. */
+int pgrav_plan_adjoint_device(void *plan, const void *w4, int w_fp64, double *grad4, void *stream);
 int pgrav_plan_stats(void *plan, pgrav_stats *st);
 /* Launch shape for later evaluations: 0 one workgroup per group, -1 auto
  * persistent grid, k > 0 a persistent grid of k workgroups (a captured graph
@@ -233,6 +246,10 @@ int pgrav_plan_graph_launch(void *plan, void *stream);
 int pgrav_plan_synchronize(void *plan);
 /* (ax, ay, az, phi) x n in the caller's body order, after G scaling. */
 int pgrav_plan_result(void *plan, int variant, double *acc4);
+/* The same into a device array (float quadruples, or double when fp64 = 1,
+ * aligned to one quadruple), queued on `stream` (NULL: the plan's stream) after
+ * the plan's last evaluation. */
+int pgrav_plan_result_device(void *plan, int variant, void *acc4, int fp64, void *stream);
 /* FP64 direct sum (nln/grav_n2.c pattern, same softening) for `ns` bodies
  * given by their original indices; out4 is ns x 4.  Periodic plans: nearest
  * image plus the Ewald correction interpolated from the FP64 table. */

@@ -14,6 +14,11 @@
  *   orc_direct               -- Newton's third law, two-body closed forms.
  *   orc_ewald                -- lattice constant 2.8372975 and symmetry
  *                               (nln/ewald_le2.c:38-43).
+ *   orc_ewald_corr           -- == orc_ewald minus the Newtonian term; the
+ *                               closed-form Jacobian == central differences;
+ *                               psi(0) == 2 x 1.4186487533, dC/dr(0) ==
+ *                               -4 pi / 3 (nln/ewald_le2.c:35-43); tabulated
+ *                               form exact at the nodes, error O(h^2).
  *   orc_cell_record          -- parallel-axis identity / tracelessness.
  */
 #include "pgrav_oracle.h"
@@ -470,6 +475,146 @@ void orc_direct_ewald(const double *sink_pos3, int64_t m, const double *src_pos3
                       const double *src_mass, int64_t n, double L, double *out4, int nthreads) {
     dsum_job b = {sink_pos3, src_pos3, src_mass, 0, m, n, 0, 0.0, L, out4};
     run_threads(ewald_worker, b, nthreads);
+}
+
+/* Value and Jacobian of the Ewald correction at x = source - sink: the sums of
+ * nln/ewald.c:13-64 and their derivatives term by term.
+ * Real space, y = x - nL, rho = |y|: t1 = erfc(alpha rho), t2 = a1 exp(-alpha^2 rho^2),
+ *   g = t1/rho + t2, t = g/rho^2: f += t y, phi -= t1/rho;
+ *   dt1/drho = -t2, dt2/drho = -2 alpha^2 rho t2, so
+ *   g' = -t2/rho - t1/rho^2 - 2 alpha^2 rho t2, t' = g'/rho^2 - 2 g/rho^3 and
+ *   df_i/dx_j += t delta_ij + t' y_i y_j / rho.
+ * Fourier space, e = exp(a2 |h|^2): f += a4 e sin(a3 h.x)/|h|^2 h, phi -= a5 e
+ *   cos(a3 h.x)/|h|^2; df_i/dx_j += a3 a4 e cos(a3 h.x)/|h|^2 h_i h_j.
+ * Newtonian term removed: C = f - x/|x|^3, dC_i/dx_j = df_i/dx_j - delta_ij/|x|^3
+ *   + 3 x_i x_j/|x|^5; psi = phi + 1/|x|, dpsi/dx = f - x/|x|^3 = C.
+ * At x = 0 the n = 0 term minus the Newtonian one is replaced by its limit:
+ *   erf(alpha rho)/rho -> a1 in psi; (erfc(alpha rho) - 1)/rho^3 + t2/rho^2 ->
+ *   -(4/3) alpha^3/sqrt(pi) = -(2/3) alpha^2 a1 on the diagonal of dC/dx. */
+static void ewald_value_jac(const double x[3], double L, double v[4], double J[12]) {
+    const int N = 4;
+    const double alpha = 2.0 / L;
+    const double a1 = 2.0 * alpha / sqrt(M_PI);
+    const double a2 = -M_PI * M_PI / (alpha * alpha * L * L);
+    const double a3 = 2.0 * M_PI / L;
+    const double a4 = 2.0 / (L * L);
+    const double a5 = 1.0 / (M_PI * L);
+    const int origin = x[0] == 0.0 && x[1] == 0.0 && x[2] == 0.0;
+    double f[3] = {0, 0, 0}, D[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double phi = M_PI / (alpha * alpha * L * L * L);
+    for (int i = -N; i <= N; i++)
+        for (int j = -N; j <= N; j++)
+            for (int k = -N; k <= N; k++) {
+                if (origin && i == 0 && j == 0 && k == 0) {
+                    phi += a1;
+                    for (int a = 0; a < 3; a++) D[a][a] -= 2.0 / 3.0 * alpha * alpha * a1;
+                    continue;
+                }
+                const double y[3] = {x[0] - L * i, x[1] - L * j, x[2] - L * k};
+                const double rho2 = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+                const double rho = sqrt(rho2);
+                if (rho > 4.0 * L)
+                    continue;
+                const double t1 = erfc(alpha * rho);
+                const double t2 = a1 * exp(-alpha * alpha * rho2);
+                const double g = t1 / rho + t2;
+                const double t = g / rho2;
+                const double gp = -t2 / rho - t1 / rho2 - 2.0 * alpha * alpha * rho * t2;
+                const double tp = gp / rho2 - 2.0 * g / (rho2 * rho);
+                for (int a = 0; a < 3; a++) {
+                    f[a] += t * y[a];
+                    D[a][a] += t;
+                    for (int b = 0; b < 3; b++) D[a][b] += tp * y[a] * y[b] / rho;
+                }
+                phi -= t1 / rho;
+            }
+    for (int i = -N; i <= N; i++)
+        for (int j = -N; j <= N; j++)
+            for (int k = -N; k <= N; k++) {
+                const double hh = (double)(i * i + j * j + k * k);
+                if (hh > 16 || hh == 0)
+                    continue;
+                const double h[3] = {(double)i, (double)j, (double)k};
+                const double hx = i * x[0] + j * x[1] + k * x[2];
+                const double e = exp(a2 * hh);
+                const double s = sin(a3 * hx), c = cos(a3 * hx);
+                for (int a = 0; a < 3; a++) {
+                    f[a] += a4 * e * s / hh * h[a];
+                    for (int b = 0; b < 3; b++) D[a][b] += a3 * a4 * e * c / hh * h[a] * h[b];
+                }
+                phi -= a5 * e * c / hh;
+            }
+    if (!origin) {
+        const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+        const double r = sqrt(r2), r3 = r * r2, r5 = r3 * r2;
+        for (int a = 0; a < 3; a++) {
+            f[a] -= x[a] / r3;
+            D[a][a] -= 1.0 / r3;
+            for (int b = 0; b < 3; b++) D[a][b] += 3.0 * x[a] * x[b] / r5;
+        }
+        phi += 1.0 / r;
+    }
+    for (int a = 0; a < 3; a++) {
+        v[a] = f[a];
+        for (int b = 0; b < 3; b++) J[3 * a + b] = D[a][b];
+        J[9 + a] = f[a];
+    }
+    v[3] = phi;
+}
+
+typedef struct {
+    const double *r3;
+    int64_t i0, i1;
+    double L;
+    int nodes;
+    double *val4, *jac12;
+} ecorr_job;
+
+static void *ecorr_worker(void *arg) {
+    const ecorr_job *j = (const ecorr_job *)arg;
+    for (int64_t p = j->i0; p < j->i1; p++) {
+        const double *r = j->r3 + 3 * p;
+        double *v = j->val4 + 4 * p, *J = j->jac12 + 12 * p;
+        if (j->nodes <= 0) {
+            ewald_value_jac(r, j->L, v, J);
+            continue;
+        }
+        const double h = 0.5 * j->L / (j->nodes - 1);
+        double node[3], d[3];
+        for (int a = 0; a < 3; a++) {
+            double q = rint(r[a] / h);
+            if (q > j->nodes - 1)
+                q = j->nodes - 1;
+            if (q < -(j->nodes - 1))
+                q = -(j->nodes - 1);
+            node[a] = q * h;
+            d[a] = r[a] - node[a];
+        }
+        ewald_value_jac(node, j->L, v, J);
+        double step[4];
+        for (int a = 0; a < 4; a++) step[a] = J[3 * a] * d[0] + J[3 * a + 1] * d[1] + J[3 * a + 2] * d[2];
+        /* psi's second-order step, trace part: (1/2) tr(dC/dr) |d|^2 / 3 with
+         * tr(dC/dr) = -4 pi / L^3 (C = grad psi, div C = -4 pi / L^3 off the lattice) */
+        step[3] += -2.0 * M_PI / 3.0 / (j->L * j->L * j->L) * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        for (int a = 0; a < 4; a++) v[a] += step[a];
+    }
+    return NULL;
+}
+
+void orc_ewald_corr(const double *r3, int64_t n, double L, int nodes, double *val4,
+                    double *jac12, int nthreads) {
+    if (nthreads < 1)
+        nthreads = 1;
+    if (nthreads > 64)
+        nthreads = 64;
+    pthread_t th[64];
+    ecorr_job jobs[64];
+    for (int t = 0; t < nthreads; t++) {
+        ecorr_job j = {r3, n * t / nthreads, n * (t + 1) / nthreads, L, nodes, val4, jac12};
+        jobs[t] = j;
+        pthread_create(&th[t], NULL, ecorr_worker, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
 }
 
 /* nln/cofm.c:650-720 (mpole_add_mono), summed over bodies about `center` */

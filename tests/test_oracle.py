@@ -218,3 +218,108 @@ def test_ewald_constants_from_reference():
     f2, p2 = oracle.ewald(-y)
     np.testing.assert_allclose(f1, -f2, rtol=1e-12)
     np.testing.assert_allclose(p1, p2, rtol=1e-12)
+
+
+def test_ewald_corr_exact_form_pins():
+    """oracle.ewald_corr (nodes = 0), the checker of the periodic tree path:
+    * values == the lattice sum of nln/ewald.c (oracle.ewald) minus the
+      Newtonian term, at random points;
+    * the closed-form Jacobian == central differences of those values, and
+      dpsi/dr == C (psi is C's potential);
+    * at r = 0: psi(0) == 2 x 1.4186487533 (nln/ewald_le2.c:38, 43; the
+      reference's literal is the simple cubic constant 2.837297479 to 1e-8),
+      dC/dr == -(4 pi / 3) 1 (nln/ewald_le2.c:35-37), C == 0;
+    * odd C and even psi, and the box scaling C(L r) = C(r)/L^2."""
+    rng = np.random.default_rng(5)
+    r = rng.uniform(-0.5, 0.5, (64, 3))
+    val, jac = oracle.ewald_corr(r)
+    for i in range(16):
+        f, phi = oracle.ewald(r[i])
+        d = np.linalg.norm(r[i])
+        np.testing.assert_allclose(val[i, :3], f - r[i] / d ** 3, rtol=0, atol=1e-13)
+        np.testing.assert_allclose(val[i, 3], phi + 1 / d, rtol=0, atol=1e-13)
+    h = 1e-5
+    for b in range(3):
+        e = np.zeros(3)
+        e[b] = h
+        vp, _ = oracle.ewald_corr(r + e)
+        vm, _ = oracle.ewald_corr(r - e)
+        fd = (vp - vm) / (2 * h)
+        np.testing.assert_allclose(jac[:, :, b], fd, rtol=0, atol=1e-8 * np.abs(jac).max())
+    np.testing.assert_allclose(jac[:, 3, :], val[:, :3], rtol=0, atol=1e-13)
+    v0, j0 = oracle.ewald_corr(np.zeros((1, 3)))
+    np.testing.assert_allclose(v0[0, 3], 2 * 1.4186487533, rtol=2e-8)
+    np.testing.assert_allclose(v0[0, 3], 2.837297479, rtol=1e-9)
+    np.testing.assert_allclose(j0[0, :3, :], -4 * np.pi / 3 * np.eye(3), rtol=0, atol=1e-12)
+    assert np.abs(v0[0, :3]).max() < 1e-15
+    vn, jn = oracle.ewald_corr(-r)
+    np.testing.assert_allclose(vn[:, :3], -val[:, :3], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(vn[:, 3], val[:, 3], rtol=0, atol=1e-12)
+    v2, _ = oracle.ewald_corr(2.0 * r[:8], L=2.0)
+    np.testing.assert_allclose(v2[:, :3], val[:8, :3] / 4, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(v2[:, 3], val[:8, 3] / 2, rtol=0, atol=1e-12)
+
+
+def test_ewald_corr_table_form_pins():
+    """The tabulated form (nearest node + first-order step, psi with the trace
+    part of its second-order step): exact at the nodes;
+    its error against the exact sum is second order -- it shrinks ~4x when the
+    node spacing halves -- and, at the 64 nodes per half box the GPU tables
+    hold, stays below 3e-4 of max |C| over the box; inside one node cell the
+    step is linear in the offset."""
+    rng = np.random.default_rng(6)
+    r = rng.uniform(-0.5, 0.5, (400, 3))
+    exact, _ = oracle.ewald_corr(r)
+    errs = {}
+    for nodes in (17, 33, 65):
+        tab, _ = oracle.ewald_corr(r, nodes=nodes)
+        errs[nodes] = np.abs(tab - exact).max()
+    assert 3.0 < errs[17] / errs[33] < 5.0, errs
+    assert 3.0 < errs[33] / errs[65] < 5.0, errs
+    tab64, j64 = oracle.ewald_corr(r, nodes=64)
+    assert np.abs(tab64 - exact).max() < 3e-4 * np.abs(exact[:, :3]).max()
+    h = 0.5 / 63
+    nodes = np.rint(r / h) * h
+    vn, jn = oracle.ewald_corr(nodes)
+    tn, _ = oracle.ewald_corr(nodes, nodes=64)
+    np.testing.assert_allclose(tn, vn, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(j64, jn, rtol=0, atol=0)
+    d = r - nodes
+    step = np.einsum("nij,nj->ni", jn, d)
+    step[:, 3] -= 2 * np.pi / 3 * (d * d).sum(1)
+    np.testing.assert_allclose(tab64, vn + step, rtol=0, atol=1e-14)
+    # the trace term removes psi's systematic second-order bias: the mean error
+    # over the box is far below the mean |second-order term|
+    assert abs((tab64 - exact)[:, 3].mean()) < 0.05 * (2 * np.pi / 3 * (d * d).sum(1)).mean()
+
+
+@pytest.mark.parametrize("soft,eps", [(0, 0.0), (1, 0.05)])
+def test_direct_grad_matches_finite_differences_of_direct(soft, eps):
+    """oracle.direct_grad (the adjoint's FP64 checker) against central
+    differences of oracle.direct: L = sum_i w_i . a_i + v_i phi_i over every
+    body, each coordinate and mass nudged by h = 1e-6 (relative), unsoftened and
+    Plummer-softened."""
+    rng = np.random.default_rng(11 + soft)
+    n = 24
+    pos = rng.normal(size=(n, 3))
+    mass = rng.random(n) + 0.2
+    w4 = rng.normal(size=(n, 4))
+    g = oracle.direct_grad(pos, mass, w4, soft_type=soft, eps=eps)
+
+    def loss(p, m):
+        r = oracle.direct(p, p, m, soft_type=soft, eps=eps)
+        return (w4 * r).sum()
+
+    h = 1e-6
+    fd = np.zeros((n, 4))
+    for i in range(n):
+        for k in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[i, k] += h
+            pm[i, k] -= h
+            fd[i, k] = (loss(pp, mass) - loss(pm, mass)) / (2 * h)
+        mp, mm = mass.copy(), mass.copy()
+        mp[i] += h
+        mm[i] -= h
+        fd[i, 3] = (loss(pos, mp) - loss(pos, mm)) / (2 * h)
+    np.testing.assert_allclose(g, fd, rtol=0, atol=1e-6 * np.abs(fd).max())
