@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "../../include/pgrav.h"
 #include "pgrav_internal.h"
@@ -137,14 +138,22 @@ void span(const void *const *f, const int *len, int nf, int64_t n, int64_t strid
     *hi = b + (n - 1) * stride;
 }
 
+// scratch: null, or nb + 1 double2 of device memory the caller provides (the
+// workgroup partial sums, then the total); null allocates them for the call
 int run(bool cosmo, const Table &T0, int64_t n, const Step &S, bool host, hipStream_t st,
-        double *ke, double *pe) {
+        double *ke, double *pe, double2 *scratch = nullptr) {
     Table T = T0;
     char *din = nullptr, *dout = nullptr;
     double2 *partial = nullptr, *tot = nullptr;
     const int64_t nb = (n + 255) / 256;
-    bool ok = check(hipMalloc(&partial, sizeof(double2) * nb), "hipMalloc") &&
-              check(hipMalloc(&tot, sizeof(double2)), "hipMalloc");
+    bool ok = true;
+    if (scratch) {
+        partial = scratch;
+        tot = scratch + nb;
+    } else {
+        ok = check(hipMalloc(&partial, sizeof(double2) * nb), "hipMalloc") &&
+             check(hipMalloc(&tot, sizeof(double2)), "hipMalloc");
+    }
     const char *ilo = nullptr, *ihi = nullptr, *olo = nullptr, *ohi = nullptr;
     if (ok && host) {
         // copy the caller's tables (the spans their fields cover) to the device,
@@ -192,9 +201,13 @@ int run(bool cosmo, const Table &T0, int64_t n, const Step &S, bool host, hipStr
         ok = check(hipMemcpyAsync((void *)olo, dout, ohi - olo, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
     if (ok)
         ok = check(hipStreamSynchronize(st), "hipStreamSynchronize");
-    for (void *p : {(void *)din, (void *)(dout != din ? dout : nullptr), (void *)partial, (void *)tot})
+    for (void *p : {(void *)din, (void *)(dout != din ? dout : nullptr)})
         if (p)
             (void)hipFree(p);
+    if (!scratch)
+        for (void *p : {(void *)partial, (void *)tot})
+            if (p)
+                (void)hipFree(p);
     if (!ok)
         return -1;
     // nln/integrate.c:56-57: kep = 0.5 * ke, pep = 0.5 * pe
@@ -206,7 +219,7 @@ int run(bool cosmo, const Table &T0, int64_t n, const Step &S, bool host, hipStr
 int integrate_common(bool cosmo, const float *inmass, const float *inpos, const float *invel,
                      const float *inacc, const float *inphi, int instride, float *outpos,
                      float *outvel, int outstride, int n, const Step &S, bool host, void *stream,
-                     double *kep, double *pep) {
+                     double *kep, double *pep, double2 *scratch = nullptr) {
     clear_error();
     if (!inmass || !inpos || !invel || !inacc || !inphi || !outpos || !outvel || !kep || !pep ||
         n < 0 || instride <= 0 || outstride <= 0 || instride % 4 || outstride % 4) {
@@ -221,7 +234,38 @@ int integrate_common(bool cosmo, const float *inmass, const float *inpos, const 
     }
     Table T{(const char *)inmass, (const char *)inpos, (const char *)invel, (const char *)inacc,
             (const char *)inphi, (char *)outpos, (char *)outvel, instride, outstride};
-    return run(cosmo, T, n, S, host, (hipStream_t)stream, kep, pep);
+    return run(cosmo, T, n, S, host, (hipStream_t)stream, kep, pep, scratch);
+}
+
+// ---- the entry point that allocates nothing: its workgroup partial sums and
+// total live in this file-scope array, one copy per GPU, held until the process
+// ends.  Host code never passes the array's name to a kernel (on the host it is
+// only a stand-in for the symbol); it passes the device address that
+// hipGetSymbolAddress returns, looked up once per device.
+constexpr int64_t NOALLOC_MAX_BODIES = int64_t(1) << 27;  // 134,217,728 bodies
+constexpr int64_t NOALLOC_SLOTS = NOALLOC_MAX_BODIES / 256 + 1;  // 8 MiB + 16 B
+static __device__ __align__(256) double2 scratch[NOALLOC_SLOTS];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double2 *scratch_ptr[MAX_DEVICES];
+
+double2 *device_scratch() {
+    int dev = -1;
+    if (!check(hipGetDevice(&dev), "hipGetDevice"))
+        return nullptr;
+    if (dev < 0 || dev >= MAX_DEVICES) {
+        fail("pgrav_integrate_device_noalloc: device index out of range");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (!check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress"))
+            return nullptr;
+        scratch_ptr[dev] = (double2 *)p;
+    }
+    return scratch_ptr[dev];
 }
 
 }  // namespace
@@ -260,6 +304,33 @@ int pgrav_integrate_device(const float *inmass, const float *inpos, const float 
     Step S{*tpos - *tvel, dtv, dt, 1.0, 1.0, 1.0};
     if (integrate_common(false, inmass, inpos, invel, inacc, inphi, instride, outpos, outvel,
                          outstride, n, S, false, stream, kep, pep))
+        return -1;
+    *tvel += dtv;
+    *tpos += dt;
+    return 0;
+}
+
+int pgrav_integrate_device_noalloc(const float *inmass, const float *inpos, const float *invel,
+                                   const float *inacc, const float *inphi, int instride,
+                                   float *outpos, float *outvel, int outstride, int n, double dt,
+                                   double dtv, double *tpos, double *tvel, double *kep, double *pep,
+                                   void *stream) {
+    clear_error();
+    if (!tpos || !tvel) {
+        fail("pgrav_integrate_device_noalloc: null time");
+        return -1;
+    }
+    if ((int64_t)n > NOALLOC_MAX_BODIES) {
+        fail("pgrav_integrate_device_noalloc: n exceeds the 134217728 bodies its static scratch "
+             "holds (use pgrav_integrate_device)");
+        return -1;
+    }
+    double2 *ws = device_scratch();
+    if (!ws)
+        return -1;
+    Step S{*tpos - *tvel, dtv, dt, 1.0, 1.0, 1.0};
+    if (integrate_common(false, inmass, inpos, invel, inacc, inphi, instride, outpos, outvel,
+                         outstride, n, S, false, stream, kep, pep, ws))
         return -1;
     *tvel += dtv;
     *tpos += dt;

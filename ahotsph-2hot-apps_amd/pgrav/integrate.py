@@ -21,6 +21,8 @@ register({
     "pgrav_integrate": (_i, [_vp] * 5 + [_i, _vp, _vp, _i, _i, _d, _d, _dp, _dp, _dp, _dp]),
     "pgrav_integrate_device": (_i, [_vp] * 5 + [_i, _vp, _vp, _i, _i, _d, _d, _dp, _dp, _dp, _dp,
                                                 _vp]),
+    "pgrav_integrate_device_noalloc": (_i, [_vp] * 5 + [_i, _vp, _vp, _i, _i, _d, _d, _dp, _dp,
+                                                        _dp, _dp, _vp]),
     "pgrav_cosmo_integrate_device": (_i, [_vp] * 5 + [_i, _vp, _vp, _i, _i] + [_d] * 5 +
                                      [_dp, _dp, _vp]),
     "pgrav_convert_v_device": (_i, [_vp, _i, _i, _d, _i, _vp]),
@@ -54,16 +56,19 @@ def integrate(table, dt, dtv, tpos, tvel, out=None):
     return tp.value, tv.value, ke.value, pe.value
 
 
-def integrate_device(ptr, n, dtype, dt, dtv, tpos, tvel, out_ptr=None, out_dtype=None, stream=None):
-    """Integrate on a device table at `ptr` (n records of `dtype`)."""
+def integrate_device(ptr, n, dtype, dt, dtv, tpos, tvel, out_ptr=None, out_dtype=None, stream=None,
+                     noalloc=False):
+    """Integrate on a device table at `ptr` (n records of `dtype`).  noalloc:
+    through pgrav_integrate_device_noalloc (no device allocation; n <= 2^27)."""
     out_ptr = ptr if out_ptr is None else out_ptr
     out_dtype = dtype if out_dtype is None else out_dtype
     tp, tv, ke, pe = _d(tpos), _d(tvel), _d(), _d()
-    rc = lib().pgrav_integrate_device(*_ptrs(ptr, dtype, IN_FIELDS), dtype.itemsize,
-                                      *_ptrs(out_ptr, out_dtype, OUT_FIELDS), out_dtype.itemsize,
-                                      n, dt, dtv, ctypes.byref(tp), ctypes.byref(tv),
-                                      ctypes.byref(ke), ctypes.byref(pe), stream)
-    _done(rc, "pgrav_integrate_device")
+    name = "pgrav_integrate_device_noalloc" if noalloc else "pgrav_integrate_device"
+    rc = getattr(lib(), name)(*_ptrs(ptr, dtype, IN_FIELDS), dtype.itemsize,
+                              *_ptrs(out_ptr, out_dtype, OUT_FIELDS), out_dtype.itemsize,
+                              n, dt, dtv, ctypes.byref(tp), ctypes.byref(tv),
+                              ctypes.byref(ke), ctypes.byref(pe), stream)
+    _done(rc, name)
     return tp.value, tv.value, ke.value, pe.value
 
 

@@ -352,6 +352,18 @@ int pgrav_integrate_device(const float *inmass, const float *inpos, const float 
                            const float *inacc, const float *inphi, int instride, float *outpos,
                            float *outvel, int outstride, int n, double dt, double dtv, double *tpos,
                            double *tvel, double *kep, double *pep, void *stream);
+/* No counterpart in the reference.  pgrav_integrate_device for callers that
+ * cannot allocate (code under HIP graph capture, tight stepping loops, harnesses
+ * that forbid hipMalloc): the same arguments and results, but no hipMalloc or
+ * hipFree -- the per-workgroup energy partials and their total live in a static
+ * device array of the library (8 MiB per GPU, held until the process ends), so
+ * n is limited to 134,217,728 bodies (larger n is an error).  Calls that overlap
+ * on one GPU share that array and race: serialise them. */
+int pgrav_integrate_device_noalloc(const float *inmass, const float *inpos, const float *invel,
+                                   const float *inacc, const float *inphi, int instride,
+                                   float *outpos, float *outvel, int outstride, int n, double dt,
+                                   double dtv, double *tpos, double *tvel, double *kep, double *pep,
+                                   void *stream);
 /* nln/integrate.c:82-135 (CosmoIntegrate) on device pointers, with the
  * cosmology's factors passed in (the reference computes them from its
  * cosmology object, libcosmo, which is not part of this path):
