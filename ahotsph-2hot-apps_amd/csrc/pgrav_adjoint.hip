@@ -65,27 +65,6 @@ __device__ __forceinline__ D1 nrsq(D1 s) {
     return D1(f, -0.5f * f * f * f * s.d);
 }
 
-// ---------------------------------------------------------------- wave sums
-// The sum of a value over the 64 lanes of a wave by DPP row operations (VALU
-// only, no LDS): pairs and quads by quad permutes, 8 and 16 lanes by the half-
-// row and row mirrors, then rows 0+1 and 2+3 by the row-15 broadcast and all
-// four by the row-31 broadcast, which leaves the total in lane 63.  The lane
-// order of the additions is fixed, so the result is reproducible.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ float dpp_add(float v) {
-    const int t = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWS, 0xf, false);
-    return v + __builtin_bit_cast(float, t);
-}
-__device__ __forceinline__ float wave_sum(float v) {
-    v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
-    v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2, 3, 0, 1]
-    v = dpp_add<0x141, 0xf>(v);  // row_half_mirror
-    v = dpp_add<0x140, 0xf>(v);  // row_mirror
-    v = dpp_add<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
-    v = dpp_add<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
-    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
-}
-
 #define ADJ_CELL 52  // per work item / node: 46 dL/dP, 3 dL/dX (pair part), pad
 #define L_BODY_KIND 3  // list kinds 0..2 are cells (orders 4, 2, 0), 3 bodies
 

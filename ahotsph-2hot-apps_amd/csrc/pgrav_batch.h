@@ -4,7 +4,6 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define EWALD_N 64  // Ewald correction table points per axis over [0, L/2]
 #define NXCD 8      // MI355X accelerator complex dies (one L2 each)
 
 namespace pgrav {
@@ -39,10 +38,10 @@ struct BatchView {
     float4 *out32 = nullptr;
     double4 *out64 = nullptr;
     int *counter = nullptr;
-    const float4 *ewald32 = nullptr;
-    const double4 *ewald64 = nullptr;  // FP64 table (direct-sum check)
+    const float *ewald32 = nullptr;   // Ewald correction table (pgrav_ewald.h), FP32 copy
+    const double *ewald64 = nullptr;  // FP64
     int unroll = 4;  // order-0 / body loop unroll of the rel32c kernels: 2, 4 or 8
-    int ewgroup = 0;  // periodic 1: cells' Ewald corrections expanded about the group centre
+    double ewratio = 0.05;  // periodic 1: see pgrav_params.ewald_ratio
     int ngroups = 0;
     int soft = 0;
     double eps = 0;
@@ -55,7 +54,7 @@ struct BatchView {
 hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, hipStream_t st);
 hipError_t launch_group64(const BatchView &v, int g, hipStream_t st);
 hipError_t launch_reset_counter(int *c, hipStream_t st);
-hipError_t launch_ewald_table(double L, double4 *t64, float4 *t32, hipStream_t st);
+hipError_t launch_ewald_table(double L, double *t64, float *t32, hipStream_t st);
 // raw moment a (nln/cofm.c:660-720 order): its exponents e[a][q] and the index
 // of the moment with e[a][q] - 1 (when e[a][q] > 0)
 struct RawIndex {
@@ -105,6 +104,6 @@ hipError_t launch_gather4(const void *src, int src_fp64, const int64_t *perm, in
 // ewald64 non-null: periodic (nearest image + Ewald correction, box edge `box`)
 hipError_t launch_direct64(const double4 *bpos, int64_t n, const int64_t *sample, int ns, int soft,
                            double eps, double G, double4 *part, int nsplit, double4 *out,
-                           const double4 *ewald64, double box, hipStream_t st);
+                           const double *ewald64, double box, hipStream_t st);
 
 }  // namespace pgrav

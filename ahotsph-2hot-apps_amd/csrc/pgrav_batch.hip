@@ -26,6 +26,7 @@
 
 #include "pgrav_batch.h"
 #include "pgrav_device.h"
+#include "pgrav_ewald.h"
 
 namespace pgrav {
 
@@ -52,12 +53,12 @@ struct BatchArgs {
     const uint32_t *idx;
     void *out;  // float4* or double4*, sorted body order
     int *counter;
-    const float4 *ewald;  // 64^3 correction table (periodic)
-    const double4 *ewald64;  // its FP64 original, read by the FP64 kernels
+    const float *ewald;  // Ewald correction table (periodic = 1, pgrav_ewald.h)
+    const double *ewald64;  // its FP64 original, read by the FP64 kernels
     const uint8_t *img;  // explicit images: the image of each entry, parallel to idx (or null)
     int img_f;           // 2 nimage + 1 images per side
     int unroll;          // order-0 / body loop unroll of the rel32c kernels: 2, 4 or 8
-    int ewgroup;         // periodic 1: cells' Ewald corrections expanded about the group centre
+    double ewratio;      // periodic 1: expansion about the group centre when radius <= ewratio x distance
     int ngroups;
     int soft;
     double eps;
@@ -73,121 +74,52 @@ __device__ __forceinline__ T pbc(T x, T box, T ibox) {
     return x;
 }
 
-// ---- Ewald correction (periodic).  The table holds, at the nodes of a
-// EWALD_N^3 grid over the positive octant [0, L/2]^3, the correction
-// T = (C(r), psi(r)) -- C = f_ewald(r) - r/|r|^3, psi = phi_ewald(r) + 1/|r| --
-// and its three partial derivatives (4 x TV per node, built in FP64 by
-// k_ewald_table).  The lookup takes the nearest node and its first-order
-// Taylor step: one 64-byte record, 12 FMAs, error O(h^2 |T''|) as trilinear
-// interpolation over 8 nodes would have.  x = sink - source, already the
-// nearest image (|x_i| <= L/2), so the Newtonian term the caller added and
-// this correction refer to the same image.
-// The potential also takes the trace part of the second-order step: psi's
-// Hessian is C's Jacobian, whose trace is -4 pi / L^3 everywhere off the lattice
-// points, so (1/2) d.H.d = -(2 pi / 3) |d|^2 / L^3 plus a traceless rest two
-// orders smaller; the forces' second-order term has no trace part (div grad C =
-// grad div C = 0).
-template <typename T>
-__device__ __forceinline__ T psi_trace_coef(T box) {
-    return T(-2.0 * M_PI / 3.0) / (box * box * box);
-}
-
-template <typename T, typename TV>
-__device__ __forceinline__ void ewald_corr(const TV *__restrict__ tab, T m, T x, T y, T z,
-                                           T box, Acc4T<T> &a) {
-    const T h = T(0.5) * box / T(EWALD_N - 1);
-    const T inv_h = T(1) / h;
-    const T sx = x < 0 ? T(-1) : T(1), sy = y < 0 ? T(-1) : T(1), sz = z < 0 ? T(-1) : T(1);
-    const T ax = fabs(x), ay = fabs(y), az = fabs(z);
-    // clamp both ways: a NaN offset must not turn into an out-of-range index
-    const int i = max(0, min((int)rint(ax * inv_h), EWALD_N - 1));
-    const int j = max(0, min((int)rint(ay * inv_h), EWALD_N - 1));
-    const int k = max(0, min((int)rint(az * inv_h), EWALD_N - 1));
-    const T du = ax - T(i) * h, dv = ay - T(j) * h, dw = az - T(k) * h;
-    const TV *p = tab + 4 * (((size_t)i * EWALD_N + j) * EWALD_N + k);
-    const TV t0 = p[0], gu = p[1], gv = p[2], gw = p[3];
-    const T cx = T(t0.x) + du * T(gu.x) + dv * T(gv.x) + dw * T(gw.x);
-    const T cy = T(t0.y) + du * T(gu.y) + dv * T(gv.y) + dw * T(gw.y);
-    const T cz = T(t0.z) + du * T(gu.z) + dv * T(gv.z) + dw * T(gw.z);
-    const T cp = T(t0.w) + du * T(gu.w) + dv * T(gv.w) + dw * T(gw.w) +
-                 psi_trace_coef(box) * (du * du + dv * dv + dw * dw);
-    // C is odd per axis; the pull on the sink is m C(r) at r = source - sink = -x
-    a.x -= m * sx * cx;
-    a.y -= m * sy * cy;
-    a.z -= m * sz * cz;
-    a.w += m * cp;
-}
-
-// The same lookup at the group centre for a cell list entry, as a first-order
-// expansion in the sink's offset s from that centre: out[0] = (C, psi) at x0 =
-// centre - source, out[1..3] = row i of the Jacobian d(C_i, psi)/dx:
-// (dC_i/dx, dC_i/dy, dC_i/dz, dpsi/dx_i).  With the table's octant symmetry (C_i
-// odd in x_i, even in the others; psi even) d(sgn_i C_i)/dx_j = sgn_i sgn_j dC_i/du_j.
-// A sink then takes C(x0 + s) = C + J s: 12 FMAs on values the tile loader left in
-// LDS, instead of a 64-byte gather per pair.  Cells are accepted only well inside
-// half a box, so every sink of the group sees the cell in the same image as the
-// centre.
-template <typename T, typename TV>
-__device__ __forceinline__ void ewald_taylor(const TV *__restrict__ tab, T x, T y, T z, T box,
-                                             V4<T> *out) {
-    const T h = T(0.5) * box / T(EWALD_N - 1);
-    const T inv_h = T(1) / h;
-    const T sx = x < 0 ? T(-1) : T(1), sy = y < 0 ? T(-1) : T(1), sz = z < 0 ? T(-1) : T(1);
-    const T ax = fabs(x), ay = fabs(y), az = fabs(z);
-    const int i = max(0, min((int)rint(ax * inv_h), EWALD_N - 1));
-    const int j = max(0, min((int)rint(ay * inv_h), EWALD_N - 1));
-    const int k = max(0, min((int)rint(az * inv_h), EWALD_N - 1));
-    const T du = ax - T(i) * h, dv = ay - T(j) * h, dw = az - T(k) * h;
-    const TV *p = tab + 4 * (((size_t)i * EWALD_N + j) * EWALD_N + k);
-    const TV t0 = p[0], gu = p[1], gv = p[2], gw = p[3];
-    V4<T> c;
-    c.x = sx * (T(t0.x) + du * T(gu.x) + dv * T(gv.x) + dw * T(gw.x));
-    c.y = sy * (T(t0.y) + du * T(gu.y) + dv * T(gv.y) + dw * T(gw.y));
-    c.z = sz * (T(t0.z) + du * T(gu.z) + dv * T(gv.z) + dw * T(gw.z));
-    c.w = T(t0.w) + du * T(gu.w) + dv * T(gv.w) + dw * T(gw.w) +
-          psi_trace_coef(box) * (du * du + dv * dv + dw * dw);
-    out[0] = c;
-    V4<T> r;
-    r.x = T(gu.x);
-    r.y = sx * sy * T(gv.x);
-    r.z = sx * sz * T(gw.x);
-    r.w = sx * T(gu.w);
-    out[1] = r;
-    r.x = sy * sx * T(gu.y);
-    r.y = T(gv.y);
-    r.z = sy * sz * T(gw.y);
-    r.w = sy * T(gv.w);
-    out[2] = r;
-    r.x = sz * sx * T(gu.z);
-    r.y = sz * sy * T(gv.z);
-    r.z = T(gw.z);
-    r.w = sz * T(gw.w);
-    out[3] = r;
-}
-
-// A cell's Ewald correction for one sink from the group-centre expansion in LDS:
-// the pull is m C at x = sink - source (as ewald_corr applies it).  s2q is the
-// sink's own second-order potential term, psi_trace_coef |s|^2.
-template <typename T>
-__device__ __forceinline__ void ewald_apply(const V4<T> *e, T m, T sx, T sy, T sz, T s2q,
-                                            Acc4T<T> &a) {
-    const V4<T> c = e[0], jx = e[1], jy = e[2], jz = e[3];
-    const T cx = c.x + jx.x * sx + jx.y * sy + jx.z * sz;
-    const T cy = c.y + jy.x * sx + jy.y * sy + jy.z * sz;
-    const T cz = c.z + jz.x * sx + jz.y * sy + jz.z * sz;
-    const T cp = c.w + jx.w * sx + jy.w * sy + jz.w * sz + s2q;
-    a.x -= m * cx;
-    a.y -= m * cy;
-    a.z -= m * cz;
-    a.w += m * cp;
-}
-
 // the table a kernel reads: FP32 arithmetic its FP32 copy, FP64 the FP64 original
-__device__ __forceinline__ const float4 *ewald_table(const BatchArgs &A, float) {
+__device__ __forceinline__ const float *ewald_table(const BatchArgs &A, float) {
     return A.ewald;
 }
-__device__ __forceinline__ const double4 *ewald_table(const BatchArgs &A, double) {
+__device__ __forceinline__ const double *ewald_table(const BatchArgs &A, double) {
     return A.ewald64;
+}
+
+// Periodic (nearest image + Ewald correction) per group, in LDS: which entries
+// of the current tile take the correction per pair, and the group's expansion
+// of the correction (20 sums, below) accumulated over the list's tiles.
+struct EwShared {
+    int pair[64];
+    double acc[20];
+};
+
+// The correction summed over a group's expanded sources (group radius small
+// against the distance to their nearest other images, no sink seeing another
+// image than the centre does), as a field of the sink's offset s from the
+// group centre: with r0 each source's offset from the centre and m its mass,
+//   P = sum m psi(r0), A = sum m C(r0), B = sum m J(r0), Tm = sum m T(r0)
+// (a cell adds its traceless second moment's coupling, (1/2) J:Q to P and
+// (1/2) T:Q to A), the sink at s = r0 - r takes
+//   C(s) = A - B s + Tm s s / 2,  psi(s) = P - A.s + s.B.s / 2 - Tm s s s / 6.
+// acc: [0] P, [1..3] A, [4..9] B (xx xy xz yy yz zz), [10..19] Tm.
+template <typename T>
+__device__ __forceinline__ void ewald_expansion_apply(const double *acc, T sx, T sy, T sz,
+                                                      Acc4T<T> &a) {
+    T t[10];
+#pragma unroll
+    for (int q = 0; q < 10; q++) t[q] = T(acc[10 + q]);
+    const T bxx = T(acc[4]), bxy = T(acc[5]), bxz = T(acc[6]), byy = T(acc[7]), byz = T(acc[8]),
+            bzz = T(acc[9]);
+    const T ax = T(acc[1]), ay = T(acc[2]), az = T(acc[3]);
+    const T txx = t[0] * sx + t[1] * sy + t[2] * sz, txy = t[1] * sx + t[3] * sy + t[4] * sz,
+            txz = t[2] * sx + t[4] * sy + t[5] * sz, tyy = t[3] * sx + t[6] * sy + t[7] * sz,
+            tyz = t[4] * sx + t[7] * sy + t[8] * sz, tzz = t[5] * sx + t[8] * sy + t[9] * sz;
+    const T ux = txx * sx + txy * sy + txz * sz, uy = txy * sx + tyy * sy + tyz * sz,
+            uz = txz * sx + tyz * sy + tzz * sz;  // Tm s s
+    const T vx = bxx * sx + bxy * sy + bxz * sz, vy = bxy * sx + byy * sy + byz * sz,
+            vz = bxz * sx + byz * sy + bzz * sz;  // B s
+    a.x += ax - vx + T(0.5) * ux;
+    a.y += ay - vy + T(0.5) * uy;
+    a.z += az - vz + T(0.5) * uz;
+    a.w += T(acc[0]) - (ax * sx + ay * sy + az * sz) + T(0.5) * (vx * sx + vy * sy + vz * sz) -
+           T(1.0 / 6.0) * (ux * sx + uy * sy + uz * sz);
 }
 
 template <typename T>
@@ -227,17 +159,69 @@ __device__ __forceinline__ V4<T> source_offset(double4 c, double4 gc, double box
     return V4<T>{T(dx), T(dy), T(dz), T(c.w)};
 }
 
+// The Ewald correction of one tile entry (periodic = 1), in the tile loader:
+// either the entry joins the group's expansion -- its (psi, C, J, T) at r0 =
+// source - centre, times its mass, plus a cell's second-moment coupling -- or
+// it is marked for per-pair lookups.  Every lane of the wave calls this (the
+// sums are wave reductions); `on` says whether this lane holds an entry.
+template <typename T>
+__device__ __forceinline__ void ewald_tile_entry(const BatchArgs &A, EwShared *ew, bool on,
+                                                 int lane, double4 src, double4 gc, bool cell,
+                                                 const T *rec) {
+    T v[20];
+#pragma unroll
+    for (int q = 0; q < 20; q++) v[q] = T(0);
+    if (on) {
+        double dx = src.x - gc.x, dy = src.y - gc.y, dz = src.z - gc.z;
+        dx -= A.box * rint(dx / A.box);
+        dy -= A.box * rint(dy / A.box);
+        dz -= A.box * rint(dz / A.box);
+        const double am = fmax(fabs(dx), fmax(fabs(dy), fabs(dz)));
+        // the expansion holds when every sink sees the centre's image of the source
+        // and the source's other images are far against the group radius
+        const bool expand = am < 0.5 * A.box - gc.w && gc.w <= A.ewratio * (A.box - am);
+        ew->pair[lane] = expand ? 0 : 1;
+        if (expand) {
+            EwField<T> F;
+            ewald_field<T>(ewald_table(A, T(0)), T(dx), T(dy), T(dz), T(A.box), F);
+            const T m = T(src.w);
+            v[0] = m * F.psi;
+#pragma unroll
+            for (int q = 0; q < 3; q++) v[1 + q] = m * F.c[q];
+#pragma unroll
+            for (int q = 0; q < 6; q++) v[4 + q] = m * F.j[q];
+#pragma unroll
+            for (int q = 0; q < 10; q++) v[10 + q] = m * F.t[q];
+            if (cell) {
+                // traceless second moment Q = sum m (d d - |d|^2/3) from the prepared
+                // record (1.5 Q, order xx xy xz yy yz zz)
+                const T k = T(2.0 / 3.0);
+                const T qxx = k * rec[4], qxy = k * rec[5], qxz = k * rec[6], qyy = k * rec[7],
+                        qyz = k * rec[8], qzz = k * rec[9];
+                const T h = T(0.5), two = T(2);
+                const T *j = F.j, *t = F.t;
+                v[0] += h * (j[0] * qxx + j[3] * qyy + j[5] * qzz +
+                             two * (j[1] * qxy + j[2] * qxz + j[4] * qyz));
+                v[1] += h * (t[0] * qxx + t[3] * qyy + t[5] * qzz + two * (t[1] * qxy + t[2] * qxz + t[4] * qyz));
+                v[2] += h * (t[1] * qxx + t[6] * qyy + t[8] * qzz + two * (t[3] * qxy + t[4] * qxz + t[7] * qyz));
+                v[3] += h * (t[2] * qxx + t[7] * qyy + t[9] * qzz + two * (t[4] * qxy + t[5] * qxz + t[8] * qyz));
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 20; q++) {
+        const T sum = wave_sum(v[q]);
+        if (lane == 0)
+            ew->acc[q] += (double)sum;
+    }
+}
+
 // PER: 0 isolated, 1 nearest image + Ewald correction, 2 explicit images
 // BU: unroll factor of the order-0 and body loops (2, 4 or 8; the autotuner picks it
 // for the group-relative compensated FP32 kernels, the others use 4)
 template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU>
-__device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds, V4<T> *ewl) {
+__device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds, EwShared *ew) {
     constexpr bool PERIODIC = PER == 1;
-    // group-relative periodic with A.ewgroup: the cells' Ewald corrections come
-    // expanded about the group centre (ewl: 4 V4 per tile entry, then one flag per
-    // entry saying the expansion is used and holds for every sink); bodies keep the
-    // per-pair lookup
-    constexpr bool EW_GROUP = PERIODIC && REL;
     const int lane = threadIdx.x;
     const int cnt = A.gcount[g];
     const double4 gc = A.gcen[g];
@@ -269,7 +253,11 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
     // periodic: this lane's share of the listed cells' second-moment traces (the
     // potential's second-order Ewald term, the same for every sink of the group)
     double trace = 0.0;
-    const T s2q = EW_GROUP ? psi_trace_coef(box) * (sx * sx + sy * sy + sz * sz) : T(0);
+    if (PERIODIC) {
+        if (lane < 20)
+            ew->acc[lane] = 0.0;
+        __syncthreads();
+    }
 
 #pragma unroll 1
     for (int kind = 0; kind < 4; kind++) {
@@ -278,9 +266,11 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
         const int64_t o0 = offk[g], o1 = offk[g + 1];
         for (int64_t t0 = o0; t0 < o1; t0 += 64) {
             const int nt = (int)min((int64_t)64, o1 - t0);
+            double4 src = make_double4(0, 0, 0, 0);
+            uint32_t e = 0;
             if (lane < nt) {
-                const uint32_t e = idx[t0 + lane];
-                double4 src = kind == 3 ? A.bpos[e] : A.cpos[e];
+                e = idx[t0 + lane];
+                src = kind == 3 ? A.bpos[e] : A.cpos[e];
                 if (PER == 2) {
                     // image (i, j, k) in nln/mac.c:91-111 order: the source sits at
                     // x + (i, j, k) box, added in FP64
@@ -292,47 +282,33 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                 lds[lane] = source_offset<T, REL, PERIODIC>(src, gc, A.box);
                 if (PERIODIC && kind < 3)
                     trace += (double)mom[(size_t)e * PREC4 + PREC_TRACE];
-                if (EW_GROUP && kind < 3 && A.ewgroup) {
-                    const V4<T> o = lds[lane];
-                    const T x0 = pbc<PERIODIC>(-o.x, box, ibox), y0 = pbc<PERIODIC>(-o.y, box, ibox),
-                            z0 = pbc<PERIODIC>(-o.z, box, ibox);
-                    ewald_taylor<T>(ewald_table(A, T(0)), x0, y0, z0, box, ewl + 4 * lane);
-                    // the expansion holds only if no sink of the group sees another image
-                    // of the cell: every sink within the group radius of the centre stays
-                    // on the centre's side of the half-box boundary in each axis
-                    const T lim = T(0.5) * box - T(gc.w);
-                    ewl[4 * 64 + lane].x = (fabs(x0) < lim && fabs(y0) < lim && fabs(z0) < lim) ? T(1) : T(0);
-                } else if (EW_GROUP) {
-                    ewl[4 * 64 + lane].x = T(0);  // per-pair lookups for this tile
-                }
             }
+            if (PERIODIC)
+                ewald_tile_entry<T>(A, ew, lane < nt, lane, src, gc, kind < 3,
+                                    mom + (size_t)e * PREC4);
             __syncthreads();
             Acc4T<T> p = {0, 0, 0, 0};
             if (kind == 0) {
                 for (int k = 0; k < nt; k++) {
-                    const uint32_t e = idx[t0 + k];
+                    const uint32_t ek = idx[t0 + k];
                     const V4<T> r = lds[k];
                     const T x = pbc<PERIODIC>(sx - r.x, box, ibox);
                     const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
                     const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
-                    cell_interact<4, false, 1, T>(mom + (size_t)e * PREC4, r.w, x, y, z, eps2_cell, p);
-                    if (EW_GROUP && ewl[4 * 64 + k].x != T(0))
-                        ewald_apply<T>(ewl + 4 * k, r.w, sx, sy, sz, s2q, p);
-                    else if (PERIODIC)
-                        ewald_corr<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
+                    cell_interact<4, false, 1, T>(mom + (size_t)ek * PREC4, r.w, x, y, z, eps2_cell, p);
+                    if (PERIODIC && ew->pair[k])
+                        ewald_pair<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
                 }
             } else if (kind == 1) {
                 for (int k = 0; k < nt; k++) {
-                    const uint32_t e = idx[t0 + k];
+                    const uint32_t ek = idx[t0 + k];
                     const V4<T> r = lds[k];
                     const T x = pbc<PERIODIC>(sx - r.x, box, ibox);
                     const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
                     const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
-                    cell_interact<2, false, 1, T>(mom + (size_t)e * PREC4, r.w, x, y, z, eps2_cell, p);
-                    if (EW_GROUP && ewl[4 * 64 + k].x != T(0))
-                        ewald_apply<T>(ewl + 4 * k, r.w, sx, sy, sz, s2q, p);
-                    else if (PERIODIC)
-                        ewald_corr<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
+                    cell_interact<2, false, 1, T>(mom + (size_t)ek * PREC4, r.w, x, y, z, eps2_cell, p);
+                    if (PERIODIC && ew->pair[k])
+                        ewald_pair<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
                 }
             } else if (kind == 2) {
 #pragma unroll BU
@@ -342,10 +318,8 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                     const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
                     const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
                     mono_interact<T>(r.w, x, y, z, eps2_cell, p);
-                    if (EW_GROUP && ewl[4 * 64 + k].x != T(0))
-                        ewald_apply<T>(ewl + 4 * k, r.w, sx, sy, sz, s2q, p);
-                    else if (PERIODIC)
-                        ewald_corr<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
+                    if (PERIODIC && ew->pair[k])
+                        ewald_pair<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
                 }
             } else {
                 // bodies: ~18 VALU per pair, so unroll to keep several LDS reads in flight
@@ -364,10 +338,10 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                         mono_interact<T>(self ? T(0) : r.w, x, y, z, self ? T(1) : T(0), p);
                     }
                     // the self pair too: at r = 0 the correction is its limit, C = 0 and
-                    // psi(0) = 2.8373/L (the table's origin node), the body's potential
-                    // from its own images and the background
-                    if (PERIODIC)
-                        ewald_corr<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
+                    // psi(0) = 2.8373/L, the body's potential from its own images and the
+                    // background
+                    if (PERIODIC && ew->pair[k])
+                        ewald_pair<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
                 }
             }
             __syncthreads();
@@ -375,14 +349,19 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
         }
     }
     if (PERIODIC) {
+        Acc4T<T> p = {0, 0, 0, 0};
+        // the expanded sources' correction at this sink's offset from the centre
+        ewald_expansion_apply<T>(ew->acc, T(sp.x - gc.x), T(sp.y - gc.y), T(sp.z - gc.z), p);
         // the cells' corrections were taken at their centres of mass; the second-
         // order remainder of the potential is (1/2) H : M2 summed over the cells,
         // with H the correction's Hessian (trace -4 pi / L^3 everywhere) and M2 the
         // cell's second moments: its trace part, -(2 pi / 3) / L^3 sum m |x - cofm|^2,
-        // is added here (the traceless part is O(1e-2) of it)
-        for (int k = 32; k >= 1; k >>= 1) trace += __shfl_xor(trace, k, 64);
+        // is added here, the traceless part with the expansion (or left out for the
+        // cells taken per pair)
+        const double tr = wave_sum(trace);
         const double b3 = A.box * A.box * A.box;
-        fold<T>(s, c, Acc4T<T>{T(0), T(0), T(0), T(-2.0 * M_PI / 3.0 / b3 * trace)}, COMP);
+        p.w += T(-2.0 * M_PI / 3.0 / b3 * tr);
+        fold<T>(s, c, p, COMP);
     }
     if (live) {
         const T Gn = T(A.G);  // nln/mac.c:269-277: results scaled by G at the end
@@ -391,15 +370,18 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
     }
 }
 
-// Minimum waves per SIMD asked of the register allocator: left to it, the FP32
-// periodic kernels keep the Ewald records in flight with ~80 VGPRs (6 waves);
-// asked for 8 they fit in 64 (the K1 persistent variant spills 4 registers).
+// Minimum waves per SIMD asked of the register allocator (FP32 kernels).  The
+// periodic kernels' tile loader evaluates the Ewald table (20 values, a Taylor
+// step, the symmetry) beside the pair loop's live state.
 #ifndef PGRAV_MINWAVES_FP32
 #define PGRAV_MINWAVES_FP32 1  // the FP32 kernels without the Ewald table
 #endif
+#ifndef PGRAV_MINWAVES_PER1
+#define PGRAV_MINWAVES_PER1 6  // periodic 1: 80 VGPRs, no spills
+#endif
 template <typename T, int PER>
 struct MinWaves {
-    static constexpr int value = (sizeof(T) == 4 && PER == 1) ? 8 : (sizeof(T) == 4 ? PGRAV_MINWAVES_FP32 : 1);
+    static constexpr int value = sizeof(T) == 4 ? (PER == 1 ? PGRAV_MINWAVES_PER1 : PGRAV_MINWAVES_FP32) : 1;
 };
 
 // one workgroup per group, groups taken in longest-first order
@@ -407,8 +389,8 @@ template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PER>::value)))
 void k_batch(BatchArgs A) {
     __shared__ V4<T> lds[64];
-    __shared__ V4<T> ewl[PER == 1 && REL ? 5 * 64 : 1];
-    eval_group<T, REL, COMP, SOFT, PER, BU>(A, A.order_np[blockIdx.x], lds, ewl);
+    __shared__ EwShared ew;
+    eval_group<T, REL, COMP, SOFT, PER, BU>(A, A.order_np[blockIdx.x], lds, &ew);
 }
 
 // persistent: a fixed grid drains per-XCD work counters.  Workgroup b runs on
@@ -432,7 +414,7 @@ void k_batch_persistent(
     const T *__restrict__ mom, const int64_t *__restrict__ gfirst, const int *__restrict__ gcount,
     const double4 *__restrict__ gcen, const int *__restrict__ lpt, const int *__restrict__ xoff,
     const int64_t *__restrict__ off, const uint32_t *__restrict__ idx, V4<T> *__restrict__ out,
-    int *__restrict__ counter, const float4 *__restrict__ ewald) {
+    int *__restrict__ counter, const float *__restrict__ ewald) {
     A.bpos = bpos;
     A.cpos = cpos;
     A.mom = mom;
@@ -447,7 +429,7 @@ void k_batch_persistent(
     A.counter = counter;
     A.ewald = ewald;
     __shared__ V4<T> lds[64];
-    __shared__ V4<T> ewl[PER == 1 && REL ? 5 * 64 : 1];
+    __shared__ EwShared ew;
     __shared__ int w;
     const int x = blockIdx.x % NXCD;
     // own slice first; once it is drained, the next group of the other XCDs' slices
@@ -474,7 +456,7 @@ void k_batch_persistent(
         __syncthreads();
         if (item < 0)
             break;
-        eval_group<T, REL, COMP, SOFT, PER, BU>(A, A.lpt[item], lds, ewl);
+        eval_group<T, REL, COMP, SOFT, PER, BU>(A, A.lpt[item], lds, &ew);
     }
     if (threadIdx.x == 0) {
         const int done = atomicAdd(&A.counter[1], 1);
@@ -489,7 +471,8 @@ void k_batch_persistent(
 template <int SOFT, int PER>
 __global__ __launch_bounds__(64) void k_group64(BatchArgs A, int g) {
     __shared__ V4<double> lds[64];
-    eval_group<double, false, false, SOFT, PER, 4>(A, g, lds, nullptr);
+    __shared__ EwShared ew;
+    eval_group<double, false, false, SOFT, PER, 4>(A, g, lds, &ew);
 }
 
 __global__ void k_reset_counter(int *c) {
@@ -500,14 +483,14 @@ __global__ void k_reset_counter(int *c) {
 // ---------------------------------------------------------------- FP64 direct sum
 // Sampled FP64 direct sum (pattern of nln/grav_n2.c:162-173): grid.y splits the
 // sources; partial sums go to part[split][sink] and are reduced in fixed order.
-// Periodic: nearest image of every source plus the Ewald correction from the
-// FP64 table (the tree kernels read its FP32 copy), so the difference to the
-// tree result is the tree's own approximation error.
+// Periodic: nearest image of every source plus the Ewald correction per pair
+// from the FP64 table, so the difference to the tree result is the tree's own
+// approximation error (and its expansion of the correction about the groups).
 template <int SOFT, bool PERIODIC>
 __global__ __launch_bounds__(64) void k_direct64(const double4 *__restrict__ bpos, int64_t n,
                                                  const int64_t *__restrict__ sample, int ns,
                                                  double eps, double4 *__restrict__ part,
-                                                 int64_t chunk, const double4 *__restrict__ tab,
+                                                 int64_t chunk, const double *__restrict__ tab,
                                                  double box) {
     __shared__ double4 lds[64];
     const int s = blockIdx.x * 64 + threadIdx.x;
@@ -535,7 +518,7 @@ __global__ __launch_bounds__(64) void k_direct64(const double4 *__restrict__ bpo
                 mono_interact<double>(self ? 0.0 : q.w, x, y, z, self ? 1.0 : 0.0, a);
             }
             if (PERIODIC)  // the self pair included: C(0) = 0, psi(0) = 2.8373/L
-                ewald_corr<double>(tab, q.w, x, y, z, box, a);
+                ewald_pair<double>(tab, q.w, x, y, z, box, a);
         }
         __syncthreads();
     }
@@ -560,84 +543,138 @@ __global__ void k_direct_reduce(const double4 *__restrict__ part, int ns, int ns
 }
 
 // ---------------------------------------------------------------- Ewald table
-// FP64 restatement of nln/ewald.c:13-64 at one point r: C(r) = f_ewald(r) - r/|r|^3
-// and psi(r) = phi_ewald(r) + 1/|r| (at r = 0 the limits 0 and 2 x 1.4186487533 / L).
-__device__ double4 ewald_point(double x0, double x1, double x2, double L) {
-    const double x[3] = {x0, x1, x2};
-    double f[3] = {0, 0, 0}, phi;
+// The correction at one point x in closed form, FP64: the lattice sums of
+// nln/ewald.c:13-64 (alpha = 2/L, |n| <= 4 within 4L, |h|^2 <= 16) and their
+// derivatives term by term, with the nearest Newtonian term taken out.
+// out = psi, C (3), J (xx xy xz yy yz zz), T (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz).
+// Real space, y = x - nL, rho = |y|, t1 = erfc(alpha rho), t2 = a1 exp(-alpha^2 rho^2):
+//   g = t1/rho + t2, g' = -t2/rho - t1/rho^2 - 2 alpha^2 rho t2,
+//   g'' = 2 t2/rho^2 + 2 t1/rho^3 + 4 alpha^4 rho^2 t2, t = g/rho^2,
+//   t' = g'/rho^2 - 2 g/rho^3, t'' = g''/rho^2 - 4 g'/rho^3 + 6 g/rho^4,
+//   u = t'/rho, u' = t''/rho - t'/rho^2:
+//   C_a += t y_a, J_ab += t d_ab + u y_a y_b,
+//   T_abc += u (d_ab y_c + d_ac y_b + d_bc y_a) + (u'/rho) y_a y_b y_c, psi -= t1/rho.
+// Fourier space, e = exp(a2 |h|^2), s, c = sin, cos(a3 h.x):
+//   C += a4 e s/|h|^2 h, J += a3 a4 e c/|h|^2 h h, T -= a3^2 a4 e s/|h|^2 h h h,
+//   psi -= a5 e c/|h|^2.
+// Newtonian term: C -= x/r^3, J -= d/r^3 - 3 x x/r^5,
+//   T += 3 (d_ab x_c + d_ac x_b + d_bc x_a)/r^5 - 15 x x x/r^7, psi += 1/r.
+// At x = 0 the n = 0 real-space term minus the Newtonian one is its limit:
+// a1 in psi, -(2/3) alpha^2 a1 on J's diagonal, nothing in C and T.
+__constant__ int EW_SYM3[10][3] = {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 1, 2},
+                                   {0, 2, 2}, {1, 1, 1}, {1, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+__constant__ int EW_SYM2[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+
+__device__ void ewald_closed(const double x[3], double L, double out[EW_REC]) {
     const int NN = 4;
     const double alpha = 2.0 / L;
-    phi = M_PI / (alpha * alpha * L * L * L);
-    const double r2x = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
-    if (r2x == 0.0)
-        return make_double4(0, 0, 0, 2.0 * 1.4186487533 / L);
     const double a1 = 2.0 * alpha / sqrt(M_PI);
     const double a2 = -M_PI * M_PI / (alpha * alpha * L * L);
     const double a3 = 2.0 * M_PI / L;
     const double a4 = 2.0 / (L * L);
     const double a5 = 1.0 / (M_PI * L);
+    const bool origin = x[0] == 0.0 && x[1] == 0.0 && x[2] == 0.0;
+    double psi = M_PI / (alpha * alpha * L * L * L), c[3] = {0, 0, 0}, j[6] = {0, 0, 0, 0, 0, 0};
+    double t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int p = -NN; p <= NN; p++)
         for (int q = -NN; q <= NN; q++)
-            for (int s = -NN; s <= NN; s++) {
-                const double xx0 = x[0] - L * p, xx1 = x[1] - L * q, xx2 = x[2] - L * s;
-                const double ax2 = xx0 * xx0 + xx1 * xx1 + xx2 * xx2;
-                const double axx = sqrt(ax2);
-                if (axx > 4.0 * L)
+            for (int r = -NN; r <= NN; r++) {
+                if (origin && p == 0 && q == 0 && r == 0) {
+                    psi += a1;
+                    j[0] -= 2.0 / 3.0 * alpha * alpha * a1;
+                    j[3] -= 2.0 / 3.0 * alpha * alpha * a1;
+                    j[5] -= 2.0 / 3.0 * alpha * alpha * a1;
                     continue;
-                const double t1 = erfc(alpha * axx);
-                const double t2 = a1 * exp(-alpha * alpha * ax2);
-                const double t = (t1 / axx + t2) / ax2;
-                f[0] += t * xx0;
-                f[1] += t * xx1;
-                f[2] += t * xx2;
-                phi -= t1 / axx;
+                }
+                const double y[3] = {x[0] - L * p, x[1] - L * q, x[2] - L * r};
+                const double rho2 = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+                const double rho = sqrt(rho2);
+                if (rho > 4.0 * L)
+                    continue;
+                const double t1 = erfc(alpha * rho);
+                const double t2 = a1 * exp(-alpha * alpha * rho2);
+                const double g = t1 / rho + t2;
+                const double gp = -t2 / rho - t1 / rho2 - 2.0 * alpha * alpha * rho * t2;
+                const double gpp = 2.0 * t2 / rho2 + 2.0 * t1 / (rho2 * rho) +
+                                   4.0 * alpha * alpha * alpha * alpha * rho2 * t2;
+                const double tt = g / rho2;
+                const double tp = gp / rho2 - 2.0 * g / (rho2 * rho);
+                const double tpp = gpp / rho2 - 4.0 * gp / (rho2 * rho) + 6.0 * g / (rho2 * rho2);
+                const double u = tp / rho, up = (tpp / rho - tp / rho2) / rho;
+                for (int a = 0; a < 3; a++) c[a] += tt * y[a];
+                for (int k = 0; k < 6; k++) {
+                    const int a = EW_SYM2[k][0], b = EW_SYM2[k][1];
+                    j[k] += (a == b ? tt : 0.0) + u * y[a] * y[b];
+                }
+                for (int k = 0; k < 10; k++) {
+                    const int a = EW_SYM3[k][0], b = EW_SYM3[k][1], cc = EW_SYM3[k][2];
+                    t[k] += u * ((a == b) * y[cc] + (a == cc) * y[b] + (b == cc) * y[a]) +
+                            up * y[a] * y[b] * y[cc];
+                }
+                psi -= t1 / rho;
             }
     for (int p = -NN; p <= NN; p++)
         for (int q = -NN; q <= NN; q++)
-            for (int s = -NN; s <= NN; s++) {
-                const double hh = (double)(p * p + q * q + s * s);
+            for (int r = -NN; r <= NN; r++) {
+                const double hh = (double)(p * p + q * q + r * r);
                 if (hh > 16 || hh == 0)
                     continue;
-                const double hx = p * x[0] + q * x[1] + s * x[2];
-                const double e = exp(a2 * hh);
-                const double t = a4 * e * sin(a3 * hx) / hh;
-                f[0] += t * p;
-                f[1] += t * q;
-                f[2] += t * s;
-                phi -= a5 * e * cos(a3 * hx) / hh;
+                const double h[3] = {(double)p, (double)q, (double)r};
+                const double hx = p * x[0] + q * x[1] + r * x[2];
+                const double e = exp(a2 * hh) / hh;
+                double sn, cs;
+                sincos(a3 * hx, &sn, &cs);
+                for (int a = 0; a < 3; a++) c[a] += a4 * e * sn * h[a];
+                for (int k = 0; k < 6; k++) j[k] += a3 * a4 * e * cs * h[EW_SYM2[k][0]] * h[EW_SYM2[k][1]];
+                for (int k = 0; k < 10; k++)
+                    t[k] -= a3 * a3 * a4 * e * sn * h[EW_SYM3[k][0]] * h[EW_SYM3[k][1]] * h[EW_SYM3[k][2]];
+                psi -= a5 * e * cs;
             }
-    const double r = sqrt(r2x), r3 = r * r2x;
-    return make_double4(f[0] - x[0] / r3, f[1] - x[1] / r3, f[2] - x[2] / r3, phi + 1.0 / r);
+    if (!origin) {
+        const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+        const double rr = sqrt(r2), r3 = rr * r2, r5 = r3 * r2, r7 = r5 * r2;
+        for (int a = 0; a < 3; a++) c[a] -= x[a] / r3;
+        for (int k = 0; k < 6; k++) {
+            const int a = EW_SYM2[k][0], b = EW_SYM2[k][1];
+            j[k] += -(a == b ? 1.0 / r3 : 0.0) + 3.0 * x[a] * x[b] / r5;
+        }
+        for (int k = 0; k < 10; k++) {
+            const int a = EW_SYM3[k][0], b = EW_SYM3[k][1], cc = EW_SYM3[k][2];
+            t[k] += 3.0 * ((a == b) * x[cc] + (a == cc) * x[b] + (b == cc) * x[a]) / r5 -
+                    15.0 * x[a] * x[b] * x[cc] / r7;
+        }
+        psi += 1.0 / rr;
+    }
+    out[0] = psi;
+    for (int a = 0; a < 3; a++) out[1 + a] = c[a];
+    for (int k = 0; k < 6; k++) out[4 + k] = j[k];
+    for (int k = 0; k < 10; k++) out[10 + k] = t[k];
 }
 
-// Table node (i, j, k) at r = (i, j, k) h, h = (L/2)/(EWALD_N-1): the correction
-// and its partial derivatives by FP64 central differences of step h/64 (T is
-// smooth there; the removed Newtonian singularity cancels inside C).
-__global__ void k_ewald_table(double L, double4 *__restrict__ t64, float4 *__restrict__ t32) {
+// one thread per node i >= j >= k of the octant grid
+__global__ void k_ewald_table(double L, double *__restrict__ t64, float *__restrict__ t32) {
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= EWALD_N * EWALD_N * EWALD_N)
+    if (id >= EWALD_NODES)
         return;
-    const int i = id / (EWALD_N * EWALD_N), j = (id / EWALD_N) % EWALD_N, k = id % EWALD_N;
+    // invert id = i(i+1)(i+2)/6 + j(j+1)/2 + k
+    int i = 0;
+    while (ew_node(i + 1, 0, 0) <= id) i++;
+    int rem = id - ew_node(i, 0, 0), j = 0;
+    while ((j + 1) * (j + 2) / 2 <= rem) j++;
+    const int k = rem - j * (j + 1) / 2;
     const double h = 0.5 * L / (EWALD_N - 1);
-    const double x = i * h, y = j * h, z = k * h, d = h / 64;
-    double4 out[4];
-    out[0] = ewald_point(x, y, z, L);
-    for (int b = 0; b < 3; b++) {
-        const double4 cp = ewald_point(x + (b == 0) * d, y + (b == 1) * d, z + (b == 2) * d, L);
-        const double4 cm = ewald_point(x - (b == 0) * d, y - (b == 1) * d, z - (b == 2) * d, L);
-        const double s = 1.0 / (2.0 * d);
-        out[1 + b] = make_double4((cp.x - cm.x) * s, (cp.y - cm.y) * s, (cp.z - cm.z) * s, (cp.w - cm.w) * s);
-    }
-    for (int c = 0; c < 4; c++) {
-        t64[4 * (size_t)id + c] = out[c];
-        t32[4 * (size_t)id + c] = make_float4((float)out[c].x, (float)out[c].y, (float)out[c].z, (float)out[c].w);
+    const double x[3] = {i * h, j * h, k * h};
+    double out[EW_REC];
+    ewald_closed(x, L, out);
+    for (int q = 0; q < EW_REC; q++) {
+        t64[(size_t)id * EW_REC + q] = out[q];
+        t32[(size_t)id * EW_REC + q] = (float)out[q];
     }
 }
 
 // ---------------------------------------------------------------- launchers
-hipError_t launch_ewald_table(double L, double4 *t64, float4 *t32, hipStream_t st) {
-    const int n = EWALD_N * EWALD_N * EWALD_N;
-    hipLaunchKernelGGL(k_ewald_table, dim3((n + 127) / 128), dim3(128), 0, st, L, t64, t32);
+hipError_t launch_ewald_table(double L, double *t64, float *t32, hipStream_t st) {
+    hipLaunchKernelGGL(k_ewald_table, dim3((EWALD_NODES + 63) / 64), dim3(64), 0, st, L, t64, t32);
     return hipGetLastError();
 }
 
@@ -717,7 +754,7 @@ hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, 
     A.img = v.img;
     A.img_f = 2 * v.nimage + 1;
     A.unroll = v.unroll;
-    A.ewgroup = v.ewgroup;
+    A.ewratio = v.ewratio;
     if (A.ngroups <= 0)
         return hipSuccess;
     if (v.periodic < 0 || v.periodic > 2 || (v.periodic == 2 && !v.img))
@@ -760,7 +797,7 @@ hipError_t launch_group64(const BatchView &v, int g, hipStream_t st) {
     A.img = v.img;
     A.img_f = 2 * v.nimage + 1;
     A.unroll = v.unroll;
-    A.ewgroup = v.ewgroup;
+    A.ewratio = v.ewratio;
     if (v.periodic < 0 || v.periodic > 2 || (v.periodic == 2 && !v.img))
         return hipErrorInvalidValue;
     const int per = v.periodic;
@@ -780,7 +817,7 @@ hipError_t launch_group64(const BatchView &v, int g, hipStream_t st) {
 template <bool PER>
 static void launch_direct_soft(dim3 grid, const double4 *bpos, int64_t n, const int64_t *sample,
                                int ns, int soft, double eps, double4 *part, int64_t chunk,
-                               const double4 *tab, double box, hipStream_t st) {
+                               const double *tab, double box, hipStream_t st) {
     if (soft == SOFT_K1)
         hipLaunchKernelGGL((k_direct64<SOFT_K1, PER>), grid, dim3(64), 0, st, bpos, n, sample, ns,
                            eps, part, chunk, tab, box);
@@ -794,7 +831,7 @@ static void launch_direct_soft(dim3 grid, const double4 *bpos, int64_t n, const 
 
 hipError_t launch_direct64(const double4 *bpos, int64_t n, const int64_t *sample, int ns, int soft,
                            double eps, double G, double4 *part, int nsplit, double4 *out,
-                           const double4 *ewald64, double box, hipStream_t st) {
+                           const double *ewald64, double box, hipStream_t st) {
     if (ns <= 0)
         return hipSuccess;
     const int64_t chunk = (n + nsplit - 1) / nsplit;

@@ -166,6 +166,46 @@ __device__ __forceinline__ void k1_interact(T m, T x, T y, T z, T eps_inv, T eps
     a.z += z * eqe;
 }
 
+// ---------------------------------------------------------------- wave sums
+// The sum of a value over the 64 lanes of a wave by DPP row operations (VALU
+// only, no LDS): pairs and quads by quad permutes, 8 and 16 lanes by the half-
+// row and row mirrors, then rows 0+1 and 2+3 by the row-15 broadcast and all
+// four by the row-31 broadcast, which leaves the total in lane 63, read back
+// into every lane.  The lane order of the additions is fixed, so the result is
+// reproducible.  Every lane of the wave must be active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_mov(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_add(float v) {
+    return v + __builtin_bit_cast(float, dpp_mov<CTRL, ROWS>(__builtin_bit_cast(int, v)));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_add(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = dpp_mov<CTRL, ROWS>((int)(b & 0xffffffffll));
+    const int hi = dpp_mov<CTRL, ROWS>((int)(b >> 32));
+    return v + __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
+    v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2, 3, 0, 1]
+    v = dpp_add<0x141, 0xf>(v);  // row_half_mirror
+    v = dpp_add<0x140, 0xf>(v);  // row_mirror
+    v = dpp_add<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_add<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+    } else {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+        return __builtin_bit_cast(T, ((long long)hi << 32) | (unsigned int)lo);
+    }
+}
+
 // Error-free transformation accumulate (Knuth 2Sum): (s, c) += v.
 template <typename T>
 __device__ __forceinline__ void two_sum_add(T &s, T &c, T v) {

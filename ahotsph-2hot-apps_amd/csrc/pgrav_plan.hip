@@ -17,6 +17,7 @@
 #include "../../include/pgrav.h"
 #include "pgrav_batch.h"
 #include "pgrav_device.h"
+#include "pgrav_ewald.h"
 #include "pgrav_host.h"
 #include "pgrav_le.h"
 #include "pgrav_le_launch.h"
@@ -288,6 +289,7 @@ void pgrav_params_default(pgrav_params *p) {
     p->mac = 0;
     p->quad_ncut = 7;   // nln/main.c:540
     p->hexa_ncut = 20;  // nln/main.c:541
+    p->ewald_ratio = 0.05;
 }
 
 void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const pgrav_params *p) {
@@ -302,6 +304,10 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
     }
     if (p->body_unroll != 0 && p->body_unroll != 2 && p->body_unroll != 4 && p->body_unroll != 8) {
         fail("pgrav_plan_create: body_unroll must be 0, 2, 4 or 8 (got %d)", p->body_unroll);
+        return nullptr;
+    }
+    if (!(p->ewald_ratio >= 0)) {
+        fail("pgrav_plan_create: ewald_ratio must be >= 0 (got %g)", p->ewald_ratio);
         return nullptr;
     }
     if (p->soft_type != SOFT_NONE && !(p->eps > 0)) {
@@ -426,7 +432,7 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
             throw std::runtime_error("memset");
         P->v.soft = p->soft_type;
         P->v.unroll = p->body_unroll ? p->body_unroll : 4;
-        P->v.ewgroup = p->ewald_expand ? 1 : 0;
+        P->v.ewratio = p->ewald_ratio;
         P->v.eps = p->eps;
         P->v.G = p->G;
         P->v.periodic = p->periodic;
@@ -443,9 +449,9 @@ void *pgrav_plan_create(const double *pos3, const double *mass, int64_t n, const
             P->le_coef = P->dalloc<double>(sizeof(double) * LE_NMON);
         }
         if (p->periodic) {
-            const size_t ne = 4 * (size_t)EWALD_N * EWALD_N * EWALD_N;  // value + 3 derivatives
-            double4 *t64 = P->dalloc<double4>(ne * sizeof(double4));
-            float4 *t32 = P->dalloc<float4>(ne * sizeof(float4));
+            const size_t ne = (size_t)EW_REC * EWALD_NODES;  // pgrav_ewald.h
+            double *t64 = P->dalloc<double>(ne * sizeof(double));
+            float *t32 = P->dalloc<float>(ne * sizeof(float));
             if (!check(launch_ewald_table(p->box, t64, t32, P->stream), "Ewald table kernel"))
                 throw std::runtime_error("ewald");
             P->v.ewald32 = t32;

@@ -33,7 +33,7 @@ class Params(ctypes.Structure):
                 ("frac_tol0", ctypes.c_double), ("ptol_boost", ctypes.c_double),
                 ("stol_max", ctypes.c_double), ("quad_ncut", ctypes.c_int),
                 ("hexa_ncut", ctypes.c_int), ("body_unroll", ctypes.c_int),
-                ("ewald_expand", ctypes.c_int), ("group_box", ctypes.c_int)]
+                ("ewald_ratio", ctypes.c_double), ("group_box", ctypes.c_int)]
 
 # acceptance tests: opening angle theta; error tolerance (the reference's AREL_MAC
 # critical radii, nln/cofm.c:276-565, taken lowest order first, nln/mac.c:1776-1800)
@@ -105,7 +105,7 @@ def make_params(theta=0.6, order=4, eps=1e-4, soft="plummer", G=1.0, group_max=6
                 persistent_blocks=-1, fp64=True, dispatch=2, mixed_order=False, nimage=2,
                 rcb_cell=32768, mac="theta", errtol=0.0, frac_tol=0.0, frac_tol0=0.0,
                 ptol_boost=0.0, stol_max=0.0, quad_ncut=7, hexa_ncut=20, group_box=0,
-                body_unroll=0, ewald_expand=False):
+                body_unroll=0, ewald_ratio=0.05):
     """pgrav_params; mac="tolerance" selects the error-tolerance acceptance, with
     errtol the absolute acceleration error per interaction (G = 1 units), frac_tol
     and frac_tol0 the relative bound and floor, quad_ncut / hexa_ncut the body
@@ -132,7 +132,7 @@ def make_params(theta=0.6, order=4, eps=1e-4, soft="plummer", G=1.0, group_max=6
     p.quad_ncut, p.hexa_ncut = quad_ncut, hexa_ncut
     p.group_box = int(group_box)
     p.body_unroll = int(body_unroll)
-    p.ewald_expand = int(ewald_expand)
+    p.ewald_ratio = float(ewald_ratio)
     return p
 
 

@@ -171,13 +171,14 @@ typedef struct {
     int hexa_ncut;         /* mac 1: ... at order 4 (default 20, nln/main.c:541) */
     int body_unroll;       /* rel32c kernels: unroll of the order-0 and body loops, 2, 4 or 8
                               (0: 4) -- a performance choice, same results */
-    int ewald_expand;      /* periodic 1, group-relative variants: 1 = a cell's Ewald
-                              correction is looked up once per (group, cell) at the group
-                              centre and expanded to first order in each sink's offset,
-                              instead of one table lookup per pair (cells near an image
-                              boundary of the group keep the per-pair lookup); faster, with
-                              an error ~ (group radius)^2 |C''| that shows at tight theta;
-                              0 (default): per pair */
+    double ewald_ratio;    /* periodic 1: a source's Ewald correction enters its group's
+                              expansion (value, Jacobian and second derivatives at the
+                              source, summed per group once and evaluated per sink, no
+                              per-pair lookup) when the group radius is at most
+                              ewald_ratio times the distance from the group to the
+                              source's nearest other image and every sink sees the same
+                              image; otherwise it is looked up per pair.  Default 0.05;
+                              0: always per pair */
     int group_box;         /* 0 (default): a cell's distance to a group is measured to the
                               group's enclosing sphere; 1: to its bounding box (every sink
                               is inside it: a tighter bound for every sink, fewer cells

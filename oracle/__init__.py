@@ -45,7 +45,7 @@ def lib():
         L.orc_direct.argtypes = [_f64p, _i64, _f64p, _f64p, _i64, _i, _d, _f64p, _i]
         L.orc_ewald.argtypes = [_f64p, _d, _f64p, ctypes.POINTER(_d)]
         L.orc_direct_ewald.argtypes = [_f64p, _i64, _f64p, _f64p, _i64, _d, _f64p, _i]
-        L.orc_ewald_corr.argtypes = [_f64p, _i64, _d, _i, _f64p, _f64p, _i]
+        L.orc_ewald_corr.argtypes = [_f64p, _i64, _d, _i, _f64p, _f64p, _f64p, _i]
         L.orc_cell_record.argtypes = [_f64p, _f64p, _i64, _f64p, _d, _f64p]
         L.orc_raw_moments.argtypes = [_f64p, _f64p, _i64, _f64p, _f64p]
         L.orc_mpole.argtypes = [_f64p, _i, _i, _d, _d, _d, _d, _i, _f64p]
@@ -158,20 +158,35 @@ def ewald(x, L=1.0):
 
 
 def ewald_corr(r, L=1.0, nodes=0, nthreads=None):
-    """Ewald correction at r = source - sink (n, 3): val (n, 4) = (C, psi) and jac
-    (n, 4, 3), rows dC_x, dC_y, dC_z, dpsi over d(x, y, z).  nodes = 0: the exact
-    lattice sum at r; nodes > 0: the nearest node of the grid of spacing
-    (L/2)/(nodes - 1) plus its first-order step d = r - node (and, in psi, the
-    trace part of the second-order step, -(2 pi / 3) |d|^2 / L^3), and the
-    node's Jacobian."""
+    """Ewald correction at r = source - sink (n, 3): val (n, 4) = (C, psi), jac
+    (n, 4, 3) with rows dC_x, dC_y, dC_z, dpsi over d(x, y, z), and third (n, 10):
+    the second derivatives of C (xxx, xxy, xxz, xyy, xyz, xzz, yyy, yyz, yzz,
+    zzz).  nodes = 0: the exact lattice sum at r; nodes > 0: the nearest node of
+    the grid of spacing (L/2)/(nodes - 1) and its Taylor steps to r (value to
+    third order in psi, second in C, first in the Jacobian; the node's T)."""
     r = _c(r, np.float64).reshape(-1, 3)
     n = r.shape[0]
     val = np.zeros((n, 4), np.float64)
     jac = np.zeros((n, 4, 3), np.float64)
+    third = np.zeros((n, 10), np.float64)
     if n:
         lib().orc_ewald_corr(r.reshape(-1), n, L, int(nodes), val.reshape(-1), jac.reshape(-1),
-                             nthreads or min(16, os.cpu_count() or 1))
-    return val, jac
+                             third.reshape(-1), nthreads or min(16, os.cpu_count() or 1))
+    return val, jac, third
+
+
+SYM3 = ((0, 0, 0), (0, 0, 1), (0, 0, 2), (0, 1, 1), (0, 1, 2), (0, 2, 2), (1, 1, 1), (1, 1, 2),
+        (1, 2, 2), (2, 2, 2))
+
+
+def sym3_full(third):
+    """(n, 10) symmetric components -> (n, 3, 3, 3)."""
+    import itertools
+    out = np.zeros(third.shape[:-1] + (3, 3, 3))
+    for q, (a, b, c) in enumerate(SYM3):
+        for p in set(itertools.permutations((a, b, c))):
+            out[..., p[0], p[1], p[2]] = third[..., q]
+    return out
 
 
 def direct_ewald(sink_pos, src_pos, src_mass, L=1.0, nthreads=None):

@@ -477,21 +477,34 @@ void orc_direct_ewald(const double *sink_pos3, int64_t m, const double *src_pos3
     run_threads(ewald_worker, b, nthreads);
 }
 
-/* Value and Jacobian of the Ewald correction at x = source - sink: the sums of
- * nln/ewald.c:13-64 and their derivatives term by term.
+/* Value and derivatives of the Ewald correction at x = source - sink: the sums
+ * of nln/ewald.c:13-64 and their derivatives term by term.
  * Real space, y = x - nL, rho = |y|: t1 = erfc(alpha rho), t2 = a1 exp(-alpha^2 rho^2),
  *   g = t1/rho + t2, t = g/rho^2: f += t y, phi -= t1/rho;
  *   dt1/drho = -t2, dt2/drho = -2 alpha^2 rho t2, so
- *   g' = -t2/rho - t1/rho^2 - 2 alpha^2 rho t2, t' = g'/rho^2 - 2 g/rho^3 and
- *   df_i/dx_j += t delta_ij + t' y_i y_j / rho.
- * Fourier space, e = exp(a2 |h|^2): f += a4 e sin(a3 h.x)/|h|^2 h, phi -= a5 e
- *   cos(a3 h.x)/|h|^2; df_i/dx_j += a3 a4 e cos(a3 h.x)/|h|^2 h_i h_j.
+ *   g' = -t2/rho - t1/rho^2 - 2 alpha^2 rho t2,
+ *   g'' = 2 t2/rho^2 + 2 t1/rho^3 + 4 alpha^4 rho^2 t2,
+ *   t' = g'/rho^2 - 2 g/rho^3, t'' = g''/rho^2 - 4 g'/rho^3 + 6 g/rho^4;
+ *   with u = t'/rho, u' = t''/rho - t'/rho^2:
+ *   df_i/dx_j += t delta_ij + u y_i y_j,
+ *   d2f_i/dx_j dx_k += u (delta_ij y_k + delta_ik y_j + delta_jk y_i) + (u'/rho) y_i y_j y_k.
+ * Fourier space, e = exp(a2 |h|^2), s = sin(a3 h.x), c = cos(a3 h.x):
+ *   f += a4 e s/|h|^2 h, phi -= a5 e c/|h|^2;
+ *   df_i/dx_j += a3 a4 e c/|h|^2 h_i h_j, d2f_i/dx_j dx_k -= a3^2 a4 e s/|h|^2 h_i h_j h_k.
  * Newtonian term removed: C = f - x/|x|^3, dC_i/dx_j = df_i/dx_j - delta_ij/|x|^3
- *   + 3 x_i x_j/|x|^5; psi = phi + 1/|x|, dpsi/dx = f - x/|x|^3 = C.
+ *   + 3 x_i x_j/|x|^5, d2C_i/dx_j dx_k = d2f_i/dx_j dx_k + 3 (delta_ij x_k + delta_ik
+ *   x_j + delta_jk x_i)/|x|^5 - 15 x_i x_j x_k/|x|^7; psi = phi + 1/|x|, dpsi/dx = C.
  * At x = 0 the n = 0 term minus the Newtonian one is replaced by its limit:
  *   erf(alpha rho)/rho -> a1 in psi; (erfc(alpha rho) - 1)/rho^3 + t2/rho^2 ->
- *   -(4/3) alpha^3/sqrt(pi) = -(2/3) alpha^2 a1 on the diagonal of dC/dx. */
-static void ewald_value_jac(const double x[3], double L, double v[4], double J[12]) {
+ *   -(4/3) alpha^3/sqrt(pi) = -(2/3) alpha^2 a1 on the diagonal of dC/dx; 0 in the
+ *   (odd) second derivatives.
+ * T (the second derivatives of C, fully symmetric as third derivatives of psi)
+ * is stored as its 10 components xxx, xxy, xxz, xyy, xyz, xzz, yyy, yyz, yzz, zzz. */
+static const int SYM3[10][3] = {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 1, 2},
+                                {0, 2, 2}, {1, 1, 1}, {1, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+
+static void ewald_value_derivs(const double x[3], double L, double v[4], double J[12],
+                               double T[10]) {
     const int N = 4;
     const double alpha = 2.0 / L;
     const double a1 = 2.0 * alpha / sqrt(M_PI);
@@ -501,6 +514,7 @@ static void ewald_value_jac(const double x[3], double L, double v[4], double J[1
     const double a5 = 1.0 / (M_PI * L);
     const int origin = x[0] == 0.0 && x[1] == 0.0 && x[2] == 0.0;
     double f[3] = {0, 0, 0}, D[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double H[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double phi = M_PI / (alpha * alpha * L * L * L);
     for (int i = -N; i <= N; i++)
         for (int j = -N; j <= N; j++)
@@ -520,11 +534,20 @@ static void ewald_value_jac(const double x[3], double L, double v[4], double J[1
                 const double g = t1 / rho + t2;
                 const double t = g / rho2;
                 const double gp = -t2 / rho - t1 / rho2 - 2.0 * alpha * alpha * rho * t2;
+                const double gpp = 2.0 * t2 / rho2 + 2.0 * t1 / (rho2 * rho) +
+                                   4.0 * alpha * alpha * alpha * alpha * rho2 * t2;
                 const double tp = gp / rho2 - 2.0 * g / (rho2 * rho);
+                const double tpp = gpp / rho2 - 4.0 * gp / (rho2 * rho) + 6.0 * g / (rho2 * rho2);
+                const double u = tp / rho, up = tpp / rho - tp / rho2;
                 for (int a = 0; a < 3; a++) {
                     f[a] += t * y[a];
                     D[a][a] += t;
-                    for (int b = 0; b < 3; b++) D[a][b] += tp * y[a] * y[b] / rho;
+                    for (int b = 0; b < 3; b++) D[a][b] += u * y[a] * y[b];
+                }
+                for (int q = 0; q < 10; q++) {
+                    const int a = SYM3[q][0], b = SYM3[q][1], c = SYM3[q][2];
+                    H[q] += u * ((a == b) * y[c] + (a == c) * y[b] + (b == c) * y[a]) +
+                            up / rho * y[a] * y[b] * y[c];
                 }
                 phi -= t1 / rho;
             }
@@ -542,15 +565,22 @@ static void ewald_value_jac(const double x[3], double L, double v[4], double J[1
                     f[a] += a4 * e * s / hh * h[a];
                     for (int b = 0; b < 3; b++) D[a][b] += a3 * a4 * e * c / hh * h[a] * h[b];
                 }
+                for (int q = 0; q < 10; q++)
+                    H[q] -= a3 * a3 * a4 * e * s / hh * h[SYM3[q][0]] * h[SYM3[q][1]] * h[SYM3[q][2]];
                 phi -= a5 * e * c / hh;
             }
     if (!origin) {
         const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
-        const double r = sqrt(r2), r3 = r * r2, r5 = r3 * r2;
+        const double r = sqrt(r2), r3 = r * r2, r5 = r3 * r2, r7 = r5 * r2;
         for (int a = 0; a < 3; a++) {
             f[a] -= x[a] / r3;
             D[a][a] -= 1.0 / r3;
             for (int b = 0; b < 3; b++) D[a][b] += 3.0 * x[a] * x[b] / r5;
+        }
+        for (int q = 0; q < 10; q++) {
+            const int a = SYM3[q][0], b = SYM3[q][1], c = SYM3[q][2];
+            H[q] += 3.0 * ((a == b) * x[c] + (a == c) * x[b] + (b == c) * x[a]) / r5 -
+                    15.0 * x[a] * x[b] * x[c] / r7;
         }
         phi += 1.0 / r;
     }
@@ -560,6 +590,23 @@ static void ewald_value_jac(const double x[3], double L, double v[4], double J[1
         J[9 + a] = f[a];
     }
     v[3] = phi;
+    for (int q = 0; q < 10; q++) T[q] = H[q];
+}
+
+/* index of the symmetric component (a, b, c) in the 10-component order */
+static int sym3_index(int a, int b, int c) {
+    int s[3] = {a, b, c};
+    for (int p = 0; p < 2; p++)
+        for (int q = 0; q < 2 - p; q++)
+            if (s[q] > s[q + 1]) {
+                const int t = s[q];
+                s[q] = s[q + 1];
+                s[q + 1] = t;
+            }
+    for (int q = 0; q < 10; q++)
+        if (SYM3[q][0] == s[0] && SYM3[q][1] == s[1] && SYM3[q][2] == s[2])
+            return q;
+    return 0;
 }
 
 typedef struct {
@@ -567,7 +614,7 @@ typedef struct {
     int64_t i0, i1;
     double L;
     int nodes;
-    double *val4, *jac12;
+    double *val4, *jac12, *third10;
 } ecorr_job;
 
 static void *ecorr_worker(void *arg) {
@@ -575,8 +622,10 @@ static void *ecorr_worker(void *arg) {
     for (int64_t p = j->i0; p < j->i1; p++) {
         const double *r = j->r3 + 3 * p;
         double *v = j->val4 + 4 * p, *J = j->jac12 + 12 * p;
+        double Tn[10];
+        double *T = j->third10 ? j->third10 + 10 * p : Tn;
         if (j->nodes <= 0) {
-            ewald_value_jac(r, j->L, v, J);
+            ewald_value_derivs(r, j->L, v, J, T);
             continue;
         }
         const double h = 0.5 * j->L / (j->nodes - 1);
@@ -590,19 +639,38 @@ static void *ecorr_worker(void *arg) {
             node[a] = q * h;
             d[a] = r[a] - node[a];
         }
-        ewald_value_jac(node, j->L, v, J);
-        double step[4];
-        for (int a = 0; a < 4; a++) step[a] = J[3 * a] * d[0] + J[3 * a + 1] * d[1] + J[3 * a + 2] * d[2];
-        /* psi's second-order step, trace part: (1/2) tr(dC/dr) |d|^2 / 3 with
-         * tr(dC/dr) = -4 pi / L^3 (C = grad psi, div C = -4 pi / L^3 off the lattice) */
-        step[3] += -2.0 * M_PI / 3.0 / (j->L * j->L * j->L) * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        for (int a = 0; a < 4; a++) v[a] += step[a];
+        double vn[4], Jn[12];
+        ewald_value_derivs(node, j->L, vn, Jn, T);
+        /* Taylor steps from the node: T d, T d d, T d d d */
+        double Td[3][3], Tdd[3], Tddd = 0;
+        for (int a = 0; a < 3; a++) {
+            Tdd[a] = 0;
+            for (int b = 0; b < 3; b++) {
+                Td[a][b] = 0;
+                for (int c = 0; c < 3; c++) Td[a][b] += T[sym3_index(a, b, c)] * d[c];
+                Tdd[a] += Td[a][b] * d[b];
+            }
+            Tddd += Tdd[a] * d[a];
+        }
+        double dJd = 0;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) dJd += d[a] * Jn[3 * a + b] * d[b];
+        /* psi + C.d + d.J.d / 2 + T d d d / 6 */
+        v[3] = vn[3] + vn[0] * d[0] + vn[1] * d[1] + vn[2] * d[2] + 0.5 * dJd + Tddd / 6.0;
+        for (int a = 0; a < 3; a++) {
+            /* C + J d + T d d / 2 */
+            v[a] = vn[a] + Jn[3 * a] * d[0] + Jn[3 * a + 1] * d[1] + Jn[3 * a + 2] * d[2] +
+                   0.5 * Tdd[a];
+            /* J + T d */
+            for (int b = 0; b < 3; b++) J[3 * a + b] = Jn[3 * a + b] + Td[a][b];
+        }
+        for (int a = 0; a < 3; a++) J[9 + a] = v[a];
     }
     return NULL;
 }
 
 void orc_ewald_corr(const double *r3, int64_t n, double L, int nodes, double *val4,
-                    double *jac12, int nthreads) {
+                    double *jac12, double *third10, int nthreads) {
     if (nthreads < 1)
         nthreads = 1;
     if (nthreads > 64)
@@ -610,7 +678,7 @@ void orc_ewald_corr(const double *r3, int64_t n, double L, int nodes, double *va
     pthread_t th[64];
     ecorr_job jobs[64];
     for (int t = 0; t < nthreads; t++) {
-        ecorr_job j = {r3, n * t / nthreads, n * (t + 1) / nthreads, L, nodes, val4, jac12};
+        ecorr_job j = {r3, n * t / nthreads, n * (t + 1) / nthreads, L, nodes, val4, jac12, third10};
         jobs[t] = j;
         pthread_create(&th[t], NULL, ecorr_worker, &jobs[t]);
     }

@@ -89,19 +89,21 @@ void orc_ewald(const double x[3], double L, double f[3], double *phi);
 void orc_direct_ewald(const double *sink_pos3, int64_t m, const double *src_pos3,
                       const double *src_mass, int64_t n, double L, double *out4, int nthreads);
 
-/* The Ewald correction of n pairs and its Jacobian.  r = source - sink (the
+/* The Ewald correction of n pairs and its derivatives.  r = source - sink (the
  * nearest image); C(r) = f(r) - r/|r|^3 and psi(r) = phi(r) + 1/|r| where f, phi
  * are the lattice sums of nln/ewald.c:13-64 (orc_ewald), i.e. the pull of a unit
  * source's periodic images and background with the nearest Newtonian term taken
- * out.  val4 = (C, psi); jac12 = dC_i/dr_j (rows i = 0..2) then dpsi/dr_j (= C_j),
- * differentiated term by term in closed form.
- *   nodes == 0: value and Jacobian at r.
+ * out.  val4 = (C, psi); jac12 = dC_i/dr_j (rows i = 0..2) then dpsi/dr_j (= C_j);
+ * third10 (may be NULL) = d2C_i/dr_j dr_k, fully symmetric, components xxx, xxy,
+ * xxz, xyy, xyz, xzz, yyy, yyz, yzz, zzz; all differentiated term by term in
+ * closed form.
+ *   nodes == 0: at r.
  *   nodes  > 0: tabulated on the grid of spacing h = (L/2)/(nodes - 1): the node
- *               nearest r per axis (signed), its value plus the first-order step
- *               J d (d = r - node) and, in psi, the trace part of the second-order
- *               step, -(2 pi / 3) |d|^2 / L^3; the node's Jacobian. */
+ *               nearest r per axis (signed) and its Taylor steps to r = node + d:
+ *               psi + C.d + d.J.d/2 + T d d d/6, C + J d + T d d/2, J + T d, and
+ *               the node's T. */
 void orc_ewald_corr(const double *r3, int64_t n, double L, int nodes, double *val4,
-                    double *jac12, int nthreads);
+                    double *jac12, double *third10, int nthreads);
 
 /* Raw moments (FP64) of a point set about `center` to order 4, summed
  * directly (what nln/cofm.c:650-768 mpole_add_mono accumulates), followed by

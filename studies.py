@@ -5,7 +5,7 @@
                              [--variants rel32c rel32 abs32 f64]
     python studies.py c4 [--ng 256] [--theta 0.6 | --thetas 0.4 0.5 0.6]
                              [--scheme ewald | images --nimage 2]
-                             [--mac tolerance --errtols 1e-2 1e-3] [--ewald-expand]
+                             [--mac tolerance --errtols 1e-2 1e-3] [--ewald-ratio 0.05]
     python studies.py c5 [--n 1048576]
     python studies.py dropin [--n 262144]
     python studies.py mac [--n 16777216] [--errtols 1e-2 1e-3 1e-4] [--leaves 16 64]
@@ -146,7 +146,7 @@ def _c4_run(args, pos, mass, label, sample, ref, **kw):
     n = pos.shape[0]
     t0 = time.perf_counter()
     plan = batch.Plan(pos, mass, periodic=args.scheme, nimage=args.nimage, box=1.0, order=4,
-                      eps=args.eps, soft="plummer", fp64=False, ewald_expand=args.ewald_expand,
+                      eps=args.eps, soft="plummer", fp64=False, ewald_ratio=args.ewald_ratio,
                       **kw)
     st = plan.stats()
     _log("c4 %s %d^3: plan %.1fs, %d groups, %.3e pairs" % (label, args.ng, time.perf_counter() - t0,
@@ -368,7 +368,9 @@ def main(argv=None):
     ap.add_argument("--leaves", type=int, nargs="+", default=[16, 64])
     ap.add_argument("--group-boxes", type=int, nargs="+", default=[0])
     ap.add_argument("--mac", choices=["theta", "tolerance"], default="theta")
-    ap.add_argument("--ewald-expand", action="store_true")
+    ap.add_argument("--ewald-ratio", type=float, default=0.05,
+                    help="periodic 1: group radius / image distance below which a source's "
+                         "Ewald correction joins the group's expansion (0: per pair)")
     ap.add_argument("--variants", nargs="+", default=["rel32c", "rel32", "abs32", "f64"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--ng", type=int, default=256)

@@ -76,38 +76,46 @@ def _pbc(x, box):
 
 
 def group_reference_periodic(H, g, pos_sorted, mass_sorted, box=1.0, soft="none", eps=0.0,
-                             G=1.0, nodes=64, expand=False, quad_phi_sign=1, perturb=()):
-    """Nearest-image periodic evaluation of group g's own lists (periodic = 1):
-    every cell and body at the image nearest each sink, plus the Ewald
-    correction m (C, psi)(r) of each (sink, cell) and (sink, body) pair, r =
+                             G=1.0, nodes=64, ratio=0.05, quad_phi_sign=1, perturb=()):
+    """Nearest-image periodic evaluation of group g's own lists (periodic = 1).
+
+    Newtonian part: every cell (multipole record rebuilt from its bodies) and
+    body at the image nearest each sink.  Ewald correction (C, psi)(r), r =
     source - sink, from oracle.ewald_corr: the exact lattice sum of nln/ewald.c
-    (nodes = 0) or its tabulated form (nearest node of `nodes` per half box plus
-    the node Jacobian's first-order step, what the GPU tables hold).  A body's
-    own pair has no Newtonian term (unsoftened) and the correction's limit at r =
-    0: C = 0 and psi(0) = 2.837297479 / L, its pull from its own images and the
-    background.  Each cell's correction is taken at its centre of mass, plus,
-    in the potential, the trace part of its second-order term, -(2 pi / 3) /
-    box^3 sum m |x - cofm|^2.
+    (nodes = 0) or its tabulated form (what the GPU tables hold, nodes = 64):
 
-    expand: cells whose image is the same for every sink -- |r0_i| < box/2 -
-    group radius per axis, r0 = cofm - group centre (nearest image) -- take the
-    correction at r0 and its first-order expansion in the sink's offset s from
-    the group centre, (C, psi)(r0) - J(r0) s, psi with -(2 pi / 3) |s|^2 /
-    box^3 (its second order's trace part); the others and all bodies the
-    per-pair form.
+    * a source whose image is the same for every sink -- |r0_i| < box/2 - group
+      radius per axis, r0 = source - group centre (nearest image) -- and whose
+      nearest other lattice image is far against the group -- group radius <=
+      ratio (box - max_i |r0_i|) -- enters the group's expansion: m (psi, C, J,
+      T) at r0 summed over those sources, and
+      each sink at offset s from the centre takes
+        C(r0 - s) = C - J s + T s s / 2,
+        psi(r0 - s) = psi - C.s + s.J.s / 2 - T s s s / 6;
+      a cell adds its second moments' coupling at r0: (1/2) T:Q to C and
+      (1/2) J:Q to psi, Q its traceless second moment about the centre of mass;
+    * the other sources take the correction per pair at r;
+    * every cell adds -(2 pi / 3) / box^3 sum m |x - cofm|^2 to the potential
+      (the trace part of its second moments against the correction's Hessian).
+    A body's own pair has no Newtonian term when unsoftened; the correction
+    covers it (at r = 0 its limit C = 0, psi = 2.837297479 / box).
 
-    perturb: planted mistakes, to show what the comparison detects:
-    "scale_c" scales C by 1 + 1e-3; "flip_jac" flips the sign of dC_x/dy and
-    dC_y/dx in every first-order step."""
+    perturb: planted mistakes, to show what the comparison detects: "scale_c"
+    scales C by 1 + 1e-3; "flip_jac" flips the sign of dC_x/dy and dC_y/dx."""
     first, cnt, gc = H.group(g)
     slots = H.group_slots(g)
     sinks = pos_sorted[slots]
     acc = np.zeros((cnt, 4))
     eps2 = eps * eps if soft == "plummer" else 0.0
-    pr, pm, prow = [], [], []  # per-pair corrections: r, source mass, sink row
-    er0, em = [], []  # group-centre expansions: r0, cell mass
     lim = 0.5 * box - gc[3]
     rows = np.arange(cnt)
+
+    def expands(r0):
+        a = np.max(np.abs(np.atleast_2d(r0)), axis=1)
+        return (a < lim) & (gc[3] <= ratio * (box - a))
+
+    pr, pm, prow = [], [], []  # per-pair corrections: r, source mass, sink row
+    ex_r0, ex_m, ex_q = [], [], []  # expanded sources: r0, mass, traceless Q (cells)
     for kind, order in ORDER.items():
         for nd in H.list(g, kind):
             c = H.cell(int(nd))
@@ -119,20 +127,22 @@ def group_reference_periodic(H, g, pos_sorted, mass_sorted, box=1.0, soft="none"
                 sel = np.all(shift == sh, axis=1)
                 acc[sel] += oracle.eval_records(sinks[sel] - sh, rec[None], order, dipole=False,
                                                 eps2=eps2, quad_phi_sign=quad_phi_sign)
-            # the correction at the centre of mass leaves, in the potential, the
-            # second-order term (1/2) H : sum m d d^T; H's trace is -4 pi / box^3
-            acc[:, 3] += -2.0 * np.pi / 3.0 / box ** 3 * (m * ((b - c["cofm"]) ** 2).sum(1)).sum()
+            dlt = b - c["cofm"]
+            m2 = np.einsum("n,na,nb->ab", m, dlt, dlt)
+            acc[:, 3] += -2.0 * np.pi / 3.0 / box ** 3 * np.trace(m2)
             r0 = _pbc(c["cofm"] - gc[:3], box)
-            if expand and np.all(np.abs(r0) < lim):
-                er0.append(r0)
-                em.append(m.sum())
+            if expands(r0)[0]:
+                ex_r0.append(r0)
+                ex_m.append(m.sum())
+                ex_q.append(m2 - np.trace(m2) / 3.0 * np.eye(3))
             else:
                 pr.append(_pbc(c["cofm"] - sinks, box))
                 pm.append(np.full(cnt, m.sum()))
                 prow.append(rows)
     bodies = H.list(g, "body")
     if bodies.size:
-        r = _pbc(pos_sorted[bodies][None, :, :] - sinks[:, None, :], box)  # (cnt, nb, 3)
+        bp = pos_sorted[bodies]
+        r = _pbc(bp[None, :, :] - sinks[:, None, :], box)  # (cnt, nb, 3)
         r2 = (r * r).sum(-1)
         mb = np.broadcast_to(mass_sorted[bodies][None, :], r2.shape)
         self_pair = r2 == 0.0
@@ -146,39 +156,57 @@ def group_reference_periodic(H, g, pos_sorted, mass_sorted, box=1.0, soft="none"
         rinv = 1.0 / np.sqrt(s2)
         acc[:, :3] += ((w * rinv ** 3)[..., None] * r).sum(1)
         acc[:, 3] -= (w * rinv).sum(1)
-        pr.append(r)
-        pm.append(mb)
-        prow.append(np.broadcast_to(rows[:, None], r2.shape))
-    h = 0.5 * box / (nodes - 1) if nodes else 0.0
+        r0 = _pbc(bp - gc[:3], box)
+        inside = expands(r0)
+        for j in np.nonzero(inside)[0]:
+            ex_r0.append(r0[j])
+            ex_m.append(mass_sorted[bodies[j]])
+            ex_q.append(np.zeros((3, 3)))
+        out = ~inside
+        if out.any():
+            pr.append(r[:, out])
+            pm.append(mb[:, out])
+            prow.append(np.broadcast_to(rows[:, None], r2.shape)[:, out])
 
-    def planted(val, jac, d):
+    def planted(val, jac):
         if "scale_c" in perturb:
             val[:, :3] *= 1.0 + 1e-3
-        if "flip_jac" in perturb and d is not None:
-            val[:, 0] -= 2.0 * jac[:, 0, 1] * d[:, 1]
-            val[:, 1] -= 2.0 * jac[:, 1, 0] * d[:, 0]
-        return val
+        if "flip_jac" in perturb:
+            jac[:, 0, 1] *= -1.0
+            jac[:, 1, 0] *= -1.0
+        return val, jac
 
     if pr:
         r = np.concatenate([np.reshape(x, (-1, 3)) for x in pr])
         m = np.concatenate([np.reshape(x, -1) for x in pm])
         row = np.concatenate([np.reshape(x, -1) for x in prow])
-        val, jac = oracle.ewald_corr(r, L=box, nodes=nodes)
-        d = r - np.clip(np.rint(r / h), -(nodes - 1), nodes - 1) * h if nodes else None
-        val = planted(val, jac, d)
+        val, jac, _ = oracle.ewald_corr(r, L=box, nodes=nodes)
+        if "flip_jac" in perturb and nodes:
+            # the node step through the flipped Jacobian entries
+            h = 0.5 * box / (nodes - 1)
+            d = r - np.clip(np.rint(r / h), -(nodes - 1), nodes - 1) * h
+            val[:, 0] -= 2.0 * jac[:, 0, 1] * d[:, 1]
+            val[:, 1] -= 2.0 * jac[:, 1, 0] * d[:, 0]
+        if "scale_c" in perturb:
+            val[:, :3] *= 1.0 + 1e-3
         np.add.at(acc, row, m[:, None] * val)
-    if er0:
-        r0 = np.array(er0)
-        val0, jac0 = oracle.ewald_corr(r0, L=box, nodes=nodes)
-        d0 = r0 - np.clip(np.rint(r0 / h), -(nodes - 1), nodes - 1) * h if nodes else None
-        val0 = planted(val0, jac0, d0)
-        if "flip_jac" in perturb:
-            jac0 = jac0.copy()
-            jac0[:, 0, 1] *= -1.0
-            jac0[:, 1, 0] *= -1.0
+    if ex_r0:
+        r0 = np.array(ex_r0)
+        m = np.array(ex_m)
+        q = np.array(ex_q)
+        val, jac, third = oracle.ewald_corr(r0, L=box, nodes=nodes)
+        val, jac = planted(val, jac)
+        T = oracle.sym3_full(third)
+        J = jac[:, :3, :]
+        # per source: its mass times the value at r0, plus a cell's second-moment
+        # coupling (Q carries the masses already)
+        A = (m[:, None] * val[:, :3]).sum(0) + 0.5 * np.einsum("nabc,nbc->a", T, q)
+        P = (m * val[:, 3]).sum() + 0.5 * np.einsum("nab,nab->", J, q)
+        B = (m[:, None, None] * J).sum(0)
+        Tm = (m[:, None, None, None] * T).sum(0)
         s = sinks - gc[:3]
-        s2q = -2.0 * np.pi / 3.0 / box ** 3 * (s * s).sum(1)  # psi's second order, trace part
-        for v0, j0, mc in zip(val0, jac0, em):
-            acc += mc * (v0[None, :] - s @ j0.T)
-            acc[:, 3] += mc * s2q
+        Ts = np.einsum("abc,nc->nab", Tm, s)
+        Tss = np.einsum("nab,nb->na", Ts, s)
+        acc[:, :3] += A[None, :] - s @ B.T + 0.5 * Tss
+        acc[:, 3] += P - s @ A + 0.5 * np.einsum("na,ab,nb->n", s, B, s) - (Tss * s).sum(1) / 6.0
     return (first if first >= 0 else slots), G * acc

@@ -225,72 +225,84 @@ def test_ewald_corr_exact_form_pins():
     * values == the lattice sum of nln/ewald.c (oracle.ewald) minus the
       Newtonian term, at random points;
     * the closed-form Jacobian == central differences of those values, and
-      dpsi/dr == C (psi is C's potential);
+      dpsi/dr == C (psi is C's potential); the closed-form second derivatives
+      == central differences of the Jacobian;
+    * div C = -4 pi / L^3 and the Laplacian of C = 0 away from the lattice points
+      (psi's Laplacian is the uniform background's);
     * at r = 0: psi(0) == 2 x 1.4186487533 (nln/ewald_le2.c:38, 43; the
       reference's literal is the simple cubic constant 2.837297479 to 1e-8),
-      dC/dr == -(4 pi / 3) 1 (nln/ewald_le2.c:35-37), C == 0;
+      dC/dr == -(4 pi / 3) 1 (nln/ewald_le2.c:35-37), C == 0 and its second
+      derivatives == 0;
     * odd C and even psi, and the box scaling C(L r) = C(r)/L^2."""
     rng = np.random.default_rng(5)
     r = rng.uniform(-0.5, 0.5, (64, 3))
-    val, jac = oracle.ewald_corr(r)
+    val, jac, third = oracle.ewald_corr(r)
     for i in range(16):
         f, phi = oracle.ewald(r[i])
         d = np.linalg.norm(r[i])
         np.testing.assert_allclose(val[i, :3], f - r[i] / d ** 3, rtol=0, atol=1e-13)
         np.testing.assert_allclose(val[i, 3], phi + 1 / d, rtol=0, atol=1e-13)
     h = 1e-5
+    T = oracle.sym3_full(third)
     for b in range(3):
         e = np.zeros(3)
         e[b] = h
-        vp, _ = oracle.ewald_corr(r + e)
-        vm, _ = oracle.ewald_corr(r - e)
+        vp, jp, _ = oracle.ewald_corr(r + e)
+        vm, jm, _ = oracle.ewald_corr(r - e)
         fd = (vp - vm) / (2 * h)
         np.testing.assert_allclose(jac[:, :, b], fd, rtol=0, atol=1e-8 * np.abs(jac).max())
+        np.testing.assert_allclose(T[:, :, :, b], (jp - jm)[:, :3, :] / (2 * h), rtol=0,
+                                   atol=1e-8 * np.abs(T).max())
     np.testing.assert_allclose(jac[:, 3, :], val[:, :3], rtol=0, atol=1e-13)
-    v0, j0 = oracle.ewald_corr(np.zeros((1, 3)))
+    np.testing.assert_allclose(np.trace(jac[:, :3, :], axis1=1, axis2=2), -4 * np.pi, rtol=1e-10)
+    np.testing.assert_allclose(np.einsum("nijj->ni", T), 0, atol=1e-9 * np.abs(T).max())
+    v0, j0, t0 = oracle.ewald_corr(np.zeros((1, 3)))
     np.testing.assert_allclose(v0[0, 3], 2 * 1.4186487533, rtol=2e-8)
     np.testing.assert_allclose(v0[0, 3], 2.837297479, rtol=1e-9)
     np.testing.assert_allclose(j0[0, :3, :], -4 * np.pi / 3 * np.eye(3), rtol=0, atol=1e-12)
-    assert np.abs(v0[0, :3]).max() < 1e-15
-    vn, jn = oracle.ewald_corr(-r)
+    assert np.abs(v0[0, :3]).max() < 1e-15 and np.abs(t0).max() < 1e-12
+    vn, jn, tn = oracle.ewald_corr(-r)
     np.testing.assert_allclose(vn[:, :3], -val[:, :3], rtol=0, atol=1e-12)
     np.testing.assert_allclose(vn[:, 3], val[:, 3], rtol=0, atol=1e-12)
-    v2, _ = oracle.ewald_corr(2.0 * r[:8], L=2.0)
+    np.testing.assert_allclose(tn, -third, rtol=0, atol=1e-10)
+    v2, _, _ = oracle.ewald_corr(2.0 * r[:8], L=2.0)
     np.testing.assert_allclose(v2[:, :3], val[:8, :3] / 4, rtol=0, atol=1e-12)
     np.testing.assert_allclose(v2[:, 3], val[:8, 3] / 2, rtol=0, atol=1e-12)
 
 
 def test_ewald_corr_table_form_pins():
-    """The tabulated form (nearest node + first-order step, psi with the trace
-    part of its second-order step): exact at the nodes;
-    its error against the exact sum is second order -- it shrinks ~4x when the
-    node spacing halves -- and, at the 64 nodes per half box the GPU tables
-    hold, stays below 3e-4 of max |C| over the box; inside one node cell the
-    step is linear in the offset."""
+    """The tabulated form (nearest node, Taylor steps with the node's value,
+    Jacobian and second derivatives): exact at the nodes; its error against the
+    exact sum falls faster than h^2 when the node spacing halves and, at the 64
+    nodes per half box the GPU tables hold, stays below 5e-6 of max |C| over the
+    box; the step from a node is the Taylor polynomial of its derivatives."""
     rng = np.random.default_rng(6)
     r = rng.uniform(-0.5, 0.5, (400, 3))
-    exact, _ = oracle.ewald_corr(r)
+    exact, _, _ = oracle.ewald_corr(r)
     errs = {}
     for nodes in (17, 33, 65):
-        tab, _ = oracle.ewald_corr(r, nodes=nodes)
+        tab, _, _ = oracle.ewald_corr(r, nodes=nodes)
         errs[nodes] = np.abs(tab - exact).max()
-    assert 3.0 < errs[17] / errs[33] < 5.0, errs
-    assert 3.0 < errs[33] / errs[65] < 5.0, errs
-    tab64, j64 = oracle.ewald_corr(r, nodes=64)
-    assert np.abs(tab64 - exact).max() < 3e-4 * np.abs(exact[:, :3]).max()
+    assert errs[17] / errs[33] > 4.0 and errs[33] / errs[65] > 4.0, errs
+    tab64, j64, t64 = oracle.ewald_corr(r, nodes=64)
+    assert np.abs(tab64 - exact).max() < 5e-6 * np.abs(exact[:, :3]).max()
     h = 0.5 / 63
     nodes = np.rint(r / h) * h
-    vn, jn = oracle.ewald_corr(nodes)
-    tn, _ = oracle.ewald_corr(nodes, nodes=64)
-    np.testing.assert_allclose(tn, vn, rtol=0, atol=1e-14)
-    np.testing.assert_allclose(j64, jn, rtol=0, atol=0)
+    vn, jn, tn = oracle.ewald_corr(nodes)
+    tv, tj, tt = oracle.ewald_corr(nodes, nodes=64)
+    np.testing.assert_allclose(tv, vn, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(tt, tn, rtol=0, atol=0)
+    np.testing.assert_array_equal(t64, tn)  # the node's second derivatives
     d = r - nodes
-    step = np.einsum("nij,nj->ni", jn, d)
-    step[:, 3] -= 2 * np.pi / 3 * (d * d).sum(1)
-    np.testing.assert_allclose(tab64, vn + step, rtol=0, atol=1e-14)
-    # the trace term removes psi's systematic second-order bias: the mean error
-    # over the box is far below the mean |second-order term|
-    assert abs((tab64 - exact)[:, 3].mean()) < 0.05 * (2 * np.pi / 3 * (d * d).sum(1)).mean()
+    T = oracle.sym3_full(tn)
+    Td = np.einsum("nabc,nc->nab", T, d)
+    Tdd = np.einsum("nab,nb->na", Td, d)
+    c = vn[:, :3] + np.einsum("nab,nb->na", jn[:, :3, :], d) + 0.5 * Tdd
+    psi = (vn[:, 3] + (vn[:, :3] * d).sum(1) + 0.5 * np.einsum("na,nab,nb->n", d, jn[:, :3, :], d)
+           + (Tdd * d).sum(1) / 6)
+    np.testing.assert_allclose(tab64[:, :3], c, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(tab64[:, 3], psi, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(j64[:, :3, :], jn[:, :3, :] + Td, rtol=0, atol=1e-12)
 
 
 @pytest.mark.parametrize("soft,eps", [(0, 0.0), (1, 0.05)])
