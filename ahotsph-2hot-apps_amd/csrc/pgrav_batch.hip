@@ -83,11 +83,14 @@ __device__ __forceinline__ const double *ewald_table(const BatchArgs &A, double)
 }
 
 // Periodic (nearest image + Ewald correction) per group, in LDS: which entries
-// of the current tile take the correction per pair, and the group's expansion
-// of the correction (20 sums, below) accumulated over the list's tiles.
+// of the current tile take the correction per pair, each lane's running sums of
+// the expansion terms (20, below) of the entries it loaded, and the group's sums.
+template <typename T, int PER>
 struct EwShared {
+    static constexpr int NP = PER == 1 ? 20 : 1;
     int pair[64];
-    double acc[20];
+    T part[NP][64];
+    double acc[NP];
 };
 
 // The correction summed over a group's expanded sources (group radius small
@@ -162,15 +165,13 @@ __device__ __forceinline__ V4<T> source_offset(double4 c, double4 gc, double box
 // The Ewald correction of one tile entry (periodic = 1), in the tile loader:
 // either the entry joins the group's expansion -- its (psi, C, J, T) at r0 =
 // source - centre, times its mass, plus a cell's second-moment coupling -- or
-// it is marked for per-pair lookups.  Every lane of the wave calls this (the
-// sums are wave reductions); `on` says whether this lane holds an entry.
-template <typename T>
-__device__ __forceinline__ void ewald_tile_entry(const BatchArgs &A, EwShared *ew, bool on,
+// it is marked for per-pair lookups.  `on` says whether this lane holds an
+// entry; the terms go to the lane's own running sums in LDS.
+template <typename T, int PER>
+__device__ __forceinline__ void ewald_tile_entry(const BatchArgs &A, EwShared<T, PER> *ew, bool on,
                                                  int lane, double4 src, double4 gc, bool cell,
                                                  const T *rec) {
     T v[20];
-#pragma unroll
-    for (int q = 0; q < 20; q++) v[q] = T(0);
     if (on) {
         double dx = src.x - gc.x, dy = src.y - gc.y, dz = src.z - gc.z;
         dx -= A.box * rint(dx / A.box);
@@ -181,6 +182,8 @@ __device__ __forceinline__ void ewald_tile_entry(const BatchArgs &A, EwShared *e
         // and the source's other images are far against the group radius
         const bool expand = am < 0.5 * A.box - gc.w && gc.w <= A.ewratio * (A.box - am);
         ew->pair[lane] = expand ? 0 : 1;
+#pragma unroll
+        for (int q = 0; q < 20; q++) v[q] = T(0);
         if (expand) {
             EwField<T> F;
             ewald_field<T>(ewald_table(A, T(0)), T(dx), T(dy), T(dz), T(A.box), F);
@@ -207,12 +210,9 @@ __device__ __forceinline__ void ewald_tile_entry(const BatchArgs &A, EwShared *e
                 v[3] += h * (t[2] * qxx + t[7] * qyy + t[9] * qzz + two * (t[4] * qxy + t[5] * qxz + t[8] * qyz));
             }
         }
-    }
+        // this lane's running sums (summed over the lanes once per group)
 #pragma unroll
-    for (int q = 0; q < 20; q++) {
-        const T sum = wave_sum(v[q]);
-        if (lane == 0)
-            ew->acc[q] += (double)sum;
+        for (int q = 0; q < 20; q++) ew->part[q][lane] += v[q];
     }
 }
 
@@ -220,7 +220,8 @@ __device__ __forceinline__ void ewald_tile_entry(const BatchArgs &A, EwShared *e
 // BU: unroll factor of the order-0 and body loops (2, 4 or 8; the autotuner picks it
 // for the group-relative compensated FP32 kernels, the others use 4)
 template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU>
-__device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds, EwShared *ew) {
+__device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds,
+                                           EwShared<T, PER> *ew) {
     constexpr bool PERIODIC = PER == 1;
     const int lane = threadIdx.x;
     const int cnt = A.gcount[g];
@@ -254,9 +255,8 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
     // potential's second-order Ewald term, the same for every sink of the group)
     double trace = 0.0;
     if (PERIODIC) {
-        if (lane < 20)
-            ew->acc[lane] = 0.0;
-        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 20; q++) ew->part[q][lane] = T(0);
     }
 
 #pragma unroll 1
@@ -284,41 +284,47 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                     trace += (double)mom[(size_t)e * PREC4 + PREC_TRACE];
             }
             if (PERIODIC)
-                ewald_tile_entry<T>(A, ew, lane < nt, lane, src, gc, kind < 3,
-                                    mom + (size_t)e * PREC4);
+                ewald_tile_entry<T, PER>(A, ew, lane < nt, lane, src, gc, kind < 3,
+                                         mom + (size_t)e * PREC4);
             __syncthreads();
             Acc4T<T> p = {0, 0, 0, 0};
+            // Offsets from the sink: an expanded source's image is the centre's for
+            // every sink of the group (its offset from the centre was wrapped in the
+            // loader and no sink crosses a half box from it), so only the sources
+            // taken per pair need the nearest image per sink.
+#define PGRAV_OFFSET(r, x, y, z, pp)                                \
+    const bool pp = PERIODIC && ew->pair[k];                        \
+    T x = sx - r.x, y = sy - r.y, z = sz - r.z;                     \
+    if (pp) {                                                       \
+        x = pbc<PERIODIC>(x, box, ibox);                            \
+        y = pbc<PERIODIC>(y, box, ibox);                            \
+        z = pbc<PERIODIC>(z, box, ibox);                            \
+    }
             if (kind == 0) {
                 for (int k = 0; k < nt; k++) {
                     const uint32_t ek = idx[t0 + k];
                     const V4<T> r = lds[k];
-                    const T x = pbc<PERIODIC>(sx - r.x, box, ibox);
-                    const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
-                    const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
+                    PGRAV_OFFSET(r, x, y, z, pp)
                     cell_interact<4, false, 1, T>(mom + (size_t)ek * PREC4, r.w, x, y, z, eps2_cell, p);
-                    if (PERIODIC && ew->pair[k])
+                    if (pp)
                         ewald_pair<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
                 }
             } else if (kind == 1) {
                 for (int k = 0; k < nt; k++) {
                     const uint32_t ek = idx[t0 + k];
                     const V4<T> r = lds[k];
-                    const T x = pbc<PERIODIC>(sx - r.x, box, ibox);
-                    const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
-                    const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
+                    PGRAV_OFFSET(r, x, y, z, pp)
                     cell_interact<2, false, 1, T>(mom + (size_t)ek * PREC4, r.w, x, y, z, eps2_cell, p);
-                    if (PERIODIC && ew->pair[k])
+                    if (pp)
                         ewald_pair<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
                 }
             } else if (kind == 2) {
 #pragma unroll BU
                 for (int k = 0; k < nt; k++) {
                     const V4<T> r = lds[k];
-                    const T x = pbc<PERIODIC>(sx - r.x, box, ibox);
-                    const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
-                    const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
+                    PGRAV_OFFSET(r, x, y, z, pp)
                     mono_interact<T>(r.w, x, y, z, eps2_cell, p);
-                    if (PERIODIC && ew->pair[k])
+                    if (pp)
                         ewald_pair<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
                 }
             } else {
@@ -326,9 +332,7 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
 #pragma unroll BU
                 for (int k = 0; k < nt; k++) {
                     const V4<T> r = lds[k];
-                    const T x = pbc<PERIODIC>(sx - r.x, box, ibox);
-                    const T y = pbc<PERIODIC>(sy - r.y, box, ibox);
-                    const T z = pbc<PERIODIC>(sz - r.z, box, ibox);
+                    PGRAV_OFFSET(r, x, y, z, pp)
                     if (SOFT == SOFT_K1) {
                         k1_interact<T>(r.w, x, y, z, eps_inv, eps2, p);
                     } else if (SOFT == SOFT_PLUMMER) {
@@ -340,15 +344,24 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                     // the self pair too: at r = 0 the correction is its limit, C = 0 and
                     // psi(0) = 2.8373/L, the body's potential from its own images and the
                     // background
-                    if (PERIODIC && ew->pair[k])
+                    if (pp)
                         ewald_pair<T>(ewald_table(A, T(0)), r.w, x, y, z, box, p);
                 }
             }
+#undef PGRAV_OFFSET
             __syncthreads();
             fold<T>(s, c, p, COMP);
         }
     }
     if (PERIODIC) {
+        // the lanes' running sums, summed per term in a fixed order (the last
+        // tile's barrier has passed)
+        if (lane < 20) {
+            double t = 0.0;
+            for (int k = 0; k < 64; k++) t += (double)ew->part[lane][k];
+            ew->acc[lane] = t;
+        }
+        __syncthreads();
         Acc4T<T> p = {0, 0, 0, 0};
         // the expanded sources' correction at this sink's offset from the centre
         ewald_expansion_apply<T>(ew->acc, T(sp.x - gc.x), T(sp.y - gc.y), T(sp.z - gc.z), p);
@@ -389,7 +402,7 @@ template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PER>::value)))
 void k_batch(BatchArgs A) {
     __shared__ V4<T> lds[64];
-    __shared__ EwShared ew;
+    __shared__ EwShared<T, PER> ew;
     eval_group<T, REL, COMP, SOFT, PER, BU>(A, A.order_np[blockIdx.x], lds, &ew);
 }
 
@@ -429,7 +442,7 @@ void k_batch_persistent(
     A.counter = counter;
     A.ewald = ewald;
     __shared__ V4<T> lds[64];
-    __shared__ EwShared ew;
+    __shared__ EwShared<T, PER> ew;
     __shared__ int w;
     const int x = blockIdx.x % NXCD;
     // own slice first; once it is drained, the next group of the other XCDs' slices
@@ -471,7 +484,7 @@ void k_batch_persistent(
 template <int SOFT, int PER>
 __global__ __launch_bounds__(64) void k_group64(BatchArgs A, int g) {
     __shared__ V4<double> lds[64];
-    __shared__ EwShared ew;
+    __shared__ EwShared<double, PER> ew;
     eval_group<double, false, false, SOFT, PER, 4>(A, g, lds, &ew);
 }
 

@@ -379,10 +379,21 @@ def main(argv=None):
     ap.add_argument("--thetas", type=float, nargs="+", default=None)
     ap.add_argument("--scheme", choices=["ewald", "images"], default="ewald")
     ap.add_argument("--nimage", type=int, default=2)
+    ap.add_argument("--copy-check-mib", type=int, default=0,
+                    help="first run a plain elementwise kernel over this many MiB (a known "
+                         "byte count for calibrating the HBM traffic counters)")
     a = ap.parse_args(argv)
     a.thetas_given = a.thetas is not None
     a.thetas = a.thetas or [0.5, 0.6, 0.7]
     import torch  # noqa: F401  (one HIP runtime in the process: torch's)
+    if a.copy_check_mib > 0:
+        src = torch.ones(a.copy_check_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 2.0, out=dst)  # reads and writes copy_check_mib MiB each
+        torch.cuda.synchronize()
+        _log("copy check: 3 x (read %d MiB, write %d MiB)" % (a.copy_check_mib, a.copy_check_mib))
+        del src, dst
     if a.config == "c2":
         a.n = a.n or 2097152
         c2(a)
