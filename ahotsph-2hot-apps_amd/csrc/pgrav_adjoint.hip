@@ -14,15 +14,16 @@
 //                    derivative along the sink's weight w through the very same
 //                    cell_interact / mono_interact code gives the sink's
 //                    gradient; the pair's source-side share (minus that, summed
-//                    over the group's sinks by a DPP wave reduction) is written
-//                    once per list entry: gpair (cells: dL/dX_c) and gentry
-//                    (bodies: dL/dx_j and dL/dm_j).
+//                    over the group's sinks, 8 entries at a time through LDS) is
+//                    written once per list entry: gpair (cells: dL/dX_c) and
+//                    gentry (bodies: dL/dx_j and dL/dm_j).
 //  k_adjoint_cells   one wave per listed cell and chunk of <= 128 listing groups
 //                    (the transposed lists): the 43 prepared-record
 //                    coefficients' gradient dL/dP (the interaction is linear in
 //                    P: the hand-written transpose of the evaluation) plus the
 //                    chunk's gpair entries, one output record per chunk.
-//  k_adjoint_bodygather  per body: its gentry entries in list order.
+//  k_adjoint_bodychunks, k_adjoint_bodygather  per body: its gentry entries in
+//                    list order, read by leaf segments, in balanced chunks.
 //  k_adjoint_nodes   per cell: its chunks in order; dL/dP -> dL/d(raw moments)
 //                    through the fixed linear map raw -> record -> prepared
 //                    record, and the total dL/dX_c including the centre
@@ -68,12 +69,16 @@ __device__ __forceinline__ D1 nrsq(D1 s) {
 #define ADJ_CELL 52  // per work item / node: 46 dL/dP, 3 dL/dX (pair part), pad
 #define L_BODY_KIND 3  // list kinds 0..2 are cells (orders 4, 2, 0), 3 bodies
 
-// dL/dP for one (sink, cell) pair: the evaluation in cell_interact is linear in
-// the prepared record P; with w, v the sink's weights these are the
-// coefficients of P in w.a + v.phi (see pgrav_device.h for the formulas).
-template <int ORDER>
-__device__ __forceinline__ void cell_beta(float x, float y, float z, float eps2, float wx_,
-                                          float wy_, float wz_, float v, float *beta) {
+// dL/dP for one (sink, cell) pair, added to beta: the evaluation in
+// cell_interact is linear in the prepared record P; with w, v the sink's
+// weights these are the coefficients of P in w.a + v.phi (see pgrav_device.h
+// for the formulas).  The order-2 and order-0 coefficients are the leading
+// ones of the order-4 form, so one branch-free form serves every list kind:
+// k2 (1 for orders 2 and 4, else 0) and k34 (1 for order 4) switch the higher
+// orders' terms off.  w = v = 0 gives zero terms.
+__device__ __forceinline__ void cell_beta_add(float x, float y, float z, float eps2, float wx_,
+                                              float wy_, float wz_, float v, float k2, float k34,
+                                              float *beta) {
     const float x2 = x * x, y2 = y * y, z2 = z * z;
     const float r2 = x2 + y2 + z2 + eps2;
     const float rinv = nrsq(r2);
@@ -84,26 +89,20 @@ __device__ __forceinline__ void cell_beta(float x, float y, float z, float eps2,
     const float rc = 2.0f * ri2, rb = 1.5f * ri2, rq = (4.0f / 3.0f) * ri2;
     const float at = ra * wdx * rc;        // dL/dT'
     const float avx = -ra * wx_, avy = -ra * wy_, avz = -ra * wz_;  // dL/dV
-#pragma unroll
-    for (int k = 0; k < 46; k++) beta[k] = 0.0f;
-    beta[3] = as;
-    if (ORDER == 0)
-        return;
-    const float c2 = rc;
-    const float u2 = as * ri2 * ri2 + at * c2;
+    beta[3] += as;
+    const float c2 = k2 * rc;
+    const float u2 = k2 * (as * ri2 * ri2 + at * rc);
     const float G2x = u2 * x + c2 * avx, G2y = u2 * y + c2 * avy, G2z = u2 * z + c2 * avz;
-    beta[4] = G2x * x;
-    beta[5] = G2x * y + G2y * x;
-    beta[6] = G2x * z + G2z * x;
-    beta[7] = G2y * y;
-    beta[8] = G2y * z + G2z * y;
-    beta[9] = G2z * z;
-    if (ORDER == 2)
-        return;
-    const float c3 = rc * rb, c4 = rc * rb * rq;
+    beta[4] += G2x * x;
+    beta[5] += G2x * y + G2y * x;
+    beta[6] += G2x * z + G2z * x;
+    beta[7] += G2y * y;
+    beta[8] += G2y * z + G2z * y;
+    beta[9] += G2z * z;
+    const float c3 = k34 * (rc * rb), c4 = c3 * rq;
     const float ri6 = ri2 * ri2 * ri2;
-    const float u3 = as * ri6 + at * c3;
-    const float u4 = as * ri6 * ri2 + at * c4;
+    const float u3 = k34 * (as * ri6) + at * c3;
+    const float u4 = k34 * (as * ri6 * ri2) + at * c4;
     const float G3[3] = {u3 * x + c3 * avx, u3 * y + c3 * avy, u3 * z + c3 * avz};
     const float G4[3] = {u4 * x + c4 * avx, u4 * y + c4 * avy, u4 * z + c4 * avz};
     const float A = x2 - z2, B = y2 - z2;
@@ -111,9 +110,12 @@ __device__ __forceinline__ void cell_beta(float x, float y, float z, float eps2,
     const float z3 = 3.0f * z2;
     const float U[7] = {x * (x2 - z3), y * (y2 - z3), y * A, z * (3.0f * x2 - z2), x * B, x * y * z,
                         z * (3.0f * y2 - z2)};
+#pragma unroll
     for (int r = 0; r < 3; r++) {
-        for (int k = 0; k < 5; k++) beta[10 + 5 * r + k] = G3[r] * b3[k];
-        for (int k = 0; k < 7; k++) beta[25 + 7 * r + k] = G4[r] * U[k];
+#pragma unroll
+        for (int k = 0; k < 5; k++) beta[10 + 5 * r + k] += G3[r] * b3[k];
+#pragma unroll
+        for (int k = 0; k < 7; k++) beta[25 + 7 * r + k] += G4[r] * U[k];
     }
 }
 
@@ -125,51 +127,90 @@ __device__ __forceinline__ int64_t sink_slot(const BatchView &V, int64_t ref, in
     return V.gslot ? (int64_t)V.gslot[V.gslot_off[ref] + lane] : ref + lane;
 }
 
+// The list entries' source-side shares, summed over the group's sinks without a
+// wave reduction per entry: entry k's per-lane values wait in red[k % 8][lane];
+// after every 8 entries (and at a tile's end) lane 8 s + p adds sinks
+// 8 p .. 8 p + 7 of entry s (in a fixed order, staggered by p across the LDS
+// banks), three DPP steps add the eight partial sums within the lane's row half,
+// and lane 8 s writes entry s: out[base + s], s < n.
+__device__ __forceinline__ void flush_entries(float4 (*red)[64], int n, float4 *__restrict__ out,
+                                              int64_t base) {
+    __syncthreads();
+    const int lane = threadIdx.x, s = lane >> 3, p = lane & 7;
+    float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const float4 v = red[s][p * 8 + ((i + p) & 7)];
+        ax += v.x;
+        ay += v.y;
+        az += v.z;
+        aw += v.w;
+    }
+    ax = dpp_add<0x141, 0xf>(dpp_add<0x4e, 0xf>(dpp_add<0xb1, 0xf>(ax)));
+    ay = dpp_add<0x141, 0xf>(dpp_add<0x4e, 0xf>(dpp_add<0xb1, 0xf>(ay)));
+    az = dpp_add<0x141, 0xf>(dpp_add<0x4e, 0xf>(dpp_add<0xb1, 0xf>(az)));
+    aw = dpp_add<0x141, 0xf>(dpp_add<0x4e, 0xf>(dpp_add<0xb1, 0xf>(aw)));
+    if (p == 0 && s < n)
+        out[base + s] = make_float4(ax, ay, az, aw);
+    __syncthreads();
+}
+
+// Every buffer arrives as its own __restrict__ argument: the kernel stores
+// entry results between its reads of the lists and records, and only pointers
+// known to be distinct let the compiler read those through scalar loads.
 template <int SOFT>
-__global__ __launch_bounds__(64) void k_adjoint(BatchView V, AdjointView A) {
+__global__ __launch_bounds__(64) void k_adjoint(
+    BatchView V, AdjointView A, const int *__restrict__ order, const int *__restrict__ gcount,
+    const double4 *__restrict__ gcen, const int64_t *__restrict__ gfirst,
+    const int64_t *__restrict__ gslot_off, const uint32_t *__restrict__ gslot,
+    const int64_t *__restrict__ off, const uint32_t *__restrict__ idx,
+    const float *__restrict__ mom, const double4 *__restrict__ bpos,
+    const double4 *__restrict__ cpos, const float4 *__restrict__ wv, float4 *__restrict__ gpair,
+    float4 *__restrict__ gentry, float4 *__restrict__ gsink) {
     __shared__ float4 lds[64];
+    __shared__ float4 red[8][64];
     // XCD-aware order: consecutive workgroups land on different XCDs, so the
     // forward's per-XCD slices of Morton-neighbouring groups keep their cells in
     // one XCD's L2 here too
-    const int g = V.order_np ? V.order_np[blockIdx.x] : (int)blockIdx.x;
+    const int g = order ? order[blockIdx.x] : (int)blockIdx.x;
     const int lane = threadIdx.x;
-    const int cnt = V.gcount[g];
-    const double4 gc = V.gcen[g];
+    const int cnt = gcount[g];
+    const double4 gc = gcen[g];
     const bool live = lane < cnt;
-    const int64_t me = sink_slot(V, V.gslot ? g : V.gfirst[g], live ? lane : 0);
-    const double4 sp = V.bpos[me];
+    const int ls = live ? lane : 0;
+    const int64_t me = gslot ? (int64_t)gslot[gslot_off[g] + ls] : gfirst[g] + ls;
+    const double4 sp = bpos[me];
     const float sx = (float)(sp.x - gc.x), sy = (float)(sp.y - gc.y), sz = (float)(sp.z - gc.z);
-    const float4 w = live ? A.wv[me] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 w = live ? wv[me] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float eps = (float)V.eps, eps2 = eps * eps;
     const float eps_inv = SOFT == SOFT_K1 ? 1.0f / eps : 0.0f;
     const float eps2c = SOFT == SOFT_PLUMMER ? eps2 : 0.0f;
+    const int64_t ncell = A.ncell_entries;
     float gx = 0.f, gy = 0.f, gz = 0.f;
 
     for (int kind = 0; kind < 4; kind++) {
-        const int64_t *offk = V.off + (int64_t)kind * (V.ngroups + 1);
+        const int64_t *offk = off + (int64_t)kind * (V.ngroups + 1);
         const int64_t o0 = offk[g], o1 = offk[g + 1];
-        const uint32_t *__restrict__ idx = V.idx;
         for (int64_t t0 = o0; t0 < o1; t0 += 64) {
             const int nt = (int)min((int64_t)64, o1 - t0);
             if (lane < nt) {
                 const uint32_t e = idx[t0 + lane];
-                const double4 s = kind == 3 ? V.bpos[e] : V.cpos[e];
+                const double4 s = kind == 3 ? bpos[e] : cpos[e];
                 lds[lane] = make_float4((float)(s.x - gc.x), (float)(s.y - gc.y),
                                         (float)(s.z - gc.z), (float)s.w);
             }
             __syncthreads();
-            for (int k = 0; k < nt; k++) {
-                const float4 r = lds[k];
-                const float X = sx - r.x, Y = sy - r.y, Z = sz - r.z;
-                // the sink offset, differentiated along this sink's weight w
-                const D1 dx(X, w.x), dy(Y, w.y), dz(Z, w.z);
-                Acc4T<D1> a = {D1(0.f), D1(0.f), D1(0.f), D1(0.f)};
-                if (kind < L_BODY_KIND) {
+            if (kind < L_BODY_KIND) {
+                for (int k = 0; k < nt; k++) {
+                    const float4 r = lds[k];
+                    // the sink offset, differentiated along this sink's weight w
+                    const D1 dx(sx - r.x, w.x), dy(sy - r.y, w.y), dz(sz - r.z, w.z);
+                    Acc4T<D1> a = {D1(0.f), D1(0.f), D1(0.f), D1(0.f)};
                     const uint32_t e = idx[t0 + k];
                     if (kind == 0)
-                        cell_interact<4, false, 1, D1, float>(V.mom32 + (size_t)e * PREC4, D1(r.w), dx, dy, dz, D1(eps2c), a);
+                        cell_interact<4, false, 1, D1, float>(mom + (size_t)e * PREC4, D1(r.w), dx, dy, dz, D1(eps2c), a);
                     else if (kind == 1)
-                        cell_interact<2, false, 1, D1, float>(V.mom32 + (size_t)e * PREC4, D1(r.w), dx, dy, dz, D1(eps2c), a);
+                        cell_interact<2, false, 1, D1, float>(mom + (size_t)e * PREC4, D1(r.w), dx, dy, dz, D1(eps2c), a);
                     else
                         mono_interact<D1>(D1(r.w), dx, dy, dz, D1(eps2c), a);
                     // d(w.a + v phi)/d(sink position) = J w - v a; the cell's centre
@@ -180,36 +221,60 @@ __global__ __launch_bounds__(64) void k_adjoint(BatchView V, AdjointView A) {
                     gx += px;
                     gy += py;
                     gz += pz;
-                    const float cx = wave_sum(-px), cy = wave_sum(-py), cz = wave_sum(-pz);
-                    if (lane == 0)
-                        A.gpair[t0 + k] = make_float4(cx, cy, cz, 0.f);
-                } else {
-                    // bodies at unit mass: the pair is linear in the source mass
-                    const bool self = X == 0.f && Y == 0.f && Z == 0.f;
-                    if (SOFT == SOFT_K1)
-                        k1_interact<D1>(D1(1.f), dx, dy, dz, D1(eps_inv), D1(eps2), a);
-                    else if (SOFT == SOFT_PLUMMER)
-                        mono_interact<D1>(D1(1.f), dx, dy, dz, D1(eps2), a);
-                    else if (!self)
-                        mono_interact<D1>(D1(1.f), dx, dy, dz, D1(0.f), a);
-                    const float px = r.w * (a.x.d - w.w * a.x.v);
-                    const float py = r.w * (a.y.d - w.w * a.y.v);
-                    const float pz = r.w * (a.z.d - w.w * a.z.v);
-                    const float pm = w.x * a.x.v + w.y * a.y.v + w.z * a.z.v + w.w * a.w.v;
-                    gx += px;
-                    gy += py;
-                    gz += pz;
-                    const float bx = wave_sum(-px), by = wave_sum(-py), bz = wave_sum(-pz),
-                                bm = wave_sum(pm);
-                    if (lane == 0)
-                        A.gentry[t0 + k - A.ncell_entries] = make_float4(bx, by, bz, bm);
+                    red[k & 7][lane] = make_float4(-px, -py, -pz, 0.f);
+                    if ((k & 7) == 7 || k == nt - 1)
+                        flush_entries(red, (k & 7) + 1, gpair, t0 + (k & ~7));
+                }
+            } else {
+                // bodies, 8 entries per block with the LDS reads of the block in flight
+                // together (the pair is ~30 VALU)
+                for (int k0 = 0; k0 < nt; k0 += 8) {
+                    const int m = min(8, nt - k0);
+#pragma unroll
+                    for (int i = 0; i < 8; i++) {
+                        const float4 r = lds[k0 + i];  // k0 + i < 64: inside the tile buffer
+                        if (i < m) {
+                            const float X = sx - r.x, Y = sy - r.y, Z = sz - r.z;
+                            float px, py, pz, pm;
+                            if (SOFT == SOFT_K1) {
+                                const D1 dx(X, w.x), dy(Y, w.y), dz(Z, w.z);
+                                Acc4T<D1> a = {D1(0.f), D1(0.f), D1(0.f), D1(0.f)};
+                                // unit mass: the pair is linear in the source mass
+                                k1_interact<D1>(D1(1.f), dx, dy, dz, D1(eps_inv), D1(eps2), a);
+                                px = r.w * (a.x.d - w.w * a.x.v);
+                                py = r.w * (a.y.d - w.w * a.y.v);
+                                pz = r.w * (a.z.d - w.w * a.z.v);
+                                pm = w.x * a.x.v + w.y * a.y.v + w.z * a.z.v + w.w * a.w.v;
+                            } else {
+                                // Plummer (or none): with f = 1/sqrt(r^2 + eps^2), a = -m f^3 x and
+                                // phi = -m f, so J w - v a = m f^3 ((3 f^2 x.w + v) x - w) and
+                                // d(w.a + v phi)/dm = -(f^3 x.w + v f); the unsoftened self pair is 0
+                                const float r2 = fmaf(X, X, fmaf(Y, Y, fmaf(Z, Z, eps2c)));
+                                const bool self = SOFT == SOFT_NONE && r2 == 0.f;
+                                const float f = self ? 0.f : __builtin_amdgcn_rsqf(r2);
+                                const float f2 = f * f, f3 = f2 * f;
+                                const float xw = w.x * X + w.y * Y + w.z * Z;
+                                const float t = 3.0f * f2 * xw + w.w;
+                                const float mf3 = r.w * f3;
+                                px = mf3 * (t * X - w.x);
+                                py = mf3 * (t * Y - w.y);
+                                pz = mf3 * (t * Z - w.z);
+                                pm = -(f3 * xw + w.w * f);
+                            }
+                            gx += px;
+                            gy += py;
+                            gz += pz;
+                            red[i][lane] = make_float4(-px, -py, -pz, pm);
+                        }
+                    }
+                    flush_entries(red, m, gentry, t0 + k0 - ncell);
                 }
             }
             __syncthreads();
         }
     }
     if (live)
-        A.gsink[me] = make_float4(gx, gy, gz, 0.f);
+        gsink[me] = make_float4(gx, gy, gz, 0.f);
 }
 
 // ---- transposed cell lists (built by the host): for every cell, the groups
@@ -223,7 +288,9 @@ __global__ __launch_bounds__(64) void k_adjoint(BatchView V, AdjointView A) {
 // the end, into the chunk's own output record.  The chunk's list entries arrive
 // 64 at a time in one vector load and are read back with v_readlane; the next
 // group's sinks load while the current group is evaluated.
-template <int SOFT>
+// BODYSET: the groups are body sets (slot table), known at compile time so that
+// the run case has no dependent load in its sink prefetch
+template <int SOFT, bool BODYSET>
 __global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, AdjointView A) {
     const int item = blockIdx.x;
     const int c = A.tcells[item];
@@ -251,45 +318,53 @@ __global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, AdjointView A
             cnt = (int)(y & 255u);
             kind = (int)(y >> 8);
         };
-        uint32_t f;
-        int cnt, kind;
-        entry(0, f, cnt, kind);
-        double4 sp = make_double4(0, 0, 0, 0);
-        float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (lane < cnt) {
-            const int64_t me = sink_slot(V, f, lane);
-            sp = V.bpos[me];
-            w = A.wv[me];
-        }
-        for (int k = 0; k < nt; k++) {
-            uint32_t fn = 0;
-            int cn = 0, kn = 0;
-            double4 spn = make_double4(0, 0, 0, 0);
-            float4 wn = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (k + 1 < nt) {
-                entry(k + 1, fn, cn, kn);
-                if (lane < cn) {
-                    const int64_t me = sink_slot(V, fn, lane);
-                    spn = V.bpos[me];
-                    wn = A.wv[me];
-                }
-            }
-            if (lane < cnt) {
-                const float x = (float)(sp.x - X.x), y = (float)(sp.y - X.y), z = (float)(sp.z - X.z);
-                float beta[46];
-                if (kind == 0)
-                    cell_beta<4>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
-                else if (kind == 1)
-                    cell_beta<2>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
-                else
-                    cell_beta<0>(x, y, z, eps2c, w.x, w.y, w.z, w.w, beta);
-#pragma unroll
-                for (int q = 3; q < 46; q++) acc[q] += beta[q];
-            }
-            cnt = cn;
-            kind = kn;
-            sp = spn;
-            w = wn;
+        // entry k's sinks (positions, weights), loaded while the entry before it
+        // is evaluated: two register sets used in turn, so the loads of one are
+        // in flight while the other is read and no copy waits for them
+        // (every lane loads, past the last entry too, from a valid slot, so the
+        // number of loads in flight is the same on every path)
+        struct In {
+            double4 sp;
+            float4 w;
+            float on;
+            int kind;
+        };
+        auto fetch = [&](int k, In &s) {
+            uint32_t f;
+            int cnt;
+            entry(min(k, nt - 1), f, cnt, s.kind);
+            if (k >= nt)
+                cnt = 0;
+            const bool on = lane < cnt;
+            const int ls = on ? lane : 0;
+            const int64_t me = BODYSET ? (int64_t)V.gslot[V.gslot_off[f] + ls] : (int64_t)f + ls;
+            s.sp = V.bpos[me];
+            s.w = A.wv[me];
+            // lanes without a sink (and entries past the last) add zero terms: the
+            // weights are scaled by 0 at use (a product, not a select, so that the
+            // load stays unconditional)
+            s.on = on ? 1.f : 0.f;
+        };
+        auto eval = [&](const In &s) {
+            const float x = (float)(s.sp.x - X.x), y = (float)(s.sp.y - X.y),
+                        z = (float)(s.sp.z - X.z);
+            const float k2 = s.kind <= 1 ? 1.f : 0.f, k34 = s.kind == 0 ? 1.f : 0.f;
+            cell_beta_add(x, y, z, eps2c, s.on * s.w.x, s.on * s.w.y, s.on * s.w.z, s.on * s.w.w, k2,
+                          k34, acc);
+        };
+        // (the scheduling barriers keep the compiler from moving a fetch above
+        // the evaluation it is meant to overlap)
+        In ea, eb;
+        fetch(0, ea);
+        for (int k = 0; k < nt; k += 2) {
+            fetch(k + 1, eb);
+            __builtin_amdgcn_sched_barrier(0);
+            eval(ea);
+            __builtin_amdgcn_sched_barrier(0);
+            fetch(k + 2, ea);
+            __builtin_amdgcn_sched_barrier(0);
+            eval(eb);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
     float *out = A.gchunk + (size_t)item * ADJ_CELL;
@@ -301,24 +376,77 @@ __global__ __launch_bounds__(64) void k_adjoint_cells(BatchView V, AdjointView A
     }
     const float sx = wave_sum(px), sy = wave_sum(py), sz = wave_sum(pz);
     if (lane == 0) {
+        out[0] = out[1] = out[2] = 0.f;  // m and the dipole are not in the record's gradient
         out[46] = sx;
         out[47] = sy;
         out[48] = sz;
+        out[49] = out[50] = out[51] = 0.f;
     }
 }
 
-// per body: its body-list entries' source gradients, in list order
+// Per body: its body-list entries' source gradients, in list order, in two
+// passes.  Entry counts per body are very uneven (a few bodies are listed by
+// thousands of groups), so a body's leaf segments are cut into at most 64
+// chunks of >= 16 segments, one thread each (k_adjoint_bodychunks), and a
+// second pass adds each body's chunk sums in order (k_adjoint_bodygather).
+// The bodies of one leaf walk the leaf's segments together: one index record
+// per segment for all of them and consecutive entries.
+__device__ __forceinline__ int64_t body_chunk_len(int64_t nseg) {
+    return max((int64_t)16, (nseg + 63) / 64);
+}
+
+__global__ void k_adjoint_bodychunks(AdjointView A) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.nbchunks)
+        return;
+    const int j = A.pbody[t];
+    const int c = A.leaf_of[j];
+    const uint32_t u = (uint32_t)(j - A.lfirst[c]);
+    const int64_t q0 = A.loff[c], len = body_chunk_len(A.loff[c + 1] - q0);
+    const int64_t s0 = q0 + (t - A.pstart[j]) * len, s1 = min(A.loff[c + 1], s0 + len);
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    // 8 segments at a time: their index records, then their entries, in flight
+    // together; summed in list order
+    for (int64_t q = s0; q < s1; q += 8) {
+        float4 g[8];
+        bool in[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint4 sg = q + i < s1 ? A.lseg[q + i] : make_uint4(0u, 1u, 0u, 0u);
+            in[i] = u >= sg.y && u < sg.z;
+            g[i] = in[i] ? A.gentry[(int64_t)sg.x + (u - sg.y)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            if (in[i]) {
+                s.x += g[i].x;
+                s.y += g[i].y;
+                s.z += g[i].z;
+                s.w += g[i].w;
+            }
+    }
+    A.gbchunk[t] = s;
+}
+
 __global__ void k_adjoint_bodygather(AdjointView A) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= A.n)
         return;
+    const int64_t t1 = A.pstart[j + 1];
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int64_t e = A.boff[j]; e < A.boff[j + 1]; e++) {
-        const float4 g = A.gentry[A.blist[e]];
-        s.x += g.x;
-        s.y += g.y;
-        s.z += g.z;
-        s.w += g.w;
+    for (int64_t t0 = A.pstart[j]; t0 < t1; t0 += 8) {
+        float4 g[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            g[i] = t0 + i < t1 ? A.gbchunk[t0 + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            if (t0 + i < t1) {
+                s.x += g[i].x;
+                s.y += g[i].y;
+                s.z += g[i].z;
+                s.w += g[i].w;
+            }
     }
     A.gbody[j] = s;
 }
@@ -428,9 +556,13 @@ hipError_t launch_adjoint(const BatchView &v, const AdjointView &A, hipStream_t 
     if (v.ngroups <= 0)
         return hipSuccess;
 #define ADJ(S)                                                                                     \
-    hipLaunchKernelGGL(k_adjoint<S>, dim3(v.ngroups), dim3(64), 0, st, v, A);                      \
-    if (A.ntcells > 0)                                                                             \
-        hipLaunchKernelGGL(k_adjoint_cells<S>, dim3(A.ntcells), dim3(64), 0, st, v, A);
+    hipLaunchKernelGGL(k_adjoint<S>, dim3(v.ngroups), dim3(64), 0, st, v, A, v.order_np, v.gcount, \
+                       v.gcen, v.gfirst, v.gslot_off, v.gslot, v.off, v.idx, v.mom32, v.bpos,     \
+                       v.cpos, A.wv, A.gpair, A.gentry, A.gsink);                                 \
+    if (A.ntcells > 0 && v.gslot)                                                                  \
+        hipLaunchKernelGGL((k_adjoint_cells<S, true>), dim3(A.ntcells), dim3(64), 0, st, v, A);    \
+    else if (A.ntcells > 0)                                                                        \
+        hipLaunchKernelGGL((k_adjoint_cells<S, false>), dim3(A.ntcells), dim3(64), 0, st, v, A);
     if (v.soft == SOFT_K1) {
         ADJ(SOFT_K1)
     } else if (v.soft == SOFT_PLUMMER) {
@@ -439,6 +571,9 @@ hipError_t launch_adjoint(const BatchView &v, const AdjointView &A, hipStream_t 
         ADJ(SOFT_NONE)
     }
 #undef ADJ
+    if (A.nbchunks > 0)
+        hipLaunchKernelGGL(k_adjoint_bodychunks, dim3((unsigned)((A.nbchunks + 127) / 128)), dim3(128), 0,
+                           st, A);
     hipLaunchKernelGGL(k_adjoint_bodygather, dim3((unsigned)((A.n + 127) / 128)), dim3(128), 0, st, A);
     hipLaunchKernelGGL(k_adjoint_nodes, dim3((unsigned)((A.nnodes + 127) / 128)), dim3(128), 0, st, A);
     hipLaunchKernelGGL(k_adjoint_bodies, dim3((unsigned)((A.n + 127) / 128)), dim3(128), 0, st, v, A);

@@ -78,10 +78,19 @@ struct AdjointView {
     const uint4 *tlist = nullptr;
     float *gchunk = nullptr;              // 52 floats per work item
     const int64_t *cchunk_off = nullptr;  // node c: work items [cchunk_off[c], cchunk_off[c+1])
-    // transposed body lists: body j's entries are blist[boff[j] .. boff[j+1]]
-    // (positions relative to ncell_entries, ascending)
-    const int64_t *boff = nullptr;
-    const uint32_t *blist = nullptr;
+    // transposed body lists by leaf: leaf c's segments lseg[loff[c] .. loff[c+1])
+    // = (entry position of body offset u0, u0, u1) -- bodies lfirst[c] + u,
+    // u0 <= u < u1, at consecutive positions (relative to ncell_entries);
+    // ascending positions
+    const int64_t *loff = nullptr;
+    const uint4 *lseg = nullptr;
+    const int64_t *lfirst = nullptr;
+    // body j's gather chunks are gbchunk[pstart[j] .. pstart[j+1]); pbody[t] is
+    // chunk t's body
+    const int64_t *pstart = nullptr;
+    const int *pbody = nullptr;
+    float4 *gbchunk = nullptr;
+    int64_t nbchunks = 0;
     float4 *gbody = nullptr;
     const double *raw = nullptr;  // 35 raw moments per node
     const float *mat = nullptr;   // 46 x 35: raw moments -> prepared record

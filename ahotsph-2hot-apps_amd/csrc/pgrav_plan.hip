@@ -584,13 +584,44 @@ static int setup_adjoint(Plan *P) {
         if (toff[c + 1] - toff[c] > 1)
             std::sort(tl.begin() + toff[c], tl.begin() + toff[c + 1],
                       [](const uint4 &a, const uint4 &b) { return a.x != b.x ? a.x < b.x : a.z < b.z; });
-    // transposed body lists: each body's entries in ascending list position
-    std::vector<int64_t> boff(n + 1, 0);
-    for (int64_t e = 0; e < nbody; e++) boff[idx[ncell + e] + 1]++;
-    for (int64_t j = 0; j < n; j++) boff[j + 1] += boff[j];
-    std::vector<uint32_t> blist(std::max<int64_t>(1, nbody));
-    std::vector<int64_t> bcur(boff.begin(), boff.end() - 1);
-    for (int64_t e = 0; e < nbody; e++) blist[bcur[idx[ncell + e]]++] = (uint32_t)e;
+    // transposed body lists, per leaf: a body list names runs of consecutive
+    // sorted bodies, so a run's part inside one leaf is a segment (first entry
+    // position, first and end body offset in the leaf), and the leaf's bodies
+    // read their entries of that segment from consecutive positions; a leaf's
+    // segments in ascending list position
+    std::vector<int64_t> lfirst(nn, 0), loff(nn + 1, 0);
+    for (int64_t c = 0; c < nn; c++) lfirst[c] = t.first[c];
+    std::vector<uint4> seg;
+    std::vector<int> seg_leaf;
+    for (int64_t e = 0; e < nbody; e++) {
+        const int64_t j = idx[ncell + e];
+        const int c = leaf[j];
+        const uint32_t u = (uint32_t)(j - t.first[c]);
+        if (e && seg_leaf.back() == c && (int64_t)idx[ncell + e - 1] == j - 1 && seg.back().z == u) {
+            seg.back().z = u + 1;
+            continue;
+        }
+        seg.push_back(make_uint4((uint32_t)e, u, u + 1, 0u));
+        seg_leaf.push_back(c);
+    }
+    for (int c : seg_leaf) loff[c + 1]++;
+    for (int64_t c = 0; c < nn; c++) loff[c + 1] += loff[c];
+    std::vector<uint4> lseg(std::max<size_t>(1, seg.size()));
+    {
+        std::vector<int64_t> lcur(loff.begin(), loff.end() - 1);
+        for (size_t s = 0; s < seg.size(); s++) lseg[lcur[seg_leaf[s]]++] = seg[s];
+    }
+    // a body's segments in chunks of max(16, ceil(segments / 64)) (the device
+    // computes the same length from loff)
+    std::vector<int64_t> pstart(n + 1, 0);
+    for (int64_t j = 0; j < n; j++) {
+        const int64_t ns = loff[leaf[j] + 1] - loff[leaf[j]];
+        const int64_t len = std::max<int64_t>(16, (ns + 63) / 64);
+        pstart[j + 1] = pstart[j] + (ns + len - 1) / len;
+    }
+    std::vector<int> pbody(std::max<int64_t>(1, pstart[n]));
+    for (int64_t j = 0; j < n; j++)
+        for (int64_t t = pstart[j]; t < pstart[j + 1]; t++) pbody[t] = (int)j;
 
     A.wv = P->dalloc<float4>(sizeof(float4) * n);
     A.gsink = P->dalloc<float4>(sizeof(float4) * n);
@@ -604,8 +635,13 @@ static int setup_adjoint(Plan *P) {
     A.tlist = P->upload(tl.data(), tl.size());
     A.gchunk = P->dalloc<float>(sizeof(float) * 52 * std::max<size_t>(1, tcells.size()));
     A.cchunk_off = P->upload(cchunk.data(), cchunk.size());
-    A.boff = P->upload(boff.data(), boff.size());
-    A.blist = P->upload(blist.data(), blist.size());
+    A.loff = P->upload(loff.data(), loff.size());
+    A.lseg = P->upload(lseg.data(), lseg.size());
+    A.lfirst = P->upload(lfirst.data(), lfirst.size());
+    A.pstart = P->upload(pstart.data(), pstart.size());
+    A.pbody = P->upload(pbody.data(), pbody.size());
+    A.nbchunks = pstart[n];
+    A.gbchunk = P->dalloc<float4>(sizeof(float4) * std::max<int64_t>(1, pstart[n]));
     A.raw = P->mv.raw;
     A.mat = P->upload(mat.data(), mat.size());
     A.rix = raw_index();
