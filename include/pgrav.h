@@ -333,6 +333,66 @@ int pgrav_host_keys(const float *pos3, int64_t n, const float *rmin, float rsize
 int pgrav_host_cell_corner(const uint64_t *key2, int ph_curve, const float *rmin, float rsize,
                            float *corner3, float *size);
 
+/* The reference's list producer for the drop-in path (host-only): the DL walk
+ * with DLRcritMAC (nln/mac.c:1773-1844) over a tree of one body per leaf, the
+ * sink inheritance of InheritSinkNlogN (nln/mac.c:477-872) and the M x N calls
+ * of mxn_hexa / mxn_quad2 (nln/mac.c:941-1272) with the pQ1 queue
+ * (nln/mac.c:394-475).  Emits what the reference hands nln/gravcuda.h: SoA-8
+ * blocks of absolute-position records (Hvec 29 floats, Qvec 13, Mvec 4 per
+ * record, nln/mac.c:117-153) and the calls that consume them, plus each body's
+ * remainder lists (the reference's CPU leaf, nln/mac.c:506-700).  The walk
+ * that drives the MAC (libtree's WalkNT) is not in the reference checkout:
+ * this one is depth first in Morton order, one MAC call per sink with every
+ * pending source.  pgrav_params: order, mac (0 opening angle, 1 error
+ * tolerance radii), quad_ncut / hexa_ncut (p2cut / p4cut), periodic 0 or 2
+ * (explicit images, nimage, box); leaf_max is 1 (the reference's tree). */
+typedef struct {
+    float dlfac;      /* DLfac (default 6, nln/main.c:536) */
+    int leaf_max_n;   /* default 56 = quad_ncut * 8 (nln/main.c:544) */
+    int gated;        /* 0: the reference's terminal expansion of every SPLIT_SRC cell into
+                         its bodies (nln/mac.c:1818-1832); 1 (default): only cells of at
+                         most leaf_max_n bodies, larger ones opened by the walk (the gate of
+                         DLRcritMACsb, nln/mac.c:1740) */
+    int hblock, min_qsink, min_hsink, min_qsrc, min_hsrc; /* mxn_s, defaults mxn_cuda */
+    int do_pQ, do_pQL, do_pH;                              /* (nln/main.c:215-225) */
+    int log_calls;    /* 1: record every MAC call (sink, sources, results, appends) */
+} pgrav_dl_params;
+
+/* arrays of pgrav_dl_array (element type, values per element) */
+#define PGRAV_DL_CELLS 0        /* float 8 per node: mass, x, y, z, rcrit, bmax, rcrit_q, rcrit_h */
+#define PGRAV_DL_RECORDS 1      /* float 29 per node: the record layout of Hvec (R = bmax) */
+#define PGRAV_DL_BODIES 2       /* float 4 per sorted slot: mass, x, y, z */
+#define PGRAV_DL_PERM 3         /* int64 1 per sorted slot: the caller's index */
+#define PGRAV_DL_CHILD_OFF 4    /* int64 nnodes + 1: node c's daughters are children[off[c]..] */
+#define PGRAV_DL_CHILDREN 5     /* int32: a node, or a body -(slot + 1) */
+#define PGRAV_DL_CALLS 6        /* int64 6 per M x N call, in walk order: routine (0 pH,
+                                   1 pQ), first body slot, m, source offset (floats into
+                                   SOURCES), source_n, record size */
+#define PGRAV_DL_SOURCES 7      /* float: every emitted SoA-8 block */
+#define PGRAV_DL_QFLUSH 8       /* int64 6 per grav_qnss_CUDA("pQ1_qnss") flush: sink base,
+                                   m, first segment (QSEG), source offset, source length, 0 */
+#define PGRAV_DL_QSEG 9         /* uint32 2 per queued sink: base, length (records) */
+#define PGRAV_DL_TAILS 10       /* int64 6 per sorted slot: the leaf's Hvec, Qvec, Mvec
+                                   remainders as (source offset, records) pairs, padded to 8 */
+#define PGRAV_DL_INTERACTIONS 11 /* int64 per sorted slot: bodies accounted for (== nimage N) */
+#define PGRAV_DL_LOG_CALLS 12   /* int64 4 per MAC call: sink, first source, n, first append */
+#define PGRAV_DL_LOG_SOURCES 13 /* int32 3 per source: entity, image, result (0 accept,
+                                   1 split sink, 2 split source) */
+#define PGRAV_DL_LOG_APPENDS 14 /* int32 3 per append: list (0 M, 1 Q, 2 H), entity, image */
+#define PGRAV_DL_NODE_INFO 15   /* int64 4 per node: first body, bodies, has child nodes, level */
+
+void pgrav_dl_params_default(pgrav_dl_params *d);
+/* Runs the walk; NULL on error (pgrav_last_error). */
+void *pgrav_dl_walk(const double *pos3, const double *mass, int64_t n, const pgrav_params *p,
+                    const pgrav_dl_params *d);
+void pgrav_dl_destroy(void *h);
+/* out16: MAC calls, pH calls, pQ calls, pQ1 flushes, source floats, M x N calls, leaf
+ * H / Q / M records, M x N H pairs, M x N Q pairs, bodies with a wrong interaction
+ * count, n, nodes, images, logged calls. */
+int pgrav_dl_stats(void *h, int64_t *out16);
+/* Copies array `which` into out (NULL: size only); returns its element count. */
+int64_t pgrav_dl_array(void *h, int which, void *out);
+
 /* ------------------------------------------------------------------ part 4
  * The step either side of the force evaluation: the leapfrog kick-drift of the
  * body table.  Fields are float arrays with a common byte stride (the
