@@ -41,6 +41,8 @@ struct BatchView {
     const float *ewald32 = nullptr;   // Ewald correction table (pgrav_ewald.h), FP32 copy
     const double *ewald64 = nullptr;  // FP64
     int unroll = 4;  // order-0 / body loop unroll of the rel32c kernels: 2, 4 or 8
+    int tile = 64;   // list entries per LDS round of the rel32c kernels: 32, 64 or 128 (the
+                     // periodic-1 kernels: 64; tiles of 32 and 128 run with the unroll of 4)
     double ewratio = 0.05;  // periodic 1: see pgrav_params.ewald_ratio
     int ngroups = 0;
     int soft = 0;

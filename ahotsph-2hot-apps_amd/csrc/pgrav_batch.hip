@@ -58,6 +58,7 @@ struct BatchArgs {
     const uint8_t *img;  // explicit images: the image of each entry, parallel to idx (or null)
     int img_f;           // 2 nimage + 1 images per side
     int unroll;          // order-0 / body loop unroll of the rel32c kernels: 2, 4 or 8
+    int tile;            // list entries staged in LDS per round (rel32c, not periodic 1): 32, 64, 128
     double ewratio;      // periodic 1: expansion about the group centre when radius <= ewratio x distance
     int ngroups;
     int soft;
@@ -219,9 +220,14 @@ __device__ __forceinline__ void ewald_tile_entry(const BatchArgs &A, EwShared<T,
 // PER: 0 isolated, 1 nearest image + Ewald correction, 2 explicit images
 // BU: unroll factor of the order-0 and body loops (2, 4 or 8; the autotuner picks it
 // for the group-relative compensated FP32 kernels, the others use 4)
-template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU>
+// TILE: list entries staged in LDS per round (32, 64 or 128 -- one barrier pair and
+// one compensated fold per round; 64 in the periodic-1 kernels, whose per-entry
+// Ewald state is a lane's)
+template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU, int TILE = 64>
 __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds,
                                            EwShared<T, PER> *ew) {
+    static_assert(TILE == 32 || TILE == 64 || TILE == 128, "tile of 32, 64 or 128 entries");
+    static_assert(PER != 1 || TILE == 64, "the periodic-1 loader keeps one entry per lane");
     constexpr bool PERIODIC = PER == 1;
     const int lane = threadIdx.x;
     const int cnt = A.gcount[g];
@@ -264,28 +270,32 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
         const int64_t *offk = A.off + (int64_t)kind * (A.ngroups + 1);
         const uint32_t *__restrict__ idx = A.idx;
         const int64_t o0 = offk[g], o1 = offk[g + 1];
-        for (int64_t t0 = o0; t0 < o1; t0 += 64) {
-            const int nt = (int)min((int64_t)64, o1 - t0);
-            double4 src = make_double4(0, 0, 0, 0);
-            uint32_t e = 0;
-            if (lane < nt) {
-                e = idx[t0 + lane];
-                src = kind == 3 ? A.bpos[e] : A.cpos[e];
-                if (PER == 2) {
-                    // image (i, j, k) in nln/mac.c:91-111 order: the source sits at
-                    // x + (i, j, k) box, added in FP64
-                    const int im = A.img[t0 + lane], f = A.img_f, h = f >> 1;
-                    src.x += (double)(im / (f * f) - h) * A.box;
-                    src.y += (double)((im / f) % f - h) * A.box;
-                    src.z += (double)(im % f - h) * A.box;
+        for (int64_t t0 = o0; t0 < o1; t0 += TILE) {
+            const int nt = (int)min((int64_t)TILE, o1 - t0);
+#pragma unroll
+            for (int h = 0; h < (TILE + 63) / 64; h++) {
+                const int u = lane + 64 * h;  // the tile entry this lane loads
+                double4 src = make_double4(0, 0, 0, 0);
+                uint32_t e = 0;
+                if (u < nt) {
+                    e = idx[t0 + u];
+                    src = kind == 3 ? A.bpos[e] : A.cpos[e];
+                    if (PER == 2) {
+                        // image (i, j, k) in nln/mac.c:91-111 order: the source sits at
+                        // x + (i, j, k) box, added in FP64
+                        const int im = A.img[t0 + u], f = A.img_f, hf = f >> 1;
+                        src.x += (double)(im / (f * f) - hf) * A.box;
+                        src.y += (double)((im / f) % f - hf) * A.box;
+                        src.z += (double)(im % f - hf) * A.box;
+                    }
+                    lds[u] = source_offset<T, REL, PERIODIC>(src, gc, A.box);
+                    if (PERIODIC && kind < 3)
+                        trace += (double)mom[(size_t)e * PREC4 + PREC_TRACE];
                 }
-                lds[lane] = source_offset<T, REL, PERIODIC>(src, gc, A.box);
-                if (PERIODIC && kind < 3)
-                    trace += (double)mom[(size_t)e * PREC4 + PREC_TRACE];
+                if (PERIODIC)
+                    ewald_tile_entry<T, PER>(A, ew, u < nt, lane, src, gc, kind < 3,
+                                             mom + (size_t)e * PREC4);
             }
-            if (PERIODIC)
-                ewald_tile_entry<T, PER>(A, ew, lane < nt, lane, src, gc, kind < 3,
-                                         mom + (size_t)e * PREC4);
             __syncthreads();
             Acc4T<T> p = {0, 0, 0, 0};
             // Offsets from the sink: an expanded source's image is the centre's for
@@ -398,12 +408,12 @@ struct MinWaves {
 };
 
 // one workgroup per group, groups taken in longest-first order
-template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU>
+template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU, int TILE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PER>::value)))
 void k_batch(BatchArgs A) {
-    __shared__ V4<T> lds[64];
+    __shared__ V4<T> lds[TILE];
     __shared__ EwShared<T, PER> ew;
-    eval_group<T, REL, COMP, SOFT, PER, BU>(A, A.order_np[blockIdx.x], lds, &ew);
+    eval_group<T, REL, COMP, SOFT, PER, BU, TILE>(A, A.order_np[blockIdx.x], lds, &ew);
 }
 
 // persistent: a fixed grid drains per-XCD work counters.  Workgroup b runs on
@@ -420,7 +430,7 @@ void k_batch(BatchArgs A) {
 // a group's results and bumps the counters before it reads the next group's
 // lists and records, and only pointers known to be distinct let the compiler
 // keep those reads on wave-uniform scalar loads.
-template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU>
+template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU, int TILE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PER>::value)))
 void k_batch_persistent(
     BatchArgs A, const double4 *__restrict__ bpos, const double4 *__restrict__ cpos,
@@ -441,7 +451,7 @@ void k_batch_persistent(
     A.out = out;
     A.counter = counter;
     A.ewald = ewald;
-    __shared__ V4<T> lds[64];
+    __shared__ V4<T> lds[TILE];
     __shared__ EwShared<T, PER> ew;
     __shared__ int w;
     const int x = blockIdx.x % NXCD;
@@ -469,7 +479,7 @@ void k_batch_persistent(
         __syncthreads();
         if (item < 0)
             break;
-        eval_group<T, REL, COMP, SOFT, PER, BU>(A, A.lpt[item], lds, &ew);
+        eval_group<T, REL, COMP, SOFT, PER, BU, TILE>(A, A.lpt[item], lds, &ew);
     }
     if (threadIdx.x == 0) {
         const int done = atomicAdd(&A.counter[1], 1);
@@ -696,23 +706,31 @@ hipError_t launch_reset_counter(int *c, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU>
+template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU, int TILE = 64>
 static void launch_one(const BatchArgs &A, int persistent_blocks, hipStream_t st) {
     // every XCD needs workgroups to drain its slice: round up to a multiple of NXCD
     persistent_blocks = (persistent_blocks + NXCD - 1) / NXCD * NXCD;
     if (persistent_blocks > 0)
-        hipLaunchKernelGGL((k_batch_persistent<T, REL, COMP, SOFT, PER, BU>), dim3(persistent_blocks),
-                           dim3(64), 0, st, A, A.bpos, A.cpos, (const T *)A.mom, A.gfirst,
-                           A.gcount, A.gcen, A.lpt, A.xoff, A.off, A.idx, (V4<T> *)A.out,
-                           A.counter, A.ewald);
+        hipLaunchKernelGGL((k_batch_persistent<T, REL, COMP, SOFT, PER, BU, TILE>),
+                           dim3(persistent_blocks), dim3(64), 0, st, A, A.bpos, A.cpos,
+                           (const T *)A.mom, A.gfirst, A.gcount, A.gcen, A.lpt, A.xoff, A.off,
+                           A.idx, (V4<T> *)A.out, A.counter, A.ewald);
     else
-        hipLaunchKernelGGL((k_batch<T, REL, COMP, SOFT, PER, BU>), dim3(A.ngroups), dim3(64), 0, st, A);
+        hipLaunchKernelGGL((k_batch<T, REL, COMP, SOFT, PER, BU, TILE>), dim3(A.ngroups), dim3(64),
+                           0, st, A);
 }
 
-// the unroll factor: a choice for the group-relative compensated FP32 kernels only
+// the unroll factor and the tile: choices for the group-relative compensated FP32
+// kernels only (tiles of 32 and 128 entries with the unroll of 4, outside periodic 1)
 template <typename T, bool REL, bool COMP, int SOFT, int PER>
 static void launch_bu(const BatchArgs &A, int pb, hipStream_t st) {
     if constexpr (REL && COMP && sizeof(T) == 4) {
+        if constexpr (PER != 1) {
+            if (A.tile == 32)
+                return launch_one<T, REL, COMP, SOFT, PER, 4, 32>(A, pb, st);
+            if (A.tile == 128)
+                return launch_one<T, REL, COMP, SOFT, PER, 4, 128>(A, pb, st);
+        }
         if (A.unroll == 2)
             return launch_one<T, REL, COMP, SOFT, PER, 2>(A, pb, st);
         if (A.unroll == 8)
@@ -767,6 +785,7 @@ hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, 
     A.img = v.img;
     A.img_f = 2 * v.nimage + 1;
     A.unroll = v.unroll;
+    A.tile = v.tile;
     A.ewratio = v.ewratio;
     if (A.ngroups <= 0)
         return hipSuccess;
@@ -810,6 +829,7 @@ hipError_t launch_group64(const BatchView &v, int g, hipStream_t st) {
     A.img = v.img;
     A.img_f = 2 * v.nimage + 1;
     A.unroll = v.unroll;
+    A.tile = v.tile;
     A.ewratio = v.ewratio;
     if (v.periodic < 0 || v.periodic > 2 || (v.periodic == 2 && !v.img))
         return hipErrorInvalidValue;

@@ -793,6 +793,17 @@ int pgrav_plan_set_unroll(void *plan, int unroll) {
     return 0;
 }
 
+int pgrav_plan_set_tile(void *plan, int tile) {
+    Plan *P = (Plan *)plan;
+    clear_error();
+    if (!P || (tile != 32 && tile != 64 && tile != 128)) {
+        fail("pgrav_plan_set_tile: the tile must be 32, 64 or 128 list entries");
+        return -1;
+    }
+    P->v.tile = tile;
+    return 0;
+}
+
 int pgrav_plan_evaluate(void *plan, int variant, void *stream) {
     Plan *P = (Plan *)plan;
     clear_error();

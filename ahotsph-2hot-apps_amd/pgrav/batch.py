@@ -68,6 +68,7 @@ register({
     "pgrav_plan_evaluate": (_i, [_vp, _i, _vp]),
     "pgrav_plan_set_launch": (_i, [_vp, _i]),
     "pgrav_plan_set_unroll": (_i, [_vp, _i]),
+    "pgrav_plan_set_tile": (_i, [_vp, _i]),
     "pgrav_plan_set_dispatch": (_i, [_vp, _i]),
     "pgrav_plan_cells": (_i, [_vp, _i64, _i64, _vp, _vp, _vp]),
     "pgrav_plan_time": (_i, [_vp, _i, _i, _vp, ctypes.POINTER(_d)]),
@@ -238,6 +239,11 @@ class Plan:
 
     def set_unroll(self, unroll):
         lib().pgrav_plan_set_unroll(self.h, int(unroll))
+        check()
+
+    def set_tile(self, tile):
+        """List entries per LDS round of the rel32c kernels: 32, 64 or 128."""
+        lib().pgrav_plan_set_tile(self.h, int(tile))
         check()
 
     def set_dispatch(self, mode):

@@ -55,6 +55,8 @@ def workload_name(args, n, variant=None, dec=None):
             dec["persistent_blocks"])
         if dec.get("body_unroll", 4) != 4:
             name += " unroll %d" % dec["body_unroll"]
+        if dec.get("tile", 64) != 64:
+            name += " tile %d" % dec["tile"]
         if dec.get("dispatch", 2) != 2:
             name += " dispatch %d" % dec["dispatch"]
     return name
@@ -118,7 +120,7 @@ def main():
                                 args.workload)
     tuned = None if args.no_tuned else autotune.load(key)
     dec = {"merge_groups": args.group_mode, "group_max": 64, "leaf_max": args.leaf,
-           "persistent_blocks": args.persistent, "body_unroll": 4, "dispatch": 2}
+           "persistent_blocks": args.persistent, "body_unroll": 4, "tile": 64, "dispatch": 2}
     if tuned:
         dec.update({k: tuned["best"][k] for k in dec if k in tuned["best"]})
         log("using tuned decomposition %s" % dec)
@@ -128,6 +130,8 @@ def main():
                       group_max=dec["group_max"], leaf_max=dec["leaf_max"],
                       persistent_blocks=dec["persistent_blocks"], body_unroll=dec["body_unroll"],
                       dispatch=dec["dispatch"], fp64=True)
+    if dec["tile"] != 64:
+        plan.set_tile(dec["tile"])
     st = plan.stats()
     log("plan: %.1fs (tree %.1fs lists %.1fs upload %.1fs) groups %d mean %.1f pairs %.3e" %
         (time.perf_counter() - t0, st["t_tree"], st["t_lists"], st["t_upload"], st["ngroups"],

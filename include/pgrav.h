@@ -235,6 +235,12 @@ int pgrav_plan_set_launch(void *plan, int persistent_blocks);
  * evaluations: 2, 4 (default) or 8 (an autotuner choice; same results; a captured
  * graph keeps the kernel it was captured with). */
 int pgrav_plan_set_unroll(void *plan, int unroll);
+/* List entries staged in LDS per round by the rel32c kernels for later
+ * evaluations: 32, 64 (default) or 128 (an autotuner choice: one barrier pair
+ * and one compensated fold per round, so results differ from 64's in the last
+ * bits only).  Tiles of 32 and 128 run with the unroll of 4; periodic-1 plans
+ * keep 64.  A captured graph keeps the kernel it was captured with. */
+int pgrav_plan_set_tile(void *plan, int tile);
 /* Group dispatch order (pgrav_params.dispatch, 0..3) for later evaluations. */
 int pgrav_plan_set_dispatch(void *plan, int mode);
 /* Enqueues one force evaluation on `stream` (NULL: the plan's stream). */

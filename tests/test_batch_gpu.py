@@ -209,6 +209,57 @@ def test_unroll_variants_give_identical_results(setup, mode):
     plan.close()
 
 
+@pytest.mark.parametrize("mode,periodic", [(1, 0), (5, 0), (1, 2)])
+def test_tile_variants(setup, mode, periodic):
+    """The autotuner's tile (list entries per LDS round: 32, 64, 128) changes only
+    where the compensated tile sums are folded: every tile matches the FP64
+    evaluation of the same lists to FP32 accuracy, repeats bit for bit, eager and
+    from a graph, and the periodic-1 kernels keep their 64."""
+    batch, ics, pos, mass = setup
+    kw = dict(order=4, eps=1e-3, merge_groups=mode, leaf_max=32, rcb_cell=4096, mac="tolerance",
+              errtol=0.1, frac_tol=0.02)
+    if periodic:
+        p = (pos - pos.min(0)) / (np.ptp(pos, 0).max() * 1.001)
+        kw.update(periodic="images", nimage=1, box=1.0)
+    else:
+        p = pos
+    plan = batch.Plan(p, mass, **kw)
+    plan.evaluate("f64")
+    ref = plan.result("f64")
+    out, err = {}, {}
+    for tile in (32, 128, 64):
+        plan.set_tile(tile)
+        for pb in (0, -1):
+            plan.set_launch(pb)
+            plan.evaluate("rel32c")
+            r = plan.result("rel32c")
+            if tile in out:
+                np.testing.assert_array_equal(r, out[tile])
+            out[tile] = r
+        da, _ = batch.rel_errors(out[tile], ref)
+        dp = np.abs(out[tile][:, 3] - ref[:, 3]) / np.median(np.abs(ref[:, 3]))
+        err[tile] = (np.percentile(da, 99.9), da.max(), dp.max())
+    print("tile errors against f64 (99.9th pct and max |da|/|a|, max |dphi| / median |phi|):",
+          err, flush=True)
+    # every body (not sampled groups): the largest relative error sits at a sink
+    # whose pull nearly cancels, the same body and size for every tile (the periodic
+    # potential crosses zero, hence its scale)
+    for tile, e in err.items():
+        assert e[0] < 1e-5 and e[1] < 1e-4 and e[2] < 1e-5, (tile, err)
+        assert e[1] < 1.5 * err[64][1] + 1e-6, (tile, err)
+    plan.set_tile(128)
+    plan.graph_capture("rel32c")
+    for rep in range(2):
+        plan.evaluate("f64")
+        plan.synchronize()
+        plan.graph_launch()
+        np.testing.assert_array_equal(plan.result("rel32c"), out[128])
+    from pgrav._lib import PgravError
+    with pytest.raises(PgravError):
+        plan.set_tile(96)
+    plan.close()
+
+
 def test_plan_rejects_bad_input(gpu_lib):
     from pgrav import batch
     from pgrav._lib import PgravError
