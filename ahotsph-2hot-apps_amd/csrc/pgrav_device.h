@@ -78,14 +78,22 @@ __device__ __forceinline__ void cell_interact(const TM *__restrict__ M, T m, T x
         T sphi = (QSIGN > 0 ? p2 : -p2);
         sphi = m + ri2 * (p1 + ri2 * sphi);
         const T tp = p1 + rc * p2;
-        const T vx = g1x + rc * g2x, vy = g1y + rc * g2y, vz = g1z + rc * g2z;
         T sa = sphi + rc * tp;
         if (QSIGN < 0) sa += T(2) * ri2 * ri2 * p2;
         const T ra = rinv * ri2;
         a.w += rinv * sphi;
-        a.x += ra * (x * sa - vx);
-        a.y += ra * (y * sa - vy);
-        a.z += ra * (z * sa - vz);
+        if (DIPOLE) {
+            const T vx = g1x + rc * g2x, vy = g1y + rc * g2y, vz = g1z + rc * g2z;
+            a.x += ra * (x * sa - vx);
+            a.y += ra * (y * sa - vy);
+            a.z += ra * (z * sa - vz);
+        } else {
+            // V = rc G2: a += (ra sa) x - (ra rc) G2, two FMAs a component
+            const T ras = ra * sa, nrarc = -(ra * rc);
+            a.x = fma(nrarc, g2x, fma(ras, x, a.x));
+            a.y = fma(nrarc, g2y, fma(ras, y, a.y));
+            a.z = fma(nrarc, g2z, fma(ras, z, a.z));
+        }
         return;
     }
     // order 3: basis A, B, xy, xz, yz
@@ -116,16 +124,25 @@ __device__ __forceinline__ void cell_interact(const TM *__restrict__ M, T m, T x
     sphi = (QSIGN > 0 ? p2 : -p2) + ri2 * sphi;
     sphi = m + ri2 * (p1 + ri2 * sphi);
     const T tp = p1 + rc * (p2 + rb * (p3 + rq * p4));
-    const T vx = g1x + rc * (g2x + rb * (g3x + rq * g4x));
-    const T vy = g1y + rc * (g2y + rb * (g3y + rq * g4y));
-    const T vz = g1z + rc * (g2z + rb * (g3z + rq * g4z));
     T sa = sphi + rc * tp;
     if (QSIGN < 0) sa += T(2) * ri2 * ri2 * p2;
     const T ra = rinv * ri2;
     a.w += rinv * sphi;
-    a.x += ra * (x * sa - vx);
-    a.y += ra * (y * sa - vy);
-    a.z += ra * (z * sa - vz);
+    // W = G2 + rb (G3 + rq G4), V = G1 + rc W
+    const T wx = g2x + rb * (g3x + rq * g4x);
+    const T wy = g2y + rb * (g3y + rq * g4y);
+    const T wz = g2z + rb * (g3z + rq * g4z);
+    if (DIPOLE) {
+        a.x += ra * (x * sa - (g1x + rc * wx));
+        a.y += ra * (y * sa - (g1y + rc * wy));
+        a.z += ra * (z * sa - (g1z + rc * wz));
+    } else {
+        // a += (ra sa) x - (ra rc) W: two FMAs a component, no zero dipole added
+        const T ras = ra * sa, nrarc = -(ra * rc);
+        a.x = fma(nrarc, wx, fma(ras, x, a.x));
+        a.y = fma(nrarc, wy, fma(ras, y, a.y));
+        a.z = fma(nrarc, wz, fma(ras, z, a.z));
+    }
 }
 
 // Monopole / body with Plummer softening eps2 (0 = none), nln/pgrav.cu:380-400,

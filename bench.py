@@ -197,6 +197,18 @@ def main():
     except (OSError, ValueError, KeyError):
         traffic = None
 
+    # FP32 flops the hardware issued per launch for this exact workload and launch
+    # (rocprofv3 SQ_INSTS_VALU_FLOPS_FP32 x 64 lanes; profiles/valu_flops.json
+    # records the run): the frac by the hardware's count beside the reference's
+    hw_flops = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "valu_flops.json")) as f:
+            rec = json.load(f).get(workload_name(args, pos.shape[0], var, dec))
+        if rec:
+            hw_flops = rec["fp32_flops_per_launch"]
+    except (OSError, ValueError, KeyError):
+        hw_flops = None
+
     cpu = None
     if not args.no_cpu:
         # the reference's own SSE4/AVX CPU kernels (oracle/_ref) on a bounded
@@ -230,7 +242,10 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel_ms": kern_ms, "algorithmic_bytes": nbytes,
-                     "hbm_GBs_algorithmic": nbytes / (kern_ms * 1e-3) / 1e9},
+                     "hbm_GBs_algorithmic": nbytes / (kern_ms * 1e-3) / 1e9,
+                     "hw_fp32_flops_per_launch": hw_flops,
+                     "hw_frac": (hw_flops / (kern_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS
+                                 if hw_flops else None)},
         "pct_fp32_peak": 100.0 * achieved / PEAK_FP32_TFLOPS,
         "errors": errors,
         "baselines": baselines,

@@ -81,7 +81,16 @@ def host_cpu():
             mhz = sum(v) / len(v) if v else None
         except (OSError, ValueError):
             mhz = None
+    # the model string a virtual machine exposes can be as short as "AMD EPYC": the
+    # family / model / stepping numbers identify the part all the same (AMD family
+    # 0x19 = 25 is Zen 3 / Zen 4, 0x1A = 26 is Zen 5)
+    fam = ls.get("CPU family")
+    arch = {"25": "Zen 3 / Zen 4 (family 19h)", "26": "Zen 5 (family 1Ah)"}.get(fam) \
+        if ls.get("Vendor ID") == "AuthenticAMD" else None
     return {"cpu_model": model or platform.processor() or "unknown",
+            "cpu_family": fam, "cpu_model_number": ls.get("Model"),
+            "cpu_stepping": ls.get("Stepping"), "microarchitecture": arch,
+            "avx512": "avx512f" in ls.get("Flags", ""),
             "vendor": ls.get("Vendor ID"), "sockets": ls.get("Socket(s)"),
             "cores_per_socket": ls.get("Core(s) per socket"),
             "threads_per_core": ls.get("Thread(s) per core"),
