@@ -65,6 +65,7 @@ struct BatchArgs {
     double eps;
     double G;
     double box;
+    double ibox;  // 1 / box
 };
 
 // nearest periodic image of a displacement (identity when not periodic)
@@ -83,13 +84,13 @@ __device__ __forceinline__ const double *ewald_table(const BatchArgs &A, double)
     return A.ewald64;
 }
 
-// Periodic (nearest image + Ewald correction) per group, in LDS: which entries
-// of the current tile take the correction per pair, each lane's running sums of
-// the expansion terms (20, below) of the entries it loaded, and the group's sums.
+// Periodic (nearest image + Ewald correction) per group, in LDS: each lane's
+// running sums of the expansion terms (20, below) of the entries it loaded, and
+// the group's sums.  (Which entries of a tile take the correction per pair is a
+// wave-uniform 64-bit mask in scalar registers.)
 template <typename T, int PER>
 struct EwShared {
     static constexpr int NP = PER == 1 ? 20 : 1;
-    int pair[64];
     T part[NP][64];
     double acc[NP];
 };
@@ -143,7 +144,7 @@ __device__ __forceinline__ void fold(Acc4T<T> &s, Acc4T<T> &c, const Acc4T<T> &p
 
 // Group-relative (REL) or absolute offset of a source centre, rounded to T.
 template <typename T, bool REL, bool PERIODIC>
-__device__ __forceinline__ V4<T> source_offset(double4 c, double4 gc, double box) {
+__device__ __forceinline__ V4<T> source_offset(double4 c, double4 gc, double box, double ibox) {
     double dx = c.x, dy = c.y, dz = c.z;
     if (REL || PERIODIC) {
         dx -= gc.x;
@@ -151,9 +152,11 @@ __device__ __forceinline__ V4<T> source_offset(double4 c, double4 gc, double box
         dz -= gc.z;
     }
     if (PERIODIC) {
-        dx -= box * rint(dx / box);
-        dy -= box * rint(dy / box);
-        dz -= box * rint(dz / box);
+        // (a product with 1 / box, not a division: the image of a source exactly half
+        // a box away is a tie either way)
+        dx -= box * rint(dx * ibox);
+        dy -= box * rint(dy * ibox);
+        dz -= box * rint(dz * ibox);
         if (!REL) {  // absolute variant: re-add the group centre after wrapping
             dx += gc.x;
             dy += gc.y;
@@ -166,23 +169,24 @@ __device__ __forceinline__ V4<T> source_offset(double4 c, double4 gc, double box
 // The Ewald correction of one tile entry (periodic = 1), in the tile loader:
 // either the entry joins the group's expansion -- its (psi, C, J, T) at r0 =
 // source - centre, times its mass, plus a cell's second-moment coupling -- or
-// it is marked for per-pair lookups.  `on` says whether this lane holds an
-// entry; the terms go to the lane's own running sums in LDS.
+// it is marked for per-pair lookups (the return value).  `on` says whether this
+// lane holds an entry; the terms go to the lane's own running sums in LDS.
 template <typename T, int PER>
-__device__ __forceinline__ void ewald_tile_entry(const BatchArgs &A, EwShared<T, PER> *ew, bool on,
+__device__ __forceinline__ bool ewald_tile_entry(const BatchArgs &A, EwShared<T, PER> *ew, bool on,
                                                  int lane, double4 src, double4 gc, bool cell,
                                                  const T *rec) {
     T v[20];
+    bool pair = false;
     if (on) {
         double dx = src.x - gc.x, dy = src.y - gc.y, dz = src.z - gc.z;
-        dx -= A.box * rint(dx / A.box);
-        dy -= A.box * rint(dy / A.box);
-        dz -= A.box * rint(dz / A.box);
+        dx -= A.box * rint(dx * A.ibox);
+        dy -= A.box * rint(dy * A.ibox);
+        dz -= A.box * rint(dz * A.ibox);
         const double am = fmax(fabs(dx), fmax(fabs(dy), fabs(dz)));
         // the expansion holds when every sink sees the centre's image of the source
         // and the source's other images are far against the group radius
         const bool expand = am < 0.5 * A.box - gc.w && gc.w <= A.ewratio * (A.box - am);
-        ew->pair[lane] = expand ? 0 : 1;
+        pair = !expand;
 #pragma unroll
         for (int q = 0; q < 20; q++) v[q] = T(0);
         if (expand) {
@@ -215,6 +219,7 @@ __device__ __forceinline__ void ewald_tile_entry(const BatchArgs &A, EwShared<T,
 #pragma unroll
         for (int q = 0; q < 20; q++) ew->part[q][lane] += v[q];
     }
+    return pair;
 }
 
 // PER: 0 isolated, 1 nearest image + Ewald correction, 2 explicit images
@@ -272,6 +277,7 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
         const int64_t o0 = offk[g], o1 = offk[g + 1];
         for (int64_t t0 = o0; t0 < o1; t0 += TILE) {
             const int nt = (int)min((int64_t)TILE, o1 - t0);
+            uint64_t pmask = 0;  // periodic 1: bit k set when entry k takes the per-pair correction
 #pragma unroll
             for (int h = 0; h < (TILE + 63) / 64; h++) {
                 const int u = lane + 64 * h;  // the tile entry this lane loads
@@ -288,13 +294,16 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
                         src.y += (double)((im / f) % f - hf) * A.box;
                         src.z += (double)(im % f - hf) * A.box;
                     }
-                    lds[u] = source_offset<T, REL, PERIODIC>(src, gc, A.box);
+                    lds[u] = source_offset<T, REL, PERIODIC>(src, gc, A.box, A.ibox);
                     if (PERIODIC && kind < 3)
                         trace += (double)mom[(size_t)e * PREC4 + PREC_TRACE];
                 }
-                if (PERIODIC)
-                    ewald_tile_entry<T, PER>(A, ew, u < nt, lane, src, gc, kind < 3,
-                                             mom + (size_t)e * PREC4);
+                if (PERIODIC) {
+                    const bool pair = ewald_tile_entry<T, PER>(A, ew, u < nt, lane, src, gc,
+                                                               kind < 3, mom + (size_t)e * PREC4);
+                    // one entry per lane (TILE 64): the wave's flags as a scalar mask
+                    pmask = __builtin_amdgcn_ballot_w64(pair);
+                }
             }
             __syncthreads();
             Acc4T<T> p = {0, 0, 0, 0};
@@ -303,7 +312,7 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
             // loader and no sink crosses a half box from it), so only the sources
             // taken per pair need the nearest image per sink.
 #define PGRAV_OFFSET(r, x, y, z, pp)                                \
-    const bool pp = PERIODIC && ew->pair[k];                        \
+    const bool pp = PERIODIC && ((pmask >> k) & 1);                 \
     T x = sx - r.x, y = sy - r.y, z = sz - r.z;                     \
     if (pp) {                                                       \
         x = pbc<PERIODIC>(x, box, ibox);                            \
@@ -400,7 +409,8 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
 #define PGRAV_MINWAVES_FP32 1  // the FP32 kernels without the Ewald table
 #endif
 #ifndef PGRAV_MINWAVES_PER1
-#define PGRAV_MINWAVES_PER1 6  // periodic 1: 80 VGPRs, no spills
+#define PGRAV_MINWAVES_PER1 7  // periodic 1: 72 VGPRs, 3 spilled once per group (38.7 ms at
+                               // the 256^3 lattice against 38.9 with 6 waves and 78 VGPRs)
 #endif
 template <typename T, int PER>
 struct MinWaves {
@@ -782,6 +792,7 @@ hipError_t launch_batch(const BatchView &v, int variant, int persistent_blocks, 
     A.eps = v.eps;
     A.G = v.G;
     A.box = v.box;
+    A.ibox = 1.0 / v.box;
     A.img = v.img;
     A.img_f = 2 * v.nimage + 1;
     A.unroll = v.unroll;
@@ -826,6 +837,7 @@ hipError_t launch_group64(const BatchView &v, int g, hipStream_t st) {
     A.eps = v.eps;
     A.G = v.G;
     A.box = v.box;
+    A.ibox = 1.0 / v.box;
     A.img = v.img;
     A.img_f = 2 * v.nimage + 1;
     A.unroll = v.unroll;
