@@ -30,6 +30,20 @@
 //  - The pQ1 queue's tail is flushed at the end of the walk (the reference never
 //    flushes it, nln/mac.c:957).
 //
+//  - subtract_background = 1 restates DLRcritMACsb (nln/mac.c:1565-1771) and the
+//    parts of the walk it brings: records of the moments less each cell's
+//    uniform cube of the mean density rho0 (mpole_add_cube, nln/cofm.c:626-648,
+//    applied at :348 with the cell's key-cube edge); the sink's near cube
+//    (clevel / cen / cr / cr2 set on inheritance, nln/mac.c:747-748, :824-843);
+//    anti-masses for the background of split cells, empty octants and bodies
+//    outside the near cube; a background flag per source, carried by the
+//    source's daughters when it is split (the walk's flags word, as the image
+//    index is); MMvec segments for small central-image cells, batched per sink
+//    by mxn_mono (nln/mac.c:877-936) into the grav_mns_queue / grav_mns_flush
+//    queue (nln/mac.c:352-457); and, at the leaf, -rho0 times the near cube's
+//    field (nln/mac.c:688-694), here from the closed form of a uniform cube's
+//    attraction in FP64.
+//
 // Everything here is host code; the GPU evaluates the emitted calls through the
 // drop-in ABI.
 #include <math.h>
@@ -55,6 +69,12 @@ const int NSSE = 8;
 const int HSZ = 29, QSZ = 13, MSZ = 4;  // floats per record (nln/pgrav.cu HSZ / QSZ / MSZ)
 enum { MAC_ACCEPT = 0, MAC_SPLIT_SINK = 1, MAC_SPLIT_SRC = 2 };
 enum { APP_M = 0, APP_Q = 1, APP_H = 2 };
+// subtract_background appends (their codes in the call log)
+enum { APP_QB = 3, APP_HB = 4, APP_ANTI_BODY = 5, APP_ANTI_CELL = 6, APP_ANTI_EMPTY = 8,
+       APP_MM = 20 };
+const int CHUBITS = 42;               // (KEYBITS - 1) / NDIM of the two-word key: "no clevel"
+const int MNSQ_MAX = 32768 + 1024;    // nln/mac.c:157-158
+const int MNSQ_MAX_M = 131072;
 
 // A walk entity: a tree node (>= 0) or a body, -(sorted slot + 1).
 inline int32_t body_ent(int64_t slot) { return (int32_t)(-(slot + 1)); }
@@ -63,7 +83,44 @@ inline int64_t ent_body(int32_t e) { return -(int64_t)e - 1; }
 struct Src {
     int32_t ent;
     int32_t img;  // offset index (nln/mac.c:91-111 numbering)
+    uint8_t bs;   // BACKGROUND_FLAG: this source's background is already accounted for
 };
+
+// The field at r of a cube [-a, a]^3 of unit density (unit G): sum over the faces
+// x = +-a of the double integral of 1/|u| over the face in the coordinates
+// u = source point - r, by the prism formula (Nagy 1966): F(c, y, z) = y ln(z + R) +
+// z ln(y + R) - c atan(y z / (c R)), R = |(c, y, z)|, is the antiderivative of
+// 1/R in y and z.  Logarithms of near-cancelling sums are taken in the
+// equivalent form ln((c^2 + y^2) / (R - z)); terms whose factor is zero are zero.
+double cube_face(double c, double y, double z) {
+    const double R = sqrt(c * c + y * y + z * z);
+    auto ylog = [&](double p, double q) {  // p ln(q + R)
+        if (p == 0.0)
+            return 0.0;
+        const double t = q >= 0 ? q + R : (c * c + p * p) / (R - q);
+        return p * log(t);
+    };
+    double f = ylog(y, z) + ylog(z, y);
+    if (c != 0.0)
+        f -= c * atan(y * z / (c * R));
+    return f;
+}
+
+void cube_field(const double r[3], double a, double g[3]) {
+    for (int ax = 0; ax < 3; ax++) {
+        const int b = (ax + 1) % 3, c = (ax + 2) % 3;
+        const double lo[3] = {-a - r[0], -a - r[1], -a - r[2]};
+        const double hi[3] = {a - r[0], a - r[1], a - r[2]};
+        double s = 0.0;
+        for (int face = 0; face < 2; face++) {
+            const double u = face ? hi[ax] : lo[ax];
+            const double sf = face ? -1.0 : 1.0;  // int u / |u|^3 du = [-1/|u|]_lo^hi
+            s += sf * (cube_face(u, hi[b], hi[c]) - cube_face(u, hi[b], lo[c]) -
+                       cube_face(u, lo[b], hi[c]) + cube_face(u, lo[b], lo[c]));
+        }
+        g[ax] = s;
+    }
+}
 
 // A cell as the reference's cell / quadcell / hexacell structs hold it (FP32).
 struct Cell {
@@ -79,6 +136,13 @@ struct Sink {
     int64_t first;  // first body slot (FirstBody)
     int64_t mcnt, qcnt, hcnt, qcnt_done, hcnt_done;
     int64_t interactions;
+    // subtract_background
+    bool done;
+    int clevel;  // the near cube's level (CHUBITS: none yet)
+    float cen[3], cr, cr2;
+    int near;
+    float fmass;
+    int64_t mmcnt, mmcnt_done, mmterms, mmterms_done;
 };
 
 // A stack of SoA-NSSE records: record k's field f is at
@@ -140,6 +204,40 @@ struct DLWalk {
     std::vector<int32_t> log_app;    // 3 per append: kind, ent, img
     int64_t stats[16] = {0};
 
+    // ---- subtract_background
+    bool sb = false;
+    double rho0 = 0.0;  // m0 / (2 r0)^3 (nln/cofm.c:62)
+    float r0 = 0.0f, nx = 0.0f;
+    struct UCell {
+        float halfsz, mass;
+    } ucell[CHUBITS];
+    std::vector<float> geom;      // 5 per node: corner x, y, z, edge, octant mask
+    std::vector<uint32_t> icorner;  // 3 per node: the key cube's integer corner at its level
+    std::vector<float> ubs;       // 4 per node: umass, uxxxx, uyyyy, uxxyy
+    std::vector<int64_t> body_leaf;
+    std::vector<uint32_t> mm;     // 2 per MMvec entry (the sink stack): base, length
+    std::vector<uint32_t> mseg;   // 2 per emitted segment
+    std::vector<int64_t> mflush;  // 8 per pMM_mnss_sK1 flush
+    std::vector<uint16_t> mss_index;
+    std::vector<uint32_t> mss_seg;
+    std::vector<int32_t> mss_source_n;
+    std::vector<int64_t> mcalls;   // 6 per host mxn_mono run
+    std::vector<int64_t> mmtails;  // 3 per body slot
+    std::vector<double> background;  // 3 per body slot
+    std::vector<float> sbsink;       // 6 per body slot
+    std::vector<float> log_sb;       // 8 per MAC call
+    std::vector<int32_t> log_flags;  // 2 per logged source
+    int64_t sbstats[8] = {0};
+    // the mns queue (nln/mac.c:160-173)
+    struct {
+        bool inuse = false;
+        int64_t sink_base = 0, m = 0;
+        std::vector<uint16_t> ss_index;
+        std::vector<uint32_t> ss_seg;  // 2 per entry
+        std::vector<int32_t> source_n;
+        std::vector<uint32_t> seg;     // 2 per segment
+    } mq;
+
     // ------------------------------------------------------------ MAC
     void position(int32_t ent, int img, float r[3]) const {
         const float *p = ent >= 0 ? cell[ent].pos : &body[4 * ent_body(ent) + 1];
@@ -148,11 +246,21 @@ struct DLWalk {
         r[1] = p[1] + o[1];
         r[2] = p[2] + o[2];
     }
-    void append(Sink &s, int kind, int32_t ent, int img) {
+    void log_append(int code, int32_t ent, int img) {
+        if (dp.log_calls) {
+            log_app.push_back(code);
+            log_app.push_back(ent);
+            log_app.push_back(img);
+        }
+    }
+    // Returns the mass written.  Codes APP_ANTI_* write `mass` at the source's
+    // position plus offset; APP_QB / APP_HB the background-flagged records.
+    float append(Sink &s, int kind, int32_t ent, int img, int code = -1, float mass = 0.0f) {
         float r[3];
         position(ent, img, r);
+        float m;
         if (kind == APP_M) {
-            const float m = ent >= 0 ? cell[ent].mass : body[4 * ent_body(ent)];
+            m = code >= APP_ANTI_BODY ? mass : (ent >= 0 ? cell[ent].mass : body[4 * ent_body(ent)]);
             const float v[4] = {m, r[0], r[1], r[2]};
             M.put(s.mcnt++, v);
         } else {
@@ -161,17 +269,25 @@ struct DLWalk {
             v[1] = r[0];
             v[2] = r[1];
             v[3] = r[2];
+            if (code == APP_QB) {
+                v[0] = v[0] + ucell[node_level(ent)].mass;  // appendQdvec, nln/mac.c:1444
+            } else if (code == APP_HB) {                    // appendHdvec, nln/mac.c:1470-1476
+                const float *u = &ubs[4 * (size_t)ent];
+                v[0] = u[0];
+                v[20] = u[1];
+                v[24] = u[2];
+                v[22] = u[3];
+            }
+            m = v[0];
             if (kind == APP_Q)
                 Q.put(s.qcnt++, v);
             else
                 H.put(s.hcnt++, v);
         }
-        if (dp.log_calls) {
-            log_app.push_back(kind);
-            log_app.push_back(ent);
-            log_app.push_back(img);
-        }
+        log_append(code >= 0 ? code : kind, ent, img);
+        return m;
     }
+    int node_level(int32_t c) const { return tree.level[c]; }
     static float sq(float a) { return a * a; }
 
     // DLRcritMAC, nln/mac.c:1774-1844
@@ -243,6 +359,161 @@ struct DLWalk {
         }
     }
 
+    // DLRcritMACsb, nln/mac.c:1565-1771.  FP32 as the reference's locals; the
+    // background flags of `sv` are updated in place (they travel with the sources).
+    static float d2(const float *a, const float *b) {
+        return sq(a[0] - b[0]) + sq(a[1] - b[1]) + sq(a[2] - b[2]);
+    }
+    // outside the sink's near cube along some axis (the anti-mass test, nln/mac.c:1671-1673)
+    static bool outside(const Sink &s, const float m[3]) {
+        return fabsf(m[0] - s.cen[0]) > s.cr || fabsf(m[1] - s.cen[1]) > s.cr ||
+               fabsf(m[2] - s.cen[2]) > s.cr;
+    }
+    // the centre of a key cube plus offset, as CellCorner + VV(+= 0.5 cellsz + offset)
+    void cube_centre(const float corner[3], float edge, int img, float m[3]) const {
+        const float *o = &offset[3 * img];
+        for (int a = 0; a < 3; a++) m[a] = corner[a] + (0.5f * edge + o[a]);
+    }
+    void mac_sb(Sink &s, int32_t sink_ent, std::vector<Src> &sv, std::vector<int> &result) {
+        const size_t app0 = log_app.size() / 3, src0 = log_src.size() / 3;
+        result.assign(sv.size(), MAC_SPLIT_SINK);
+        std::vector<uint8_t> bs_in(sv.size());
+        for (size_t i = 0; i < sv.size(); i++) bs_in[i] = sv[i].bs;
+        stats[0]++;
+        const int central = nimage >> 1;
+        if (!s.done) {
+            for (size_t i = 0; i < sv.size(); i++) {
+                const int32_t e = sv[i].ent;
+                const int img = sv[i].img;
+                bool bs = sv[i].bs != 0;
+                float r[3];
+                position(e, img, r);
+                if (e >= 0) {
+                    const Cell &c = cell[e];
+                    const int lev = tree.level[e];
+                    const bool isquad = p2cut && c.daughters >= p2cut;
+                    const bool ishexa = p4cut && c.daughters >= p4cut;
+                    const float smallest = ishexa ? c.rcrit_h : (isquad ? c.rcrit_q : c.rcrit);
+                    float dr2, bmax;
+                    if (s.clevel != CHUBITS && lev < s.clevel) {
+                        dr2 = d2(r, s.cen);
+                        bmax = 1.2f * s.cr;
+                    } else {
+                        dr2 = d2(r, s.pos);
+                        bmax = s.bmax;
+                    }
+                    if (!bs && lev == s.clevel && d2(r, s.cen) < s.cr2) {
+                        bs = true;
+                        sv[i].bs = 1;
+                        s.near++;
+                    }
+                    if (isquad && dr2 > sq(c.rcrit_q + bmax)) {
+                        s.fmass += append(s, APP_Q, e, img, bs ? APP_QB : APP_Q);
+                        s.interactions += c.daughters;
+                        result[i] = MAC_ACCEPT;
+                    } else if (ishexa && dr2 > sq(c.rcrit_h + bmax)) {
+                        s.fmass += append(s, APP_H, e, img, bs ? APP_HB : APP_H);
+                        s.interactions += c.daughters;
+                        result[i] = MAC_ACCEPT;
+                    } else {
+                        result[i] = (dp.dlfac * s.bmax > smallest && s.daughters > dp.leaf_max_n)
+                                        ? MAC_SPLIT_SINK
+                                        : MAC_SPLIT_SRC;
+                    }
+                } else {
+                    // body-body, and the anti-mass of the body's leaf cube unless inside
+                    // the near cube (nln/mac.c:1658-1686)
+                    s.fmass += append(s, APP_M, e, img);
+                    s.interactions++;
+                    result[i] = MAC_ACCEPT;
+                    if (!bs) {
+                        const float *g = &geom[5 * (size_t)body_leaf[ent_body(e)]];
+                        float mx[3];
+                        cube_centre(g, g[3], img, mx);
+                        if (outside(s, mx)) {
+                            const double c3 = g[3];
+                            const float am = (float)(-rho0 * c3 * c3 * c3);
+                            s.fmass += append(s, APP_M, e, img, APP_ANTI_BODY, am);
+                            sbstats[3]++;
+                        } else {
+                            s.near++;
+                        }
+                    }
+                }
+                if (result[i] == MAC_SPLIT_SRC && !bs) {
+                    const int lev = tree.level[e];
+                    const float *g = &geom[5 * (size_t)e];
+                    const int sf = (int)g[4];
+                    if (lev >= s.clevel) {
+                        // the whole cell's background, once: its daughters carry the flag
+                        s.fmass += append(s, APP_M, e, img, APP_ANTI_CELL, -ucell[lev].mass);
+                        sv[i].bs = 1;
+                        sbstats[4]++;
+                    } else if (sf != 0xff && lev > 4) {
+                        // empty octants (nln/mac.c:1707-1734), in key-digit order
+                        const uint32_t *ic = &icorner[3 * (size_t)e];
+                        const float factor = (float)tree.size / (float)(1u << (lev + 1));
+                        for (int j = 0; j < 8; j++) {
+                            if (sf >> j & 1)
+                                continue;
+                            float corner[3];
+                            for (int a = 0; a < 3; a++)
+                                corner[a] = (float)tree.x0[a] +
+                                            factor * (float)(2 * ic[a] + (uint32_t)(j >> a & 1));
+                            float mx[3];
+                            cube_centre(corner, factor, img, mx);
+                            if (outside(s, mx)) {
+                                s.fmass += append(s, APP_M, e, img, APP_ANTI_EMPTY + j,
+                                                  -ucell[lev + 1].mass);
+                                sbstats[5]++;
+                            } else {
+                                s.near++;
+                            }
+                        }
+                    }
+                }
+                // terminal traversal of a small cell (nln/mac.c:1736-1753)
+                if (e >= 0 && result[i] != MAC_ACCEPT && cell[e].daughters <= dp.leaf_max_n) {
+                    const int64_t nb = cell[e].daughters, b0 = tree.first[e];
+                    if (dp.do_pM && img == central) {
+                        if (mm.size() < 2 * (size_t)(s.mmcnt + 1))
+                            mm.resize(std::max(2 * (size_t)(s.mmcnt + 1), 2 * mm.size()));
+                        sbstats[2]++;
+                        mm[2 * s.mmcnt] = (uint32_t)b0;
+                        mm[2 * s.mmcnt + 1] = (uint32_t)nb;
+                        s.mmcnt++;
+                        s.mmterms += nb;
+                        log_append(APP_MM, e, img);
+                    } else {
+                        for (int64_t b = b0; b < b0 + nb; b++) append(s, APP_M, body_ent(b), img);
+                    }
+                    result[i] = MAC_ACCEPT;
+                    s.interactions += nb;
+                }
+            }
+        }
+        if (dp.log_calls) {
+            for (size_t i = 0; i < sv.size(); i++) {
+                log_src.push_back(sv[i].ent);
+                log_src.push_back(sv[i].img);
+                log_src.push_back(result[i]);
+                log_flags.push_back(bs_in[i]);
+                log_flags.push_back(sv[i].bs);
+            }
+            log_calls.push_back(sink_ent);
+            log_calls.push_back((int64_t)src0);
+            log_calls.push_back((int64_t)sv.size());
+            log_calls.push_back((int64_t)app0);
+        }
+    }
+    void log_sink_sb(const Sink &s) {
+        if (!dp.log_calls)
+            return;
+        const float v[8] = {s.cen[0], s.cen[1], s.cen[2], s.cr, s.cr2, (float)s.clevel,
+                            s.done ? 1.0f : 0.0f, (float)s.near};
+        log_sb.insert(log_sb.end(), v, v + 8);
+    }
+
     // ------------------------------------------------------------ emission
     int64_t emit_blocks(SoA &st, int64_t b0, int64_t b1) {
         const int64_t off = (int64_t)src.size();
@@ -301,6 +572,88 @@ struct DLWalk {
         qq.m += m;
         stats[10] += m * (int64_t)n;
     }
+    // grav_mns_flush, nln/mac.c:430-457: one grav_mnss_CUDA("pMM_mnss_sK1") call
+    // (source_base 0: the segments index the body table)
+    void mns_flush() {
+        if (!mq.inuse)
+            return;
+        const int64_t v[8] = {mq.sink_base, mq.m, (int64_t)mss_index.size(),
+                              (int64_t)mss_source_n.size(), (int64_t)mq.source_n.size(),
+                              (int64_t)mseg.size() / 2, (int64_t)mq.seg.size() / 2, 0};
+        mflush.insert(mflush.end(), v, v + 8);
+        mss_index.insert(mss_index.end(), mq.ss_index.begin(), mq.ss_index.begin() + mq.m);
+        mss_seg.insert(mss_seg.end(), mq.ss_seg.begin(), mq.ss_seg.end());
+        mss_source_n.insert(mss_source_n.end(), mq.source_n.begin(), mq.source_n.end());
+        mseg.insert(mseg.end(), mq.seg.begin(), mq.seg.end());
+        sbstats[0]++;
+        mq.sink_base += mq.m;
+        mq.inuse = false;
+        mq.m = 0;
+        mq.ss_index.clear();
+        mq.ss_seg.clear();
+        mq.source_n.clear();
+        mq.seg.clear();
+    }
+    // grav_mns_queue, nln/mac.c:351-392: sinks base .. base + m take the segment run
+    // `seg` (seg_n segments, source_n bodies); a gap before them becomes one
+    // zero-work entry
+    void mns_queue(int64_t base_abs, int64_t m, const uint32_t *seg, int64_t seg_n,
+                   int64_t source_n) {
+        if (!mq.inuse) {
+            mq.sink_base = base_abs;
+            mq.inuse = true;
+        }
+        const int64_t base = base_abs - mq.sink_base;
+        if (base_abs < mq.sink_base + mq.m)
+            throw std::runtime_error("repeated or out-of-sequence sink in grav_mns_queue");
+        auto push_entry = [&](uint32_t b, uint32_t len, int32_t sn) {
+            mq.ss_seg.push_back(b);
+            mq.ss_seg.push_back(len);
+            mq.source_n.push_back(sn);
+        };
+        if (base_abs > mq.sink_base + mq.m) {
+            const int64_t gap = base_abs - (mq.sink_base + mq.m);
+            const uint16_t k = (uint16_t)mq.source_n.size();
+            push_entry((uint32_t)(mq.seg.size() / 2), 0, 0);
+            for (int64_t i = 0; i < gap; i++) mq.ss_index.push_back(k);
+            mq.m += gap;
+        }
+        if (base + m >= MNSQ_MAX_M)
+            throw std::runtime_error("mnsq.ss_index overflow");
+        const uint16_t k = (uint16_t)mq.source_n.size();
+        push_entry((uint32_t)(mq.seg.size() / 2), (uint32_t)seg_n, (int32_t)source_n);
+        mq.seg.insert(mq.seg.end(), seg, seg + 2 * seg_n);
+        for (int64_t i = 0; i < m; i++) mq.ss_index.push_back(k);
+        mq.m += m;
+        sbstats[6] += m * source_n;
+        if ((int64_t)mq.seg.size() / 2 >= MNSQ_MAX)
+            throw std::runtime_error("mnsq.seg_n overflow");
+        if ((int64_t)mq.source_n.size() >= MNSQ_MAX / 16)
+            throw std::runtime_error("mnsq.ss_len overflow");
+    }
+    // mxn_mono, nln/mac.c:877-936: the sink's MMvec segments since the last run,
+    // queued for the GPU when the sink starts past the queue, else run on the host
+    // for each body (recorded as a call)
+    void mxn_mono(Sink &s) {
+        const int64_t seg_n = s.mmcnt - s.mmcnt_done, source_n = s.mmterms - s.mmterms_done;
+        if (dp.do_pM && s.first >= mq.sink_base + mq.m) {
+            // the reference passes the stack's start (MMvec); the walk never queues a
+            // sink whose ancestors ran mxn_mono, so mmcnt_done is 0 here
+            if (s.mmcnt_done != 0)
+                throw std::runtime_error("mxn_mono: queue with segments already done");
+            mns_queue(s.first, s.daughters, mm.data(), seg_n, source_n);
+            if ((int64_t)mq.seg.size() / 2 >= 32768)
+                mns_flush();
+        } else {
+            const int64_t v[6] = {s.first, s.daughters, (int64_t)mseg.size() / 2, seg_n,
+                                  source_n, 0};
+            mcalls.insert(mcalls.end(), v, v + 6);
+            mseg.insert(mseg.end(), mm.begin() + 2 * s.mmcnt_done, mm.begin() + 2 * s.mmcnt);
+            sbstats[1]++;
+        }
+        s.mmcnt_done = s.mmcnt;
+        s.mmterms_done = s.mmterms;
+    }
     // mxn_hexa, nln/mac.c:1180-1272 (the GPU branch: a slot is always granted;
     // with do_pH = 0 the reference runs the same work on the CPU, emitted alike)
     void mxn_hexa(Sink &s) {
@@ -332,6 +685,28 @@ struct DLWalk {
     }
     // the leaf: nln/mac.c:506-700 -- each list's remainder, padded to NSSE
     void leaf(Sink &s, int64_t slot) {
+        if (sb) {
+            if (slot >= tree.n - 1)
+                mns_flush();  // at the last body (nln/mac.c:505-506)
+            int64_t *mt = &mmtails[3 * slot];
+            mt[0] = (int64_t)mseg.size() / 2;
+            mt[1] = s.mmcnt - s.mmcnt_done;
+            mt[2] = s.mmterms - s.mmterms_done;
+            mseg.insert(mseg.end(), mm.begin() + 2 * s.mmcnt_done, mm.begin() + 2 * s.mmcnt);
+            // the near cube's background, -rho0 times a uniform cube's field
+            // (nln/mac.c:688-694)
+            double r[3], g[3];
+            for (int a = 0; a < 3; a++) r[a] = (double)(s.pos[a] - s.cen[a]);
+            cube_field(r, (double)s.cr, g);
+            for (int a = 0; a < 3; a++) background[3 * slot + a] = -rho0 * g[a];
+            float *q = &sbsink[6 * slot];
+            q[0] = s.cen[0];
+            q[1] = s.cen[1];
+            q[2] = s.cen[2];
+            q[3] = s.cr;
+            q[4] = s.fmass;
+            q[5] = (float)s.near;
+        }
         int64_t *t = &tails[6 * slot];
         const int64_t hn = (s.hcnt + NSSE - 1) / NSSE * NSSE;
         H.zero(s.hcnt, hn);
@@ -371,7 +746,128 @@ struct DLWalk {
             s.isbody = true;
             s.first = b;
         }
+        s.clevel = CHUBITS;
         return s;
+    }
+
+    // subtract_background set-up: the mean density and the ucell table (SetupCofm,
+    // nln/cofm.c:62; SetupGrav, nln/mac.c:237-250, FP32 as there), each node's key
+    // cube (CellCorner arithmetic, nln/physics_generic.c:237-258: corner = Rmin +
+    // (Rsize / 2^level) * integer corner, in float; Rmin = the box origin, Rsize =
+    // the box) and its octants holding bodies
+    void setup_sb() {
+        const int64_t n = tree.n, nn = tree.nnodes();
+        double m0 = 0.0;
+        for (int64_t i = 0; i < n; i++) m0 += tree.mass[i];
+        r0 = (float)(0.5 * tp.box);
+        rho0 = m0 / ((2.0 * r0) * (2.0 * r0) * (2.0 * r0));
+        nx = dp.nx > 0.0f ? dp.nx : (float)cbrt((double)n);
+        float a1 = r0;
+        const float e2 = 2.0f * a1;
+        float a3 = (float)(rho0 * (double)(e2 * e2 * e2));
+        for (int i = 0; i < CHUBITS; i++) {
+            ucell[i].halfsz = a1;
+            ucell[i].mass = a3;
+            a1 *= 0.5f;
+            a3 *= 0.125f;
+        }
+        // integer corners from the first body's key coordinates (the tree's keys)
+        const double scale = (double)(1 << 21) / tree.size;
+        auto icoord = [&](int64_t b, int a) {
+            int64_t q = (int64_t)((tree.pos[3 * b + a] - tree.x0[a]) * scale);
+            return (uint32_t)std::max<int64_t>(0, std::min<int64_t>((1 << 21) - 1, q));
+        };
+        geom.assign(5 * (size_t)nn, 0.0f);
+        icorner.assign(3 * (size_t)nn, 0);
+        for (int64_t c = 0; c < nn; c++) {
+            const int L = tree.level[c];
+            const float factor = (float)tree.size / (float)(1u << L);
+            for (int a = 0; a < 3; a++) {
+                const uint32_t ic = icoord(tree.first[c], a) >> (21 - L);
+                icorner[3 * c + a] = ic;
+                geom[5 * c + a] = (float)tree.x0[a] + factor * (float)ic;
+            }
+            geom[5 * c + 3] = factor;
+            int sf = 0;
+            if (tree.child[c] >= 0) {
+                for (int k = 0; k < tree.nchild[c]; k++) {
+                    const int32_t d = tree.child[c] + k;
+                    int j = 0;
+                    for (int a = 0; a < 3; a++) j |= (int)(icoord(tree.first[d], a) >> (21 - L - 1) & 1) << a;
+                    sf |= 1 << j;
+                }
+            } else {
+                sf = 0xff;  // a leaf (one body, or coincident ones): no empty octants
+            }
+            geom[5 * c + 4] = (float)sf;
+        }
+        body_leaf.assign(n, 0);
+        mmtails.assign(3 * (size_t)n, 0);
+        background.assign(3 * (size_t)n, 0.0);
+        sbsink.assign(6 * (size_t)n, 0.0f);
+        ubs.assign(4 * (size_t)nn, 0.0f);
+        // the AREL radii of the background-subtracted cells (nln/cofm.c:356-360,
+        // :425-434): cells of more than 8 bodies bound the odd terms by the sums of
+        // |third| and |fifth| raw moments about the centre of mass, with B4 = B6 = 0
+        if (tp.mac == 1) {
+            const double bmax0 = 0.5 * tree.size;
+            for (int64_t c = 0; c < nn; c++) {
+                if (tree.count[c] <= 8)
+                    continue;
+                const double *raw = &tree.raw[35 * (size_t)c];
+                const double *cm = &tree.cofm[3 * c];
+                double s3 = 0.0;
+                for (int a = 3; a >= 0; a--)
+                    for (int b = 3 - a; b >= 0; b--) {
+                        const int cc = 3 - a - b;
+                        s3 += multinomial(3, a, b, cc) * fabs(raw[raw_index(a, b, cc)]);
+                    }
+                double m5[21] = {0};
+                for (int64_t i = tree.first[c]; i < tree.first[c] + tree.count[c]; i++) {
+                    const double dx = tree.pos[3 * i] - cm[0], dy = tree.pos[3 * i + 1] - cm[1],
+                                 dz = tree.pos[3 * i + 2] - cm[2];
+                    int k = 0;
+                    for (int a = 5; a >= 0; a--)
+                        for (int b = 5 - a; b >= 0; b--, k++)
+                            m5[k] += tree.mass[i] * pow(dx, a) * pow(dy, b) * pow(dz, 5 - a - b);
+                }
+                double s5 = 0.0;
+                int k = 0;
+                for (int a = 5; a >= 0; a--)
+                    for (int b = 5 - a; b >= 0; b--, k++) s5 += multinomial(5, a, b, 5 - a - b) * fabs(m5[k]);
+                const double sb35[2] = {s3 / 9.0, s5 / 27.0};
+                const double B2 = raw[raw_index(2, 0, 0)] + raw[raw_index(0, 2, 0)] + raw[raw_index(0, 0, 2)];
+                const double B4 = raw[raw_index(4, 0, 0)] + raw[raw_index(0, 4, 0)] +
+                                  raw[raw_index(0, 0, 4)] + 2 * raw[raw_index(2, 2, 0)] +
+                                  2 * raw[raw_index(2, 0, 2)] + 2 * raw[raw_index(0, 2, 2)];
+                arel_rcrit(raw[0], tree.bmax[c], B2, B4, 0.0, tree.count[c], tp, bmax0,
+                           &tree.rcrit[3 * (size_t)c], sb35);
+            }
+        }
+    }
+    static double multinomial(int n, int a, int b, int c) {
+        auto f = [](int k) {
+            double r = 1;
+            for (int i = 2; i <= k; i++) r *= i;
+            return r;
+        };
+        return f(n) / (f(a) * f(b) * f(c));
+    }
+    // mpole_add_cube (nln/cofm.c:626-648) on raw moments about the centre: a cube
+    // of edge a and density rho, its moments taken about its own centre
+    static void add_cube(double a, double rho, double *u) {
+        const double a3 = a * a * a * rho, a5 = a * a * a3, a7 = a * a * a5;
+        u[raw_index(0, 0, 0)] += a3;
+        for (int ax = 0; ax < 3; ax++) {
+            int e[3] = {0, 0, 0};
+            e[ax] = 2;
+            u[raw_index(e[0], e[1], e[2])] += a5 / 12.0;
+            e[ax] = 4;
+            u[raw_index(e[0], e[1], e[2])] += a7 / 80.0;
+        }
+        u[raw_index(2, 2, 0)] += a7 / 144.0;
+        u[raw_index(2, 0, 2)] += a7 / 144.0;
+        u[raw_index(0, 2, 2)] += a7 / 144.0;
     }
     // InheritSinkNlogN from -> to (nln/mac.c:737-846): the M x N work of `from`
     // the first time, then the counts
@@ -383,6 +879,9 @@ struct DLWalk {
             if (p2cut && dp.hblock && from.daughters >= dp.min_qsink &&
                 from.qcnt - from.qcnt_done >= dp.min_qsrc)
                 mxn_quad2(from);
+            if (dp.hblock && from.daughters >= dp.min_msink &&
+                from.mmterms - from.mmterms_done >= dp.min_msrc)
+                mxn_mono(from);
             from.processed = true;
         }
         Sink to = make_sink(to_ent);
@@ -391,14 +890,53 @@ struct DLWalk {
         to.hcnt = from.hcnt;
         to.qcnt_done = from.qcnt_done;
         to.hcnt_done = from.hcnt_done;
+        to.mmcnt = from.mmcnt;
+        to.mmcnt_done = from.mmcnt_done;
+        to.mmterms = from.mmterms;
+        to.mmterms_done = from.mmterms_done;
         to.interactions = from.interactions;
+        if (sb) {
+            // the near cube (nln/mac.c:745-748, :824-843): fixed at the first cell
+            // of <= 32 bodies or of half-size <= 2 mean spacings on the path, or
+            // at the body (its leaf cube) if none came before
+            const bool ccp = to_ent >= 0 && (cell[to_ent].daughters <= 32 ||
+                                             4.0001f * r0 / nx >= ucell[tree.level[to_ent]].halfsz);
+            if (ccp && from.clevel == CHUBITS) {
+                const float hs = ucell[tree.level[to_ent]].halfsz;
+                to.clevel = tree.level[to_ent];
+                memcpy(to.cen, cell[to_ent].pos, sizeof to.cen);
+                to.cr = 3.0f * hs;
+                to.cr2 = sq(3.99f * hs);
+            } else {
+                to.clevel = from.clevel;
+                memcpy(to.cen, from.cen, sizeof to.cen);
+                to.cr = from.cr;
+                to.cr2 = from.cr2;
+            }
+            if (to.isbody && from.clevel == CHUBITS) {
+                const int64_t leaf = body_leaf[to.first];
+                const float *g = &geom[5 * (size_t)leaf];
+                for (int a = 0; a < 3; a++) to.cen[a] = g[a] + 0.5f * g[3];
+                to.clevel = tree.level[leaf];
+                to.cr = 1.5f * g[3];
+                to.cr2 = sq(1.995f * g[3]);
+            }
+            to.near = from.near;
+            to.fmass = from.fmass;
+            to.done = to.interactions == (int64_t)nimage * tree.n ? true : from.done;
+        }
         return to;
     }
     void walk(Sink &s, int32_t ent, std::vector<Src> pending) {
         std::vector<Src> pass;
         std::vector<int> res;
         while (!pending.empty()) {
-            mac(s, ent, pending, res);
+            if (sb) {
+                mac_sb(s, ent, pending, res);
+                log_sink_sb(s);
+            } else {
+                mac(s, ent, pending, res);
+            }
             std::vector<Src> opened;
             for (size_t i = 0; i < pending.size(); i++) {
                 if (res[i] == MAC_SPLIT_SINK) {
@@ -406,7 +944,7 @@ struct DLWalk {
                 } else if (res[i] == MAC_SPLIT_SRC) {
                     const int32_t e = pending[i].ent;
                     for (int64_t k = child_off[e]; k < child_off[e + 1]; k++)
-                        opened.push_back({children[k], pending[i].img});
+                        opened.push_back({children[k], pending[i].img, pending[i].bs});
                 }
             }
             pending.swap(opened);
@@ -439,6 +977,8 @@ struct DLWalk {
             body[4 * i] = (float)tree.mass[i];
             for (int a = 0; a < 3; a++) body[4 * i + 1 + a] = (float)tree.pos[3 * i + a];
         }
+        if (sb)
+            setup_sb();
         cell.resize(nn);
         rec.assign((size_t)HSZ * nn, 0.0f);
         for (int64_t c = 0; c < nn; c++) {
@@ -467,8 +1007,23 @@ struct DLWalk {
                 C.rcrit_h = P >= 4 ? rk(4) : C.rcrit_q;
             }
             double r29[HSZ];
-            raw_to_record<double>(raw, &tree.cofm[3 * c], tree.bmax[c] > 0 ? tree.bmax[c] : 1.0, r29);
+            const double R = tree.bmax[c] > 0 ? tree.bmax[c] : 1.0;
+            raw_to_record<double>(raw, &tree.cofm[3 * c], R, r29);
             r29[5] = r29[6] = r29[7] = 0.0;  // no dipole about the centre of mass
+            if (sb) {
+                // the hexacell's unsubtracted umass / uxxxx / uyyyy / uxxyy, then the
+                // record of the moments less the cell's cube (nln/cofm.c:345-349, :540-549)
+                float *u = &ubs[4 * (size_t)c];
+                u[0] = (float)raw[0];
+                u[1] = (float)r29[20];
+                u[2] = (float)r29[24];
+                u[3] = (float)r29[22];
+                double sub[35];
+                memcpy(sub, raw, sizeof sub);
+                add_cube((double)geom[5 * c + 3], -rho0, sub);
+                raw_to_record<double>(sub, &tree.cofm[3 * c], R, r29);
+                r29[5] = r29[6] = r29[7] = 0.0;
+            }
             for (int k = 0; k < HSZ; k++) rec[(size_t)HSZ * c + k] = (float)r29[k];
         }
         // daughters of every node as walk entities: child nodes, single-body leaves as
@@ -479,11 +1034,17 @@ struct DLWalk {
             if (tree.child[c] >= 0) {
                 for (int k = 0; k < tree.nchild[c]; k++) {
                     const int32_t d = tree.child[c] + k;
-                    children.push_back(tree.child[d] < 0 && tree.count[d] == 1 ? body_ent(tree.first[d]) : d);
+                    const bool single = tree.child[d] < 0 && tree.count[d] == 1;
+                    children.push_back(single ? body_ent(tree.first[d]) : d);
+                    if (sb && single)
+                        body_leaf[tree.first[d]] = d;
                 }
             } else if (tree.count[c] > 1) {
-                for (int64_t b = tree.first[c]; b < tree.first[c] + tree.count[c]; b++)
+                for (int64_t b = tree.first[c]; b < tree.first[c] + tree.count[c]; b++) {
                     children.push_back(body_ent(b));
+                    if (sb)
+                        body_leaf[b] = c;
+                }
             }
             child_off[c + 1] = (int64_t)children.size();
         }
@@ -493,10 +1054,12 @@ struct DLWalk {
         // nln/mac.c:279-291)
         const int32_t root = tree.child[0] < 0 && tree.count[0] == 1 ? body_ent(0) : 0;
         std::vector<Src> src0;
-        for (int i = 0; i < nimage; i++) src0.push_back({root, i});
+        for (int i = 0; i < nimage; i++) src0.push_back({root, i, 0});
         Sink s = make_sink(root);
         walk(s, root, src0);
         qns_flush();
+        if (sb)
+            mns_flush();
         stats[4] = (int64_t)src.size();
         stats[5] = (int64_t)calls.size() / 6;
     }
@@ -526,6 +1089,10 @@ void pgrav_dl_params_default(pgrav_dl_params *d) {
     d->do_pQL = 1;
     d->do_pH = 1;
     d->log_calls = 0;
+    d->subtract_background = 0;
+    d->do_pM = 1;
+    d->min_msink = 8;
+    d->min_msrc = 128;
 }
 
 void *pgrav_dl_walk(const double *pos3, const double *mass, int64_t n, const pgrav_params *p,
@@ -548,6 +1115,23 @@ void *pgrav_dl_walk(const double *pos3, const double *mass, int64_t n, const pgr
         fail("pgrav_dl_walk: periodic 1 (nearest image + Ewald table) has no DL walk; use 0 or 2");
         return nullptr;
     }
+    if (d->subtract_background && p->periodic != 2) {
+        fail("pgrav_dl_walk: subtract_background needs the periodic box of explicit images "
+             "(periodic 2)");
+        return nullptr;
+    }
+    if (d->min_msink < 0 || d->min_msrc < 0 || !(d->nx >= 0.0f)) {
+        fail("pgrav_dl_walk: non-negative M x N limits and nx");
+        return nullptr;
+    }
+    if (d->subtract_background && d->do_pM) {
+        // MMvec segments are evaluated with the first body's mass (nln/mac.c:637-639)
+        for (int64_t i = 1; i < n; i++)
+            if (mass[i] != mass[0]) {
+                fail("pgrav_dl_walk: MMvec segments (do_pM) need equal body masses");
+                return nullptr;
+            }
+    }
     DLWalk *W = new (std::nothrow) DLWalk();
     if (!W) {
         fail("out of host memory");
@@ -560,6 +1144,7 @@ void *pgrav_dl_walk(const double *pos3, const double *mass, int64_t n, const pgr
         return nullptr;
     }
     W->dp = *d;
+    W->sb = d->subtract_background != 0;
     W->tp.leaf_max = 1;  // the reference's tree: a body per leaf (coincident ones aside)
     W->p4cut = W->tp.order >= 4 ? W->tp.hexa_ncut : 0;
     W->p2cut = W->tp.order >= 2 ? W->tp.quad_ncut : 0;
@@ -642,6 +1227,25 @@ int64_t pgrav_dl_array(void *h, int which, void *out) {
             perm[4 * c + 3] = W->tree.level[c];
         }
         take(perm, 4);
+        break;
+    case PGRAV_DL_NODE_GEOM: take(W->geom, 5); break;
+    case PGRAV_DL_UBS: take(W->ubs, 4); break;
+    case PGRAV_DL_BODY_LEAF: take(W->body_leaf, 1); break;
+    case PGRAV_DL_MFLUSH: take(W->mflush, 8); break;
+    case PGRAV_DL_MSS_INDEX: take(W->mss_index, 1); break;
+    case PGRAV_DL_MSS_SEG: take(W->mss_seg, 2); break;
+    case PGRAV_DL_MSS_SOURCE_N: take(W->mss_source_n, 1); break;
+    case PGRAV_DL_MSEG: take(W->mseg, 2); break;
+    case PGRAV_DL_MCALLS: take(W->mcalls, 6); break;
+    case PGRAV_DL_MMTAILS: take(W->mmtails, 3); break;
+    case PGRAV_DL_BACKGROUND: take(W->background, 3); break;
+    case PGRAV_DL_SB_SINK: take(W->sbsink, 6); break;
+    case PGRAV_DL_LOG_SB: take(W->log_sb, 8); break;
+    case PGRAV_DL_LOG_FLAGS: take(W->log_flags, 2); break;
+    case PGRAV_DL_SB_STATS:
+        p = W->sbstats;
+        bytes = sizeof W->sbstats;
+        count = 8;
         break;
     default:
         fail("pgrav_dl_array: unknown array %d", which);

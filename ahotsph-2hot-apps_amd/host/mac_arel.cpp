@@ -68,7 +68,7 @@ double arel_ptol(double m, double bmax, const TreeParams &p, double bmax0) {
 }
 
 void arel_rcrit(double m, double b, double B2, double B4, double B6, int64_t count,
-                const TreeParams &p, double bmax0, double out[3]) {
+                const TreeParams &p, double bmax0, double out[3], const double *sb35) {
     const bool quad = p.quad_ncut > 0 && count >= p.quad_ncut;
     const bool hexa = p.hexa_ncut > 0 && count >= p.hexa_ncut;
     out[0] = 0.0;
@@ -111,7 +111,9 @@ void arel_rcrit(double m, double b, double B2, double B4, double B6, int64_t cou
 
     // order 2: E_2 with B3 <= b B2
     if (quad) {
-        const double B3q = b * B2;
+        const double B3q = sb35 ? sb35[0] : b * B2;
+        if (sb35)
+            B4 = 0.0;
         const double a[7] = {3 * B4, -4 * B3q, 0.0, 0.0, pt * b * b, -2 * pt * b, pt};
         ab = root_above(a, 6, ab, 0.01);
         auto relq = [&](double t, double start) {
@@ -126,7 +128,9 @@ void arel_rcrit(double m, double b, double B2, double B4, double B6, int64_t cou
     }
     // order 4: E_4 with B5 <= b B4
     if (hexa) {
-        const double B5 = b * B4;
+        const double B5 = sb35 ? sb35[1] : b * B4;
+        if (sb35)
+            B6 = 0.0;
         const double a[9] = {5 * B6, -6 * B5, 0.0, 0.0, 0.0, 0.0, pt * b * b, -2 * pt * b, pt};
         ab = root_above(a, 8, ab, 0.001);
         auto relh = [&](double t, double start) {

@@ -116,8 +116,11 @@ int raw_index(int a, int b, int c);
 // three critical radii from (m, bmax, B2, B4, B6), and every node's (Tree::rcrit).
 // bmax0: the system half-extent (the reference's r0) that ptol_boost divides by.
 double arel_ptol(double m, double bmax, const TreeParams &p, double bmax0);
+// sb35, when given (the subtract_background form for cells of more than 8 bodies,
+// nln/cofm.c:356-360, :425-434): order 2 takes B3 = sb35[0] with B4 = 0, order 4
+// B5 = sb35[1] with B6 = 0.
 void arel_rcrit(double m, double bmax, double B2, double B4, double B6, int64_t count,
-                const TreeParams &p, double bmax0, double out[3]);
+                const TreeParams &p, double bmax0, double out[3], const double *sb35 = nullptr);
 void compute_rcrit(Tree &t, const TreeParams &p, int nthreads);
 // estimated VALU instructions of one wave over a group's lists (entries per kind)
 double wave_cost(int64_t hexa, int64_t quad, int64_t mono, int64_t body);

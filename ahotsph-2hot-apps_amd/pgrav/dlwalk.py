@@ -5,6 +5,10 @@
     W.sources    # every SoA-8 block the calls, the pQ1 flushes and the leaves read
     W.tails      # (n, 6): each body's leaf remainders (H, Q, M offset / records)
 
+With dl=dict(subtract_background=1) (periodic="images") the walk is DLRcritMACsb
+(nln/mac.c:1565-1771): W.mflush / W.mss_* / W.mseg are the pMM_mnss_sK1 queue's
+flushes, W.mmtails each leaf's MMvec remainder, W.background the near cube's field.
+
 The walk is the reference's DL traversal with DLRcritMAC (nln/mac.c:1773-1844),
 the sink inheritance of InheritSinkNlogN and the M x N batching of mxn_hexa /
 mxn_quad2 (nln/mac.c:394-475, :477-1272); bodies are in the Morton-sorted order
@@ -22,7 +26,9 @@ class DLParams(ctypes.Structure):
     _fields_ = [("dlfac", ctypes.c_float), ("leaf_max_n", ctypes.c_int), ("gated", ctypes.c_int),
                 ("hblock", ctypes.c_int), ("min_qsink", ctypes.c_int), ("min_hsink", ctypes.c_int),
                 ("min_qsrc", ctypes.c_int), ("min_hsrc", ctypes.c_int), ("do_pQ", ctypes.c_int),
-                ("do_pQL", ctypes.c_int), ("do_pH", ctypes.c_int), ("log_calls", ctypes.c_int)]
+                ("do_pQL", ctypes.c_int), ("do_pH", ctypes.c_int), ("log_calls", ctypes.c_int),
+                ("subtract_background", ctypes.c_int), ("do_pM", ctypes.c_int),
+                ("min_msink", ctypes.c_int), ("min_msrc", ctypes.c_int), ("nx", ctypes.c_float)]
 
 
 _vp, _i, _i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
@@ -42,7 +48,15 @@ ARRAYS = {
     "qseg": (9, np.uint32, 2), "tails": (10, np.int64, 6), "interactions": (11, np.int64, 1),
     "log_calls": (12, np.int64, 4), "log_sources": (13, np.int32, 3),
     "log_appends": (14, np.int32, 3), "node_info": (15, np.int64, 4),
+    # subtract_background
+    "node_geom": (16, np.float32, 5), "ubs": (17, np.float32, 4), "body_leaf": (18, np.int64, 1),
+    "mflush": (19, np.int64, 8), "mss_index": (20, np.uint16, 1), "mss_seg": (21, np.uint32, 2),
+    "mss_source_n": (22, np.int32, 1), "mseg": (23, np.uint32, 2), "mcalls": (24, np.int64, 6),
+    "mmtails": (25, np.int64, 3), "background": (26, np.float64, 3), "sb_sink": (27, np.float32, 6),
+    "log_sb": (28, np.float32, 8), "log_flags": (29, np.int32, 2), "sb_stats": (30, np.int64, 1),
 }
+SB_STATS = ("mns_flushes", "host_mono", "mm_segments", "anti_body", "anti_cell", "anti_empty",
+            "mns_pairs")
 STATS = ("mac_calls", "ph_calls", "pq_calls", "pq1_flushes", "source_floats", "mxn_calls",
          "leaf_h", "leaf_q", "leaf_m", "mxn_h_pairs", "mxn_q_pairs", "bad_interactions", "n",
          "nnodes", "nimage", "logged_calls")
@@ -110,6 +124,12 @@ class DLWalk:
     def block(self, off, nrec, sz):
         """nrec records of sz floats as SoA-8 blocks at source offset off (flat)."""
         return self.sources[off:off + nrec * sz]
+
+    def sb_stats(self):
+        return dict(zip(SB_STATS, (int(v) for v in self.sb_stats_raw())))
+
+    def sb_stats_raw(self):
+        return self.array("sb_stats")
 
     def mac_calls(self):
         """The logged MAC calls: [(sink entity, [(entity, image, result)], [(list, entity,

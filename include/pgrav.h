@@ -362,6 +362,17 @@ typedef struct {
     int hblock, min_qsink, min_hsink, min_qsrc, min_hsrc; /* mxn_s, defaults mxn_cuda */
     int do_pQ, do_pQL, do_pH;                              /* (nln/main.c:215-225) */
     int log_calls;    /* 1: record every MAC call (sink, sources, results, appends) */
+    /* subtract_background (nln/main.c:543): 1 selects DLRcritMACsb (nln/mac.c:1565-1771)
+       with periodic 2: cell records carry the moments less the cell's uniform cube of the
+       mean density (mpole_add_cube, nln/cofm.c:348, :626-648), the walk adds anti-masses
+       for the background of split cells, empty octants and bodies outside each sink's
+       near cube, the leaf adds the near cube's own field (cubic_acc, nln/mac.c:688-694),
+       and small central-image cells become MMvec segments (nln/mac.c:1736-1741) batched by
+       mxn_mono into the pMM_mnss_sK1 queue (nln/mac.c:352-457, :877-936). */
+    int subtract_background;
+    int do_pM, min_msink, min_msrc;                        /* mxn_cuda: 1, 8, 128 */
+    float nx;         /* mac.nx, the cube root of the run's global body count (nln/main.c:856),
+                         which sets where near cubes start (nln/mac.c:747); 0: cbrt(n) */
 } pgrav_dl_params;
 
 /* arrays of pgrav_dl_array (element type, values per element) */
@@ -386,6 +397,41 @@ typedef struct {
                                    1 split sink, 2 split source) */
 #define PGRAV_DL_LOG_APPENDS 14 /* int32 3 per append: list (0 M, 1 Q, 2 H), entity, image */
 #define PGRAV_DL_NODE_INFO 15   /* int64 4 per node: first body, bodies, has child nodes, level */
+/* subtract_background = 1 only (empty otherwise) */
+#define PGRAV_DL_NODE_GEOM 16   /* float 5 per node: key-cube corner x, y, z, edge, octants
+                                   holding bodies (bit j: octant x = j & 1, y = j >> 1 & 1,
+                                   z = j >> 2, the key digit of CellCorner) */
+#define PGRAV_DL_UBS 17         /* float 4 per node: the unsubtracted mass, qxxxx, qyyyy,
+                                   qxxyy (hexacell umass / uxxxx / uyyyy / uxxyy) */
+#define PGRAV_DL_BODY_LEAF 18   /* int64 per sorted slot: the leaf node holding the body */
+#define PGRAV_DL_MFLUSH 19      /* int64 8 per grav_mnss_CUDA("pMM_mnss_sK1") flush: sink
+                                   base, m, first ss_index (MSS_INDEX), first ss entry
+                                   (MSS_SEG / MSS_SOURCE_N), ss_len, first segment (MSEG),
+                                   seg_n, 0 */
+#define PGRAV_DL_MSS_INDEX 20   /* uint16 per queued sink */
+#define PGRAV_DL_MSS_SEG 21     /* uint32 2 per ss entry: first segment, segments */
+#define PGRAV_DL_MSS_SOURCE_N 22 /* int32 per ss entry: bodies in its segments */
+#define PGRAV_DL_MSEG 23        /* uint32 2 per MMvec segment: first body slot, bodies */
+#define PGRAV_DL_MCALLS 24      /* int64 6 per mxn_mono run on the host (not queued): first
+                                   slot, m, first segment (MSEG), segments, bodies, 0 */
+#define PGRAV_DL_MMTAILS 25     /* int64 3 per sorted slot: the leaf's MMvec remainder:
+                                   first segment (MSEG), segments, bodies */
+#define PGRAV_DL_BACKGROUND 26  /* double 3 per sorted slot: -rho0 cubic_acc(pos - cen, cr),
+                                   the near cube's field, FP64 */
+#define PGRAV_DL_SB_SINK 27     /* float 6 per sorted slot: the leaf sink's cen x, y, z, cr,
+                                   fmass (signed mass appended to its lists), near */
+#define PGRAV_DL_LOG_SB 28      /* float 8 per MAC call: the sink's cen x, y, z, cr, cr2,
+                                   clevel (42: none yet), done, near after the call */
+#define PGRAV_DL_LOG_FLAGS 29   /* int32 2 per logged source: background flag in, out */
+#define PGRAV_DL_SB_STATS 30    /* int64 8: pMM_mnss_sK1 flushes, host mxn_mono runs, MMvec
+                                   segments, body / cell / empty-octant anti-masses, queued
+                                   sink-body pairs, 0 */
+/* Appends in LOG_APPENDS with subtract_background: 1 / 2 Q / H with subtracted moments,
+ * 3 / 4 Q / H of a background-flagged cell (Q mass + the cube's, H the unsubtracted
+ * umass / uxxxx / uyyyy / uxxyy), 5 a body's anti-mass (-rho0 edge^3 of its leaf cube),
+ * 6 a split cell's anti-mass (-ucell mass of its level), 8 + j the anti-mass of empty
+ * octant j, 20 an MMvec segment (entity: the cell); every anti-mass sits at the source's
+ * own position plus offset (appendMvec writes r, nln/mac.c:1421-1430). */
 
 void pgrav_dl_params_default(pgrav_dl_params *d);
 /* Runs the walk; NULL on error (pgrav_last_error). */
