@@ -750,27 +750,11 @@ struct DLWalk {
         return s;
     }
 
-    // subtract_background set-up: the mean density and the ucell table (SetupCofm,
-    // nln/cofm.c:62; SetupGrav, nln/mac.c:237-250, FP32 as there), each node's key
-    // cube (CellCorner arithmetic, nln/physics_generic.c:237-258: corner = Rmin +
-    // (Rsize / 2^level) * integer corner, in float; Rmin = the box origin, Rsize =
-    // the box) and its octants holding bodies
-    void setup_sb() {
-        const int64_t n = tree.n, nn = tree.nnodes();
-        double m0 = 0.0;
-        for (int64_t i = 0; i < n; i++) m0 += tree.mass[i];
-        r0 = (float)(0.5 * tp.box);
-        rho0 = m0 / ((2.0 * r0) * (2.0 * r0) * (2.0 * r0));
-        nx = dp.nx > 0.0f ? dp.nx : (float)cbrt((double)n);
-        float a1 = r0;
-        const float e2 = 2.0f * a1;
-        float a3 = (float)(rho0 * (double)(e2 * e2 * e2));
-        for (int i = 0; i < CHUBITS; i++) {
-            ucell[i].halfsz = a1;
-            ucell[i].mass = a3;
-            a1 *= 0.5f;
-            a3 *= 0.125f;
-        }
+    // Each node's key cube (CellCorner arithmetic, nln/physics_generic.c:237-258:
+    // corner = Rmin + (Rsize / 2^level) * integer corner, in float; Rmin = the box
+    // origin, Rsize = the box) and its octants holding bodies.
+    void setup_geom() {
+        const int64_t nn = tree.nnodes();
         // integer corners from the first body's key coordinates (the tree's keys)
         const double scale = (double)(1 << 21) / tree.size;
         auto icoord = [&](int64_t b, int a) {
@@ -801,47 +785,123 @@ struct DLWalk {
             }
             geom[5 * c + 4] = (float)sf;
         }
+    }
+
+    // subtract_background: the mean density and the ucell table (SetupCofm,
+    // nln/cofm.c:62; SetupGrav, nln/mac.c:237-250, FP32 as there)
+    void setup_sb() {
+        const int64_t n = tree.n, nn = tree.nnodes();
+        double m0 = 0.0;
+        for (int64_t i = 0; i < n; i++) m0 += tree.mass[i];
+        r0 = (float)(0.5 * tp.box);
+        rho0 = m0 / ((2.0 * r0) * (2.0 * r0) * (2.0 * r0));
+        nx = dp.nx > 0.0f ? dp.nx : (float)cbrt((double)n);
+        float a1 = r0;
+        const float e2 = 2.0f * a1;
+        float a3 = (float)(rho0 * (double)(e2 * e2 * e2));
+        for (int i = 0; i < CHUBITS; i++) {
+            ucell[i].halfsz = a1;
+            ucell[i].mass = a3;
+            a1 *= 0.5f;
+            a3 *= 0.125f;
+        }
         body_leaf.assign(n, 0);
         mmtails.assign(3 * (size_t)n, 0);
         background.assign(3 * (size_t)n, 0.0);
         sbsink.assign(6 * (size_t)n, 0.0f);
         ubs.assign(4 * (size_t)nn, 0.0f);
-        // the AREL radii of the background-subtracted cells (nln/cofm.c:356-360,
-        // :425-434): cells of more than 8 bodies bound the odd terms by the sums of
-        // |third| and |fifth| raw moments about the centre of mass, with B4 = B6 = 0
-        if (tp.mac == 1) {
-            const double bmax0 = 0.5 * tree.size;
+    }
+
+    // The expansion basis of every cell: its centre, the raw moments about it, bmax
+    // and the critical radii.  By default the tree's (centre of mass); with
+    // geometric_center the key cube's centre (nln/cofm.c:141-146: moments about it,
+    // the dipole no longer zero) and bmax by CofmFromDaugh's recursion about it
+    // (nln/cofm.c:183-222: daughters' distance plus bmax, bodies' distance, capped
+    // by the cube's half-diagonal).  The error-tolerance radii are taken about the
+    // basis centre; with subtract_background, cells of more than 8 bodies bound the
+    // odd terms by the sums of |third| and |fifth| moments with B4 = B6 = 0
+    // (nln/cofm.c:356-360, :425-434).
+    std::vector<double> bcen, braw, bbmax, brcrit;
+    void cell_basis() {
+        const int64_t nn = tree.nnodes();
+        const bool geo = dp.geometric_center != 0;
+        bcen = tree.cofm;
+        braw = tree.raw;
+        bbmax = tree.bmax;
+        brcrit = tree.rcrit;
+        if (geo) {
             for (int64_t c = 0; c < nn; c++) {
-                if (tree.count[c] <= 8)
-                    continue;
-                const double *raw = &tree.raw[35 * (size_t)c];
-                const double *cm = &tree.cofm[3 * c];
-                double s3 = 0.0;
-                for (int a = 3; a >= 0; a--)
-                    for (int b = 3 - a; b >= 0; b--) {
-                        const int cc = 3 - a - b;
-                        s3 += multinomial(3, a, b, cc) * fabs(raw[raw_index(a, b, cc)]);
-                    }
-                double m5[21] = {0};
+                const float *g = &geom[5 * (size_t)c];
+                for (int a = 0; a < 3; a++) bcen[3 * c + a] = (double)(g[a] + 0.5f * g[3]);
+                double *u = &braw[35 * (size_t)c];
+                std::fill(u, u + 35, 0.0);
+                const double *cc = &bcen[3 * c];
                 for (int64_t i = tree.first[c]; i < tree.first[c] + tree.count[c]; i++) {
-                    const double dx = tree.pos[3 * i] - cm[0], dy = tree.pos[3 * i + 1] - cm[1],
-                                 dz = tree.pos[3 * i + 2] - cm[2];
-                    int k = 0;
-                    for (int a = 5; a >= 0; a--)
-                        for (int b = 5 - a; b >= 0; b--, k++)
-                            m5[k] += tree.mass[i] * pow(dx, a) * pow(dy, b) * pow(dz, 5 - a - b);
+                    const double d[3] = {tree.pos[3 * i] - cc[0], tree.pos[3 * i + 1] - cc[1],
+                                         tree.pos[3 * i + 2] - cc[2]};
+                    for (int a = 0; a <= 4; a++)
+                        for (int b = 0; a + b <= 4; b++)
+                            for (int e = 0; a + b + e <= 4; e++)
+                                u[raw_index(a, b, e)] += tree.mass[i] * pow(d[0], a) * pow(d[1], b) * pow(d[2], e);
                 }
-                double s5 = 0.0;
+            }
+            // children follow their parents in the node order: deepest first backwards
+            for (int64_t c = nn - 1; c >= 0; c--) {
+                const double *cc = &bcen[3 * c];
+                double bm = 0.0;
+                auto dist = [&](const double *p) {
+                    const double dx = p[0] - cc[0], dy = p[1] - cc[1], dz = p[2] - cc[2];
+                    return sqrt(dx * dx + dy * dy + dz * dz);
+                };
+                if (tree.child[c] >= 0) {
+                    for (int k = 0; k < tree.nchild[c]; k++) {
+                        const int32_t d = tree.child[c] + k;
+                        const bool single = tree.child[d] < 0 && tree.count[d] == 1;
+                        bm = std::max(bm, single ? dist(&tree.pos[3 * tree.first[d]])
+                                                 : dist(&bcen[3 * d]) + bbmax[d]);
+                    }
+                } else {
+                    for (int64_t i = tree.first[c]; i < tree.first[c] + tree.count[c]; i++)
+                        bm = std::max(bm, dist(&tree.pos[3 * i]));
+                }
+                const float half = 0.5f * geom[5 * (size_t)c + 3];
+                bbmax[c] = std::min(bm, (double)sqrtf(3.0f * half * half));
+            }
+        }
+        if (tp.mac != 1 || (!geo && !sb))
+            return;
+        const double bmax0 = 0.5 * tree.size;
+        for (int64_t c = 0; c < nn; c++) {
+            const double *raw = &braw[35 * (size_t)c];
+            const double *cm = &bcen[3 * c];
+            const double B2 = raw[raw_index(2, 0, 0)] + raw[raw_index(0, 2, 0)] + raw[raw_index(0, 0, 2)];
+            const double B4 = raw[raw_index(4, 0, 0)] + raw[raw_index(0, 4, 0)] +
+                              raw[raw_index(0, 0, 4)] + 2 * raw[raw_index(2, 2, 0)] +
+                              2 * raw[raw_index(2, 0, 2)] + 2 * raw[raw_index(0, 2, 2)];
+            double B6 = 0.0, m5[21] = {0};
+            for (int64_t i = tree.first[c]; i < tree.first[c] + tree.count[c]; i++) {
+                const double dx = tree.pos[3 * i] - cm[0], dy = tree.pos[3 * i + 1] - cm[1],
+                             dz = tree.pos[3 * i + 2] - cm[2];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                B6 += tree.mass[i] * r2 * r2 * r2;
+                int k = 0;
+                for (int a = 5; a >= 0; a--)
+                    for (int b = 5 - a; b >= 0; b--, k++)
+                        m5[k] += tree.mass[i] * pow(dx, a) * pow(dy, b) * pow(dz, 5 - a - b);
+            }
+            double *out = &brcrit[3 * (size_t)c];
+            if (sb && tree.count[c] > 8) {
+                double s3 = 0.0, s5 = 0.0;
+                for (int a = 3; a >= 0; a--)
+                    for (int b = 3 - a; b >= 0; b--)
+                        s3 += multinomial(3, a, b, 3 - a - b) * fabs(raw[raw_index(a, b, 3 - a - b)]);
                 int k = 0;
                 for (int a = 5; a >= 0; a--)
                     for (int b = 5 - a; b >= 0; b--, k++) s5 += multinomial(5, a, b, 5 - a - b) * fabs(m5[k]);
                 const double sb35[2] = {s3 / 9.0, s5 / 27.0};
-                const double B2 = raw[raw_index(2, 0, 0)] + raw[raw_index(0, 2, 0)] + raw[raw_index(0, 0, 2)];
-                const double B4 = raw[raw_index(4, 0, 0)] + raw[raw_index(0, 4, 0)] +
-                                  raw[raw_index(0, 0, 4)] + 2 * raw[raw_index(2, 2, 0)] +
-                                  2 * raw[raw_index(2, 0, 2)] + 2 * raw[raw_index(0, 2, 2)];
-                arel_rcrit(raw[0], tree.bmax[c], B2, B4, 0.0, tree.count[c], tp, bmax0,
-                           &tree.rcrit[3 * (size_t)c], sb35);
+                arel_rcrit(raw[0], bbmax[c], B2, B4, B6, tree.count[c], tp, bmax0, out, sb35);
+            } else {
+                arel_rcrit(raw[0], bbmax[c], B2, B4, B6, tree.count[c], tp, bmax0, out);
             }
         }
     }
@@ -977,21 +1037,26 @@ struct DLWalk {
             body[4 * i] = (float)tree.mass[i];
             for (int a = 0; a < 3; a++) body[4 * i + 1 + a] = (float)tree.pos[3 * i + a];
         }
+        if (sb || dp.geometric_center)
+            setup_geom();
         if (sb)
             setup_sb();
+        cell_basis();
         cell.resize(nn);
         rec.assign((size_t)HSZ * nn, 0.0f);
+        const bool geo = dp.geometric_center != 0;
         for (int64_t c = 0; c < nn; c++) {
             Cell &C = cell[c];
-            const double *raw = &tree.raw[35 * (size_t)c];
+            const double *raw = &braw[35 * (size_t)c];
+            const double *cen = &bcen[3 * c];
             C.mass = (float)raw[0];
-            for (int a = 0; a < 3; a++) C.pos[a] = (float)tree.cofm[3 * c + a];
-            C.bmax = (float)tree.bmax[c];
+            for (int a = 0; a < 3; a++) C.pos[a] = (float)cen[a];
+            C.bmax = (float)bbmax[c];
             C.daughters = tree.count[c];
             if (tp.mac == 1) {
                 // radii of orders 0, 2, 4 (a cell too small for an order has none: its
                 // test never passes -- isquad / ishexa are false for it anyway)
-                const double *rc = &tree.rcrit[3 * (size_t)c];
+                const double *rc = &brcrit[3 * (size_t)c];
                 C.rcrit = (float)rc[0];
                 C.rcrit_q = std::isnan(rc[1]) ? (float)rc[0] : (float)rc[1];
                 C.rcrit_h = std::isnan(rc[2]) ? C.rcrit_q : (float)rc[2];
@@ -1000,16 +1065,17 @@ struct DLWalk {
                 // nln/cofm.c:568-569) and each lower order k where its truncation
                 // bound (b / r)^(k+1) is the top order's, theta^(P+1)
                 const int P = tp.order;
-                const double b = tree.bmax[c];
+                const double b = bbmax[c];
                 auto rk = [&](int k) { return (float)(b * pow(tp.theta, -(double)(P + 1) / (k + 1))); };
                 C.rcrit = rk(0);
                 C.rcrit_q = P >= 2 ? rk(2) : C.rcrit;
                 C.rcrit_h = P >= 4 ? rk(4) : C.rcrit_q;
             }
             double r29[HSZ];
-            const double R = tree.bmax[c] > 0 ? tree.bmax[c] : 1.0;
-            raw_to_record<double>(raw, &tree.cofm[3 * c], R, r29);
-            r29[5] = r29[6] = r29[7] = 0.0;  // no dipole about the centre of mass
+            const double R = bbmax[c] > 0 ? bbmax[c] : 1.0;
+            raw_to_record<double>(raw, cen, R, r29);
+            if (!geo)
+                r29[5] = r29[6] = r29[7] = 0.0;  // no dipole about the centre of mass
             if (sb) {
                 // the hexacell's unsubtracted umass / uxxxx / uyyyy / uxxyy, then the
                 // record of the moments less the cell's cube (nln/cofm.c:345-349, :540-549)
@@ -1021,8 +1087,9 @@ struct DLWalk {
                 double sub[35];
                 memcpy(sub, raw, sizeof sub);
                 add_cube((double)geom[5 * c + 3], -rho0, sub);
-                raw_to_record<double>(sub, &tree.cofm[3 * c], R, r29);
-                r29[5] = r29[6] = r29[7] = 0.0;
+                raw_to_record<double>(sub, cen, R, r29);
+                if (!geo)
+                    r29[5] = r29[6] = r29[7] = 0.0;
             }
             for (int k = 0; k < HSZ; k++) rec[(size_t)HSZ * c + k] = (float)r29[k];
         }

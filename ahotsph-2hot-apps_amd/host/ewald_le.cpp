@@ -124,7 +124,18 @@ static double binom(int n, int r) {
     return c;
 }
 
-void le_matrix(int nimage, std::vector<double> &W) {
+// x^a y^b z^c integrated over the unit cube [-1/2, 1/2]^3
+static double unit_cube_moment(const int *e) {
+    double v = 1.0;
+    for (int a = 0; a < 3; a++) {
+        if (e[a] & 1)
+            return 0.0;
+        v /= (double)(e[a] + 1) * pow(2.0, e[a]);
+    }
+    return v;
+}
+
+void le_matrix(int nimage, std::vector<double> &W, bool sb) {
     const int NM = LE_NMON;
     W.assign((size_t)NM * NM, 0.0);
     double ck[LE_DEG + 1] = {0};
@@ -146,6 +157,18 @@ void le_matrix(int nimage, std::vector<double> &W) {
             const double sgn = ((ga[0] + ga[1] + ga[2]) & 1) ? -1.0 : 1.0;
             W[(size_t)bi * NM + gi] += -ck[k] * kap * mult * sgn;
         }
+    }
+    if (sb) {
+        // the sources less the box's uniform cube of the total mass mu_0: mu_gamma ->
+        // mu_gamma - mu_0 kappa_gamma (calculate_cartesian_moments with msb,
+        // nln/ewald_le2.c:605-646; kappa_0 = 1 gives Q[0] = 0, :688-691), folded into
+        // column 0 of W
+        std::vector<double> wk(NM, 0.0);
+        for (int bi = 0; bi < NM; bi++)
+            for (int gi = 0; gi < NM; gi++)
+                wk[bi] += W[(size_t)bi * NM + gi] * unit_cube_moment(LE_MON.e[gi]);
+        for (int bi = 0; bi < NM; bi++) W[(size_t)bi * NM + 0] -= wk[bi];
+        return;
     }
     // background (nln/ewald_le2.c:26-48): per unit source mass the constant
     // phicorr + xi and -(2 pi / 3) |x - x_j|^2; phicorr = sum over the explicit
@@ -170,10 +193,10 @@ void le_matrix(int nimage, std::vector<double> &W) {
 }
 
 void le_evaluate_host(const double *pos3, const double *mass, int64_t n, double box, int nimage,
-                      double G, double *acc4) {
+                      double G, double *acc4, bool sb) {
     const int NM = LE_NMON;
     std::vector<double> W;
-    le_matrix(nimage, W);
+    le_matrix(nimage, W, sb);
     auto powers = [&](const double *p, double px[][LE_DEG + 1]) {
         for (int a = 0; a < 3; a++) {
             const double x = (p[a] - 0.5 * box) / box;
@@ -206,6 +229,19 @@ void le_evaluate_host(const double *pos3, const double *mass, int64_t n, double 
                 gr[1] += C[b] * e[1] * px[0][e[0]] * px[1][e[1] - 1] * px[2][e[2]];
             if (e[2])
                 gr[2] += C[b] * e[2] * px[0][e[0]] * px[1][e[1]] * px[2][e[2] - 1];
+        }
+        if (sb) {
+            // a -= (4 pi / 3) dipole (nln/ewald_le2.c:371-375: Q[4], Q[7], Q[11] are
+            // -4, 6, 6 times the dipole over the mass); the potential keeps no
+            // self term (that sits in ewald_background, not called)
+            for (int a = 0; a < 3; a++) {
+                int e[3] = {0, 0, 0};
+                e[a] = 1;
+                gr[a] += 4.0 * M_PI / 3.0 * mu[le_index(e[0], e[1], e[2])];
+            }
+            for (int a = 0; a < 3; a++) acc4[4 * i + a] = -G / (box * box) * gr[a];
+            acc4[4 * i + 3] = G / box * phi;
+            continue;
         }
         for (int a = 0; a < 3; a++) acc4[4 * i + a] = -G / (box * box) * gr[a];
         acc4[4 * i + 3] = G / box * (phi - LE_XI * mass[i]);

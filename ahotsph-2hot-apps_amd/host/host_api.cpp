@@ -347,6 +347,22 @@ int pgrav_host_ewald_le(const double *pos3, const double *mass, int64_t n, doubl
     return 0;
 }
 
+int pgrav_host_ewald_le_sb(const double *pos3, const double *mass, int64_t n, double box,
+                           int nimage, double G, double *acc4) {
+    clear_error();
+    if (!pos3 || !mass || !acc4 || n < 0 || !(box > 0) || nimage < 0 || nimage > MAX_NIMAGE) {
+        fail("pgrav_host_ewald_le_sb: bad arguments");
+        return -1;
+    }
+    try {
+        le_evaluate_host(pos3, mass, n, box, nimage, G, acc4, true);
+    } catch (const std::exception &e) {
+        fail("pgrav_host_ewald_le_sb: %s", e.what());
+        return -1;
+    }
+    return 0;
+}
+
 int pgrav_host_le_lattice(int k, int nimage, double *ck) {
     clear_error();
     if (!ck || k < 4 || k > 8 || (k & 1) || nimage < 0) {

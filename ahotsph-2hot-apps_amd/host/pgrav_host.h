@@ -129,8 +129,12 @@ double wave_cost(int64_t hexa, int64_t quad, int64_t mono, int64_t body);
 // lattice coefficient c_k, the moments-to-coefficients matrix (LE_NMON^2, row
 // major) and an FP64 evaluation at the bodies (acc4 per body, G applied).
 double le_lattice_coeff(int k, int nimage);
-void le_matrix(int nimage, std::vector<double> &W);
+// sb: the subtract_background form (nln/ewald_le2.c:371-375, :605-646; no
+// ewald_background, nln/main.c:1839): the moments less the box's uniform cube of
+// the same mass, no background terms, no self term, and the acceleration (only)
+// less (4 pi / 3) G / L^2 times the dipole moment about the box centre.
+void le_matrix(int nimage, std::vector<double> &W, bool sb = false);
 void le_evaluate_host(const double *pos3, const double *mass, int64_t n, double box, int nimage,
-                      double G, double *acc4);
+                      double G, double *acc4, bool sb = false);
 
 }  // namespace pgrav

@@ -97,6 +97,7 @@ register({
     "pgrav_host_sorted": (_i, [_vp, _vp, _vp]),
     "pgrav_host_lpt": (_i, [_vp, _vp]),
     "pgrav_host_ewald_le": (_i, [_vp, _vp, _i64, _d, _i, _d, _vp]),
+    "pgrav_host_ewald_le_sb": (_i, [_vp, _vp, _i64, _d, _i, _d, _vp]),
     "pgrav_host_le_lattice": (_i, [_i, _i, ctypes.POINTER(_d)]),
 })
 
@@ -443,13 +444,15 @@ class HostPlan:
         return o
 
 
-def ewald_le_host(pos, mass, box=1.0, nimage=2, G=1.0):
+def ewald_le_host(pos, mass, box=1.0, nimage=2, G=1.0, subtract_background=False):
     """Far images + background of the explicit-image scheme at each body, FP64 on
-    the host: (n, 4) = (a, phi), what FixCubeEwaldLE adds (nln/main.c:1836-1860)."""
+    the host: (n, 4) = (a, phi), what FixCubeEwaldLE adds (nln/main.c:1836-1860);
+    subtract_background: its form for the background-subtracted lists."""
     pos, mass = _as_inputs(pos, mass)
     out = np.zeros((pos.shape[0], 4))
-    rc = lib().pgrav_host_ewald_le(pos.ctypes.data, mass.ctypes.data, pos.shape[0], float(box),
-                                   int(nimage), float(G), out.ctypes.data)
+    fn = lib().pgrav_host_ewald_le_sb if subtract_background else lib().pgrav_host_ewald_le
+    rc = fn(pos.ctypes.data, mass.ctypes.data, pos.shape[0], float(box), int(nimage), float(G),
+            out.ctypes.data)
     check()
     if rc:
         raise PgravError("pgrav_host_ewald_le failed")

@@ -28,7 +28,8 @@ class DLParams(ctypes.Structure):
                 ("min_qsrc", ctypes.c_int), ("min_hsrc", ctypes.c_int), ("do_pQ", ctypes.c_int),
                 ("do_pQL", ctypes.c_int), ("do_pH", ctypes.c_int), ("log_calls", ctypes.c_int),
                 ("subtract_background", ctypes.c_int), ("do_pM", ctypes.c_int),
-                ("min_msink", ctypes.c_int), ("min_msrc", ctypes.c_int), ("nx", ctypes.c_float)]
+                ("min_msink", ctypes.c_int), ("min_msrc", ctypes.c_int), ("nx", ctypes.c_float),
+                ("geometric_center", ctypes.c_int)]
 
 
 _vp, _i, _i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64

@@ -317,6 +317,13 @@ int pgrav_host_sorted(void *h, int64_t *perm, double *pos3);
  * (k = 4, 6, 8): the x^k coefficient of the far images' 1/r sum. */
 int pgrav_host_ewald_le(const double *pos3, const double *mass, int64_t n, double box, int nimage,
                         double G, double *acc4);
+/* The same pass in the subtract_background form (FixCubeEwaldLE with
+ * subtract_background, nln/main.c:1839-1860; ewald_le2.c:371-375, :605-646): the
+ * far images of the bodies less the box's uniform background, the acceleration
+ * less (4 pi / 3) G / L^2 times the dipole about the box centre; no background or
+ * self terms.  For the lists of pgrav_dl_walk with subtract_background = 1. */
+int pgrav_host_ewald_le_sb(const double *pos3, const double *mass, int64_t n, double box,
+                           int nimage, double G, double *acc4);
 int pgrav_host_le_lattice(int k, int nimage, double *ck);
 int pgrav_host_lpt(void *h, int32_t *order);
 
@@ -373,6 +380,9 @@ typedef struct {
     int do_pM, min_msink, min_msrc;                        /* mxn_cuda: 1, 8, 128 */
     float nx;         /* mac.nx, the cube root of the run's global body count (nln/main.c:856),
                          which sets where near cubes start (nln/mac.c:747); 0: cbrt(n) */
+    int geometric_center; /* geometric_center (nln/main.c:542): cells expanded about their
+                         key cube's centre (dipole moments kept, nln/cofm.c:141-146) instead
+                         of their centre of mass; the near cubes then align with the cells */
 } pgrav_dl_params;
 
 /* arrays of pgrav_dl_array (element type, values per element) */

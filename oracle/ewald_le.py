@@ -114,6 +114,39 @@ def far_field(pos, mass, box=1.0, nimage=2, G=1.0):
     return out
 
 
+def far_field_sb(pos, mass, box=1.0, nimage=2, G=1.0):
+    """The subtract_background form (FixCubeEwaldLE with subtract_background,
+    nln/main.c:1839-1860): the sources are the bodies less a uniform cube of their
+    total mass filling the box (calculate_cartesian_moments with msb adds the unit
+    cube's moments times -M, nln/ewald_le2.c:605-646, and sets Q[0] = 0, :688-691),
+    no ewald_background, and the acceleration less (4 pi / 3) times the normalised
+    dipole (nln/ewald_le2.c:371-375).  Here the cube's part of each sum is its
+    integral, by a 5-point Gauss-Legendre product rule (exact for degree 8)."""
+    pos = np.asarray(pos, np.float64)
+    mass = np.asarray(mass, np.float64)
+    M = mass.sum()
+    x = (pos - 0.5 * box) / box
+    t = dict(zip((4, 6, 8), lattice_t(nimage)))
+    gx, gw = np.polynomial.legendre.leggauss(5)
+    gx, gw = 0.5 * gx, 0.5 * gw
+    cube = np.array(list(itertools.product(gx, gx, gx)))
+    cw = np.array([a * b * c for a, b, c in itertools.product(gw, gw, gw)])
+    q = (mass[:, None] * x).sum(0) / M
+    out = np.zeros((pos.shape[0], 4))
+    for i in range(pos.shape[0]):
+        pp = 0.0
+        grad = np.zeros(3)
+        for k in (8, 6, 4):
+            v, g = poly(K[k], x[i] - x)
+            vc, gc = poly(K[k], x[i] - cube)
+            pp += t[k] * ((mass * v).sum() / M - (cw * vc).sum())
+            grad += t[k] * ((mass[:, None] * g).sum(0) / M - (cw[:, None] * gc).sum(0))
+        gm = G * M
+        out[i, :3] = gm / box ** 2 * (-grad - 4.0 * np.pi / 3.0 * q)
+        out[i, 3] = gm / box * pp
+    return out
+
+
 def near_images(pos, mass, box=1.0, nimage=2, G=1.0):
     """FP64 direct sum over the (2 nimage + 1)^3 explicit images of every body (the
     walk's coverage, nln/mac.c:91-111, 734), the self pair of the central image

@@ -165,6 +165,9 @@ def test_sb_records_and_unsubtracted_fields():
     (4096, 4, "theta", 1, dict(clustered=1)),
     (8192, 4, "theta", 1, dict(clustered=1, nx=512.0)),
     (4096, 4, "tolerance", 1, dict(clustered=1, nx=256.0)),
+    # cells expanded about their key cube's centre
+    (12, 4, "tolerance", 1, dict(geometric_center=1)),
+    (4096, 4, "theta", 1, dict(clustered=1, nx=512.0, geometric_center=1)),
 ])
 def test_sb_walk_calls_match_oracle(ng, order, mac, nimage, dl):
     dl = dict(dl)
@@ -302,3 +305,180 @@ def test_sb_node_geometry():
     for j in range(0, body.shape[0], 37):
         b0, nb = int(info[bl[j], 0]), int(info[bl[j], 1])
         assert b0 <= j < b0 + nb
+
+
+# ---------------------------------------------------------------- the whole scheme
+
+def _geo_walk(ng, rms, nimage, log=1):
+    from pgrav import ics
+    from pgrav.dlwalk import DLWalk
+    pos, mass = ics.lattice_grf(ng, seed=4, rms=rms)
+    W = DLWalk(pos, mass, dl=dict(subtract_background=1, geometric_center=1, log_calls=log),
+               order=4, theta=0.6, periodic="images", box=1.0, nimage=nimage)
+    return pos, mass, W
+
+
+def _pts_field(p, q, m):
+    d = q - p
+    r2 = (d * d).sum(1)
+    ok = r2 > 0
+    return (m[ok, None] * d[ok] / (r2[ok] ** 1.5)[:, None]).sum(0)
+
+
+def test_sb_lists_partition_bodies_less_background():
+    """Every list entry of a body's path evaluated exactly -- a cell's bodies (less
+    its key cube of density rho0 where the record carries the subtraction), the
+    anti-masses as the cubes they stand for, MMvec bodies -- plus the leaf's near
+    cube and the subtract_background far field is the Ewald sum (nln/ewald.c,
+    oracle): the walk covers bodies less background over the explicit images once
+    each.  With geometric_center the signed mass of each body's lists plus its near
+    cube is zero and every near cube holds 27 cells (the check the reference keeps
+    disabled, nln/mac.c:697-707)."""
+    from pgrav import batch
+    pos, mass, W = _geo_walk(12, 0.02, 2)
+    cells, info, geom, body = W.cells, W.node_info, W.node_geom, W.bodies.astype(np.float64)
+    rho0 = mass.sum()
+    off = _offsets(W).astype(np.float64)
+    assert np.all(W.sb_sink[:, 5] == 27)
+    parent = {}
+    co, ch = W.child_off, W.children
+    for c in range(co.size - 1):
+        for e in ch[co[c]:co[c + 1]]:
+            parent[int(e)] = c
+    apps = {}
+    for sink, _, ap in W.mac_calls():
+        apps.setdefault(sink, []).extend(ap.tolist())
+
+    def cube(p, corner, edge, o):
+        return -rho0 * sbmac.cube_field(p - (corner + 0.5 * edge + o), 0.5 * edge)
+    rng = np.random.default_rng(0)
+    sample = rng.choice(body.shape[0], 16, replace=False)
+    perm = W.perm
+    far = batch.ewald_le_host(pos, mass, 1.0, 2, subtract_background=True)
+    ex = oracle.direct_ewald(pos[perm[sample]], pos, mass, 1.0)
+    errs = []
+    for si, j in enumerate(sample):
+        p = body[j, 1:4]
+        e, acc, msum = -(int(j) + 1), np.zeros(3), 0.0
+        path = []
+        while True:
+            path.append(e)
+            if e not in parent:
+                break
+            e = parent[e]
+        for s in path:
+            for code, ent, img in apps.get(s, []):
+                o = off[img]
+                if ent < 0:
+                    b = -ent - 1
+                    assert code == sbmac.M  # no body anti-mass in a lattice this regular
+                    acc += _pts_field(p, body[b:b + 1, 1:4] + o, body[b:b + 1, 0])
+                    msum += body[b, 0]
+                    continue
+                b0, nb = int(info[ent, 0]), int(info[ent, 1])
+                corner, edge = geom[ent, :3].astype(np.float64), float(geom[ent, 3])
+                if code in (sbmac.Q, sbmac.H, sbmac.QB, sbmac.HB, sbmac.MM):
+                    acc += _pts_field(p, body[b0:b0 + nb, 1:4] + o, body[b0:b0 + nb, 0])
+                    msum += body[b0:b0 + nb, 0].sum()
+                    if code in (sbmac.Q, sbmac.H):
+                        acc += cube(p, corner, edge, o)
+                        msum -= rho0 * edge ** 3
+                elif code == sbmac.ANTI_CELL:
+                    acc += cube(p, corner, edge, o)
+                    msum -= rho0 * edge ** 3
+                else:
+                    raise AssertionError("unexpected append %d" % code)
+        cr = float(W.sb_sink[j, 3])
+        assert abs(msum - rho0 * (2 * cr) ** 3) < 1e-6 * rho0
+        acc += W.background[j] + far[perm[j], :3]
+        errs.append(np.linalg.norm(acc - ex[si, :3]))
+    rms = np.sqrt((ex[:, :3] ** 2).sum(1).mean())
+    errs = np.array(errs)
+    assert np.median(errs) < 1e-4 * rms and errs.max() < 1e-3 * rms, (np.median(errs) / rms,
+                                                                       errs.max() / rms)
+
+
+def _evaluate_sb(W, eps_inv):
+    """FP64 oracle evaluation of everything the sb walk emits: the M x N calls, the
+    pQ1 and pMM_mnss_sK1 flushes, the host mxn_mono runs, the leaves (K1-softened
+    M lists, as smooth_type 4 with MMvec, nln/mac.c:221-224) and the near cube."""
+    body = W.bodies
+    n = body.shape[0]
+    acc = np.zeros((n, 4))
+    src = W.sources
+    bp = body[:, 1:4]
+    m0 = float(body[0, 0])
+    for routine, first, m, so, sn, sz in W.calls:
+        acc[first:first + m] += oracle.pmn(4 if routine == 0 else 2, bp[first:first + m],
+                                           src[so:so + sn * sz], sn, sz)
+    for base, m, s0, so, slen, _ in W.qflush:
+        acc[base:base + m] += oracle.pq1_mnss(bp[base:base + m], W.qseg[s0:s0 + m],
+                                              src[so:so + slen * 13])
+    for base, m, i0, e0, ss_len, g0, seg_n, _ in W.mflush:
+        out, bad = oracle.pmm1_mnss_sk1(bp[base:base + m], W.mss_index[i0:i0 + m],
+                                        W.mss_seg[e0:e0 + ss_len], W.mseg[g0:g0 + seg_n], bp,
+                                        W.mss_source_n[e0:e0 + ss_len], m0, eps_inv)
+        assert bad == 0
+        acc[base:base + m] += out
+    for first, m, s0, ns, nsrc, _ in W.mcalls:
+        acc[first:first + m] += oracle.pmm_sk1(bp[first:first + m], W.mseg[s0:s0 + ns], bp, m0,
+                                               eps_inv)
+    for j in range(n):
+        ho, hn, qo, qn, mo, mn = W.tails[j]
+        p = bp[j:j + 1]
+        if hn:
+            acc[j] += oracle.pmn(4, p, src[ho:ho + hn * 29], hn, 29)[0]
+        if qn:
+            acc[j] += oracle.pmn(2, p, src[qo:qo + qn * 13], qn, 13)[0]
+        if mn:
+            acc[j] += oracle.pm_sk1(p, src[mo:mo + mn * 4], mn, eps_inv)[0]
+        s0, ns, nsrc = W.mmtails[j]
+        if ns:
+            acc[j] += oracle.pmm_sk1(p, W.mseg[s0:s0 + ns], bp, m0, eps_inv)[0]
+    acc[:, :3] += W.background
+    return acc
+
+
+@pytest.mark.parametrize("rms,bound", [(0.02, 3e-2), (0.3, 1e-2)])
+def test_sb_forces_match_ewald(rms, bound):
+    """The emitted lists through the oracle kernels plus the far field against the
+    Ewald sum.  What remains is the scheme's own approximation: the subtracted
+    cells' truncated expansions and the split cells' background as point masses
+    (the exact-entry evaluation above closes to 1e-5)."""
+    from pgrav import batch
+    pos, mass, W = _geo_walk(12, rms, 1, log=0)
+    acc = _evaluate_sb(W, 1e5)
+    perm = W.perm
+    acc[:, :3] += batch.ewald_le_host(pos, mass, 1.0, 1, subtract_background=True)[perm, :3]
+    rng = np.random.default_rng(0)
+    s = rng.choice(pos.shape[0], 48, replace=False)
+    ex = oracle.direct_ewald(pos[perm[s]], pos, mass, 1.0)
+    rmsf = np.sqrt((ex[:, :3] ** 2).sum(1).mean())
+    e = np.linalg.norm(acc[s, :3] - ex[:, :3], axis=1) / rmsf
+    assert np.median(e) < bound, np.median(e)
+
+
+def test_geometric_center_records():
+    """geometric_center: each cell's record is its bodies' expansion about the key
+    cube's centre (dipole included) and bmax bounds its bodies' distance from that
+    centre within the cube's half-diagonal (nln/cofm.c:141-146, :183-222)."""
+    pos, mass, W = _walk(clustered=4096, log=0, subtract_background=0, geometric_center=1)
+    info, geom, cells, rec, body = W.node_info, W.node_geom, W.cells, W.records, W.bodies
+    checked = 0
+    for c in range(info.shape[0]):
+        b0, nb = int(info[c, 0]), int(info[c, 1])
+        if nb < 2:
+            continue
+        edge = float(geom[c, 3])
+        cen = geom[c, :3].astype(np.float64) + 0.5 * edge
+        np.testing.assert_allclose(cells[c, 1:4], cen, rtol=0, atol=1e-7)
+        p = body[b0:b0 + nb, 1:4].astype(np.float64)
+        dmax = np.sqrt(((p - cen) ** 2).sum(1)).max()
+        # (the float bodies sit within 1e-7 of the FP64 positions the library used)
+        assert dmax - 1e-7 <= cells[c, 5] <= 0.5 * edge * np.sqrt(3) * (1 + 1e-6)
+        if nb >= 20 and checked < 60:
+            want = oracle.cell_record(p, np.full(nb, float(body[b0, 0])), cen, float(cells[c, 5]))
+            got = rec[c].astype(np.float64)
+            np.testing.assert_allclose(got, want, rtol=2e-6, atol=2e-6 * np.abs(want).max())
+            checked += 1
+    assert checked > 20

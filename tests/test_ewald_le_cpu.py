@@ -99,6 +99,46 @@ def test_images_plus_far_field_is_the_ewald_sum(nimage, tol):
     assert np.abs(tot[:, 3] - ex[:, 3]).max() < 10 * tol * np.abs(ex[:, 3]).mean()
 
 
+@pytest.mark.parametrize("nimage", [1, 2])
+def test_host_far_field_sb_matches_oracle(nimage):
+    """The subtract_background form: the library's moment expansion with the cube
+    folded into W against the oracle's pair sum less the cube's integral."""
+    rng = np.random.default_rng(20 + nimage)
+    pos = rng.random((50, 3)) * 2.0
+    mass = rng.random(50) + 0.1
+    ref = ewald_le.far_field_sb(pos, mass, box=2.0, nimage=nimage, G=0.7)
+    got = batch.ewald_le_host(pos, mass, box=2.0, nimage=nimage, G=0.7, subtract_background=True)
+    scale = np.abs(ref[:, :3]).max()
+    np.testing.assert_allclose(got[:, :3], ref[:, :3], rtol=0, atol=1e-10 * scale)
+    np.testing.assert_allclose(got[:, 3], ref[:, 3], rtol=0, atol=1e-10 * np.abs(ref[:, 3]).max())
+
+
+@pytest.mark.parametrize("nimage,tol", [(1, 1e-3), (2, 2e-6)])
+def test_images_less_background_plus_sb_far_field_is_the_ewald_sum(nimage, tol):
+    """The subtract_background scheme's accounting: the explicit images of the
+    bodies, less the uniform background over the same (2 nimage + 1)^3 boxes (a
+    cube of density -M / L^3, its field in closed form), plus the sb far field, is
+    the Ewald sum (nln/ewald.c, oracle) -- the dipole term turns the cubic
+    summation into the Ewald (tin-foil) convention."""
+    from oracle import sbmac
+    rng = np.random.default_rng(30 + nimage)
+    n = 40
+    pos = rng.random((n, 3))
+    pos[:8] *= 0.3  # a net dipole about the box centre
+    mass = rng.random(n) + 0.5
+    M = mass.sum()
+    half = (2 * nimage + 1) * 0.5
+    bg = np.array([-M * sbmac.cube_field(p - 0.5, half) for p in pos])
+    far = batch.ewald_le_host(pos, mass, 1.0, nimage, subtract_background=True)
+    tot = ewald_le.near_images(pos, mass, 1.0, nimage)[:, :3] + bg + far[:, :3]
+    ex = oracle.direct_ewald(pos, pos, mass, 1.0)
+    rms = np.sqrt((ex[:, :3] ** 2).sum(1).mean())
+    assert np.linalg.norm(tot - ex[:, :3], axis=1).max() < tol * rms
+    # without the dipole term the sum misses (4 pi / 3) G P / L^3
+    dip = 4 * np.pi / 3 * (mass[:, None] * (pos - 0.5)).sum(0)
+    assert np.linalg.norm(tot + dip - ex[:, :3], axis=1).max() > 10 * tol * rms
+
+
 @pytest.mark.parametrize("nimage", [0, 1])
 def test_image_lists_cover_n_per_image(nimage):
     """Each sink meets every body once per explicit image (nln/mac.c:734:
