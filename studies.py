@@ -19,8 +19,10 @@ c3  clustered bodies (pgrav.ics.clustered), order 4, Plummer eps 1e-4, for each
     opening angle: kernel time of the FP32 group-relative compensated (rel32c),
     plain-sum (rel32) and absolute-coordinate (abs32, the reference's arithmetic)
     variants and of the batched FP64 variant, each with its force error against
-    the FP64 direct sum on 8192 sampled bodies; --group-boxes 1 measures the
-    acceptance distance from each group's bounding box instead of its sphere.
+    the FP64 direct sum on 8192 sampled bodies and, for the FP32 variants, against
+    the FP64 kernel on the same lists over all bodies (the FP32 rounding alone);
+    --group-boxes 1 measures the acceptance distance from each group's bounding
+    box instead of its sphere; --offset K shifts the system by K system sizes.
 c4  periodic unit box, ng^3 lattice displaced by a Gaussian random field
     (pgrav.ics.lattice_grf): eager and graph-replayed evaluation time, error
     against the periodic FP64 direct sum (nearest image + 64^3 Ewald table) on
@@ -115,6 +117,9 @@ def c2(args):
 
 def c3(args):
     pos, mass = ics.clustered(args.n, seed=2)
+    # --offset moves the whole system by that many system sizes: the lists and the
+    # FP64 results do not change, the absolute FP32 coordinates lose bits
+    pos = pos + args.offset * (np.ptp(pos, axis=0).max())
     rng = np.random.default_rng(12345)
     sample = rng.choice(pos.shape[0], min(8192, pos.shape[0]), replace=False)
     ref = None
@@ -131,12 +136,25 @@ def c3(args):
         row = {"config": "c3", "n": int(pos.shape[0]), "theta": theta, "group_box": gbox,
                "decomposition": dec, "groups": st["ngroups"], "pairs": st["pairs_total"],
                "pairs_hexa": st["pairs_hexa"], "pairs_body": st["pairs_body"], "variants": {}}
-        for var in args.variants:
+        # the batched FP64 kernel first: every FP32 variant is also compared with it on
+        # all bodies (same lists, so what differs is the FP32 arithmetic alone)
+        f64 = None
+        for var in sorted(args.variants, key=lambda v: v != "f64"):
             plan.evaluate(var)
             ms = plan.time(var, reps=3)
-            e = _errors(plan.result(var)[sample], ref)
+            res = plan.result(var)
+            e = _errors(res[sample], ref)
             row["variants"][var] = {"kernel_ms": ms, "interactions_per_s": st["pairs_total"] / (ms * 1e-3),
                                     "errors": e}
+            if var == "f64":
+                f64 = res.copy()
+            elif f64 is not None:
+                da, dp = batch.rel_errors(res, f64)
+                row["variants"][var]["vs_f64_same_lists"] = {
+                    "median": float(np.median(da)), "p99": float(np.percentile(da, 99)),
+                    "max": float(da.max()), "phi_median": float(np.median(dp))}
+                _log("   %-6s vs f64, all bodies: median %.2e  p99 %.2e  max %.2e" %
+                     (var, np.median(da), np.percentile(da, 99), da.max()))
             _log("   %-6s %9.3f ms  median %.2e  p99 %.2e" % (var, ms, e["median"], e["p99"]))
         plan.close()
         print(json.dumps(row), flush=True)
@@ -373,6 +391,8 @@ def main(argv=None):
                          "Ewald correction joins the group's expansion (0: per pair)")
     ap.add_argument("--variants", nargs="+", default=["rel32c", "rel32", "abs32", "f64"])
     ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--offset", type=float, default=0.0,
+                    help="c3: shift the system by this many system sizes")
     ap.add_argument("--ng", type=int, default=256)
     ap.add_argument("--theta", type=float, default=0.6)
     ap.add_argument("--eps", type=float, default=1e-4)
