@@ -603,11 +603,14 @@ void grav_mnss_CUDA(const char *routine, const int sink_base, const int m,
     if (!upload(s, total))
         return;
     char *d = (char *)s.dev.p;
+    int64_t max_source_n = 0;
+    for (int k = 0; k < ss_len; k++) max_source_n = std::max<int64_t>(max_source_n, source_n[k]);
     check(launch_mnss((const float *)S.devpos.p + 3 * (size_t)sink_base,
                       (float *)S.devacc.p + 4 * (size_t)sink_base, m, (const uint16_t *)d,
                       (const uint32_t *)(d + o_ss_seg), ss_len, (const uint32_t *)(d + o_seg),
                       seg_len, (const float *)S.devpos.p + 3 * (size_t)source_base,
-                      (const int *)(d + o_srcn), mmass, e, (int *)S.errflag.p, s.stream),
+                      (const int *)(d + o_srcn), max_source_n, mmass, e, (int *)S.errflag.p,
+                      s.stream),
           "pMM_mnss_sK1 kernel");
 }
 
@@ -655,11 +658,14 @@ void grav_qnss_CUDA(const char *routine, const int base, const int m, const segm
         fail("source_n %d is not a multiple of NSSE = 8", source_n);
         return;
     }
-    for (int i = 0; i < m; i++)
+    int64_t max_len = 0;
+    for (int i = 0; i < m; i++) {
         if ((int64_t)seg[i].base + seg[i].length > source_n) {
             fail("seg[%d] runs past the %d queued sources", i, source_n);
             return;
         }
+        max_len = std::max<int64_t>(max_len, seg[i].length);
+    }
     size_t o_src = align128((size_t)m * 8);
     size_t bytes = (size_t)source_n * 13 * sizeof(float);
     size_t total = o_src + bytes;
@@ -677,7 +683,7 @@ void grav_qnss_CUDA(const char *routine, const int base, const int m, const segm
         return;
     check(launch_qnss(quad_phi_sign(), (const float *)S.devpos.p + 3 * (size_t)base,
                       (float *)S.devacc.p + 4 * (size_t)base, m, (const uint32_t *)s.dev.p,
-                      (const float4 *)s.pos4.p, (const float *)s.mom.p, s.stream),
+                      max_len, (const float4 *)s.pos4.p, (const float *)s.mom.p, s.stream),
           "pQ1_qnss kernel");
 }
 

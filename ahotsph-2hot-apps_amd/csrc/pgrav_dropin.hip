@@ -146,61 +146,109 @@ __global__ __launch_bounds__(64) void k_mns(const float *__restrict__ sinkpos,
         flush_acc(acc + 4 * s, a);
 }
 
-// pMM1_mnss_sK1 (nln/pgrav.cu:713-790): sink i uses segment run ss_seg[ss_index[i]].
-__global__ __launch_bounds__(256) void k_mnss(const float *__restrict__ sinkpos,
-                                             float *__restrict__ acc, int m,
-                                             const uint16_t *__restrict__ ss_index,
-                                             const uint32_t *__restrict__ ss_seg, int ss_len,
-                                             const uint32_t *__restrict__ s_seg, int s_seg_len,
-                                             const float *__restrict__ srcpos,
-                                             const int *__restrict__ s_source_n, float mmass,
-                                             float eps_inv, int *__restrict__ errflag) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= m)
-        return;
-    const int si = ss_index[s];
-    if (si >= ss_len || ss_seg[2 * si] + ss_seg[2 * si + 1] > (uint32_t)s_seg_len) {
-        atomicAdd(errflag, 1);
-        return;
-    }
-    const uint32_t *seg = s_seg + 2 * (size_t)ss_seg[2 * si];
-    const int seg_len = (int)ss_seg[2 * si + 1];
-    const float px = sinkpos[3 * s], py = sinkpos[3 * s + 1], pz = sinkpos[3 * s + 2];
-    const float eps2 = 1.0f / (eps_inv * eps_inv);
-    Acc4 a = {0.f, 0.f, 0.f, 0.f};
-    int count = 0;
-    for (int g = 0; g < seg_len; g++) {
-        const uint32_t base = seg[2 * g], len = seg[2 * g + 1];
-        count += (int)len;
-        for (uint32_t j = 0; j < len; j++) {
-            const float *q = srcpos + 3 * (size_t)(base + j);
-            k1_interact(mmass, px - q[0], py - q[1], pz - q[2], eps_inv, eps2, a);
+// The batched queue kernels below give each sink one lane and cut its source list
+// into `split` contiguous slices, W per workgroup (one wave each): a flush of a
+// few hundred sinks then still launches thousands of waves instead of a handful
+// of latency-bound ones.  The W slices of a workgroup are summed in LDS in wave
+// order and added once into the accumulator.
+template <int W>
+__device__ __forceinline__ void sum_slices(float4 (*part)[64], int lane, int w, bool live,
+                                           const Acc4 &a, float *acc) {
+    part[w][lane] = make_float4(a.x, a.y, a.z, a.w);
+    __syncthreads();
+    if (w == 0 && live) {
+        Acc4 t = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            const float4 v = part[k][lane];
+            t.x += v.x;
+            t.y += v.y;
+            t.z += v.z;
+            t.w += v.w;
         }
+        flush_acc(acc, t);
     }
-    flush_acc(acc + 4 * s, a);
-    if (count != s_source_n[si])
-        atomicAdd(errflag, 1);  // nln/pgrav.cu:779-789
 }
 
-// pQ1_mnss (nln/pgrav.cu:293-368): sink i uses run seg[i] of the packed buffer.
-template <int QSIGN>
-__global__ __launch_bounds__(64) void k_qnss(const float *__restrict__ sinkpos,
-                                             float *__restrict__ acc, int m,
-                                             const uint32_t *__restrict__ seg,
-                                             const float4 *__restrict__ pos4,
-                                             const float *__restrict__ mom) {
-    const int s = blockIdx.x * 64 + threadIdx.x;
-    if (s >= m)
-        return;
-    const uint32_t base = seg[2 * s], len = seg[2 * s + 1];
-    const float px = sinkpos[3 * s], py = sinkpos[3 * s + 1], pz = sinkpos[3 * s + 2];
+// pMM1_mnss_sK1 (nln/pgrav.cu:713-790): sink i uses segment run ss_seg[ss_index[i]]
+// of the segment array; slice k of a sink covers bodies [k c, (k + 1) c) of the
+// run's bodies in segment order, c = ceil(bodies / split).  The count check of
+// the reference (its device printf, nln/pgrav.cu:779-789) is made once per sink.
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_mnss(const float *__restrict__ sinkpos,
+                                                 float *__restrict__ acc, int m,
+                                                 const uint16_t *__restrict__ ss_index,
+                                                 const uint32_t *__restrict__ ss_seg, int ss_len,
+                                                 const uint32_t *__restrict__ s_seg, int s_seg_len,
+                                                 const float *__restrict__ srcpos,
+                                                 const int *__restrict__ s_source_n, float mmass,
+                                                 float eps_inv, int *__restrict__ errflag,
+                                                 int split) {
+    __shared__ float4 part[W][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s = blockIdx.x * 64 + lane;
+    const int slice = blockIdx.y * W + w;
+    bool live = s < m;
     Acc4 a = {0.f, 0.f, 0.f, 0.f};
-    for (uint32_t j = 0; j < len; j++) {
-        const float4 q = pos4[base + j];
-        cell_interact<2, true, QSIGN>(mom + (size_t)(base + j) * PREC2, q.w, px - q.x, py - q.y,
-                                      pz - q.z, 0.0f, a);
+    if (live) {
+        const int si = ss_index[s];
+        if (si >= ss_len || ss_seg[2 * si] + ss_seg[2 * si + 1] > (uint32_t)s_seg_len) {
+            if (slice == 0)
+                atomicAdd(errflag, 1);
+            live = false;
+        } else {
+            const uint32_t *seg = s_seg + 2 * (size_t)ss_seg[2 * si];
+            const int seg_len = (int)ss_seg[2 * si + 1];
+            uint32_t total = 0;
+            for (int g = 0; g < seg_len; g++) total += seg[2 * g + 1];
+            if (slice == 0 && (int)total != s_source_n[si])
+                atomicAdd(errflag, 1);
+            const uint32_t chunk = (total + split - 1) / split;
+            const uint32_t j0 = (uint32_t)min((uint64_t)total, (uint64_t)slice * chunk);
+            const uint32_t j1 = min(total, j0 + chunk);
+            const float px = sinkpos[3 * s], py = sinkpos[3 * s + 1], pz = sinkpos[3 * s + 2];
+            const float eps2 = 1.0f / (eps_inv * eps_inv);
+            uint32_t off = 0;
+            for (int g = 0; g < seg_len && off < j1; g++) {
+                const uint32_t base = seg[2 * g], len = seg[2 * g + 1];
+                const uint32_t lo = max(j0, off), hi = min(j1, off + len);
+                for (uint32_t j = lo; j < hi; j++) {
+                    const float *q = srcpos + 3 * (size_t)(base + (j - off));
+                    k1_interact(mmass, px - q[0], py - q[1], pz - q[2], eps_inv, eps2, a);
+                }
+                off += len;
+            }
+        }
     }
-    flush_acc(acc + 4 * s, a);
+    sum_slices<W>(part, lane, w, live, a, acc + 4 * (size_t)(live ? s : 0));
+}
+
+// pQ1_mnss (nln/pgrav.cu:293-368): sink i uses run seg[i] of the packed buffer;
+// slice k of a sink covers [k c, (k + 1) c) of its run, c = ceil(length / split).
+template <int QSIGN, int W>
+__global__ __launch_bounds__(64 * W) void k_qnss(const float *__restrict__ sinkpos,
+                                                 float *__restrict__ acc, int m,
+                                                 const uint32_t *__restrict__ seg,
+                                                 const float4 *__restrict__ pos4,
+                                                 const float *__restrict__ mom, int split) {
+    __shared__ float4 part[W][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s = blockIdx.x * 64 + lane;
+    const bool live = s < m;
+    Acc4 a = {0.f, 0.f, 0.f, 0.f};
+    if (live) {
+        const uint32_t base = seg[2 * s], len = seg[2 * s + 1];
+        const uint32_t chunk = (len + split - 1) / split;
+        const uint32_t j0 = (uint32_t)min((uint64_t)len, (uint64_t)(blockIdx.y * W + w) * chunk);
+        const uint32_t j1 = min(len, j0 + chunk);
+        const float px = sinkpos[3 * s], py = sinkpos[3 * s + 1], pz = sinkpos[3 * s + 2];
+        for (uint32_t j = j0; j < j1; j++) {
+            const float4 q = pos4[base + j];
+            cell_interact<2, true, QSIGN>(mom + (size_t)(base + j) * PREC2, q.w, px - q.x,
+                                          py - q.y, pz - q.z, 0.0f, a);
+        }
+    }
+    sum_slices<W>(part, lane, w, live, a, acc + 4 * (size_t)(live ? s : 0));
 }
 
 // pQQ1 (nln/pgrav.cu:498-572) with the defects of A5 fixed: bound on the
@@ -292,27 +340,56 @@ hipError_t launch_mns(const float *sinkpos, float *acc, int m, const uint32_t *i
     return hipGetLastError();
 }
 
+// Slices per sink for the queue kernels: at least ~2048 waves per call while
+// a slice keeps >= `minslice` sources of the longest list; W slices per workgroup.
+static int queue_split(int m, int64_t maxlen, int minslice) {
+    const int64_t blocks = (m + 63) / 64;
+    int split = 1;
+    while (split < 256 && blocks * split < 2048 && maxlen / (2 * split) >= minslice) split *= 2;
+    return split;
+}
+
 hipError_t launch_mnss(const float *sinkpos, float *acc, int m, const uint16_t *ss_index,
                        const uint32_t *ss_seg, int ss_len, const uint32_t *seg, int seg_len,
-                       const float *srcpos, const int *source_n, float mmass, float eps_inv,
-                       int *errflag, hipStream_t st) {
+                       const float *srcpos, const int *source_n, int64_t max_source_n,
+                       float mmass, float eps_inv, int *errflag, hipStream_t st) {
     if (m <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_mnss, dim3((m + 255) / 256), dim3(256), 0, st, sinkpos, acc, m, ss_index,
-                       ss_seg, ss_len, seg, seg_len, srcpos, source_n, mmass, eps_inv, errflag);
+    const int split = queue_split(m, max_source_n, 32);
+    const unsigned bx = (unsigned)((m + 63) / 64);
+#define MNSS(W)                                                                                   \
+    hipLaunchKernelGGL((k_mnss<W>), dim3(bx, split / (W)), dim3(64 * (W)), 0, st, sinkpos, acc, m, \
+                       ss_index, ss_seg, ss_len, seg, seg_len, srcpos, source_n, mmass, eps_inv,  \
+                       errflag, split)
+    if (split >= 4)
+        MNSS(4);
+    else if (split == 2)
+        MNSS(2);
+    else
+        MNSS(1);
+#undef MNSS
     return hipGetLastError();
 }
 
 hipError_t launch_qnss(int qsign, const float *sinkpos, float *acc, int m, const uint32_t *seg,
-                       const float4 *pos4, const float *mom, hipStream_t st) {
+                       int64_t max_len, const float4 *pos4, const float *mom, hipStream_t st) {
     if (m <= 0)
         return hipSuccess;
-    if (qsign < 0)
-        hipLaunchKernelGGL(k_qnss<-1>, dim3((m + 63) / 64), dim3(64), 0, st, sinkpos, acc, m, seg,
-                           pos4, mom);
-    else
-        hipLaunchKernelGGL(k_qnss<1>, dim3((m + 63) / 64), dim3(64), 0, st, sinkpos, acc, m, seg,
-                           pos4, mom);
+    const int split = queue_split(m, max_len, 16);
+    const unsigned bx = (unsigned)((m + 63) / 64);
+#define QNSS(Q, W)                                                                                \
+    hipLaunchKernelGGL((k_qnss<Q, W>), dim3(bx, split / (W)), dim3(64 * (W)), 0, st, sinkpos, acc, \
+                       m, seg, pos4, mom, split)
+    if (qsign < 0) {
+        if (split >= 4) QNSS(-1, 4);
+        else if (split == 2) QNSS(-1, 2);
+        else QNSS(-1, 1);
+    } else {
+        if (split >= 4) QNSS(1, 4);
+        else if (split == 2) QNSS(1, 2);
+        else QNSS(1, 1);
+    }
+#undef QNSS
     return hipGetLastError();
 }
 

@@ -22,12 +22,13 @@ hipError_t launch_mn(int order, int soft, int qsign, const float *sinkpos, float
                      const float4 *pos4, const float *mom, int n, float e, hipStream_t st);
 hipError_t launch_mns(const float *sinkpos, float *acc, int m, const uint32_t *idx, int nidx,
                       const float *srcpos, float mmass, float eps_inv, hipStream_t st);
+// max_source_n / max_len: the longest source list of any sink (sets the split)
 hipError_t launch_mnss(const float *sinkpos, float *acc, int m, const uint16_t *ss_index,
                        const uint32_t *ss_seg, int ss_len, const uint32_t *seg, int seg_len,
-                       const float *srcpos, const int *source_n, float mmass, float eps_inv,
-                       int *errflag, hipStream_t st);
+                       const float *srcpos, const int *source_n, int64_t max_source_n,
+                       float mmass, float eps_inv, int *errflag, hipStream_t st);
 hipError_t launch_qnss(int qsign, const float *sinkpos, float *acc, int m, const uint32_t *seg,
-                       const float4 *pos4, const float *mom, hipStream_t st);
+                       int64_t max_len, const float4 *pos4, const float *mom, hipStream_t st);
 hipError_t launch_qns(int qsign, const float *sinkpos, float *acc, int m, const int *idx,
                       int source_n, const float4 *pos4, const float *mom, hipStream_t st);
 

@@ -8,6 +8,7 @@
                              [--mac tolerance --errtols 1e-2 1e-3] [--ewald-ratio 0.05]
     python studies.py c5 [--n 1048576]
     python studies.py dropin [--n 262144]
+    python studies.py dropin_sb [--ng 32]
     python studies.py mac [--n 16777216] [--errtols 1e-2 1e-3 1e-4] [--leaves 16 64]
 
 c2  the plain method on the GPU at scale: clustered bodies (pgrav.ics.clustered),
@@ -43,6 +44,10 @@ dropin  the reference's own GPU dispatch through the drop-in ABI (nln/gravcuda.h
     the end of WalkTerminateSinkCUDA with the lists already built, against the
     batched kernel on the same lists (K1 bodies, unsoftened cells, as
     pgrav.cu).
+dropin_sb  the subtract_background walk (pgrav_dl_walk, geometric centres, one
+    image layer) on an ng^3 lattice (rms displacement 0.3 spacings): its pH / pQ
+    calls, pQ1 flushes and pMM_mnss_sK1 flushes through the drop-in ABI, all and
+    each class alone, timed from WalkInitSinkCUDA to WalkTerminateSinkCUDA.
 mac  clustered bodies as c3, the error-tolerance acceptance (mac = 1, the
     reference's AREL_MAC radii taken lowest order first) against the opening
     angle: errtol = each of --errtols times the median |a| of the sampled bodies,
@@ -343,6 +348,72 @@ def dropin(args):
     print(json.dumps(row), flush=True)
 
 
+def dropin_sb(args):
+    """The subtract_background walk's GPU work through the drop-in ABI: its pH / pQ
+    calls, pQ1 flushes and pMM_mnss_sK1 flushes (the batched p-p queue), each class
+    timed on its own from the first call to the end of WalkTerminateSinkCUDA."""
+    import pgrav
+    from pgrav.dlwalk import DLWalk
+    pos, mass = ics.lattice_grf(args.ng, seed=4, rms=0.3)
+    n = pos.shape[0]
+    t0 = time.perf_counter()
+    W = DLWalk(pos, mass, dl=dict(subtract_background=1, geometric_center=1), order=4,
+               theta=args.theta, periodic="images", box=1.0, nimage=1)
+    st, sb = W.stats(), W.sb_stats()
+    _log("dropin_sb %d^3: walk %.1fs, %d pH + %d pQ calls, %d pQ1 flushes, %d pMM_mnss_sK1 "
+         "flushes (%.3e pairs)" % (args.ng, time.perf_counter() - t0, st["ph_calls"],
+                                   st["pq_calls"], st["pq1_flushes"], sb["mns_flushes"],
+                                   sb["mns_pairs"]))
+    body = W.bodies
+    btab = np.zeros((n, 14), np.float32)
+    btab[:, 0] = body[:, 0]
+    btab[:, 1:4] = body[:, 1:4]
+    flat = btab.reshape(-1)
+    src = W.sources
+    calls = [(("pH", "pQ")[r], int(f), int(m), np.ascontiguousarray(src[so:so + sn * sz]),
+              int(sn), int(sz)) for r, f, m, so, sn, sz in W.calls]
+    qfl = [(int(b), int(m), np.ascontiguousarray(W.qseg[s0:s0 + m]).reshape(-1),
+            np.ascontiguousarray(src[so:so + sl * 13]), int(sl))
+           for b, m, s0, so, sl, _ in W.qflush]
+    mfl = [(int(b), int(m), W.mss_index[i0:i0 + m], W.mss_seg[e0:e0 + sl].reshape(-1),
+            W.mseg[g0:g0 + sn].reshape(-1), W.mss_source_n[e0:e0 + sl])
+           for b, m, i0, e0, sl, g0, sn, _ in W.mflush]
+    gc = pgrav.GravCuda()
+
+    def slot():
+        q = -1
+        while q < 0:
+            q, _ = gc.qalloc()
+        return q
+
+    def run(kind):
+        btab[:, 7:11] = 0
+        t1 = time.perf_counter()
+        gc.walk_init_sink(flat, 14)
+        if kind in ("mn", "all"):
+            for r, f, m, buf, sn, sz in calls:
+                gc.grav_mn(r, f, m, buf, sn, sz, 0.0, slot())
+        if kind in ("qnss", "all"):
+            for b, m, seg, buf, sl in qfl:
+                gc.grav_qnss("pQ1_qnss", b, m, seg, buf, sl, slot())
+        if kind in ("mnss", "all"):
+            for b, m, idx, ssg, seg, srcn in mfl:
+                gc.grav_mnss("pMM_mnss_sK1", b, m, idx, ssg, seg, 0, srcn, float(body[0, 0]),
+                             1e5, slot())
+        gc.walk_terminate_sink(flat, 14)
+        return (time.perf_counter() - t1) * 1e3
+    row = {"config": "dropin_sb", "ng": args.ng, "n": int(n), "theta": args.theta,
+           "ph_calls": st["ph_calls"], "pq_calls": st["pq_calls"], "pq1_flushes": st["pq1_flushes"],
+           "mns_flushes": sb["mns_flushes"], "mns_pairs": sb["mns_pairs"],
+           "mxn_h_pairs": st["mxn_h_pairs"], "mxn_q_pairs": st["mxn_q_pairs"], "ms": {}}
+    for kind in ("all", "mn", "qnss", "mnss"):
+        best = min(run(kind) for _ in range(3))
+        row["ms"][kind] = best
+        _log("   %-5s %9.2f ms" % (kind, best))
+    row["mnss_pairs_per_s_wall"] = sb["mns_pairs"] / (row["ms"]["mnss"] * 1e-3)
+    print(json.dumps(row), flush=True)
+
+
 def mac(args):
     pos, mass = ics.clustered(args.n, seed=2)
     rng = np.random.default_rng(12345)
@@ -381,7 +452,7 @@ def mac(args):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", choices=["c2", "c3", "c4", "c5", "dropin", "mac"])
+    ap.add_argument("config", choices=["c2", "c3", "c4", "c5", "dropin", "dropin_sb", "mac"])
     ap.add_argument("--errtols", type=float, nargs="+", default=[1e-2, 1e-3, 1e-4])
     ap.add_argument("--leaves", type=int, nargs="+", default=[16, 64])
     ap.add_argument("--group-boxes", type=int, nargs="+", default=[0])
@@ -427,6 +498,10 @@ def main(argv=None):
     elif a.config == "dropin":
         a.n = a.n or 262144
         dropin(a)
+    elif a.config == "dropin_sb":
+        if a.ng == 256:
+            a.ng = 32  # the host walk emits every body's leaf lists: keep it small
+        dropin_sb(a)
     elif a.config == "mac":
         a.n = a.n or 16777216
         mac(a)

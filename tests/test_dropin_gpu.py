@@ -153,6 +153,54 @@ def test_grav_mnss_queue_layout(gpu_lib):
     assert np.all(got[~live] == 0)
 
 
+def test_queue_kernels_split_long_lists(gpu_lib):
+    """Few sinks with long source lists: the queue kernels cut every sink's list
+    over many waves (up to 128 slices here) and must still equal the oracle."""
+    pgrav = gpu_lib
+    rng = np.random.default_rng(11)
+    g = pgrav.GravCuda()
+    N = 6000
+    btab = _bodies(rng, N, scale=0.5)
+    # pMM_mnss_sK1: 70 sinks in two runs of ~5000 bodies, segments of 1..56 bodies
+    ss_seg, seg, source_n = [], [], []
+    for _ in range(2):
+        segs, tot = [], 0
+        while tot < 5000:
+            b, ln = int(rng.integers(0, N - 56)), int(rng.integers(1, 57))
+            segs.append((b, ln))
+            tot += ln
+        ss_seg.append((len(seg), len(segs)))
+        source_n.append(tot)
+        seg += segs
+    ss_index = [0] * 33 + [1] * 37
+    eps = 0.02
+    flat = btab.reshape(-1)
+    g.walk_init_sink(flat, STRIDE)
+    q, _ = g.qalloc()
+    g.grav_mnss("pMM_mnss_sK1", 500, 70, ss_index, np.array(ss_seg).reshape(-1),
+                np.array(seg).reshape(-1), 0, source_n, 0.0005, 1.0 / eps, q)
+    g.walk_terminate_sink(flat, STRIDE)
+    ref, bad = oracle.pmm1_mnss_sk1(btab[500:570, 1:4], ss_index, np.array(ss_seg).reshape(-1),
+                                    np.array(seg).reshape(-1), btab[:, 1:4], source_n, 0.0005,
+                                    1.0 / eps)
+    assert bad == 0
+    _close(btab[500:570, 7:11], ref)
+    # pQ1_qnss: 90 sinks over runs of 2000 and 1200 order-2 records
+    btab2 = _bodies(rng, 200)
+    recs = _cells(rng, 3200)
+    buf = oracle.to_soa(recs[:, :13], 8)
+    sg = np.zeros((90, 2), np.uint32)
+    sg[:50] = (0, 2000)
+    sg[50:] = (2000, 1200)
+    flat2 = btab2.reshape(-1)
+    g.walk_init_sink(flat2, STRIDE)
+    q, _ = g.qalloc()
+    g.grav_qnss("pQ1_qnss", 10, 90, sg.reshape(-1), buf, 3200, q)
+    g.walk_terminate_sink(flat2, STRIDE)
+    ref2 = oracle.pq1_mnss(btab2[10:100, 1:4], sg.reshape(-1), buf)
+    _close(btab2[10:100, 7:11], ref2)
+
+
 def test_grav_mnss_count_mismatch_is_reported(gpu_lib):
     pgrav = gpu_lib
     g = pgrav.GravCuda()
