@@ -19,6 +19,8 @@ namespace pgrav {
 
 namespace {
 
+constexpr int64_t INTEGRATE_MAX_BLOCKS = 8192;  // workgroups of the grid-stride step
+
 struct Table {
     const char *mass, *pos, *vel, *acc, *phi;
     char *opos, *ovel;
@@ -35,13 +37,15 @@ __device__ __forceinline__ float ld(const char *p, int64_t i, int64_t stride, in
     return ((const float *)(p + i * stride))[k];
 }
 
+// Grid-stride: a few thousand workgroups of 256 threads walk the table, each
+// thread summing its bodies' energies; one LDS reduction per workgroup.
 template <bool COSMO>
 __global__ __launch_bounds__(256) void k_integrate(Table T, int64_t n, Step S, double2 *partial) {
 #pragma clang fp contract(off)
     __shared__ double ske[256], spe[256];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double ke = 0.0, pe = 0.0;
-    if (i < n) {
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
         const float mf = ld(T.mass, i, T.in_stride, 0), phif = ld(T.phi, i, T.in_stride, 0);
         const double m = mf;
         double vc2 = 0.0;
@@ -53,10 +57,11 @@ __global__ __launch_bounds__(256) void k_integrate(Table T, int64_t n, Step S, d
             vc2 = vc2 + vc * vc;
             vout[k] = (float)(v + S.kick * a);
         }
-        ke = m * vc2;
+        double kei = m * vc2;
         if (COSMO)
-            ke = ke / S.ke_scale;  // / (a0 * a0)
-        pe = (double)(mf * phif);  // *inmass * *inphi: a float product
+            kei = kei / S.ke_scale;  // / (a0 * a0)
+        ke += kei;
+        pe += (double)(mf * phif);  // *inmass * *inphi: a float product
         float pout[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -145,7 +150,7 @@ int run(bool cosmo, const Table &T0, int64_t n, const Step &S, bool host, hipStr
     Table T = T0;
     char *din = nullptr, *dout = nullptr;
     double2 *partial = nullptr, *tot = nullptr;
-    const int64_t nb = (n + 255) / 256;
+    const int64_t nb = std::min<int64_t>((n + 255) / 256, INTEGRATE_MAX_BLOCKS);
     bool ok = true;
     if (scratch) {
         partial = scratch;

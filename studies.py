@@ -9,6 +9,7 @@
     python studies.py c5 [--n 1048576]
     python studies.py dropin [--n 262144]
     python studies.py dropin_sb [--ng 32]
+    python studies.py steps [--n 16777216]
     python studies.py mac [--n 16777216] [--errtols 1e-2 1e-3 1e-4] [--leaves 16 64]
 
 c2  the plain method on the GPU at scale: clustered bodies (pgrav.ics.clustered),
@@ -48,6 +49,8 @@ dropin_sb  the subtract_background walk (pgrav_dl_walk, geometric centres, one
     image layer) on an ng^3 lattice (rms displacement 0.3 spacings): its pH / pQ
     calls, pQ1 flushes and pMM_mnss_sK1 flushes through the drop-in ABI, all and
     each class alone, timed from WalkInitSinkCUDA to WalkTerminateSinkCUDA.
+steps  the moment update under a fixed tree and the kick-drift of the body
+    table at the benchmark's N, inputs on the device, 5 calls each.
 mac  clustered bodies as c3, the error-tolerance acceptance (mac = 1, the
     reference's AREL_MAC radii taken lowest order first) against the opening
     angle: errtol = each of --errtols times the median |a| of the sampled bodies,
@@ -414,6 +417,53 @@ def dropin_sb(args):
     print(json.dumps(row), flush=True)
 
 
+def steps(args):
+    """The kernels either side of the force evaluation at the benchmark's N: the
+    moment update under a fixed tree (pgrav_plan_update_device: body gather, one
+    moment launch per level, record preparation) and the kick-drift of the
+    reference's 56-byte body table (pgrav_integrate_device), each timed over 5
+    calls with the inputs already on the device."""
+    import torch
+    from pgrav import integrate as integ
+    pos, mass = ics.clustered(args.n, seed=2)
+    n = pos.shape[0]
+    dec = _decomposition(n, 0.6, "clustered")
+    plan = batch.Plan(pos, mass, theta=0.6, order=4, eps=1e-4, soft="plummer", fp64=False, **dec)
+    st = plan.stats()
+    rng = np.random.default_rng(7)
+    moved = pos + 1e-6 * rng.standard_normal(pos.shape)
+    dpos = torch.from_numpy(moved).cuda()
+    dmass = torch.from_numpy(np.ascontiguousarray(mass, np.float64)).cuda()
+    torch.cuda.synchronize()
+    plan.update_device(dpos.data_ptr(), dmass.data_ptr())
+    plan.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        plan.update_device(dpos.data_ptr(), dmass.data_ptr())
+    plan.synchronize()
+    upd = (time.perf_counter() - t0) / 5 * 1e3
+    _log("steps: update (gather, per-level moments, records) %.2f ms per call, %d nodes" %
+         (upd, st["nnodes"]))
+    plan.close()
+    tab = np.zeros(n, integ.BODY)
+    tab["mass"] = mass
+    tab["pos"] = pos
+    tab["vel"] = 1e-3 * rng.standard_normal((n, 3))
+    tab["acc"] = 1e-2 * rng.standard_normal((n, 3))
+    dtab = torch.from_numpy(tab.view(np.uint8)).cuda()
+    torch.cuda.synchronize()
+    integ.integrate_device(dtab.data_ptr(), n, integ.BODY, 1e-3, 1e-3, 0.0, 0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        integ.integrate_device(dtab.data_ptr(), n, integ.BODY, 1e-3, 1e-3, 0.0, 0.0)
+    torch.cuda.synchronize()
+    itg = (time.perf_counter() - t0) / 5 * 1e3
+    _log("steps: integrate %.3f ms per call (%d bodies x 56 B)" % (itg, n))
+    print(json.dumps({"config": "steps", "n": int(n), "nodes": st["nnodes"], "update_ms": upd,
+                      "integrate_ms": itg, "table_bytes": int(56 * n)}), flush=True)
+
+
 def mac(args):
     pos, mass = ics.clustered(args.n, seed=2)
     rng = np.random.default_rng(12345)
@@ -452,7 +502,8 @@ def mac(args):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", choices=["c2", "c3", "c4", "c5", "dropin", "dropin_sb", "mac"])
+    ap.add_argument("config", choices=["c2", "c3", "c4", "c5", "dropin", "dropin_sb", "mac",
+                                       "steps"])
     ap.add_argument("--errtols", type=float, nargs="+", default=[1e-2, 1e-3, 1e-4])
     ap.add_argument("--leaves", type=int, nargs="+", default=[16, 64])
     ap.add_argument("--group-boxes", type=int, nargs="+", default=[0])
@@ -505,6 +556,9 @@ def main(argv=None):
     elif a.config == "mac":
         a.n = a.n or 16777216
         mac(a)
+    elif a.config == "steps":
+        a.n = a.n or 16777216
+        steps(a)
     else:
         a.n = a.n or 1048576
         c5(a)
