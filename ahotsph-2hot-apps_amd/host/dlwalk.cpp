@@ -68,10 +68,9 @@ namespace {
 const int NSSE = 8;
 const int HSZ = 29, QSZ = 13, MSZ = 4;  // floats per record (nln/pgrav.cu HSZ / QSZ / MSZ)
 enum { MAC_ACCEPT = 0, MAC_SPLIT_SINK = 1, MAC_SPLIT_SRC = 2 };
-enum { APP_M = 0, APP_Q = 1, APP_H = 2 };
-// subtract_background appends (their codes in the call log)
-enum { APP_QB = 3, APP_HB = 4, APP_ANTI_BODY = 5, APP_ANTI_CELL = 6, APP_ANTI_EMPTY = 8,
-       APP_MM = 20 };
+// list appends, and (subtract_background) their variants: the codes of the call log
+enum { APP_M = 0, APP_Q = 1, APP_H = 2, APP_QB = 3, APP_HB = 4, APP_ANTI_BODY = 5,
+       APP_ANTI_CELL = 6, APP_ANTI_EMPTY = 8, APP_MM = 20 };
 const int CHUBITS = 42;               // (KEYBITS - 1) / NDIM of the two-word key: "no clevel"
 const int MNSQ_MAX = 32768 + 1024;    // nln/mac.c:157-158
 const int MNSQ_MAX_M = 131072;
@@ -228,6 +227,11 @@ struct DLWalk {
     std::vector<float> log_sb;       // 8 per MAC call
     std::vector<int32_t> log_flags;  // 2 per logged source
     int64_t sbstats[8] = {0};
+    // tail_spans: per list, the source offset of each stack block's latest
+    // snapshot on the current sink path (-1: none), the leaves' runs of them
+    std::vector<int64_t> blk_off[3];
+    std::vector<int64_t> tail_runs;   // 3 per run: list, source offset, records
+    std::vector<int64_t> tail_index;  // 2 per body slot: first run, runs
     // the mns queue (nln/mac.c:160-173)
     struct {
         bool inuse = false;
@@ -374,10 +378,11 @@ struct DLWalk {
         const float *o = &offset[3 * img];
         for (int a = 0; a < 3; a++) m[a] = corner[a] + (0.5f * edge + o[a]);
     }
+    std::vector<uint8_t> bs_in;  // mac_sb's incoming flags (for the log)
     void mac_sb(Sink &s, int32_t sink_ent, std::vector<Src> &sv, std::vector<int> &result) {
         const size_t app0 = log_app.size() / 3, src0 = log_src.size() / 3;
         result.assign(sv.size(), MAC_SPLIT_SINK);
-        std::vector<uint8_t> bs_in(sv.size());
+        bs_in.resize(sv.size());
         for (size_t i = 0; i < sv.size(); i++) bs_in[i] = sv[i].bs;
         stats[0]++;
         const int central = nimage >> 1;
@@ -683,6 +688,53 @@ struct DLWalk {
         }
         s.qcnt_done = n1 * NSSE;
     }
+    // tail_spans: copy stack blocks [c0 / NSSE, ceil(c1 / NSSE)) of list `kind`
+    // once, records past c1 zeroed (they belong to no sink on this path), and
+    // point the path's block table at the copies; `undo` restores it on return
+    void snapshot(int kind, SoA &st, int64_t c0, int64_t c1,
+                  std::vector<std::pair<int64_t, int64_t>> &undo) {
+        if (c1 <= c0)
+            return;
+        const int64_t b0 = c0 / NSSE, b1 = (c1 + NSSE - 1) / NSSE;
+        std::vector<int64_t> &bo = blk_off[kind];
+        if ((int64_t)bo.size() < b1)
+            bo.resize(std::max<int64_t>(b1, 2 * (int64_t)bo.size()), -1);
+        for (int64_t b = b0; b < b1; b++) {
+            undo.push_back({kind * ((int64_t)1 << 56) + b, bo[b]});
+            const int64_t off = (int64_t)src.size();
+            const float *p = st.block(b);
+            src.insert(src.end(), p, p + (size_t)st.sz * NSSE);
+            if (b == b1 - 1 && c1 % NSSE)
+                for (int f = 0; f < st.sz; f++)
+                    for (int r = (int)(c1 % NSSE); r < NSSE; r++) src[off + f * NSSE + r] = 0.0f;
+            bo[b] = off;
+        }
+    }
+    void undo_snapshots(const std::vector<std::pair<int64_t, int64_t>> &undo) {
+        for (size_t k = undo.size(); k-- > 0;) {
+            const int kind = (int)(undo[k].first >> 56);
+            blk_off[kind][undo[k].first & (((int64_t)1 << 56) - 1)] = undo[k].second;
+        }
+    }
+    // a leaf's remainder of list `kind`, records [c0, c1) padded to NSSE, as runs of
+    // snapshot blocks
+    void tail_from_snapshots(int kind, int sz, int64_t c0, int64_t c1) {
+        const int64_t b0 = c0 / NSSE, b1 = (c1 + NSSE - 1) / NSSE;
+        for (int64_t b = b0; b < b1; b++) {
+            const int64_t off = blk_off[kind][b];
+            if (off < 0)
+                throw std::runtime_error("leaf remainder block without a snapshot");
+            const size_t nr = tail_runs.size();
+            if (b > b0 && tail_runs[nr - 3] == kind &&
+                tail_runs[nr - 2] + tail_runs[nr - 1] / NSSE * sz * NSSE == off) {
+                tail_runs[nr - 1] += NSSE;
+            } else {
+                const int64_t v[3] = {kind, off, NSSE};
+                tail_runs.insert(tail_runs.end(), v, v + 3);
+            }
+        }
+    }
+
     // the leaf: nln/mac.c:506-700 -- each list's remainder, padded to NSSE
     void leaf(Sink &s, int64_t slot) {
         if (sb) {
@@ -709,17 +761,26 @@ struct DLWalk {
         }
         int64_t *t = &tails[6 * slot];
         const int64_t hn = (s.hcnt + NSSE - 1) / NSSE * NSSE;
-        H.zero(s.hcnt, hn);
-        t[0] = emit_blocks(H, s.hcnt_done / NSSE, hn / NSSE);
-        t[1] = hn - s.hcnt_done;
         const int64_t qn = (s.qcnt + NSSE - 1) / NSSE * NSSE;
-        Q.zero(s.qcnt, qn);
-        t[2] = emit_blocks(Q, s.qcnt_done / NSSE, qn / NSSE);
-        t[3] = qn - s.qcnt_done;
         const int64_t mn = (s.mcnt + NSSE - 1) / NSSE * NSSE;
-        M.zero(s.mcnt, mn);
-        t[4] = emit_blocks(M, 0, mn / NSSE);
+        t[1] = hn - s.hcnt_done;
+        t[3] = qn - s.qcnt_done;
         t[5] = mn;
+        if (dp.tail_spans) {
+            tail_index[2 * slot] = (int64_t)tail_runs.size() / 3;
+            tail_from_snapshots(0, HSZ, s.hcnt_done, s.hcnt);
+            tail_from_snapshots(1, QSZ, s.qcnt_done, s.qcnt);
+            tail_from_snapshots(2, MSZ, 0, s.mcnt);
+            tail_index[2 * slot + 1] = (int64_t)tail_runs.size() / 3 - tail_index[2 * slot];
+            t[0] = t[2] = t[4] = -1;
+        } else {
+            H.zero(s.hcnt, hn);
+            t[0] = emit_blocks(H, s.hcnt_done / NSSE, hn / NSSE);
+            Q.zero(s.qcnt, qn);
+            t[2] = emit_blocks(Q, s.qcnt_done / NSSE, qn / NSSE);
+            M.zero(s.mcnt, mn);
+            t[4] = emit_blocks(M, 0, mn / NSSE);
+        }
         stats[6] += t[1];
         stats[7] += t[3];
         stats[8] += t[5];
@@ -987,9 +1048,24 @@ struct DLWalk {
         }
         return to;
     }
-    void walk(Sink &s, int32_t ent, std::vector<Src> pending) {
-        std::vector<Src> pass;
+    // Per-depth buffers of the walk, reused from sink to sink (the source lists of
+    // a periodic walk run to thousands of entries per sink; fresh vectors for
+    // every sink spent most of the walk in the allocator and page faults).
+    struct Level {
+        std::vector<Src> pending, pass, opened;
         std::vector<int> res;
+        std::vector<std::pair<int64_t, int64_t>> undo;
+    };
+    std::vector<Level> lv;  // sized once in run(): a deeper walk never moves a caller's level
+    void walk(Sink &s, int32_t ent, const std::vector<Src> &in, size_t depth) {
+        if (depth >= lv.size())
+            throw std::runtime_error("the sink tree is deeper than the walk's level buffers");
+        Level &L = lv[depth];
+        std::vector<Src> &pending = L.pending, &pass = L.pass, &opened = L.opened;
+        std::vector<int> &res = L.res;
+        pending.assign(in.begin(), in.end());
+        pass.clear();
+        const int64_t h0 = s.hcnt, q0 = s.qcnt, m0 = s.mcnt;
         while (!pending.empty()) {
             if (sb) {
                 mac_sb(s, ent, pending, res);
@@ -997,7 +1073,7 @@ struct DLWalk {
             } else {
                 mac(s, ent, pending, res);
             }
-            std::vector<Src> opened;
+            opened.clear();
             for (size_t i = 0; i < pending.size(); i++) {
                 if (res[i] == MAC_SPLIT_SINK) {
                     pass.push_back(pending[i]);
@@ -1009,14 +1085,22 @@ struct DLWalk {
             }
             pending.swap(opened);
         }
+        std::vector<std::pair<int64_t, int64_t>> &undo = L.undo;
+        undo.clear();
+        if (dp.tail_spans) {
+            snapshot(0, H, h0, s.hcnt, undo);
+            snapshot(1, Q, q0, s.qcnt, undo);
+            snapshot(2, M, m0, s.mcnt, undo);
+        }
         if (s.isbody) {
             leaf(s, ent_body(ent));
-            return;
+        } else {
+            for (int64_t k = child_off[ent]; k < child_off[ent + 1]; k++) {
+                Sink d = inherit(s, children[k]);
+                walk(d, children[k], pass, depth + 1);
+            }
         }
-        for (int64_t k = child_off[ent]; k < child_off[ent + 1]; k++) {
-            Sink d = inherit(s, children[k]);
-            walk(d, children[k], pass);
-        }
+        undo_snapshots(undo);
     }
 
     void run() {
@@ -1117,13 +1201,17 @@ struct DLWalk {
         }
         tails.assign(6 * (size_t)n, 0);
         inter.assign(n, 0);
+        if (dp.tail_spans)
+            tail_index.assign(2 * (size_t)n, 0);
         // the root: one source per image (WalkInitSrc / WalkInitSrcPeriodic,
         // nln/mac.c:279-291)
         const int32_t root = tree.child[0] < 0 && tree.count[0] == 1 ? body_ent(0) : 0;
         std::vector<Src> src0;
         for (int i = 0; i < nimage; i++) src0.push_back({root, i, 0});
         Sink s = make_sink(root);
-        walk(s, root, src0);
+        // one level per tree level (at most 21 below the root) and the bodies
+        lv.assign(24, Level());
+        walk(s, root, src0, 0);
         qns_flush();
         if (sb)
             mns_flush();
@@ -1309,6 +1397,8 @@ int64_t pgrav_dl_array(void *h, int which, void *out) {
     case PGRAV_DL_SB_SINK: take(W->sbsink, 6); break;
     case PGRAV_DL_LOG_SB: take(W->log_sb, 8); break;
     case PGRAV_DL_LOG_FLAGS: take(W->log_flags, 2); break;
+    case PGRAV_DL_TAIL_RUNS: take(W->tail_runs, 3); break;
+    case PGRAV_DL_TAIL_INDEX: take(W->tail_index, 2); break;
     case PGRAV_DL_SB_STATS:
         p = W->sbstats;
         bytes = sizeof W->sbstats;

@@ -29,7 +29,7 @@ class DLParams(ctypes.Structure):
                 ("do_pQL", ctypes.c_int), ("do_pH", ctypes.c_int), ("log_calls", ctypes.c_int),
                 ("subtract_background", ctypes.c_int), ("do_pM", ctypes.c_int),
                 ("min_msink", ctypes.c_int), ("min_msrc", ctypes.c_int), ("nx", ctypes.c_float),
-                ("geometric_center", ctypes.c_int)]
+                ("geometric_center", ctypes.c_int), ("tail_spans", ctypes.c_int)]
 
 
 _vp, _i, _i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
@@ -55,6 +55,8 @@ ARRAYS = {
     "mss_source_n": (22, np.int32, 1), "mseg": (23, np.uint32, 2), "mcalls": (24, np.int64, 6),
     "mmtails": (25, np.int64, 3), "background": (26, np.float64, 3), "sb_sink": (27, np.float32, 6),
     "log_sb": (28, np.float32, 8), "log_flags": (29, np.int32, 2), "sb_stats": (30, np.int64, 1),
+    # tail_spans
+    "tail_runs": (31, np.int64, 3), "tail_index": (32, np.int64, 2),
 }
 SB_STATS = ("mns_flushes", "host_mono", "mm_segments", "anti_body", "anti_cell", "anti_empty",
             "mns_pairs")
@@ -125,6 +127,21 @@ class DLWalk:
     def block(self, off, nrec, sz):
         """nrec records of sz floats as SoA-8 blocks at source offset off (flat)."""
         return self.sources[off:off + nrec * sz]
+
+    def leaf_runs(self, j):
+        """Body slot j's leaf remainders as {list: [(source offset, records)]} (list 0
+        Hvec, 1 Qvec, 2 Mvec), whichever way the walk emitted them."""
+        out = {0: [], 1: [], 2: []}
+        if self.dl.tail_spans:
+            first, cnt = (int(v) for v in self.tail_index[j])
+            for kind, off, nrec in self.tail_runs[first:first + cnt]:
+                out[int(kind)].append((int(off), int(nrec)))
+        else:
+            t = self.tails[j]
+            for k in range(3):
+                if t[2 * k + 1]:
+                    out[k].append((int(t[2 * k]), int(t[2 * k + 1])))
+        return out
 
     def sb_stats(self):
         return dict(zip(SB_STATS, (int(v) for v in self.sb_stats_raw())))

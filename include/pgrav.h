@@ -383,6 +383,11 @@ typedef struct {
     int geometric_center; /* geometric_center (nln/main.c:542): cells expanded about their
                          key cube's centre (dipole moments kept, nln/cofm.c:141-146) instead
                          of their centre of mass; the near cubes then align with the cells */
+    int tail_spans;   /* 0: each body's leaf remainders copied whole into SOURCES (TAILS);
+                         1: every sink's new stack blocks emitted once, each body's
+                         remainders given as runs of them (TAIL_RUNS / TAIL_INDEX; TAILS then
+                         holds -1 offsets and the record counts) -- memory O(appends)
+                         instead of O(N x list length) */
 } pgrav_dl_params;
 
 /* arrays of pgrav_dl_array (element type, values per element) */
@@ -436,6 +441,11 @@ typedef struct {
 #define PGRAV_DL_SB_STATS 30    /* int64 8: pMM_mnss_sK1 flushes, host mxn_mono runs, MMvec
                                    segments, body / cell / empty-octant anti-masses, queued
                                    sink-body pairs, 0 */
+/* tail_spans = 1 only */
+#define PGRAV_DL_TAIL_RUNS 31   /* int64 3 per run: list (0 Hvec, 1 Qvec, 2 Mvec), source
+                                   offset (floats into SOURCES), records (SoA-8 blocks x 8) */
+#define PGRAV_DL_TAIL_INDEX 32  /* int64 2 per sorted slot: first run, runs (H, then Q, then
+                                   M, each in stack order) */
 /* Appends in LOG_APPENDS with subtract_background: 1 / 2 Q / H with subtracted moments,
  * 3 / 4 Q / H of a background-flagged cell (Q mass + the cube's, H the unsubtracted
  * umass / uxxxx / uyyyy / uxxyy), 5 a body's anti-mass (-rho0 edge^3 of its leaf cube),

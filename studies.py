@@ -360,8 +360,8 @@ def dropin_sb(args):
     pos, mass = ics.lattice_grf(args.ng, seed=4, rms=0.3)
     n = pos.shape[0]
     t0 = time.perf_counter()
-    W = DLWalk(pos, mass, dl=dict(subtract_background=1, geometric_center=1), order=4,
-               theta=args.theta, periodic="images", box=1.0, nimage=1)
+    W = DLWalk(pos, mass, dl=dict(subtract_background=1, geometric_center=1, tail_spans=1),
+               order=4, theta=args.theta, periodic="images", box=1.0, nimage=1)
     st, sb = W.stats(), W.sb_stats()
     _log("dropin_sb %d^3: walk %.1fs, %d pH + %d pQ calls, %d pQ1 flushes, %d pMM_mnss_sK1 "
          "flushes (%.3e pairs)" % (args.ng, time.perf_counter() - t0, st["ph_calls"],

@@ -226,3 +226,40 @@ def test_dl_walk_emitted_records():
         exp = np.array([dlmac.record(k, ent, img, cells, recs, bodies, off) for k, ent, img in want])
         np.testing.assert_array_equal(got[:len(want)], exp)
         assert not got[len(want):].any()
+
+
+@pytest.mark.parametrize("sb", [0, 1])
+def test_tail_spans_equal_copied_tails(sb):
+    """tail_spans = 1 emits every sink's new stack blocks once and gives each body's
+    leaf remainders as runs of them: per body and list, the records the runs hold
+    are exactly the copied remainders of tail_spans = 0, and everything else the
+    walk emits is unchanged."""
+    from pgrav import ics
+    from pgrav.dlwalk import DLWalk
+    if sb:
+        pos, mass = ics.lattice_grf(10, seed=4, rms=0.3)
+        kw = dict(periodic="images", box=1.0, nimage=1)
+        dl = dict(subtract_background=1, geometric_center=1, **MXN)
+    else:
+        pos, mass = ics.plummer(2048, seed=5)
+        kw, dl = {}, dict(MXN)
+    A = DLWalk(pos, mass, dl=dl, order=4, theta=0.6, **kw)
+    B = DLWalk(pos, mass, dl=dict(dl, tail_spans=1), order=4, theta=0.6, **kw)
+    np.testing.assert_array_equal(A.calls[:, :3], B.calls[:, :3])
+    np.testing.assert_array_equal(A.interactions, B.interactions)
+    np.testing.assert_array_equal(A.tails[:, 1::2], B.tails[:, 1::2])
+    assert (B.tails[:, 0::2] == -1).all()
+    sa, sbf = A.sources, B.sources
+    for k in range(A.calls.shape[0]):
+        _, _, _, oa, n, sz = A.calls[k]
+        ob = B.calls[k, 3]
+        np.testing.assert_array_equal(sa[oa:oa + n * sz], sbf[ob:ob + n * sz])
+    sizes = (29, 13, 4)
+    for j in range(A.bodies.shape[0]):
+        ra, rb = A.leaf_runs(j), B.leaf_runs(j)
+        for kind in range(3):
+            sz = sizes[kind]
+            want = np.concatenate([sa[o:o + n * sz] for o, n in ra[kind]] or [np.zeros(0, np.float32)])
+            got = np.concatenate([sbf[o:o + n * sz] for o, n in rb[kind]] or [np.zeros(0, np.float32)])
+            np.testing.assert_array_equal(got, want, err_msg="body %d list %d" % (j, kind))
+    assert B.stats()["source_floats"] < A.stats()["source_floats"]
