@@ -319,6 +319,29 @@ int qallocCUDA(int *inuse) {
     std::lock_guard<std::mutex> lk(g_mutex);
     if (!ensure_init())
         return -1;
+    if (!inuse) {
+        // nln/mac.c's callers pass NULL: hand out slots round robin and stop at the
+        // first idle one -- the slot used NQ calls ago has usually finished, so one
+        // stream query instead of one per busy slot
+        static int next = 0;
+        for (int k = 0; k < NQ; k++) {
+            const int i = (next + k) % NQ;
+            Slot &s = S.slot[i];
+            if (s.inuse) {
+                const hipError_t e = hipStreamQuery(s.stream);
+                if (e == hipErrorNotReady)
+                    continue;
+                if (e != hipSuccess) {
+                    check(e, "hipStreamQuery");
+                    return -1;
+                }
+                s.inuse = false;
+            }
+            next = (i + 1) % NQ;
+            return i;
+        }
+        return -1;
+    }
     int busy = 0;
     for (int i = 0; i < NQ; i++) {
         Slot &s = S.slot[i];

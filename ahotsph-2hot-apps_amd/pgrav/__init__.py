@@ -90,7 +90,10 @@ class GravCuda:
     def vecwidth(self):
         return self._call("vecwidthCUDA")
 
-    def qalloc(self):
+    def qalloc(self, count=True):
+        """(slot, busy slots); count=False passes NULL as nln/mac.c does (busy: -1)."""
+        if not count:
+            return self._call("qallocCUDA", None), -1
         inuse = ctypes.c_int(0)
         q = self._call("qallocCUDA", ctypes.cast(ctypes.byref(inuse), ctypes.c_void_p))
         return q, inuse.value

@@ -42,7 +42,9 @@ void CUDA_Finalize(void);
 int vecwidthCUDA(void);
 /* nln/gravcuda.h:17 (impl nln/pgrav.cu:1024, 814-856): returns a free slot
  * index in [0,32) or -1 when all 32 slots still have work in flight; *inuse
- * (may be NULL) receives the number of busy slots. */
+ * (may be NULL) receives the number of busy slots.  With inuse NULL (what
+ * nln/mac.c passes) slots are handed out round robin and polling stops at the
+ * first idle one. */
 int qallocCUDA(int *inuse);
 /* nln/gravcuda.h:18 (impl nln/pgrav.cu:878-906): uploads the cell table for
  * grav_qns_CUDA.  qtab points at nobj hexacell records (nln/physics.h:140-157)

@@ -322,7 +322,7 @@ def dropin(args):
         for first, cnt, soa, source_n, seg, nb in work:
             q = -1
             while q < 0:
-                q, _ = gc.qalloc()
+                q, _ = gc.qalloc(count=False)
             if source_n:
                 gc.grav_mn("pH", first, cnt, soa, source_n, 29, 0.0, q)
             if nb:
@@ -386,7 +386,7 @@ def dropin_sb(args):
     def slot():
         q = -1
         while q < 0:
-            q, _ = gc.qalloc()
+            q, _ = gc.qalloc(count=False)
         return q
 
     def run(kind):
