@@ -19,6 +19,13 @@
 // read-modify-write -- a few tenths of a millisecond at N = 16.8 M.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <stdexcept>
+#include <vector>
+
+#include "../../include/pgrav.h"
+#include "pgrav_host.h"
+#include "pgrav_internal.h"
 #include "pgrav_le.h"
 #include "pgrav_le_launch.h"
 
@@ -26,9 +33,22 @@ namespace pgrav {
 
 namespace {
 
+// The bodies as the kernels read them: sorted double4 (x, y, z, m) of a plan, or
+// the caller's separate FP64 positions (n x 3) and masses.
+struct Bodies4 {
+    const double4 *__restrict__ b;
+    __device__ double4 operator()(int64_t i) const { return b[i]; }
+};
+struct Bodies3 {
+    const double *__restrict__ p, *__restrict__ m;
+    __device__ double4 operator()(int64_t i) const {
+        return make_double4(p[3 * i], p[3 * i + 1], p[3 * i + 2], m[i]);
+    }
+};
+
 constexpr int LE_TILE = 128;
-__global__ __launch_bounds__(256) void k_le_partials(const double4 *__restrict__ bpos, int64_t n,
-                                                     double box, int64_t chunk,
+template <typename B>
+__global__ __launch_bounds__(256) void k_le_partials(B bpos, int64_t n, double box, int64_t chunk,
                                                      double *__restrict__ partial) {
     __shared__ double pm[LE_TILE][LE_DEG + 1], py[LE_TILE][LE_DEG + 1], pz[LE_TILE][LE_DEG + 1];
     const int t = threadIdx.x;
@@ -49,7 +69,7 @@ __global__ __launch_bounds__(256) void k_le_partials(const double4 *__restrict__
     for (int64_t s = b0; s < b1; s += LE_TILE) {
         const int m = (int)min((int64_t)LE_TILE, b1 - s);
         if (t < m) {
-            const double4 b = bpos[s + t];
+            const double4 b = bpos(s + t);
             const double x = (b.x - 0.5 * box) * ibox, y = (b.y - 0.5 * box) * ibox,
                          z = (b.z - 0.5 * box) * ibox;
             double px = b.w, qy = 1.0, qz = 1.0;
@@ -90,9 +110,11 @@ __global__ __launch_bounds__(256) void k_le_reduce(const double *__restrict__ pa
 }
 
 // C = W mu; Wt is W transposed (Wt[g][beta]) so that the threads read along a row
+// sb: coef[LE_NMON + a] = (4 pi / 3) times the dipole moment mu_{e_a}: the
+// acceleration-only term of the subtract_background form (nln/ewald_le2.c:371-375)
 __global__ __launch_bounds__(256) void k_le_coeffs(const double *__restrict__ mu,
                                                    const double *__restrict__ Wt,
-                                                   double *__restrict__ coef) {
+                                                   double *__restrict__ coef, int sb) {
     __shared__ double m[LE_NMON];
     const int t = threadIdx.x;
     if (t < LE_NMON)
@@ -103,12 +125,13 @@ __global__ __launch_bounds__(256) void k_le_coeffs(const double *__restrict__ mu
         for (int g = 0; g < LE_NMON; g++) s += Wt[(size_t)g * LE_NMON + t] * m[g];
         coef[t] = s;
     }
+    if (sb && t < 3)
+        coef[LE_NMON + t] = 4.0 * M_PI / 3.0 * m[le_index(t == 0, t == 1, t == 2)];
 }
 
-template <typename V>
-__global__ __launch_bounds__(256) void k_le_apply(const double4 *__restrict__ bpos, int64_t n,
-                                                  double box, double G,
-                                                  const double *__restrict__ coef,
+template <typename B, typename V>
+__global__ __launch_bounds__(256) void k_le_apply(B bpos, int64_t n, double box, double G,
+                                                  const double *__restrict__ coef, int sb,
                                                   V *__restrict__ out) {
     __shared__ double c[LE_NMON];
     for (int k = threadIdx.x; k < LE_NMON; k += blockDim.x) c[k] = coef[k];
@@ -116,7 +139,7 @@ __global__ __launch_bounds__(256) void k_le_apply(const double4 *__restrict__ bp
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    const double4 b = bpos[i];
+    const double4 b = bpos(i);
     const double ibox = 1.0 / box;
     double px[LE_DEG + 1], py[LE_DEG + 1], pz[LE_DEG + 1];
     px[0] = py[0] = pz[0] = 1.0;
@@ -140,13 +163,44 @@ __global__ __launch_bounds__(256) void k_le_apply(const double4 *__restrict__ bp
         if (cc)
             gz += c[k] * cc * px[a] * py[bb] * pz[cc - 1];
     }
+    if (sb) {
+        // the dipole term on the acceleration only; no self term (it belongs to
+        // ewald_background, which the subtract_background form does not call)
+        gx += coef[LE_NMON];
+        gy += coef[LE_NMON + 1];
+        gz += coef[LE_NMON + 2];
+    }
     const double sa = -G * ibox * ibox, sp = G * ibox;
     V r = out[i];
     r.x += sa * gx;
     r.y += sa * gy;
     r.z += sa * gz;
-    r.w += sp * (phi - LE_XI * b.w);
+    r.w += sp * (sb ? phi : phi - LE_XI * b.w);
     out[i] = r;
+}
+
+template <typename B>
+hipError_t launch_le(B bodies, int64_t n, double box, double G, const double *Wt, double *partial,
+                     int nblocks, double *coef, int sb, float4 *out32, double4 *out64,
+                     hipStream_t st) {
+    if (n <= 0)
+        return hipSuccess;
+    if (nblocks < 1 || nblocks > LE_MAX_BLOCKS || (out32 == nullptr) == (out64 == nullptr))
+        return hipErrorInvalidValue;
+    const int64_t chunk = (n + nblocks - 1) / nblocks;
+    double *mu = partial + (size_t)nblocks * LE_NMON;
+    hipLaunchKernelGGL(k_le_partials<B>, dim3(nblocks), dim3(256), 0, st, bodies, n, box, chunk,
+                       partial);
+    hipLaunchKernelGGL(k_le_reduce, dim3(LE_NMON), dim3(256), 0, st, partial, nblocks, mu);
+    hipLaunchKernelGGL(k_le_coeffs, dim3(1), dim3(256), 0, st, mu, Wt, coef, sb);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (out32)
+        hipLaunchKernelGGL((k_le_apply<B, float4>), grid, dim3(256), 0, st, bodies, n, box, G, coef,
+                           sb, out32);
+    else
+        hipLaunchKernelGGL((k_le_apply<B, double4>), grid, dim3(256), 0, st, bodies, n, box, G,
+                           coef, sb, out64);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -154,21 +208,78 @@ __global__ __launch_bounds__(256) void k_le_apply(const double4 *__restrict__ bp
 hipError_t launch_ewald_le(const double4 *bpos, int64_t n, double box, double G, const double *Wt,
                            double *partial, int nblocks, double *coef, float4 *out32,
                            double4 *out64, hipStream_t st) {
-    if (n <= 0)
-        return hipSuccess;
-    if (nblocks < 1 || nblocks > LE_MAX_BLOCKS || (out32 == nullptr) == (out64 == nullptr))
-        return hipErrorInvalidValue;
-    const int64_t chunk = (n + nblocks - 1) / nblocks;
-    double *mu = partial + (size_t)nblocks * LE_NMON;
-    hipLaunchKernelGGL(k_le_partials, dim3(nblocks), dim3(256), 0, st, bpos, n, box, chunk, partial);
-    hipLaunchKernelGGL(k_le_reduce, dim3(LE_NMON), dim3(256), 0, st, partial, nblocks, mu);
-    hipLaunchKernelGGL(k_le_coeffs, dim3(1), dim3(256), 0, st, mu, Wt, coef);
-    const dim3 grid((unsigned)((n + 255) / 256));
-    if (out32)
-        hipLaunchKernelGGL(k_le_apply<float4>, grid, dim3(256), 0, st, bpos, n, box, G, coef, out32);
-    else
-        hipLaunchKernelGGL(k_le_apply<double4>, grid, dim3(256), 0, st, bpos, n, box, G, coef, out64);
-    return hipGetLastError();
+    return launch_le(Bodies4{bpos}, n, box, G, Wt, partial, nblocks, coef, 0, out32, out64, st);
 }
 
 }  // namespace pgrav
+
+// ---- the far field for caller device arrays: FixCubeEwaldLE as a GPU pass
+namespace {
+
+constexpr int LE_CACHE_DEVICES = 64;
+std::mutex le_mu;
+// per device: W transposed for nimage 0..MAX_NIMAGE, plain and subtract_background
+double *le_wt[LE_CACHE_DEVICES][pgrav::MAX_NIMAGE + 1][2];
+
+const double *device_wt(int nimage, int sb) {
+    int dev = -1;
+    if (!pgrav::check(hipGetDevice(&dev), "hipGetDevice"))
+        return nullptr;
+    if (dev < 0 || dev >= LE_CACHE_DEVICES) {
+        pgrav::fail("pgrav_ewald_le_device: device index out of range");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(le_mu);
+    double *&w = le_wt[dev][nimage][sb];
+    if (!w) {
+        std::vector<double> W, Wt((size_t)LE_NMON * LE_NMON);
+        pgrav::le_matrix(nimage, W, sb != 0);
+        for (int b = 0; b < LE_NMON; b++)
+            for (int g = 0; g < LE_NMON; g++) Wt[(size_t)g * LE_NMON + b] = W[(size_t)b * LE_NMON + g];
+        double *d = nullptr;
+        if (!pgrav::check(hipMalloc(&d, sizeof(double) * Wt.size()), "hipMalloc") ||
+            !pgrav::check(hipMemcpy(d, Wt.data(), sizeof(double) * Wt.size(), hipMemcpyHostToDevice),
+                          "hipMemcpy"))
+            return nullptr;
+        w = d;  // kept until the process ends
+    }
+    return w;
+}
+
+}  // namespace
+
+extern "C" int pgrav_ewald_le_device(const double *pos3, const double *mass, int64_t n, double box,
+                                     int nimage, double G, int subtract_background, double *acc4,
+                                     void *stream) {
+    pgrav::clear_error();
+    if (!pos3 || !mass || !acc4 || n < 0 || !(box > 0) || nimage < 0 ||
+        nimage > pgrav::MAX_NIMAGE) {
+        pgrav::fail("pgrav_ewald_le_device: bad arguments");
+        return -1;
+    }
+    if (n == 0)
+        return 0;
+    try {
+        const int sb = subtract_background ? 1 : 0;
+        const double *Wt = device_wt(nimage, sb);
+        if (!Wt)
+            return -1;
+        hipStream_t st = (hipStream_t)stream;
+        const int nblocks = (int)std::min<int64_t>(2048, (n + 511) / 512);
+        // stream-ordered scratch: the workgroup partials, the moments and the coefficients
+        double *scr = nullptr;
+        const size_t words = (size_t)LE_NMON * (nblocks + 1) + LE_NMON + 3;
+        if (!pgrav::check(hipMallocAsync((void **)&scr, sizeof(double) * words, st), "hipMallocAsync"))
+            return -1;
+        double *partial = scr, *coef = scr + (size_t)LE_NMON * (nblocks + 1);
+        const hipError_t e = pgrav::launch_le(pgrav::Bodies3{pos3, mass}, n, box, G, Wt, partial,
+                                              nblocks, coef, sb, nullptr, (double4 *)acc4, st);
+        const bool ok = pgrav::check(e, "far-field kernels");
+        if (!pgrav::check(hipFreeAsync(scr, st), "hipFreeAsync") || !ok)
+            return -1;
+        return 0;
+    } catch (const std::exception &ex) {
+        pgrav::fail("pgrav_ewald_le_device: %s", ex.what());
+        return -1;
+    }
+}

@@ -99,6 +99,7 @@ register({
     "pgrav_host_lpt": (_i, [_vp, _vp]),
     "pgrav_host_ewald_le": (_i, [_vp, _vp, _i64, _d, _i, _d, _vp]),
     "pgrav_host_ewald_le_sb": (_i, [_vp, _vp, _i64, _d, _i, _d, _vp]),
+    "pgrav_ewald_le_device": (_i, [_vp, _vp, _i64, _d, _i, _d, _i, _vp, _vp]),
     "pgrav_host_le_lattice": (_i, [_i, _i, ctypes.POINTER(_d)]),
 })
 
@@ -457,6 +458,17 @@ class HostPlan:
         lib().pgrav_host_lpt(self.h, o.ctypes.data)
         check()
         return o
+
+
+def ewald_le_device(pos_ptr, mass_ptr, n, acc_ptr, box=1.0, nimage=2, G=1.0,
+                    subtract_background=False, stream=None):
+    """The far field on the GPU for device arrays (FP64: n x 3 positions, n masses);
+    adds into acc_ptr (n x 4 FP64), queued on `stream` (hipStream_t or None)."""
+    rc = lib().pgrav_ewald_le_device(pos_ptr, mass_ptr, int(n), float(box), int(nimage), float(G),
+                                     int(bool(subtract_background)), acc_ptr, stream)
+    check()
+    if rc:
+        raise PgravError("pgrav_ewald_le_device failed")
 
 
 def ewald_le_host(pos, mass, box=1.0, nimage=2, G=1.0, subtract_background=False):

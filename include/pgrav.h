@@ -326,6 +326,16 @@ int pgrav_host_ewald_le(const double *pos3, const double *mass, int64_t n, doubl
  * self terms.  For the lists of pgrav_dl_walk with subtract_background = 1. */
 int pgrav_host_ewald_le_sb(const double *pos3, const double *mass, int64_t n, double box,
                            int nimage, double G, double *acc4);
+/* Both forms on the GPU for caller device arrays (no reference counterpart: the
+ * reference runs FixCubeEwaldLE on the host): pos3 (n x 3), mass (n) and acc4 (n x 4)
+ * are FP64 device pointers, bodies in [0, box)^3; acc4 += the far field times G
+ * (the same terms as pgrav_host_ewald_le, or pgrav_host_ewald_le_sb with
+ * subtract_background = 1), queued on `stream` (a hipStream_t; NULL: the null
+ * stream).  Scratch is stream-ordered (hipMallocAsync), the lattice matrix is
+ * built once per device and form. */
+int pgrav_ewald_le_device(const double *pos3, const double *mass, int64_t n, double box,
+                          int nimage, double G, int subtract_background, double *acc4,
+                          void *stream);
 int pgrav_host_le_lattice(int k, int nimage, double *ck);
 int pgrav_host_lpt(void *h, int32_t *order);
 
