@@ -406,20 +406,25 @@ __device__ __forceinline__ void eval_group(const BatchArgs &A, int g, V4<T> *lds
 // periodic kernels' tile loader evaluates the Ewald table (20 values, a Taylor
 // step, the symmetry) beside the pair loop's live state.
 #ifndef PGRAV_MINWAVES_FP32
-#define PGRAV_MINWAVES_FP32 1  // the FP32 kernels without the Ewald table
+#define PGRAV_MINWAVES_FP32 8  // the FP32 kernels without the Ewald table, unroll <= 4: left
+                               // alone they take 106 SGPRs (the scalar-loaded records) and 7
+                               // waves; held to 8 (78 SGPRs, a few spilled to VGPR lanes, no
+                               // scratch) the clustered 16.8 M launch runs 1.5 % faster.  Unroll
+                               // 8 would spill VGPRs to scratch at 8 waves and keeps the default.
 #endif
 #ifndef PGRAV_MINWAVES_PER1
 #define PGRAV_MINWAVES_PER1 7  // periodic 1: 72 VGPRs, 3 spilled once per group (38.7 ms at
                                // the 256^3 lattice against 38.9 with 6 waves and 78 VGPRs)
 #endif
-template <typename T, int PER>
+template <typename T, int PER, int BU>
 struct MinWaves {
-    static constexpr int value = sizeof(T) == 4 ? (PER == 1 ? PGRAV_MINWAVES_PER1 : PGRAV_MINWAVES_FP32) : 1;
+    static constexpr int value =
+        sizeof(T) == 4 ? (PER == 1 ? PGRAV_MINWAVES_PER1 : (BU >= 8 ? 1 : PGRAV_MINWAVES_FP32)) : 1;
 };
 
 // one workgroup per group, groups taken in longest-first order
 template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU, int TILE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PER>::value)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PER, BU>::value)))
 void k_batch(BatchArgs A) {
     __shared__ V4<T> lds[TILE];
     __shared__ EwShared<T, PER> ew;
@@ -441,7 +446,7 @@ void k_batch(BatchArgs A) {
 // lists and records, and only pointers known to be distinct let the compiler
 // keep those reads on wave-uniform scalar loads.
 template <typename T, bool REL, bool COMP, int SOFT, int PER, int BU, int TILE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PER>::value)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MinWaves<T, PER, BU>::value)))
 void k_batch_persistent(
     BatchArgs A, const double4 *__restrict__ bpos, const double4 *__restrict__ cpos,
     const T *__restrict__ mom, const int64_t *__restrict__ gfirst, const int *__restrict__ gcount,
