@@ -13,6 +13,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -126,6 +127,35 @@ struct Slot {
     DevBuf mom;   // prepared source records
 };
 
+// The per-sink-group entry points (grav_mn_CUDA, pinteractCUDA, grav_mns_CUDA,
+// grav_qns_CUDA) append their lists to a pinned batch instead of launching: the
+// reference's one copy and one launch per call cost ~20 us of host API time each,
+// more than most of these calls' kernels take.  A batch is uploaded in one copy
+// and evaluated by one prep and one evaluation kernel per kind once its lists
+// reach BATCH_BYTES, and before anything reads the results or replaces a table
+// the calls use (WalkTerminateSinkCUDA, WalkInitSinkCUDA, WalkInitSrcCUDA,
+// WalkTerminateSrcCUDA, pgrav_synchronize, pgrav_set_quad_phi_sign,
+// CUDA_Finalize).  Two batches alternate, so the host fills one while the
+// other's copy runs.  The queue entry points (grav_mnss_CUDA, grav_qnss_CUDA)
+// carry thousands of sinks per call and keep their own launches.
+enum { BK_M, BK_MK1, BK_Q, BK_H, BK_MNS, BK_QNS, BK_KINDS };
+static const size_t BATCH_BYTES = (size_t)16 << 20;
+static const int CHUNK[BK_KINDS] = {512, 512, 256, 128, 256, 256};  // sources per task
+
+struct Batch {
+    HostBuf host;
+    DevBuf dev;
+    hipEvent_t copied = nullptr;  // recorded after the copy out of `host`
+    bool pending = false;
+    size_t used = 0;         // bytes of list data at the front of `host`
+    size_t table_bytes = 0;  // bytes the call, task and block tables will add
+    int ncalls = 0;
+    std::vector<DeferredCall> calls[BK_KINDS];
+    std::vector<DeferredTask> tasks[BK_KINDS];
+    std::vector<int> blocks[BK_KINDS];  // M x N kinds: the call of each prep block
+    int prep[BK_KINDS] = {};            // M x N kinds: prepared records so far
+};
+
 struct State {
     bool init = false;
     int device = 0;
@@ -139,6 +169,13 @@ struct State {
     // WalkInitSrcCUDA table (prepared order-2 records)
     DevBuf qraw, qpos4, qmom;
     int64_t qcells = 0;
+    // deferred calls: two batches filled in turn, their stream, the prepared
+    // records of the batch being evaluated
+    bool defer = true;
+    Batch batch[2];
+    int bcur = 0;
+    hipStream_t bstream = nullptr;
+    DevBuf bpos4, bmom;
 };
 
 static State S;
@@ -162,6 +199,11 @@ static bool ensure_init() {
                    "hipStreamCreate") ||
             !check(hipEventCreateWithFlags(&S.slot[i].staged, hipEventDisableTiming),
                    "hipEventCreate"))
+            return false;
+    if (!check(hipStreamCreateWithFlags(&S.bstream, hipStreamNonBlocking), "hipStreamCreate"))
+        return false;
+    for (Batch &b : S.batch)
+        if (!check(hipEventCreateWithFlags(&b.copied, hipEventDisableTiming), "hipEventCreate"))
             return false;
     if (!S.errflag.ensure(sizeof(int)) || !check(hipMemset(S.errflag.p, 0, sizeof(int)), "hipMemset"))
         return false;
@@ -227,7 +269,12 @@ static bool sinks_ready(int64_t first, int64_t m) {
     return true;
 }
 
+static bool flush_batch();
+
+// Runs the deferred calls and waits for every queued evaluation.
 static void wait_all() {
+    if (S.init && (!flush_batch() || !check(hipStreamSynchronize(S.bstream), "hipStreamSynchronize")))
+        return;
     for (int i = 0; i < NQ; i++) {
         Slot &s = S.slot[i];
         if (s.inuse) {
@@ -249,6 +296,125 @@ static void release_slots() {
 
 static size_t align128(size_t x) { return (x + 127) & ~(size_t)127; }
 
+// Uploads the current batch and queues its kernels on the batch stream, then
+// switches to the other batch.
+static bool flush_batch() {
+    Batch &b = S.batch[S.bcur];
+    if (!b.ncalls)
+        return true;
+    // the tables go behind the lists, into the same copy
+    char *h = (char *)b.host.p;
+    size_t off = align128(b.used);
+    size_t o_calls[BK_KINDS], o_tasks[BK_KINDS], o_blocks[BK_KINDS];
+    for (int k = 0; k < BK_KINDS; k++) {
+        o_calls[k] = off;
+        memcpy(h + off, b.calls[k].data(), b.calls[k].size() * sizeof(DeferredCall));
+        off = align128(off + b.calls[k].size() * sizeof(DeferredCall));
+        o_tasks[k] = off;
+        memcpy(h + off, b.tasks[k].data(), b.tasks[k].size() * sizeof(DeferredTask));
+        off = align128(off + b.tasks[k].size() * sizeof(DeferredTask));
+        o_blocks[k] = off;
+        memcpy(h + off, b.blocks[k].data(), b.blocks[k].size() * sizeof(int));
+        off = align128(off + b.blocks[k].size() * sizeof(int));
+    }
+    if (off > b.host.cap) {
+        fail("deferred batch overran its staging buffer (%zu > %zu bytes)", off, b.host.cap);
+        return false;
+    }
+    // a buffer still read by queued kernels is replaced only after they ran
+    const size_t npos = (size_t)b.prep[BK_M] + b.prep[BK_MK1] + b.prep[BK_Q] + b.prep[BK_H];
+    const size_t nmom = (size_t)b.prep[BK_Q] * PREC2 + (size_t)b.prep[BK_H] * PREC4;
+    if (off > b.dev.cap || 16 * npos > S.bpos4.cap || 4 * nmom > S.bmom.cap) {
+        if (!check(hipStreamSynchronize(S.bstream), "hipStreamSynchronize"))
+            return false;
+        if (!b.dev.ensure(off) || !S.bpos4.ensure(16 * npos + 16) ||
+            !S.bmom.ensure(4 * nmom + 16))
+            return false;
+    }
+    if (!check(hipMemcpyAsync(b.dev.p, b.host.p, off, hipMemcpyHostToDevice, S.bstream),
+               "hipMemcpyAsync") ||
+        !check(hipEventRecord(b.copied, S.bstream), "hipEventRecord"))
+        return false;
+    b.pending = true;
+    const char *d = (const char *)b.dev.p;
+    const float *pos = (const float *)S.devpos.p;
+    float *acc = (float *)S.devacc.p;
+    const int qs = quad_phi_sign();
+    size_t p4 = 0, pm = 0;
+    static const int order_of[4] = {0, 0, 2, 4};
+    bool ok = true;
+    for (int k = BK_M; k <= BK_H && ok; k++) {
+        ok = check(launch_mn_batch(order_of[k], k == BK_MK1 ? SOFT_K1 : SOFT_NONE, qs,
+                                   (const float *)d, (const DeferredCall *)(d + o_calls[k]),
+                                   (const int *)(d + o_blocks[k]), (int)b.blocks[k].size(),
+                                   (const DeferredTask *)(d + o_tasks[k]), (int)b.tasks[k].size(),
+                                   (float4 *)S.bpos4.p + p4, (float *)S.bmom.p + pm, pos, acc,
+                                   S.bstream),
+                   "deferred M x N kernels");
+        p4 += b.prep[k];
+        pm += (size_t)b.prep[k] * (k == BK_H ? PREC4 : (k == BK_Q ? PREC2 : 0));
+    }
+    ok = ok &&
+         check(launch_mns_batch((const uint32_t *)d, (const DeferredCall *)(d + o_calls[BK_MNS]),
+                                (const DeferredTask *)(d + o_tasks[BK_MNS]),
+                                (int)b.tasks[BK_MNS].size(), pos, acc, S.bstream),
+               "deferred pMM_sK1 kernel") &&
+         check(launch_qns_batch(qs, (const int *)d, (const DeferredCall *)(d + o_calls[BK_QNS]),
+                                (const DeferredTask *)(d + o_tasks[BK_QNS]),
+                                (int)b.tasks[BK_QNS].size(), (const float4 *)S.qpos4.p,
+                                (const float *)S.qmom.p, pos, acc, S.bstream),
+               "deferred pQQ1 kernel");
+    for (int k = 0; k < BK_KINDS; k++) {
+        b.calls[k].clear();
+        b.tasks[k].clear();
+        b.blocks[k].clear();
+        b.prep[k] = 0;
+    }
+    b.used = 0;
+    b.table_bytes = 0;
+    b.ncalls = 0;
+    S.bcur ^= 1;
+    return ok;
+}
+
+// Room for `data` more bytes of lists and `tables` more bytes of tables in the
+// current batch; flushes it first when it is full.  Returns the batch, or null.
+static Batch *batch_room(size_t data, size_t tables) {
+    for (int pass = 0; pass < 2; pass++) {
+        Batch &b = S.batch[S.bcur];
+        if (b.pending) {  // the copy out of this buffer from its last flush
+            if (!check(hipEventSynchronize(b.copied), "hipEventSynchronize"))
+                return nullptr;
+            b.pending = false;
+        }
+        // lists, tables and the 128-B alignment of 3 tables per kind
+        const size_t need = align128(b.used + data) + b.table_bytes + tables + 3 * 128 * BK_KINDS;
+        if (b.ncalls && (b.used + data > BATCH_BYTES || need > b.host.cap)) {
+            if (!flush_batch())
+                return nullptr;
+            continue;
+        }
+        if (!b.host.ensure(std::max(need, BATCH_BYTES + ((size_t)1 << 20))))
+            return nullptr;
+        return &b;
+    }
+    return nullptr;
+}
+
+// Appends one call's tasks (64 sinks x CHUNK[kind] sources each) and table bytes.
+static void batch_tasks(Batch &b, int kind, int m, int n) {
+    const int call = (int)b.calls[kind].size() - 1;
+    const int ch = CHUNK[kind];
+    for (int s0 = 0; s0 < m; s0 += 64)
+        for (int i0 = 0; i0 < n; i0 += ch) b.tasks[kind].push_back(make_int4(call, s0, i0, std::min(n, i0 + ch)));
+    b.ncalls++;
+}
+
+static size_t task_bytes(int kind, int m, int n) {
+    return sizeof(DeferredCall) + sizeof(DeferredTask) * (size_t)((m + 63) / 64) *
+                                      (size_t)((n + CHUNK[kind] - 1) / CHUNK[kind]);
+}
+
 }  // namespace pgrav
 
 using namespace pgrav;
@@ -263,11 +429,25 @@ int pgrav_set_error_mode(int mode) {
 }
 const char *pgrav_last_error(void) { return g_failed ? g_error.c_str() : nullptr; }
 void pgrav_clear_error(void) { clear_error(); }
-void pgrav_set_quad_phi_sign(int sign) { g_qsign = sign < 0 ? -1 : 1; }
+void pgrav_set_quad_phi_sign(int sign) {
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if (S.init)
+        flush_batch();  // calls already made keep the sign they were made with
+    g_qsign = sign < 0 ? -1 : 1;
+}
+
+int pgrav_set_dropin_deferral(int enable) {
+    std::lock_guard<std::mutex> lk(g_mutex);
+    const int old = S.defer ? 1 : 0;
+    if (S.init && S.defer && !enable)
+        flush_batch();
+    S.defer = enable != 0;
+    return old;
+}
 
 int pgrav_synchronize(void) {
     std::lock_guard<std::mutex> lk(g_mutex);
-    if (!ensure_init())
+    if (!ensure_init() || !flush_batch())
         return -1;
     return check(hipDeviceSynchronize(), "hipDeviceSynchronize") ? 0 : -1;
 }
@@ -298,6 +478,19 @@ void CUDA_Finalize(void) {
     S.qraw.release();
     S.qpos4.release();
     S.qmom.release();
+    for (Batch &b : S.batch) {
+        b.host.release();
+        b.dev.release();
+        if (b.copied)
+            (void)hipEventDestroy(b.copied);
+        b.copied = nullptr;
+        b.pending = false;
+    }
+    S.bpos4.release();
+    S.bmom.release();
+    if (S.bstream)
+        (void)hipStreamDestroy(S.bstream);
+    S.bstream = nullptr;
     for (int i = 0; i < NQ; i++) {
         if (S.slot[i].stream)
             (void)hipStreamDestroy(S.slot[i].stream);
@@ -492,8 +685,29 @@ static void mn_common(int order, int soft, const float *p, int m, int stride, co
         fail("source_n %d is not a multiple of NSSE = 8", source_n);
         return;
     }
-    Slot &s = S.slot[q];
     size_t bytes = (size_t)source_n * sz * sizeof(float);
+    if (S.defer) {
+        if (m <= 0 || source_n <= 0)
+            return;
+        const int kind = order == 0 ? (soft == SOFT_K1 ? BK_MK1 : BK_M) : (order == 2 ? BK_Q : BK_H);
+        const int nblk = (source_n + 127) / 128;
+        const size_t tb = task_bytes(kind, m, source_n) + sizeof(int) * (size_t)nblk;
+        Batch *b = batch_room(bytes + 64, tb);
+        if (!b)
+            return;
+        const size_t at = (b->used + 63) & ~(size_t)63;
+        memcpy((char *)b->host.p + at, f, bytes);  // the caller may reuse f on return
+        b->used = at + bytes;
+        DeferredCall c = {off, (int64_t)(at / 4), m, source_n, sz, b->prep[kind], e, 0.0f};
+        b->calls[kind].push_back(c);
+        const int call = (int)b->calls[kind].size() - 1;
+        for (int k = 0; k < nblk; k++) b->blocks[kind].push_back(call);
+        b->prep[kind] += 128 * nblk;
+        b->table_bytes += tb;
+        batch_tasks(*b, kind, m, source_n);
+        return;
+    }
+    Slot &s = S.slot[q];
     const int mstride = order == 4 ? PREC4 : PREC2;
     if (!stage(s, bytes + 16, bytes + 16, 16 * (size_t)(source_n + 1),
                4 * (size_t)mstride * (source_n + 1)))
@@ -574,8 +788,30 @@ void grav_mns_CUDA(const char *routine, const int base, const int m, const segme
     if (total != source_n)
         S.seg_mismatch++;  // assert(sum_n == source_n), nln/pgrav.cu:636; reported at terminate
     // the segments expanded into body indices: one coalesced load per 64 bodies
-    Slot &s = S.slot[q];
     size_t bytes = (size_t)total * sizeof(uint32_t);
+    if (S.defer) {
+        if (m <= 0 || total <= 0)
+            return;
+        if (total > INT32_MAX) {
+            fail("pMM_sK1 list of %lld bodies", (long long)total);
+            return;
+        }
+        const size_t tb = task_bytes(BK_MNS, m, (int)total);
+        Batch *b = batch_room(bytes + 64, tb);
+        if (!b)
+            return;
+        const size_t at = (b->used + 63) & ~(size_t)63;
+        uint32_t *h = (uint32_t *)((char *)b->host.p + at);
+        for (int g = 0; g < seg_n; g++)
+            for (uint32_t j = 0; j < seg[g].length; j++) *h++ = seg[g].base + j;
+        b->used = at + bytes;
+        DeferredCall c = {(int64_t)base, (int64_t)(at / 4), m, (int)total, 0, 0, e, mmass};
+        b->calls[BK_MNS].push_back(c);
+        b->table_bytes += tb;
+        batch_tasks(*b, BK_MNS, m, (int)total);
+        return;
+    }
+    Slot &s = S.slot[q];
     if (!stage(s, bytes + 16, bytes + 16))
         return;
     uint32_t *h = (uint32_t *)s.host.p;
@@ -655,8 +891,24 @@ void grav_qns_CUDA(const char *routine, const int base, const int m, const int *
             fail("source_list[%d] = %d outside the cell table", i, source_list[i]);
             return;
         }
-    Slot &s = S.slot[q];
     size_t bytes = (size_t)source_n * sizeof(int);
+    if (S.defer) {
+        if (m <= 0 || source_n <= 0)
+            return;
+        const size_t tb = task_bytes(BK_QNS, m, source_n);
+        Batch *b = batch_room(bytes + 64, tb);
+        if (!b)
+            return;
+        const size_t at = (b->used + 63) & ~(size_t)63;
+        memcpy((char *)b->host.p + at, source_list, bytes);
+        b->used = at + bytes;
+        DeferredCall c = {(int64_t)base, (int64_t)(at / 4), m, source_n, 0, 0, 0.0f, 0.0f};
+        b->calls[BK_QNS].push_back(c);
+        b->table_bytes += tb;
+        batch_tasks(*b, BK_QNS, m, source_n);
+        return;
+    }
+    Slot &s = S.slot[q];
     if (!stage(s, bytes + 16, bytes + 16))
         return;
     memcpy(s.host.p, source_list, bytes);

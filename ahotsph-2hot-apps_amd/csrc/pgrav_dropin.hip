@@ -26,14 +26,11 @@ __device__ __forceinline__ float soa8(const float *__restrict__ f, int i, int k,
 // in FP32, so no inf - inf can appear in the order-4 basis.
 // the order is a template parameter so that the record stays in registers
 template <int ORDER>
-__global__ __launch_bounds__(128) void k_prep_soa8(const float *__restrict__ f, int source_n,
-                                                   int sz, int dipole, float4 *__restrict__ pos4,
-                                                   float *__restrict__ mom) {
+__device__ __forceinline__ void prep_source(const float *__restrict__ f, int i, int sz,
+                                            int dipole, float4 *__restrict__ pos4,
+                                            float *__restrict__ mom) {
     constexpr int NF = ORDER == 0 ? 4 : (ORDER == 2 ? 13 : 29);
     constexpr int MSTRIDE = ORDER == 4 ? PREC4 : PREC2;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= source_n)
-        return;
     float q[29];
 #pragma unroll
     for (int k = 0; k < 29; k++) q[k] = k < NF ? soa8(f, i, k, sz) : 0.0f;
@@ -50,6 +47,15 @@ __global__ __launch_bounds__(128) void k_prep_soa8(const float *__restrict__ f, 
 #pragma unroll
         for (int k = 0; k < MSTRIDE; k++) mom[(size_t)i * MSTRIDE + k] = rec[k];
     }
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(128) void k_prep_soa8(const float *__restrict__ f, int source_n,
+                                                   int sz, int dipole, float4 *__restrict__ pos4,
+                                                   float *__restrict__ mom) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < source_n)
+        prep_source<ORDER>(f, i, sz, dipole, pos4, mom);
 }
 
 // AoS hexacell table (nln/physics.h:140-157, byte stride `stride`) -> prepared
@@ -77,22 +83,14 @@ __device__ __forceinline__ void flush_acc(float *__restrict__ acc, const Acc4 &a
     atomicAdd(acc + 3, a.w);
 }
 
-// pH / pQ / pM / pM_sK1 (nln/pgrav.cu:49-468): m sinks x n sources.
-// ORDER 0 with SOFT == SOFT_K1 is pM_sK1 (e = 1/eps).
+// Sources [i0, i1) of a prepared list against one sink.  Every lane reads the
+// same record, so the loads are scalar (wave-uniform addresses).
 template <int ORDER, int QSIGN, int SOFT>
-__global__ __launch_bounds__(64) void k_mn(const float *__restrict__ sinkpos,
-                                           float *__restrict__ acc, int m,
-                                           const float4 *__restrict__ pos4,
-                                           const float *__restrict__ mom, int n, int chunk,
-                                           float e) {
-    const int s = blockIdx.x * 64 + threadIdx.x;
-    const int i0 = blockIdx.y * chunk;
-    const int i1 = min(n, i0 + chunk);
-    const bool live = s < m;
-    const int sc = live ? s : 0;  // dead lanes compute on a valid sink, never store
-    const float px = sinkpos[3 * sc], py = sinkpos[3 * sc + 1], pz = sinkpos[3 * sc + 2];
+__device__ __forceinline__ void mn_range(float px, float py, float pz,
+                                         const float4 *__restrict__ pos4,
+                                         const float *__restrict__ mom, int i0, int i1, float e,
+                                         Acc4 &a) {
     const float eps2k1 = SOFT == SOFT_K1 ? 1.0f / (e * e) : 0.0f;
-    Acc4 a = {0.f, 0.f, 0.f, 0.f};
     for (int i = i0; i < i1; i++) {
         const float4 q = pos4[i];
         const float x = px - q.x, y = py - q.y, z = pz - q.z;
@@ -106,28 +104,36 @@ __global__ __launch_bounds__(64) void k_mn(const float *__restrict__ sinkpos,
             cell_interact<ORDER, true, QSIGN>(M, q.w, x, y, z, 0.0f, a);
         }
     }
+}
+
+// pH / pQ / pM / pM_sK1 (nln/pgrav.cu:49-468): m sinks x n sources.
+// ORDER 0 with SOFT == SOFT_K1 is pM_sK1 (e = 1/eps).
+template <int ORDER, int QSIGN, int SOFT>
+__global__ __launch_bounds__(64) void k_mn(const float *__restrict__ sinkpos,
+                                           float *__restrict__ acc, int m,
+                                           const float4 *__restrict__ pos4,
+                                           const float *__restrict__ mom, int n, int chunk,
+                                           float e) {
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    const int i0 = blockIdx.y * chunk;
+    const int i1 = min(n, i0 + chunk);
+    const bool live = s < m;
+    const int sc = live ? s : 0;  // dead lanes compute on a valid sink, never store
+    Acc4 a = {0.f, 0.f, 0.f, 0.f};
+    mn_range<ORDER, QSIGN, SOFT>(sinkpos[3 * sc], sinkpos[3 * sc + 1], sinkpos[3 * sc + 2], pos4,
+                                 mom, i0, i1, e, a);
     if (live)
         flush_acc(acc + 4 * s, a);
 }
 
-// pMM_sK1 (nln/pgrav.cu:575-637): every sink against the bodies of a
-// segment list into the sink table.  The host expands the segments into body
-// indices; grid.y splits that list so that a call runs on several waves, and
-// each wave stages its bodies through LDS 64 at a time (one coalesced load per
-// tile, however short the segments are).
-__global__ __launch_bounds__(64) void k_mns(const float *__restrict__ sinkpos,
-                                            float *__restrict__ acc, int m,
-                                            const uint32_t *__restrict__ idx, int nidx, int chunk,
-                                            const float *__restrict__ srcpos, float mmass,
-                                            float eps_inv) {
-    __shared__ float3 tile[64];
-    const int s = blockIdx.x * 64 + threadIdx.x;
-    const bool live = s < m;
-    const int sc = live ? s : 0;
-    const float px = sinkpos[3 * sc], py = sinkpos[3 * sc + 1], pz = sinkpos[3 * sc + 2];
+// Bodies idx[j0..j1) of a pMM_sK1 list against one sink, staged through LDS 64
+// at a time (one coalesced load per tile, however short the segments are).
+// Every lane of the workgroup (one wave) takes part in the staging.
+__device__ __forceinline__ void mns_range(float px, float py, float pz,
+                                          const uint32_t *__restrict__ idx, int j0, int j1,
+                                          const float *__restrict__ srcpos, float mmass,
+                                          float eps_inv, float3 *tile, Acc4 &a) {
     const float eps2 = 1.0f / (eps_inv * eps_inv);
-    Acc4 a = {0.f, 0.f, 0.f, 0.f};
-    const int j0 = blockIdx.y * chunk, j1 = min(nidx, j0 + chunk);
     for (int t0 = j0; t0 < j1; t0 += 64) {
         const int nt = min(64, j1 - t0);
         if ((int)threadIdx.x < nt) {
@@ -142,6 +148,24 @@ __global__ __launch_bounds__(64) void k_mns(const float *__restrict__ sinkpos,
         }
         __syncthreads();
     }
+}
+
+// pMM_sK1 (nln/pgrav.cu:575-637): every sink against the bodies of a
+// segment list into the sink table.  The host expands the segments into body
+// indices; grid.y splits that list so that a call runs on several waves.
+__global__ __launch_bounds__(64) void k_mns(const float *__restrict__ sinkpos,
+                                            float *__restrict__ acc, int m,
+                                            const uint32_t *__restrict__ idx, int nidx, int chunk,
+                                            const float *__restrict__ srcpos, float mmass,
+                                            float eps_inv) {
+    __shared__ float3 tile[64];
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    const bool live = s < m;
+    const int sc = live ? s : 0;
+    Acc4 a = {0.f, 0.f, 0.f, 0.f};
+    const int j0 = blockIdx.y * chunk, j1 = min(nidx, j0 + chunk);
+    mns_range(sinkpos[3 * sc], sinkpos[3 * sc + 1], sinkpos[3 * sc + 2], idx, j0, j1, srcpos,
+              mmass, eps_inv, tile, a);
     if (live && j0 < j1)
         flush_acc(acc + 4 * s, a);
 }
@@ -251,6 +275,21 @@ __global__ __launch_bounds__(64 * W) void k_qnss(const float *__restrict__ sinkp
     sum_slices<W>(part, lane, w, live, a, acc + 4 * (size_t)(live ? s : 0));
 }
 
+// Cells idx[i0..i1) of the WalkInitSrcCUDA table against one sink (scalar loads:
+// every lane reads the same index and record).
+template <int QSIGN>
+__device__ __forceinline__ void qns_range(float px, float py, float pz,
+                                          const int *__restrict__ idx, int i0, int i1,
+                                          const float4 *__restrict__ pos4,
+                                          const float *__restrict__ mom, Acc4 &a) {
+    for (int i = i0; i < i1; i++) {
+        const int c = idx[i];
+        const float4 q = pos4[c];
+        cell_interact<2, true, QSIGN>(mom + (size_t)c * PREC2, q.w, px - q.x, py - q.y, pz - q.z,
+                                      0.0f, a);
+    }
+}
+
 // pQQ1 (nln/pgrav.cu:498-572) with the defects of A5 fixed: bound on the
 // sink count and idx[i] selecting the record.
 template <int QSIGN>
@@ -262,16 +301,86 @@ __global__ __launch_bounds__(64) void k_qns(const float *__restrict__ sinkpos,
     const int s = blockIdx.x * 64 + threadIdx.x;
     const bool live = s < m;
     const int sc = live ? s : 0;
-    const float px = sinkpos[3 * sc], py = sinkpos[3 * sc + 1], pz = sinkpos[3 * sc + 2];
     Acc4 a = {0.f, 0.f, 0.f, 0.f};
-    for (int i = 0; i < source_n; i++) {
-        const int c = idx[i];
-        const float4 q = pos4[c];
-        cell_interact<2, true, QSIGN>(mom + (size_t)c * PREC2, q.w, px - q.x, py - q.y, pz - q.z,
-                                      0.0f, a);
-    }
+    qns_range<QSIGN>(sinkpos[3 * sc], sinkpos[3 * sc + 1], sinkpos[3 * sc + 2], idx, 0, source_n,
+                     pos4, mom, a);
     if (live)
         flush_acc(acc + 4 * s, a);
+}
+
+// ------------------------------------------------------- deferred-call batches
+// One upload of many calls' lists and one launch per kind, instead of a copy and
+// one or two launches per call.  Each workgroup (one wave, 64 sinks) takes one
+// task: a call, a block of its sinks and a range of its sources; the call and
+// task records are wave-uniform, so they come through scalar loads.
+template <int ORDER>
+__global__ __launch_bounds__(128) void k_prep_batch(const float *__restrict__ data,
+                                                    const DeferredCall *__restrict__ calls,
+                                                    const int *__restrict__ block_call,
+                                                    float4 *__restrict__ pos4,
+                                                    float *__restrict__ mom) {
+    constexpr int MSTRIDE = ORDER == 4 ? PREC4 : PREC2;
+    const DeferredCall c = calls[block_call[blockIdx.x]];
+    const int i = (int)(blockIdx.x - c.prep / 128) * 128 + threadIdx.x;
+    if (i < c.n)
+        prep_source<ORDER>(data + c.src, i, c.sz, 1, pos4 + c.prep,
+                           mom + (size_t)c.prep * MSTRIDE);
+}
+
+template <int ORDER, int QSIGN, int SOFT>
+__global__ __launch_bounds__(64) void k_mn_batch(const float *__restrict__ sinkpos,
+                                                 float *__restrict__ acc,
+                                                 const DeferredCall *__restrict__ calls,
+                                                 const DeferredTask *__restrict__ tasks,
+                                                 const float4 *__restrict__ pos4,
+                                                 const float *__restrict__ mom) {
+    constexpr int MSTRIDE = ORDER == 4 ? PREC4 : PREC2;
+    const DeferredTask t = tasks[blockIdx.x];
+    const DeferredCall c = calls[t.x];
+    const int s = t.y + threadIdx.x;
+    const bool live = s < c.m;
+    const float *p = sinkpos + 3 * (c.sink + (live ? s : 0));
+    Acc4 a = {0.f, 0.f, 0.f, 0.f};
+    mn_range<ORDER, QSIGN, SOFT>(p[0], p[1], p[2], pos4 + c.prep, mom + (size_t)c.prep * MSTRIDE,
+                                 t.z, t.w, c.e, a);
+    if (live)
+        flush_acc(acc + 4 * (c.sink + s), a);
+}
+
+__global__ __launch_bounds__(64) void k_mns_batch(const float *__restrict__ sinkpos,
+                                                  float *__restrict__ acc,
+                                                  const DeferredCall *__restrict__ calls,
+                                                  const DeferredTask *__restrict__ tasks,
+                                                  const uint32_t *__restrict__ data) {
+    __shared__ float3 tile[64];
+    const DeferredTask t = tasks[blockIdx.x];
+    const DeferredCall c = calls[t.x];
+    const int s = t.y + threadIdx.x;
+    const bool live = s < c.m;
+    const float *p = sinkpos + 3 * (c.sink + (live ? s : 0));
+    Acc4 a = {0.f, 0.f, 0.f, 0.f};
+    mns_range(p[0], p[1], p[2], data + c.src, t.z, t.w, sinkpos, c.mass, c.e, tile, a);
+    if (live)
+        flush_acc(acc + 4 * (c.sink + s), a);
+}
+
+template <int QSIGN>
+__global__ __launch_bounds__(64) void k_qns_batch(const float *__restrict__ sinkpos,
+                                                  float *__restrict__ acc,
+                                                  const DeferredCall *__restrict__ calls,
+                                                  const DeferredTask *__restrict__ tasks,
+                                                  const int *__restrict__ data,
+                                                  const float4 *__restrict__ pos4,
+                                                  const float *__restrict__ mom) {
+    const DeferredTask t = tasks[blockIdx.x];
+    const DeferredCall c = calls[t.x];
+    const int s = t.y + threadIdx.x;
+    const bool live = s < c.m;
+    const float *p = sinkpos + 3 * (c.sink + (live ? s : 0));
+    Acc4 a = {0.f, 0.f, 0.f, 0.f};
+    qns_range<QSIGN>(p[0], p[1], p[2], data + c.src, t.z, t.w, pos4, mom, a);
+    if (live)
+        flush_acc(acc + 4 * (c.sink + s), a);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -403,6 +512,60 @@ hipError_t launch_qns(int qsign, const float *sinkpos, float *acc, int m, const 
     else
         hipLaunchKernelGGL(k_qns<1>, dim3((m + 63) / 64), dim3(64), 0, st, sinkpos, acc, m, idx,
                            source_n, pos4, mom);
+    return hipGetLastError();
+}
+
+hipError_t launch_mn_batch(int order, int soft, int qsign, const float *data,
+                           const DeferredCall *calls, const int *block_call, int nblocks,
+                           const DeferredTask *tasks, int ntasks, float4 *pos4, float *mom,
+                           const float *sinkpos, float *acc, hipStream_t st) {
+    if (nblocks <= 0 || ntasks <= 0)
+        return hipSuccess;
+#define PREP(O)                                                                                 \
+    hipLaunchKernelGGL(k_prep_batch<O>, dim3(nblocks), dim3(128), 0, st, data, calls, block_call, \
+                       pos4, mom)
+#define MNB(O, Q, S)                                                                            \
+    hipLaunchKernelGGL((k_mn_batch<O, Q, S>), dim3(ntasks), dim3(64), 0, st, sinkpos, acc, calls, \
+                       tasks, pos4, mom)
+    if (order == 0) {
+        PREP(0);
+        if (soft == SOFT_K1) MNB(0, 1, SOFT_K1);
+        else MNB(0, 1, SOFT_NONE);
+    } else if (order == 2) {
+        PREP(2);
+        if (qsign < 0) MNB(2, -1, SOFT_NONE);
+        else MNB(2, 1, SOFT_NONE);
+    } else {
+        PREP(4);
+        if (qsign < 0) MNB(4, -1, SOFT_NONE);
+        else MNB(4, 1, SOFT_NONE);
+    }
+#undef PREP
+#undef MNB
+    return hipGetLastError();
+}
+
+hipError_t launch_mns_batch(const uint32_t *data, const DeferredCall *calls,
+                            const DeferredTask *tasks, int ntasks, const float *sinkpos,
+                            float *acc, hipStream_t st) {
+    if (ntasks <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_mns_batch, dim3(ntasks), dim3(64), 0, st, sinkpos, acc, calls, tasks,
+                       data);
+    return hipGetLastError();
+}
+
+hipError_t launch_qns_batch(int qsign, const int *data, const DeferredCall *calls,
+                            const DeferredTask *tasks, int ntasks, const float4 *pos4,
+                            const float *mom, const float *sinkpos, float *acc, hipStream_t st) {
+    if (ntasks <= 0)
+        return hipSuccess;
+    if (qsign < 0)
+        hipLaunchKernelGGL(k_qns_batch<-1>, dim3(ntasks), dim3(64), 0, st, sinkpos, acc, calls,
+                           tasks, data, pos4, mom);
+    else
+        hipLaunchKernelGGL(k_qns_batch<1>, dim3(ntasks), dim3(64), 0, st, sinkpos, acc, calls,
+                           tasks, data, pos4, mom);
     return hipGetLastError();
 }
 

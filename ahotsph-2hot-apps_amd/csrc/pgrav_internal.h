@@ -5,6 +5,20 @@
 
 namespace pgrav {
 
+// One deferred call of the drop-in path (grav_mn_CUDA, pinteractCUDA,
+// grav_mns_CUDA, grav_qns_CUDA) as the batched kernels read it.
+struct DeferredCall {
+    int64_t sink;  // first sink (body index)
+    int64_t src;   // 4-byte word offset of the call's list in the batch data
+    int m, n;      // sinks; sources (M x N, pQQ1) or bodies (pMM_sK1)
+    int sz;        // M x N: floats per SoA-8 source record
+    int prep;      // M x N: first prepared record of the call (a multiple of 128)
+    float e;       // M x N: the call's e (pM_sK1: 1/eps); pMM_sK1: 1/eps
+    float mass;    // pMM_sK1: the body mass
+};
+// (call, first sink, first source, end source): one workgroup of a batched kernel
+typedef int4 DeferredTask;
+
 // Error reporting (pgrav_abi.hip).  The reference aborts the job on every
 // runtime error (Error() in libsw); pgrav_set_error_mode(1) turns that into a
 // recorded message that pgrav_last_error() returns, for callers that want to
@@ -31,6 +45,21 @@ hipError_t launch_qnss(int qsign, const float *sinkpos, float *acc, int m, const
                        int64_t max_len, const float4 *pos4, const float *mom, hipStream_t st);
 hipError_t launch_qns(int qsign, const float *sinkpos, float *acc, int m, const int *idx,
                       int source_n, const float4 *pos4, const float *mom, hipStream_t st);
+
+// The batched kernels of the deferred calls: `data` is the uploaded batch (lists
+// at the offsets of the calls), sinkpos / acc the whole device body table.
+// M x N kinds first prepare every call's sources (block_call: the call of each
+// 128-source prep block) into pos4 / mom, then evaluate the tasks.
+hipError_t launch_mn_batch(int order, int soft, int qsign, const float *data,
+                           const DeferredCall *calls, const int *block_call, int nblocks,
+                           const DeferredTask *tasks, int ntasks, float4 *pos4, float *mom,
+                           const float *sinkpos, float *acc, hipStream_t st);
+hipError_t launch_mns_batch(const uint32_t *data, const DeferredCall *calls,
+                            const DeferredTask *tasks, int ntasks, const float *sinkpos,
+                            float *acc, hipStream_t st);
+hipError_t launch_qns_batch(int qsign, const int *data, const DeferredCall *calls,
+                            const DeferredTask *tasks, int ntasks, const float4 *pos4,
+                            const float *mom, const float *sinkpos, float *acc, hipStream_t st);
 
 int quad_phi_sign();
 

@@ -73,6 +73,12 @@ def set_quad_phi_sign(sign):
     lib().pgrav_set_quad_phi_sign(int(sign))
 
 
+def set_dropin_deferral(enable):
+    """1 (default): the per-sink-group drop-in calls are batched and evaluated by
+    one launch per kind; 0: one copy and launch per call.  Returns the old value."""
+    return lib().pgrav_set_dropin_deferral(int(bool(enable)))
+
+
 class GravCuda:
     """Thin object view of the drop-in ABI (nln/gravcuda.h) used by tests and
     by host drivers written in Python.  Arrays are kept alive by the caller."""

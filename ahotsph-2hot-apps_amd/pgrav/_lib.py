@@ -56,6 +56,7 @@ SIGNATURES = {
     "pgrav_clear_error": (None, []),
     "pgrav_set_quad_phi_sign": (None, [_i]),
     "pgrav_synchronize": (_i, []),
+    "pgrav_set_dropin_deferral": (_i, [_i]),
 }
 
 

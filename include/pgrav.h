@@ -114,7 +114,16 @@ void pgrav_clear_error(void);
  * of nln/do_grav_sse4.c:1836 and nln/pgrav_vec.c:117; -1: the sign of
  * nln/pgrav.cu:114,277 ("phi -= eqe").  Accelerations are identical. */
 void pgrav_set_quad_phi_sign(int sign);
+/* Runs the deferred calls (below), then waits for the device. */
 int pgrav_synchronize(void);
+/* 1 (default): grav_mn_CUDA, pinteractCUDA, grav_mns_CUDA and grav_qns_CUDA
+ * copy their lists into a pinned batch and return; a batch is uploaded in one
+ * copy and evaluated by one kernel per kind when it holds 16 MB of lists, and
+ * before WalkTerminateSinkCUDA (and every other call that reads the results or
+ * replaces a table) returns.  Results are those of one launch per call, summed
+ * in another order.  0: one copy and launch per call on the call's slot, as
+ * nln/pgrav.cu:1043-1084 does.  Returns the previous setting. */
+int pgrav_set_dropin_deferral(int enable);
 
 /* ------------------------------------------------------------------ part 3
  * Batched path.  No single reference counterpart: it replaces the host walk's

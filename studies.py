@@ -314,24 +314,38 @@ def dropin(args):
     btab[:, 1:4] = spos
     flat = btab.reshape(-1)
     gc = pgrav.GravCuda()
-    best = None
-    for rep in range(3):
-        btab[:, 7:11] = 0
-        t1 = time.perf_counter()
-        gc.walk_init_sink(flat, 14)
-        for first, cnt, soa, source_n, seg, nb in work:
-            q = -1
-            while q < 0:
-                q, _ = gc.qalloc(count=False)
-            if source_n:
-                gc.grav_mn("pH", first, cnt, soa, source_n, 29, 0.0, q)
-            if nb:
-                gc.grav_mns("pMM_sK1", first, cnt, seg, nb, float(mass[0]), 1.0 / eps, q)
-        gc.walk_terminate_sink(flat, 14)
-        ms = (time.perf_counter() - t1) * 1e3
-        best = ms if best is None else min(best, ms)
-        _log("dropin rep %d: %.1f ms" % (rep, ms))
-    got = btab[:, 7:11].astype(np.float64)
+    times, results = {}, {}
+    for mode in (0, 1):  # one launch per call, then the deferred batches
+        pgrav.set_dropin_deferral(mode)
+        for rep in range(3):
+            btab[:, 7:11] = 0
+            t1 = time.perf_counter()
+            gc.walk_init_sink(flat, 14)
+            for first, cnt, soa, source_n, seg, nb in work:
+                q = -1
+                while q < 0:
+                    q, _ = gc.qalloc(count=False)
+                if source_n:
+                    gc.grav_mn("pH", first, cnt, soa, source_n, 29, 0.0, q)
+                if nb:
+                    gc.grav_mns("pMM_sK1", first, cnt, seg, nb, float(mass[0]), 1.0 / eps, q)
+            gc.walk_terminate_sink(flat, 14)
+            ms = (time.perf_counter() - t1) * 1e3
+            times[mode] = min(times.get(mode, ms), ms)
+            _log("dropin deferral %d rep %d: %.1f ms" % (mode, rep, ms))
+        results[mode] = btab[:, 7:11].astype(np.float64)
+    # the same loop with an empty library call in place of each call: the share of
+    # the wall time that is Python and ctypes, not the library
+    t1 = time.perf_counter()
+    for first, cnt, soa, source_n, seg, nb in work:
+        gc.qalloc(count=False)
+        if source_n:
+            gc.vecwidth()
+        if nb:
+            gc.vecwidth()
+    loop_ms = (time.perf_counter() - t1) * 1e3
+    best = times[1]
+    got = results[1]
     plan = batch.Plan(pos, mass, fp64=True, **kw)
     plan.evaluate("abs32")
     kern = plan.time("abs32", reps=3)
@@ -340,8 +354,11 @@ def dropin(args):
     plan.close()
     da = np.linalg.norm(got[:, :3] - ref[:, :3], axis=1) / np.linalg.norm(ref[:, :3], axis=1)
     pairs = st["pairs_hexa"] + st["pairs_body"]
+    dd = np.linalg.norm(results[0][:, :3] - got[:, :3], axis=1) / np.linalg.norm(ref[:, :3], axis=1)
     row = {"config": "dropin", "n": int(n), "groups": G, "pairs_hexa": st["pairs_hexa"],
-           "pairs_body": st["pairs_body"], "dropin_ms": best,
+           "pairs_body": st["pairs_body"], "dropin_ms": best, "per_call_ms": times[0],
+           "per_call_vs_deferred_max_rel": float(dd.max()), "python_loop_ms": loop_ms,
+           "list_mb": float(sum(w[2].nbytes * (w[3] > 0) + 4 * w[5] for w in work) / 1e6),
            "dropin_interactions_per_s": pairs / (best * 1e-3), "calls": int(sum(
                (w[3] > 0) + (w[5] > 0) for w in work)),
            "batched_abs32_kernel_ms": kern, "batched_interactions_per_s": pairs / (kern * 1e-3),

@@ -17,6 +17,14 @@ pytestmark = pytest.mark.gpu
 STRIDE = 14  # sizeof(body)/sizeof(float), nln/physics.h:33-44
 
 
+@pytest.fixture(params=[1, 0], ids=["deferred", "per_call"], autouse=True)
+def deferral(gpu_lib, request):
+    """Every test runs with the calls batched (the default) and launched one by one."""
+    old = gpu_lib.set_dropin_deferral(request.param)
+    yield request.param
+    gpu_lib.set_dropin_deferral(old)
+
+
 def _bodies(rng, n, scale=3.0):
     b = np.zeros((n, STRIDE), np.float32)
     b[:, 0] = 1.0 / n
@@ -318,3 +326,90 @@ def test_py_pgrav_monopole_known_answer(gpu_lib):
     accp3 = np.zeros(4, np.float32)
     pgrav_cu.monopole(sink, accp3, 4, np.ascontiguousarray(swiz))
     np.testing.assert_allclose(accp3, exact, rtol=0, atol=2e-6 * np.abs(exact).max())
+
+
+def _many_calls(pgrav, rng, btab, calls):
+    """Runs (routine, first, m, f, n, sz, e) calls on rotating slots; returns the
+    results added into the body table."""
+    g = pgrav.GravCuda()
+    flat = btab.reshape(-1)
+    before = btab[:, 7:11].copy()
+    g.walk_init_sink(flat, STRIDE)
+    for routine, first, m, f, n, sz, e in calls:
+        q = -1
+        while q < 0:
+            q, _ = g.qalloc(count=False)
+        if routine == "pMM_sK1":
+            g.grav_mns(routine, first, m, f, n, 0.001, e, q)
+        else:
+            g.grav_mn(routine, first, m, f, n, sz, e, q)
+    g.walk_terminate_sink(flat, STRIDE)
+    out = btab[:, 7:11] - before
+    btab[:, 7:11] = before
+    return out.astype(np.float64)
+
+
+def test_deferred_batches_fill_and_flush(gpu_lib, deferral):
+    """Lists over several 16 MB batches, a call larger than a batch, every kind mixed
+    and sinks shared between calls: the batched results equal one launch per call
+    up to the FP32 order of the sums, and an oracle spot check pins both."""
+    if deferral == 0:
+        pytest.skip("compares the two modes; runs once")
+    pgrav = gpu_lib
+    rng = np.random.default_rng(21)
+    nb = 4096
+    btab = _bodies(rng, nb, scale=2.0)
+    recs = _cells(rng, 64)
+    pools = {sz: oracle.to_soa(np.tile(recs[:, :sz], (40, 1)), 8) for sz in (4, 13, 29)}
+    calls = []
+    for k in range(150):
+        kind = ("pH", "pQ", "pM", "pM_sK1", "pMM_sK1")[k % 5]
+        first = int(rng.integers(0, nb - 100))
+        m = int(rng.integers(1, 100))
+        if kind == "pMM_sK1":
+            seg = np.array([[rng.integers(0, nb - 600), rng.integers(1, 600)],
+                            [rng.integers(0, nb - 50), rng.integers(0, 50)]], np.uint32)
+            calls.append((kind, first, m, seg.reshape(-1), int(seg[:, 1].sum()), 0, 4.0))
+            continue
+        sz = {"pH": 29, "pQ": 13, "pM": 4, "pM_sK1": 4}[kind]
+        n = 8 * int(rng.integers(1, 320))
+        calls.append((kind, first, m, pools[sz], n, sz, 2.0 if kind == "pM_sK1" else 0.0))
+    big = oracle.to_soa(np.tile(recs[:, :29], (2600, 1)), 8)  # 166,400 sources: 19 MB
+    calls.insert(70, ("pH", 5, 40, big, big.size // 29, 29, 0.0))
+    got = _many_calls(pgrav, rng, btab, calls)
+    old = pgrav.set_dropin_deferral(0)
+    try:
+        ref = _many_calls(pgrav, rng, btab, calls)
+    finally:
+        pgrav.set_dropin_deferral(old)
+    scale = np.abs(ref).max(axis=1, keepdims=True) + 1e-30
+    assert np.abs(got - ref).max() / scale.max() < 1e-5
+    assert (np.abs(got - ref) / scale).max() < 1e-4
+    # oracle spot check: the first pH call alone
+    kind, first, m, f, n, sz, e = calls[0]
+    one = _many_calls(pgrav, rng, btab, [calls[0]])
+    o = oracle.pmn(4, btab[:, 1:4], f, n, sz)
+    _close(one[first:first + m], o[first:first + m])
+
+
+def test_deferred_calls_flush_before_the_source_table_changes(gpu_lib, deferral):
+    """pQQ1 calls queued against one WalkInitSrcCUDA table are evaluated before the
+    next table replaces it."""
+    pgrav = gpu_lib
+    rng = np.random.default_rng(22)
+    g = pgrav.GravCuda()
+    btab = _bodies(rng, 90)
+    flat = btab.reshape(-1)
+    recs_a, recs_b = _cells(rng, 40), _cells(rng, 40)
+    idx = rng.permutation(40)[:25].astype(np.int32)
+    g.walk_init_sink(flat, STRIDE)
+    for recs in (recs_a, recs_b):
+        raw = np.frombuffer(_hexacell_table(recs).tobytes(), np.uint8).copy()
+        g.walk_init_src(raw, 160, 40)
+        q, _ = g.qalloc()
+        g.grav_qns("pQ1", 3, 80, idx, q)
+    g.walk_terminate_sink(flat, STRIDE)
+    g.walk_terminate_src()
+    ref = sum(oracle.pqq1(btab[3:83, 1:4], idx, r[:, :13].astype(np.float32))
+              for r in (recs_a, recs_b))
+    _close(btab[3:83, 7:11], ref)
