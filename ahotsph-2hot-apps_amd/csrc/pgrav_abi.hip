@@ -410,9 +410,20 @@ static void batch_tasks(Batch &b, int kind, int m, int n) {
     b.ncalls++;
 }
 
+static int64_t task_count(int kind, int64_t m, int64_t n) {
+    return ((m + 63) / 64) * ((n + CHUNK[kind] - 1) / CHUNK[kind]);
+}
+
 static size_t task_bytes(int kind, int m, int n) {
-    return sizeof(DeferredCall) + sizeof(DeferredTask) * (size_t)((m + 63) / 64) *
-                                      (size_t)((n + CHUNK[kind] - 1) / CHUNK[kind]);
+    return sizeof(DeferredCall) + sizeof(DeferredTask) * (size_t)task_count(kind, m, n);
+}
+
+// A call with more tasks than this fills the GPU by itself and runs at once on
+// its slot (its task table alone would take more than 1 MB of the batch).
+static const int64_t DEFER_MAX_TASKS = 65536;
+
+static int mn_kind(int order, int soft) {
+    return order == 0 ? (soft == SOFT_K1 ? BK_MK1 : BK_M) : (order == 2 ? BK_Q : BK_H);
 }
 
 }  // namespace pgrav
@@ -686,10 +697,10 @@ static void mn_common(int order, int soft, const float *p, int m, int stride, co
         return;
     }
     size_t bytes = (size_t)source_n * sz * sizeof(float);
-    if (S.defer) {
+    if (S.defer && task_count(mn_kind(order, soft), m, source_n) <= DEFER_MAX_TASKS) {
         if (m <= 0 || source_n <= 0)
             return;
-        const int kind = order == 0 ? (soft == SOFT_K1 ? BK_MK1 : BK_M) : (order == 2 ? BK_Q : BK_H);
+        const int kind = mn_kind(order, soft);
         const int nblk = (source_n + 127) / 128;
         const size_t tb = task_bytes(kind, m, source_n) + sizeof(int) * (size_t)nblk;
         Batch *b = batch_room(bytes + 64, tb);
@@ -785,17 +796,17 @@ void grav_mns_CUDA(const char *routine, const int base, const int m, const segme
         }
         total += seg[g].length;
     }
+    if (total > INT32_MAX) {
+        fail("pMM_sK1 segments hold %lld bodies (at most %d)", (long long)total, INT32_MAX);
+        return;
+    }
     if (total != source_n)
         S.seg_mismatch++;  // assert(sum_n == source_n), nln/pgrav.cu:636; reported at terminate
     // the segments expanded into body indices: one coalesced load per 64 bodies
     size_t bytes = (size_t)total * sizeof(uint32_t);
-    if (S.defer) {
+    if (S.defer && task_count(BK_MNS, m, total) <= DEFER_MAX_TASKS) {
         if (m <= 0 || total <= 0)
             return;
-        if (total > INT32_MAX) {
-            fail("pMM_sK1 list of %lld bodies", (long long)total);
-            return;
-        }
         const size_t tb = task_bytes(BK_MNS, m, (int)total);
         Batch *b = batch_room(bytes + 64, tb);
         if (!b)
@@ -892,7 +903,7 @@ void grav_qns_CUDA(const char *routine, const int base, const int m, const int *
             return;
         }
     size_t bytes = (size_t)source_n * sizeof(int);
-    if (S.defer) {
+    if (S.defer && task_count(BK_QNS, m, source_n) <= DEFER_MAX_TASKS) {
         if (m <= 0 || source_n <= 0)
             return;
         const size_t tb = task_bytes(BK_QNS, m, source_n);

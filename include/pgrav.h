@@ -56,8 +56,9 @@ void WalkTerminateSrcCUDA(void);
  * (stride floats per body, position at +1..3); copies positions to the device
  * and zeroes the device accumulator. */
 void WalkInitSinkCUDA(float *btab, int stride, int64_t nobj);
-/* nln/gravcuda.h:21 (impl nln/pgrav.cu:969-1020): waits for every slot, then
- * btab[i*stride+7..10] += (ax, ay, az, phi) accumulated since WalkInitSinkCUDA. */
+/* nln/gravcuda.h:21 (impl nln/pgrav.cu:969-1020): runs the deferred calls and
+ * waits for every slot, then btab[i*stride+7..10] += (ax, ay, az, phi)
+ * accumulated since WalkInitSinkCUDA. */
 void WalkTerminateSinkCUDA(float *btab, int stride, int64_t nobj);
 /* nln/gravcuda.h:22-32 (impl nln/pgrav.cu:1026-1087): routine "pM", "pQ",
  * "pH" or "pM_sK1"; p points into the body table passed to WalkInitSinkCUDA
@@ -122,7 +123,9 @@ int pgrav_synchronize(void);
  * before WalkTerminateSinkCUDA (and every other call that reads the results or
  * replaces a table) returns.  Results are those of one launch per call, summed
  * in another order.  0: one copy and launch per call on the call's slot, as
- * nln/pgrav.cu:1043-1084 does.  Returns the previous setting. */
+ * nln/pgrav.cu:1043-1084 does.  A call of more than 65,536 tasks (64 sinks by
+ * 128-512 sources each) runs at once in either mode.  Returns the previous
+ * setting. */
 int pgrav_set_dropin_deferral(int enable);
 
 /* ------------------------------------------------------------------ part 3

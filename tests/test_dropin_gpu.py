@@ -413,3 +413,26 @@ def test_deferred_calls_flush_before_the_source_table_changes(gpu_lib, deferral)
     ref = sum(oracle.pqq1(btab[3:83, 1:4], idx, r[:, :13].astype(np.float32))
               for r in (recs_a, recs_b))
     _close(btab[3:83, 7:11], ref)
+
+
+def test_call_too_large_to_defer_runs_at_once(gpu_lib):
+    """A call of more than 65,536 tasks (here 313 sink blocks x 235 source chunks)
+    takes the one-launch path even when deferral is on; sampled sinks against the
+    oracle."""
+    pgrav = gpu_lib
+    rng = np.random.default_rng(23)
+    g = pgrav.GravCuda()
+    m, n = 20000, 120000
+    btab = _bodies(rng, m, scale=1.0)
+    src = np.zeros((n, 4), np.float32)
+    src[:, 0] = rng.uniform(0.5, 1.5, n) / n
+    src[:, 1:4] = rng.uniform(-3, 3, (n, 3))
+    f = oracle.to_soa(src, 8)
+    flat = btab.reshape(-1)
+    g.walk_init_sink(flat, STRIDE)
+    q, _ = g.qalloc()
+    g.grav_mn("pM", 0, m, f, n, 4, 0.0, q)
+    g.walk_terminate_sink(flat, STRIDE)
+    pick = rng.choice(m, 300, replace=False)
+    ref = oracle.pmn(0, btab[pick, 1:4], f, n, 4)
+    _close(btab[pick, 7:11], ref, tol=1e-4)
