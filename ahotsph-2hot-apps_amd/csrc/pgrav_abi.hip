@@ -272,9 +272,14 @@ static bool sinks_ready(int64_t first, int64_t m) {
 static bool flush_batch();
 
 // Runs the deferred calls and waits for every queued evaluation.
+// A batch that fails to run has its error recorded; every stream is still
+// drained, so that no kernel reads a buffer the caller goes on to release.
 static void wait_all() {
-    if (S.init && (!flush_batch() || !check(hipStreamSynchronize(S.bstream), "hipStreamSynchronize")))
-        return;
+    if (S.init) {
+        flush_batch();
+        if (!check(hipStreamSynchronize(S.bstream), "hipStreamSynchronize"))
+            return;
+    }
     for (int i = 0; i < NQ; i++) {
         Slot &s = S.slot[i];
         if (s.inuse) {
