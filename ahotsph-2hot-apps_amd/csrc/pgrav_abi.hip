@@ -310,21 +310,27 @@ static bool flush_batch() {
     // the tables go behind the lists, into the same copy
     char *h = (char *)b.host.p;
     size_t off = align128(b.used);
-    size_t o_calls[BK_KINDS], o_tasks[BK_KINDS], o_blocks[BK_KINDS];
-    for (int k = 0; k < BK_KINDS; k++) {
-        o_calls[k] = off;
-        memcpy(h + off, b.calls[k].data(), b.calls[k].size() * sizeof(DeferredCall));
-        off = align128(off + b.calls[k].size() * sizeof(DeferredCall));
-        o_tasks[k] = off;
-        memcpy(h + off, b.tasks[k].data(), b.tasks[k].size() * sizeof(DeferredTask));
-        off = align128(off + b.tasks[k].size() * sizeof(DeferredTask));
-        o_blocks[k] = off;
-        memcpy(h + off, b.blocks[k].data(), b.blocks[k].size() * sizeof(int));
-        off = align128(off + b.blocks[k].size() * sizeof(int));
-    }
-    if (off > b.host.cap) {
-        fail("deferred batch overran its staging buffer (%zu > %zu bytes)", off, b.host.cap);
+    size_t end = off;
+    for (int k = 0; k < BK_KINDS; k++)
+        end += align128(b.calls[k].size() * sizeof(DeferredCall)) +
+               align128(b.tasks[k].size() * sizeof(DeferredTask)) +
+               align128(b.blocks[k].size() * sizeof(int));
+    if (end > b.host.cap) {  // batch_room reserves this room; checked before any write
+        fail("deferred batch overran its staging buffer (%zu > %zu bytes)", end, b.host.cap);
         return false;
+    }
+    size_t o_calls[BK_KINDS], o_tasks[BK_KINDS], o_blocks[BK_KINDS];
+    auto put = [&](const void *src, size_t bytes) {
+        const size_t at = off;
+        if (bytes)
+            memcpy(h + at, src, bytes);
+        off = align128(off + bytes);
+        return at;
+    };
+    for (int k = 0; k < BK_KINDS; k++) {
+        o_calls[k] = put(b.calls[k].data(), b.calls[k].size() * sizeof(DeferredCall));
+        o_tasks[k] = put(b.tasks[k].data(), b.tasks[k].size() * sizeof(DeferredTask));
+        o_blocks[k] = put(b.blocks[k].data(), b.blocks[k].size() * sizeof(int));
     }
     // a buffer still read by queued kernels is replaced only after they ran
     const size_t npos = (size_t)b.prep[BK_M] + b.prep[BK_MK1] + b.prep[BK_Q] + b.prep[BK_H];
