@@ -430,8 +430,13 @@ static size_t task_bytes(int kind, int m, int n) {
 }
 
 // A call with more tasks than this fills the GPU by itself and runs at once on
-// its slot (its task table alone would take more than 1 MB of the batch).
+// its slot (its task table alone would take more than 1 MB of the batch), as
+// does a call whose list is larger than a batch: the batches keep their size.
 static const int64_t DEFER_MAX_TASKS = 65536;
+
+static bool deferrable(int kind, int64_t m, int64_t n, size_t bytes) {
+    return S.defer && bytes <= BATCH_BYTES && task_count(kind, m, n) <= DEFER_MAX_TASKS;
+}
 
 static int mn_kind(int order, int soft) {
     return order == 0 ? (soft == SOFT_K1 ? BK_MK1 : BK_M) : (order == 2 ? BK_Q : BK_H);
@@ -708,7 +713,7 @@ static void mn_common(int order, int soft, const float *p, int m, int stride, co
         return;
     }
     size_t bytes = (size_t)source_n * sz * sizeof(float);
-    if (S.defer && task_count(mn_kind(order, soft), m, source_n) <= DEFER_MAX_TASKS) {
+    if (deferrable(mn_kind(order, soft), m, source_n, bytes)) {
         if (m <= 0 || source_n <= 0)
             return;
         const int kind = mn_kind(order, soft);
@@ -815,7 +820,7 @@ void grav_mns_CUDA(const char *routine, const int base, const int m, const segme
         S.seg_mismatch++;  // assert(sum_n == source_n), nln/pgrav.cu:636; reported at terminate
     // the segments expanded into body indices: one coalesced load per 64 bodies
     size_t bytes = (size_t)total * sizeof(uint32_t);
-    if (S.defer && task_count(BK_MNS, m, total) <= DEFER_MAX_TASKS) {
+    if (deferrable(BK_MNS, m, total, bytes)) {
         if (m <= 0 || total <= 0)
             return;
         const size_t tb = task_bytes(BK_MNS, m, (int)total);
@@ -914,7 +919,7 @@ void grav_qns_CUDA(const char *routine, const int base, const int m, const int *
             return;
         }
     size_t bytes = (size_t)source_n * sizeof(int);
-    if (S.defer && task_count(BK_QNS, m, source_n) <= DEFER_MAX_TASKS) {
+    if (deferrable(BK_QNS, m, source_n, bytes)) {
         if (m <= 0 || source_n <= 0)
             return;
         const size_t tb = task_bytes(BK_QNS, m, source_n);

@@ -124,8 +124,8 @@ int pgrav_synchronize(void);
  * replaces a table) returns.  Results are those of one launch per call, summed
  * in another order.  0: one copy and launch per call on the call's slot, as
  * nln/pgrav.cu:1043-1084 does.  A call of more than 65,536 tasks (64 sinks by
- * 128-512 sources each) runs at once in either mode.  Returns the previous
- * setting. */
+ * 128-512 sources each) or of more than 16 MB of list runs at once in either
+ * mode.  Returns the previous setting. */
 int pgrav_set_dropin_deferral(int enable);
 
 /* ------------------------------------------------------------------ part 3

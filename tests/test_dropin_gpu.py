@@ -350,9 +350,10 @@ def _many_calls(pgrav, rng, btab, calls):
 
 
 def test_deferred_batches_fill_and_flush(gpu_lib, deferral):
-    """Lists over several 16 MB batches, a call larger than a batch, every kind mixed
-    and sinks shared between calls: the batched results equal one launch per call
-    up to the FP32 order of the sums, and an oracle spot check pins both."""
+    """Lists over several 16 MB batches, a call larger than a batch (which runs at
+    once between the deferred ones), every kind mixed and sinks shared between
+    calls: the batched results equal one launch per call up to the FP32 order of
+    the sums, and an oracle spot check pins both."""
     if deferral == 0:
         pytest.skip("compares the two modes; runs once")
     pgrav = gpu_lib
