@@ -50,6 +50,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -170,6 +171,36 @@ struct SoA {
     }
 };
 
+// The emitted source blocks, the walk's largest output, in fixed chunks of 64 MB
+// that are filled in order: it grows without reallocating, where a doubling
+// vector would copy itself at every growth and touch about twice its final size
+// of fresh memory.  Offsets are global, as in one array.
+struct SourceStore {
+    static constexpr size_t CH = (size_t)1 << 24;  // floats per chunk
+    std::vector<std::unique_ptr<float[]>> chunk;
+    size_t n = 0;
+    size_t size() const { return n; }
+    void append(const float *p, size_t k) {
+        while (k) {
+            if (n == chunk.size() * CH)
+                chunk.emplace_back(new float[CH]);
+            const size_t at = n % CH, t = std::min(CH - at, k);
+            memcpy(chunk.back().get() + at, p, t * sizeof(float));
+            p += t;
+            k -= t;
+            n += t;
+        }
+    }
+    float &operator[](size_t i) { return chunk[i / CH][i % CH]; }
+    void copy_out(float *dst) const {
+        for (size_t c = 0, left = n; left; c++) {
+            const size_t t = std::min(CH, left);
+            memcpy(dst + c * CH, chunk[c].get(), t * sizeof(float));
+            left -= t;
+        }
+    }
+};
+
 struct DLWalk {
     TreeParams tp;
     pgrav_dl_params dp;
@@ -185,7 +216,7 @@ struct DLWalk {
     SoA H{HSZ}, Q{QSZ}, M{MSZ};
     // outputs
     std::vector<int64_t> calls;   // 6 per M x N call: routine (0 pH, 1 pQ), first, m, src_off, source_n, sz
-    std::vector<float> src;       // every emitted source block
+    SourceStore src;              // every emitted source block
     std::vector<int64_t> qflush;  // 6 per pQ1 flush: sink_base, m, seg_off, src_off, source_len, 0
     std::vector<uint32_t> qseg;   // 2 per queued sink: base, length
     std::vector<int64_t> tails;   // 6 per body slot: h_off, h_n, q_off, q_n, m_off, m_n
@@ -524,7 +555,7 @@ struct DLWalk {
         const int64_t off = (int64_t)src.size();
         for (int64_t b = b0; b < b1; b++) {
             const float *p = st.block(b);
-            src.insert(src.end(), p, p + (size_t)st.sz * NSSE);
+            src.append(p, (size_t)st.sz * NSSE);
         }
         return off;
     }
@@ -543,7 +574,7 @@ struct DLWalk {
                               (int64_t)(qq.source.size() / QSZ), 0};
         qflush.insert(qflush.end(), v, v + 6);
         qseg.insert(qseg.end(), qq.seg.begin(), qq.seg.end());
-        src.insert(src.end(), qq.source.begin(), qq.source.end());
+        src.append(qq.source.data(), qq.source.size());
         stats[3]++;
         qq.sink_base += qq.m;
         qq.inuse = false;
@@ -703,7 +734,7 @@ struct DLWalk {
             undo.push_back({kind * ((int64_t)1 << 56) + b, bo[b]});
             const int64_t off = (int64_t)src.size();
             const float *p = st.block(b);
-            src.insert(src.end(), p, p + (size_t)st.sz * NSSE);
+            src.append(p, (size_t)st.sz * NSSE);
             if (b == b1 - 1 && c1 % NSSE)
                 for (int f = 0; f < st.sz; f++)
                     for (int r = (int)(c1 % NSSE); r < NSSE; r++) src[off + f * NSSE + r] = 0.0f;
@@ -1365,7 +1396,11 @@ int64_t pgrav_dl_array(void *h, int which, void *out) {
     case PGRAV_DL_CHILD_OFF: take(W->child_off, 1); break;
     case PGRAV_DL_CHILDREN: take(W->children, 1); break;
     case PGRAV_DL_CALLS: take(W->calls, 6); break;
-    case PGRAV_DL_SOURCES: take(W->src, 1); break;
+    case PGRAV_DL_SOURCES:
+        count = (int64_t)W->src.size();
+        if (out)
+            W->src.copy_out((float *)out);
+        return count;
     case PGRAV_DL_QFLUSH: take(W->qflush, 6); break;
     case PGRAV_DL_QSEG: take(W->qseg, 2); break;
     case PGRAV_DL_TAILS: take(W->tails, 6); break;
